@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Throughput of the RRTMGP LW+SW radiation hot path on one MI355X.
+
+A step is one rrtmgp_lw_main_run (all sky, 3 Gauss angles, 128 g-points)
+plus one rrtmgp_sw_main_run (all sky, 112 g-points) over 262,144 columns x
+127 layers, with every input already resident in HBM (device-pointer C-ABI
+entries).  value = columns per second of the LW+SW pair.
+
+The JSON line also carries:
+  roofline     : the dominant kernel's algorithmic bytes / its mean launch
+                 time (HIP events on the launch stream) against 8 TB/s HBM
+  cpu_baseline : the FP64 oracle timed on the host cores over a bounded
+                 sample of the same workload, and the flux / heating-rate
+                 error of the GPU result on that sample against it
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "ccpp-physics_amd"))
+
+HBM_PEAK_GBS = 8000.0
+NCOL = 262_144
+NLAY = 127
+UNIQUE = 32_768        # distinct synthetic columns, tiled to NCOL on the device
+METRIC = "LW+SW columns/s at 127 layers (table vs NN) @ HR RMSE<=0.1 K/day vs FP64"
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--ncol", type=int, default=NCOL)
+    p.add_argument("--cpu-sample", type=int, default=2048)
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def build_inputs(kd_lw, kd_sw, ncol, torch):
+    from ccpprad import synthetic as syn, capi, device
+    uniq = min(UNIQUE, ncol)
+    pr = syn.make_profiles(uniq, NLAY, seed=20261203, clouds=True, kd=kd_lw)
+    arr = capi.profile_fields(pr)
+    reps = (ncol + uniq - 1) // uniq
+    dev = {}
+    for k, v in arr.items():
+        t = torch.from_numpy(np.ascontiguousarray(v)).cuda()
+        if t.dim() == 1:
+            t = t.repeat(reps)[:ncol].contiguous()
+        else:
+            t = t.repeat(1, reps)[:, :ncol].contiguous()
+        dev[k] = t
+    dev["idx"] = torch.arange(1, ncol + 1, dtype=torch.int32, device="cuda")
+    return pr, arr, dev
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU column sharding is described "
+                                   "in DESIGN.md and not launched", "n_gpus": a.gpus}))
+        sys.exit(2)
+    import torch
+    from ccpprad import synthetic as syn, capi, lib, device
+
+    kd_lw, kd_sw = syn.make_kdist(True), syn.make_kdist(False)
+    lut_lw, lut_sw = syn.make_cloud_lut(True), syn.make_cloud_lut(False)
+    ncol = a.ncol
+    blksz = 64
+    pr, arr, dev = build_inputs(kd_lw, kd_sw, ncol, torch)
+    lw = lib.LWScheme(kd_lw, lut_lw, NLAY, blksz=blksz)
+    sw = lib.SWScheme(kd_sw, lut_sw, NLAY, blksz=blksz)
+    lw_out = device.lw_outputs(ncol, NLAY)
+    sw_out = device.sw_outputs(ncol, NLAY)
+    lw_args = dict(dev)
+    lw_args.update(lw_out)
+    sw_args = dict(dev)
+    sw_args.update(sw_out)
+    lw_ints = capi.lw_default_ints(ncol, NLAY, blksz=blksz, nangle=3)
+    sw_ints = capi.sw_default_ints(ncol, ncol, NLAY, blksz=blksz)
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+
+    def step(validate=False):
+        lw.run_dev(lw_ints, lw_args, stream=sp, validate=validate)
+        sw.run_dev(sw_ints, sw_args, stream=sp, validate=validate)
+
+    print(f"# setup done: {ncol} columns x {NLAY} layers", file=sys.stderr, flush=True)
+    step(validate=True)
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / a.steps
+    value = ncol / (ms / 1e3)
+    print(f"# timed {a.steps} steps: {ms:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # ---- per-kernel times with HIP events on the launch stream (one extra step)
+    L = lib.load()
+    for s in (lw, sw):
+        L.ccpprad_timing_enable(s.ctx, 1)
+    step()
+    kt = {}
+    names = {0: "gas_optics", 1: "lw_solve", 2: "sw_solve", 3: "cloud_optics", 4: "mcica"}
+    for tag, s in (("lw", lw), ("sw", sw)):
+        for kid, nm in names.items():
+            tot, cnt = C.c_double(), C.c_int()
+            L.ccpprad_timing_query(s.ctx, kid, C.byref(tot), C.byref(cnt))
+            if cnt.value:
+                kt[f"{tag}_{nm}"] = (tot.value, cnt.value)
+        L.ccpprad_timing_enable(s.ctx, 0)
+    # algorithmic bytes per launch (column chunk of 65,536 at most)
+    G_LW, G_SW, B_LW, B_SW, V = kd_lw.ngpt, kd_sw.ngpt, kd_lw.nband, kd_sw.nband, NLAY + 1
+    chunk = min(ncol, 65536)
+    per_col = {
+        # tau + pfrac read once (FP32), cloud absorption depth per band, mask
+        # bits, T/p inputs (FP64), fluxes x4 + HR out (FP64)
+        "lw_lw_solve": NLAY * G_LW * 8 + NLAY * B_LW * 4 + NLAY * 16 + (NLAY + 2 * V + 2) * 8 + (4 * V + NLAY) * 8,
+        # tau_abs + tau_ray (FP32), cloud tau/ssa/g per band, mask bits,
+        # fluxes + HR out
+        "sw_sw_solve": NLAY * G_SW * 8 + NLAY * B_SW * 12 + NLAY * 16 + (5 + V) * 8 + (4 * V + NLAY) * 8,
+        # p, T, 6 vmr, p_lev inputs (FP64) -> tau + pfrac (FP32)
+        "lw_gas_optics": (8 * NLAY + V) * 8 + NLAY * G_LW * 8,
+        "sw_gas_optics": (8 * NLAY + V) * 8 + NLAY * G_SW * 8,
+    }
+    dom = max(kt.items(), key=lambda kv: kv[1][0])
+    dname, (dtot, dcnt) = dom
+    avg_ms = dtot / dcnt
+    bytes_launch = per_col.get(dname, 0) * chunk
+    achieved = bytes_launch / (avg_ms / 1e3) / 1e9 if bytes_launch else None
+    roof = {"kernel": dname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+            "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_launch, "columns_per_launch": chunk,
+            "kernel_ms_per_step": {k: v[0] for k, v in kt.items()}}
+
+    # ---- CPU baseline: FP64 oracle on a bounded sample, and GPU error on it
+    cpu = None
+    acc = None
+    if not a.no_cpu:
+        from oracle import oracle as O
+        ns = min(a.cpu_sample, UNIQUE, ncol)
+        sub = pr.subset(np.arange(ns))
+        sarr = capi.profile_fields(sub)
+        sarr["idx"] = np.arange(1, ns + 1, dtype=np.int32)
+        nth = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        O.set_threads(nth)
+        lw_ref = dict(sarr)
+        lw_ref.update(O.lw_outputs(ns, NLAY))
+        sw_ref = dict(sarr)
+        sw_ref.update(O.sw_outputs(ns, NLAY))
+        t0 = time.perf_counter()
+        O.lw_main_run(kd_lw, lut_lw, capi.lw_default_ints(ns, NLAY, blksz=blksz, nangle=3), lw_ref)
+        O.sw_main_run(kd_sw, lut_sw, capi.sw_default_ints(ns, ns, NLAY, blksz=blksz), sw_ref)
+        tc = time.perf_counter() - t0
+        cpu = {"value": ns / tc, "unit": "columns/s", "cores": O.nthreads(), "kind": "port",
+               "sample": f"FP64 oracle, LW+SW all-sky, first {ns} of the {UNIQUE} distinct columns x {NLAY} "
+                         f"layers, {tc:.2f} s"}
+        # GPU results on the same columns (the first ns columns of the device run)
+        g = {k: v[..., :ns].cpu().numpy() if v.dim() == 2 else v for k, v in
+             {"lwu": lw_out["fluxlwUP_allsky"], "lwd": lw_out["fluxlwDOWN_allsky"], "lwh": lw_out["htrlw"],
+              "swu": sw_out["fluxswUP_allsky"], "swd": sw_out["fluxswDOWN_allsky"], "swh": sw_out["htrsw"]}.items()}
+
+        def e(x, y):
+            d = x - y
+            return float(np.abs(d).max()), float(np.sqrt(np.mean(d * d)))
+
+        acc = {
+            "lw_flux_maxabs_Wm2": max(e(g["lwu"], lw_ref["fluxlwUP_allsky"])[0], e(g["lwd"], lw_ref["fluxlwDOWN_allsky"])[0]),
+            "sw_flux_maxabs_Wm2": max(e(g["swu"], sw_ref["fluxswUP_allsky"])[0], e(g["swd"], sw_ref["fluxswDOWN_allsky"])[0]),
+            "lw_hr_maxabs_Kday": e(g["lwh"] * 86400, lw_ref["htrlw"] * 86400)[0],
+            "lw_hr_rmse_Kday": e(g["lwh"] * 86400, lw_ref["htrlw"] * 86400)[1],
+            "sw_hr_maxabs_Kday": e(g["swh"] * 86400, sw_ref["htrsw"] * 86400)[0],
+            "sw_hr_rmse_Kday": e(g["swh"] * 86400, sw_ref["htrsw"] * 86400)[1],
+            "columns": ns,
+        }
+    line = {
+        "metric": METRIC, "value": value, "unit": "columns/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"LW+SW all-sky table path, {ncol} columns x {NLAY} layers "
+                               f"(LW 16 bands/128 g, 3 Gauss angles; SW 14 bands/112 g; McICA max-random; "
+                               f"FP32 optics/transport, FP64 reductions and heating rate)",
+                   "ncol": ncol, "nlay": NLAY, "path": "table", "blksz": blksz},
+        "roofline": roof, "cpu_baseline": cpu, "accuracy_vs_fp64_oracle": acc,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,162 @@
+"""ctypes binding of libccpprad_hip.so and the Python mirror of the CCPP
+scheme interface (rrtmgp_lw_main / rrtmgp_sw_main init + run).
+
+The product path is the HIP library: if it is missing, importing this module
+raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import numpy as np
+
+from . import capi, synthetic as syn
+from .build import HIP_LIB
+
+P_D = C.POINTER(C.c_double)
+P_F = C.POINTER(C.c_float)
+
+_lib = None
+
+# Every symbol the C-ABI header declares (checked by the CPU test suite).
+EXPORTS = ["ccpprad_lw_init", "ccpprad_sw_init", "ccpprad_lw_init_files", "ccpprad_sw_init_files",
+           "ccpprad_lw_run", "ccpprad_sw_run", "ccpprad_lw_run_dev", "ccpprad_sw_run_dev",
+           "ccpprad_finalize", "ccpprad_heating_rate_dev", "ccpprad_lw_gas_optics_dev",
+           "ccpprad_sw_gas_optics_dev", "ccpprad_ngpt", "ccpprad_nband", "ccpprad_version"]
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(HIP_LIB):
+        raise RuntimeError(f"{HIP_LIB} is not built: run __graft_entry__.build()")
+    L = C.CDLL(HIP_LIB)
+    vp = C.c_void_p
+    L.ccpprad_lw_init.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp), C.c_char_p, C.c_int]
+    L.ccpprad_sw_init.argtypes = L.ccpprad_lw_init.argtypes
+    L.ccpprad_lw_init_files.argtypes = [C.c_char_p, C.c_char_p, vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp),
+                                        C.c_char_p, C.c_int]
+    L.ccpprad_sw_init_files.argtypes = L.ccpprad_lw_init_files.argtypes
+    L.ccpprad_lw_run.argtypes = [vp, vp, C.c_char_p, C.c_int]
+    L.ccpprad_sw_run.argtypes = [vp, vp, C.c_char_p, C.c_int]
+    L.ccpprad_lw_run_dev.argtypes = [vp, vp, vp, C.c_int, C.c_char_p, C.c_int]
+    L.ccpprad_sw_run_dev.argtypes = [vp, vp, vp, C.c_int, C.c_char_p, C.c_int]
+    L.ccpprad_finalize.argtypes = [vp]
+    L.ccpprad_finalize.restype = None
+    L.ccpprad_heating_rate_dev.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_char_p, C.c_int]
+    L.ccpprad_lw_gas_optics_dev.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp, C.c_char_p, C.c_int]
+    L.ccpprad_sw_gas_optics_dev.argtypes = L.ccpprad_lw_gas_optics_dev.argtypes
+    L.ccpprad_ngpt.argtypes = [vp]
+    L.ccpprad_nband.argtypes = [vp]
+    L.ccpprad_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _gas_names(names):
+    arr = (C.c_char_p * len(names))(*[n.encode() for n in names])
+    return arr
+
+
+def _dp(t):
+    """Raw pointer of a torch tensor or numpy array (None -> NULL)."""
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data
+    return t.data_ptr()
+
+
+class _Scheme:
+    is_lw = True
+
+    def __init__(self, kd: syn.KDist, lut: syn.CloudLUT | None, nlay: int, blksz: int = 1,
+                 gas_names=syn.GAS_NAMES):
+        self.L = load()
+        self.kd, self.lut, self.nlay = kd, lut, nlay
+        hk = capi.kdist_struct(kd)
+        hl = capi.lut_struct(lut) if lut is not None else None
+        names = _gas_names(list(gas_names))
+        ctx = C.c_void_p()
+        buf, n = capi.errbuf()
+        fn = self.L.ccpprad_lw_init if self.is_lw else self.L.ccpprad_sw_init
+        rc = fn(C.addressof(hk.struct), C.addressof(hl.struct) if hl else None, names, len(gas_names),
+                nlay, blksz, C.byref(ctx), buf, n)
+        capi.check(rc, buf)
+        self.ctx = ctx
+
+    @property
+    def ngpt(self):
+        return self.L.ccpprad_ngpt(self.ctx)
+
+    @property
+    def nband(self):
+        return self.L.ccpprad_nband(self.ctx)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.ccpprad_finalize(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LWScheme(_Scheme):
+    """rrtmgp_lw_main: init builds device tables; run/run_dev execute _run."""
+    is_lw = True
+
+    def run_dev(self, ints: dict, arrays: dict, stream=None, validate=True):
+        """arrays: name -> torch CUDA tensor (FP64, Fortran layout).  Outputs
+        are written in place."""
+        h = capi.build_args(capi.LWArgs, ints, arrays)
+        buf, n = capi.errbuf()
+        rc = self.L.ccpprad_lw_run_dev(self.ctx, C.addressof(h.struct), stream, int(validate), buf, n)
+        capi.check(rc, buf)
+
+    def run(self, ints: dict, arrays: dict):
+        """Host numpy arrays (the Fortran drop-in path)."""
+        h = capi.build_args(capi.LWArgs, ints, arrays)
+        buf, n = capi.errbuf()
+        rc = self.L.ccpprad_lw_run(self.ctx, C.addressof(h.struct), buf, n)
+        capi.check(rc, buf)
+
+    def gas_optics_dev(self, ncol, nlay, p_lay, p_lev, t_lay, vmr, tau, pfrac, stream=None):
+        buf, n = capi.errbuf()
+        rc = self.L.ccpprad_lw_gas_optics_dev(self.ctx, ncol, nlay, _dp(p_lay), _dp(p_lev), _dp(t_lay),
+                                              _dp(vmr), _dp(tau), _dp(pfrac), stream, buf, n)
+        capi.check(rc, buf)
+
+
+class SWScheme(_Scheme):
+    is_lw = False
+
+    def run_dev(self, ints: dict, arrays: dict, solcon=1361.0, stream=None, validate=True):
+        h = capi.build_args(capi.SWArgs, ints, arrays, solcon=solcon)
+        buf, n = capi.errbuf()
+        rc = self.L.ccpprad_sw_run_dev(self.ctx, C.addressof(h.struct), stream, int(validate), buf, n)
+        capi.check(rc, buf)
+
+    def run(self, ints: dict, arrays: dict, solcon=1361.0):
+        h = capi.build_args(capi.SWArgs, ints, arrays, solcon=solcon)
+        buf, n = capi.errbuf()
+        rc = self.L.ccpprad_sw_run(self.ctx, C.addressof(h.struct), buf, n)
+        capi.check(rc, buf)
+
+    def gas_optics_dev(self, ncol, nlay, p_lay, p_lev, t_lay, vmr, tau_abs, tau_ray, stream=None):
+        buf, n = capi.errbuf()
+        rc = self.L.ccpprad_sw_gas_optics_dev(self.ctx, ncol, nlay, _dp(p_lay), _dp(p_lev), _dp(t_lay),
+                                              _dp(vmr), _dp(tau_abs), _dp(tau_ray), stream, buf, n)
+        capi.check(rc, buf)
+
+
+def heating_rate_dev(ncol, nlay, up, dn, p_lev, htr, stream=None):
+    L = load()
+    buf, n = capi.errbuf()
+    rc = L.ccpprad_heating_rate_dev(ncol, nlay, _dp(up), _dp(dn), _dp(p_lev), _dp(htr), stream, buf, n)
+    capi.check(rc, buf)

@@ -1,0 +1,689 @@
+// C-ABI of the MI355X RRTMGP hot path: context (device tables), scheme
+// run entries (device and host pointers), component entries.
+// See include/ccpprad.h for the reference interface each entry replaces.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+#include "../../include/ccpprad.h"
+#include "ccpprad_dev.h"
+
+#include "kernels.h"
+
+using namespace ccpprad;
+
+// --------------------------------------------------------------------------
+// helpers
+// --------------------------------------------------------------------------
+static void seterr(char *errmsg, int errlen, const char *fmt, ...)
+{
+    if (!errmsg || errlen <= 0) return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(errmsg, (size_t)errlen, fmt, ap);
+    va_end(ap);
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            seterr(errmsg, errlen, "%s failed: %s", #expr, hipGetErrorString(e_));          \
+            return 2;                                                                       \
+        }                                                                                   \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct ccpprad_ctx {
+    int is_lw = 0;
+    int nlay = 0, blksz = 0, device = 0;
+    TablesView T{};
+    LutView L{};
+    int have_lut = 0;
+    std::vector<void *> allocs;         // table allocations
+    uint32_t *mtpow = nullptr;
+    int *alb_cat = nullptr;
+    int ibd = 0, ibd_uv = 0;
+    double *toa64 = nullptr;
+    float *toa32 = nullptr;
+    const double *solar_dev = nullptr;
+    int *err = nullptr;                 // device error word
+    int *last_wipe = nullptr;
+    int gas_slot[MAX_GAS];              // k-dist gas index (0-based) of each CCPP vmr slot
+    // work buffers
+    DevBuf w_tau, w_aux, w_mask, w_ctau, w_cssa, w_cg, w_sF, w_sA, w_sQ, w_sb, w_blk, w_colf, w_day, w_idx;
+    // host-path staging
+    std::vector<DevBuf> stage;
+    hipStream_t stream = nullptr;
+    int chunk = 65536;
+    // optional per-kernel event timing
+    int timing = 0;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_rec;
+};
+
+enum { K_GAS = 0, K_LWSOLVE = 1, K_SWSOLVE = 2, K_CLOUD = 3, K_MCICA = 4, K_MLP = 5, K_N = 6 };
+
+static hipEvent_t ev_take(ccpprad_ctx *c)
+{
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+// Brackets one launch with events on its stream when timing is on.
+struct KScope {
+    ccpprad_ctx *c;
+    int id;
+    hipStream_t st;
+    hipEvent_t a = nullptr;
+    KScope(ccpprad_ctx *c_, int id_, hipStream_t st_) : c(c_), id(id_), st(st_)
+    {
+        if (c->timing && (a = ev_take(c))) (void)hipEventRecord(a, st);
+    }
+    ~KScope()
+    {
+        if (!a) return;
+        hipEvent_t b = ev_take(c);
+        if (!b) return;
+        (void)hipEventRecord(b, st);
+        c->ev_rec.push_back({id, {a, b}});
+    }
+};
+
+extern "C" int ccpprad_timing_enable(ccpprad_ctx *c, int enable)
+{
+    if (!c) return 1;
+    c->timing = enable;
+    c->ev_rec.clear();
+    c->ev_used = 0;
+    return 0;
+}
+
+extern "C" int ccpprad_timing_query(ccpprad_ctx *c, int kernel_id, double *total_ms, int *count)
+{
+    if (!c || !total_ms || !count) return 1;
+    double t = 0.0;
+    int n = 0;
+    for (auto &r : c->ev_rec) {
+        if (r.first != kernel_id) continue;
+        if (hipEventSynchronize(r.second.second) != hipSuccess) return 2;
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, r.second.first, r.second.second) != hipSuccess) return 2;
+        t += ms;
+        ++n;
+    }
+    *total_ms = t;
+    *count = n;
+    return 0;
+}
+
+// Internal to the library (hidden): staging buffers and the private stream
+// of the host-pointer entries.
+extern "C" __attribute__((visibility("hidden"))) int ccpprad_ctx_stage(ccpprad_ctx *c, int slot, size_t bytes, void **p)
+{
+    if ((int)c->stage.size() <= slot) c->stage.resize(slot + 1);
+    if (c->stage[slot].ensure(bytes ? bytes : 8) != hipSuccess) return 1;
+    *p = c->stage[slot].p;
+    return 0;
+}
+
+extern "C" __attribute__((visibility("hidden"))) void *ccpprad_ctx_stream(ccpprad_ctx *c)
+{
+    if (!c->stream) (void)hipStreamCreate(&c->stream);
+    return (void *)c->stream;
+}
+
+template <typename T>
+static T *upload(ccpprad_ctx *c, const T *h, size_t n, hipError_t *e)
+{
+    T *d = nullptr;
+    *e = hipMalloc(&d, sizeof(T) * (n ? n : 1));
+    if (*e != hipSuccess) return nullptr;
+    c->allocs.push_back(d);
+    if (n) *e = hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice);
+    return d;
+}
+
+static std::vector<float> to_f32(const double *p, size_t n)
+{
+    std::vector<float> v(n);
+    for (size_t i = 0; i < n; ++i) v[i] = (float)p[i];
+    return v;
+}
+
+static const char *KGAS[MAX_GAS] = {"h2o", "co2", "o3", "n2o", "ch4", "o2"};
+
+static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lut *lut,
+                    const char *const *gas_names, int ngases, int nLay, int blksz, ccpprad_ctx **out,
+                    char *errmsg, int errlen)
+{
+    if (!kd || !out) { seterr(errmsg, errlen, "null k-distribution or context pointer"); return 1; }
+    if (!!kd->is_lw != !!want_lw) { seterr(errmsg, errlen, "k-distribution is for the other spectral range"); return 1; }
+    if (kd->ngpt > MAX_GPT || kd->ngpt < 1) { seterr(errmsg, errlen, "ngpt=%d outside 1..%d", kd->ngpt, MAX_GPT); return 1; }
+    if (kd->nband > MAX_BAND || kd->nflav > MAX_FLAV || kd->ngas > MAX_GAS || kd->ngas < 1) {
+        seterr(errmsg, errlen, "table dimensions exceed the device limits"); return 1;
+    }
+    if (kd->minor_lower.n > MAX_MINOR || kd->minor_upper.n > MAX_MINOR) { seterr(errmsg, errlen, "too many minor intervals"); return 1; }
+    if (nLay < 2 || nLay > MAX_LAY) { seterr(errmsg, errlen, "nLay=%d outside 2..%d", nLay, MAX_LAY); return 1; }
+    if (blksz < 1) { seterr(errmsg, errlen, "rrtmgp_phys_blksz must be positive"); return 1; }
+    if (want_lw && (!kd->planck_frac || !kd->totplnk || kd->nplanck < 2)) { seterr(errmsg, errlen, "LW tables lack Planck data"); return 1; }
+    if (!want_lw && (!kd->krayl_lower || !kd->krayl_upper || !kd->solar_src)) { seterr(errmsg, errlen, "SW tables lack Rayleigh/solar data"); return 1; }
+    // active gas names (CCPP active_gases_array) must be known to the k-distribution
+    for (int i = 0; i < ngases && gas_names; ++i) {
+        bool ok = false;
+        for (int g = 0; g < kd->ngas && g < MAX_GAS; ++g)
+            if (gas_names[i] && strcmp(gas_names[i], KGAS[g]) == 0) ok = true;
+        if (!ok) { seterr(errmsg, errlen, "gas '%s' is not in the k-distribution", gas_names[i] ? gas_names[i] : "(null)"); return 1; }
+    }
+    ccpprad_ctx *c = new ccpprad_ctx();
+    c->is_lw = want_lw;
+    c->nlay = nLay;
+    c->blksz = blksz;
+    (void)hipGetDevice(&c->device);
+    const char *ch = getenv("CCPPRAD_CHUNK");
+    if (ch && atoi(ch) > 0) c->chunk = atoi(ch);
+    hipError_t e = hipSuccess;
+    TablesView &T = c->T;
+    T.is_lw = want_lw; T.ngas = kd->ngas; T.nband = kd->nband; T.ngpt = kd->ngpt; T.ntemp = kd->ntemp;
+    T.npres = kd->npres; T.neta = kd->neta; T.nflav = kd->nflav; T.nplanck = kd->nplanck;
+    T.press_ref_log1 = log(kd->press_ref[0]);
+    T.press_ref_log_delta = (log(kd->press_ref[0]) - log(kd->press_ref[kd->npres - 1])) / (kd->npres - 1);
+    T.press_ref_trop_log = log(kd->press_ref_trop);
+    T.temp_ref_min = kd->temp_ref[0];
+    T.temp_ref_delta = (kd->temp_ref[kd->ntemp - 1] - kd->temp_ref[0]) / (kd->ntemp - 1);
+    T.totplnk_delta = want_lw ? (kd->temp_ref[kd->ntemp - 1] - kd->temp_ref[0]) / (kd->nplanck - 1) : 1.0;
+    T.press_min = std::min(kd->press_ref[0], kd->press_ref[kd->npres - 1]);
+    T.press_max = std::max(kd->press_ref[0], kd->press_ref[kd->npres - 1]);
+    T.temp_min = kd->temp_ref[0];
+    T.temp_max = kd->temp_ref[kd->ntemp - 1];
+    T.temp_ref = upload(c, kd->temp_ref, kd->ntemp, &e);
+    const size_t nk = (size_t)kd->ngpt * kd->neta * (kd->npres + 1) * kd->ntemp;
+    { auto v = to_f32(kd->kmajor, nk); T.kmajor = upload(c, v.data(), nk, &e); }
+    if (want_lw) {
+        auto v = to_f32(kd->planck_frac, nk); T.pfrac_tab = upload(c, v.data(), nk, &e);
+        auto p = to_f32(kd->totplnk, (size_t)kd->nplanck * kd->nband); T.totplnk = upload(c, p.data(), p.size(), &e);
+        std::vector<float> fit(2 * kd->nband, 0.0f);
+        if (kd->optimal_angle_fit) for (int i = 0; i < 2 * kd->nband; ++i) fit[i] = (float)kd->optimal_angle_fit[i];
+        else for (int b = 0; b < kd->nband; ++b) fit[2 * b + 1] = 1.66f;
+        T.opt_fit = upload(c, fit.data(), fit.size(), &e);
+    } else {
+        const size_t nr = (size_t)kd->ngpt * kd->neta * kd->ntemp;
+        auto a = to_f32(kd->krayl_lower, nr); T.krayl[0] = upload(c, a.data(), nr, &e);
+        auto b = to_f32(kd->krayl_upper, nr); T.krayl[1] = upload(c, b.data(), nr, &e);
+        c->solar_dev = upload(c, kd->solar_src, kd->ngpt, &e);
+        c->toa64 = upload<double>(c, nullptr, kd->ngpt, &e);
+        c->toa32 = upload<float>(c, nullptr, kd->ngpt, &e);
+        T.toa_norm = c->toa32;
+        T.toa_norm64 = c->toa64;
+    }
+    T.vmr_ref = upload(c, kd->vmr_ref, (size_t)2 * (kd->ngas + 1) * kd->ntemp, &e);
+    std::vector<int> flav(2 * kd->nflav);
+    for (int i = 0; i < 2 * kd->nflav; ++i) flav[i] = kd->flavor[i];
+    T.flavor = upload(c, flav.data(), flav.size(), &e);
+    std::vector<int> gf(2 * kd->ngpt), g2b(kd->ngpt, 0), bl(2 * kd->nband);
+    for (int g = 0; g < kd->ngpt; ++g) { gf[2 * g] = kd->gpoint_flavor[2 * g] - 1; gf[2 * g + 1] = kd->gpoint_flavor[2 * g + 1] - 1; }
+    for (int b = 0; b < kd->nband; ++b) {
+        bl[2 * b] = kd->band_lims_gpt[2 * b] - 1; bl[2 * b + 1] = kd->band_lims_gpt[2 * b + 1] - 1;
+        for (int g = bl[2 * b]; g <= bl[2 * b + 1] && g < kd->ngpt; ++g) if (g >= 0) g2b[g] = b;
+    }
+    T.gpt_flav = upload(c, gf.data(), gf.size(), &e);
+    T.g2b = upload(c, g2b.data(), g2b.size(), &e);
+    T.band_lims = upload(c, bl.data(), bl.size(), &e);
+    // minor absorbers: parameters and per-g CSR lists, per regime
+    for (int r = 0; r < 2; ++r) {
+        const ccpprad_minor &m = r == 0 ? kd->minor_lower : kd->minor_upper;
+        T.nminor[r] = m.n;
+        T.ncontrib[r] = m.ncontrib;
+        std::vector<int> par(5 * std::max(1, m.n), 0);
+        std::vector<std::vector<std::pair<int, int>>> per(kd->ngpt);
+        for (int i = 0; i < m.n; ++i) {
+            const int gs = m.limits_gpt[2 * i] - 1, ge = m.limits_gpt[2 * i + 1] - 1;
+            if (gs < 0 || ge >= kd->ngpt || ge < gs || m.idx_minor[i] < 0 || m.idx_minor[i] > kd->ngas ||
+                m.idx_scaling[i] < 0 || m.idx_scaling[i] > kd->ngas ||
+                m.kminor_start[i] - 1 + (ge - gs) >= m.ncontrib) {
+                delete c;
+                seterr(errmsg, errlen, "invalid minor-absorber interval %d", i);
+                return 1;
+            }
+            par[5 * i] = m.idx_minor[i];
+            par[5 * i + 1] = m.scales_with_density[i];
+            par[5 * i + 2] = m.idx_scaling[i];
+            par[5 * i + 3] = m.scale_by_complement[i];
+            par[5 * i + 4] = kd->gpoint_flavor[2 * gs + r] - 1;
+            for (int g = gs; g <= ge; ++g) per[g].push_back({i, m.kminor_start[i] - 1 + (g - gs)});
+        }
+        std::vector<int> ptr(kd->ngpt + 1, 0), items;
+        for (int g = 0; g < kd->ngpt; ++g) {
+            ptr[g + 1] = ptr[g] + (int)per[g].size();
+            for (auto &p : per[g]) { items.push_back(p.first); items.push_back(p.second); }
+        }
+        if (items.empty()) items.push_back(0);
+        T.minor_par[r] = upload(c, par.data(), par.size(), &e);
+        T.minor_gptr[r] = upload(c, ptr.data(), ptr.size(), &e);
+        T.minor_items[r] = upload(c, items.data(), items.size(), &e);
+        const size_t nkm = (size_t)std::max(1, m.ncontrib) * kd->neta * kd->ntemp;
+        std::vector<float> km(nkm, 0.0f);
+        if (m.ncontrib) for (size_t i = 0; i < nkm; ++i) km[i] = (float)m.kminor[i];
+        T.kminor[r] = upload(c, km.data(), km.size(), &e);
+    }
+    // McICA: 69069^i mod 2^32
+    {
+        std::vector<uint32_t> pw(624);
+        uint32_t x = 1u;
+        for (int i = 0; i < 624; ++i) { pw[i] = x; x *= 69069u; }
+        c->mtpow = upload(c, pw.data(), pw.size(), &e);
+    }
+    // SW albedo band categories (rrtmgp_sw_main.F90:368-387)
+    {
+        std::vector<int> cat(kd->nband, 3);
+        for (int b = 0; b < kd->nband; ++b) {
+            const double lo = kd->band_lims_wvn[2 * b];
+            if (lo < 12850.0) cat[b] = 0;
+            if (lo == 12850.0) { cat[b] = 1; c->ibd = b + 1; }
+            if (lo >= 16000.0) cat[b] = 2;
+            if (lo == 29000.0) c->ibd_uv = b + 1;
+        }
+        c->alb_cat = upload(c, cat.data(), cat.size(), &e);
+        if (!want_lw && (!c->ibd || !c->ibd_uv)) {
+            delete c;
+            seterr(errmsg, errlen, "SW band limits lack the 12850 / 29000 cm-1 bands");
+            return 1;
+        }
+    }
+    if (lut) {
+        if (lut->nband != kd->nband) { delete c; seterr(errmsg, errlen, "cloud LUT band count differs from the k-distribution"); return 1; }
+        LutView &L = c->L;
+        int icergh = lut->icergh > 0 ? lut->icergh : 1;
+        if (icergh > lut->nrghice) { delete c; seterr(errmsg, errlen, "ice roughness category out of range"); return 1; }
+        L.nband = lut->nband; L.nsize_liq = lut->nsize_liq; L.nsize_ice = lut->nsize_ice;
+        L.liq_lwr = (float)lut->radliq_lwr; L.liq_upr = (float)lut->radliq_upr;
+        L.ice_lwr = (float)lut->radice_lwr; L.ice_upr = (float)lut->radice_upr;
+        L.liq_step = (float)((lut->radliq_upr - lut->radliq_lwr) / (lut->nsize_liq - 1));
+        L.ice_step = (float)((lut->radice_upr - lut->radice_lwr) / (lut->nsize_ice - 1));
+        const size_t nl = (size_t)lut->nsize_liq * lut->nband, ni = (size_t)lut->nsize_ice * lut->nband;
+        auto a = to_f32(lut->extliq, nl); L.extliq = upload(c, a.data(), nl, &e);
+        auto b = to_f32(lut->ssaliq, nl); L.ssaliq = upload(c, b.data(), nl, &e);
+        auto cc = to_f32(lut->asyliq, nl); L.asyliq = upload(c, cc.data(), nl, &e);
+        const size_t io = ni * (icergh - 1);
+        auto d1 = to_f32(lut->extice + io, ni); L.extice = upload(c, d1.data(), ni, &e);
+        auto d2 = to_f32(lut->ssaice + io, ni); L.ssaice = upload(c, d2.data(), ni, &e);
+        auto d3 = to_f32(lut->asyice + io, ni); L.asyice = upload(c, d3.data(), ni, &e);
+        c->have_lut = 1;
+    }
+    c->err = upload<int>(c, nullptr, 2, &e);
+    c->last_wipe = c->err + 1;
+    if (e != hipSuccess) {
+        seterr(errmsg, errlen, "device table upload failed: %s", hipGetErrorString(e));
+        ccpprad_finalize(c);
+        return 2;
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" int ccpprad_lw_init(const ccpprad_kdist *kd, const ccpprad_cloud_lut *lut, const char *const *gas_names,
+                               int ngases, int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg, int errlen)
+{
+    return ctx_init(1, kd, lut, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen);
+}
+
+extern "C" int ccpprad_sw_init(const ccpprad_kdist *kd, const ccpprad_cloud_lut *lut, const char *const *gas_names,
+                               int ngases, int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg, int errlen)
+{
+    return ctx_init(0, kd, lut, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen);
+}
+
+extern "C" void ccpprad_finalize(ccpprad_ctx *c)
+{
+    if (!c) return;
+    for (void *p : c->allocs) (void)hipFree(p);
+    DevBuf *bufs[] = {&c->w_tau, &c->w_aux, &c->w_mask, &c->w_ctau, &c->w_cssa, &c->w_cg, &c->w_sF,
+                      &c->w_sA, &c->w_sQ, &c->w_sb, &c->w_blk, &c->w_colf, &c->w_day, &c->w_idx};
+    for (DevBuf *b : bufs) b->release();
+    for (auto &b : c->stage) b.release();
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int ccpprad_ngpt(const ccpprad_ctx *c) { return c ? c->T.ngpt : 0; }
+extern "C" int ccpprad_nband(const ccpprad_ctx *c) { return c ? c->T.nband : 0; }
+extern "C" const char *ccpprad_version(void) { return "ccpprad-mi355x 0.1 (gfx950)"; }
+
+// Gauss-Jacobi(0,1) secants/weights for n = 1..4 (n = 1: diffusivity 1.66, 0.5).
+static void gauss_table(int n, float *D, float *W)
+{
+    static const double Dt[4][4] = {{1.66, 0, 0, 0},
+                                    {1.1835034190722739, 2.8164965809277263, 0, 0},
+                                    {1.0971985826868363, 1.6933850767089306, 4.7094163405312410, 0},
+                                    {1.0605625725391020, 1.3828256085317800, 2.4014818012578690, 7.1551300134919480}};
+    static const double Wt[4][4] = {{0.5, 0, 0, 0},
+                                    {0.3180413817439772, 0.1819586182560228, 0, 0},
+                                    {0.2009319137389255, 0.2292411063595887, 0.0698269799014858, 0},
+                                    {0.1355069134315015, 0.2034645680102685, 0.1298475476082326, 0.0311809709499974}};
+    for (int i = 0; i < MAX_ANG; ++i) { D[i] = (float)Dt[n - 1][i]; W[i] = (float)Wt[n - 1][i]; }
+}
+
+static bool need(const void *p) { return p != nullptr; }
+
+// ---------------------------------------------------------------------------
+// LW run, device pointers
+// ---------------------------------------------------------------------------
+extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void *stream, int validate,
+                                  char *errmsg, int errlen)
+{
+    if (errmsg && errlen > 0) errmsg[0] = 0;
+    if (!c || !a) { seterr(errmsg, errlen, "null context or arguments"); return 1; }
+    if (!c->is_lw) { seterr(errmsg, errlen, "context is not a LW context"); return 1; }
+    if (!a->doLWrad) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    const int nC = a->nCol, nL = a->nLay;
+    if (a->doGP_lwscat) { seterr(errmsg, errlen, "doGP_lwscat (LW two-stream scattering) is not supported"); return 1; }
+    if (nL != c->nlay) { seterr(errmsg, errlen, "nLay=%d differs from init (%d)", nL, c->nlay); return 1; }
+    if (nC < 0) { seterr(errmsg, errlen, "nCol negative"); return 1; }
+    if (nC == 0) return 0;
+    const int bs = a->rrtmgp_phys_blksz;
+    if (bs < 1 || nC % bs) { seterr(errmsg, errlen, "nCol=%d must be a multiple of rrtmgp_phys_blksz=%d", nC, bs); return 1; }
+    if (a->nGauss_angles < 1 || a->nGauss_angles > MAX_ANG) { seterr(errmsg, errlen, "nGauss_angles must be 1..4"); return 1; }
+    const void *req[] = {a->semis, a->tsfg, a->p_lay, a->p_lev, a->t_lay, a->t_lev, a->vmr_h2o, a->vmr_co2, a->vmr_o3,
+                         a->vmr_n2o, a->vmr_ch4, a->vmr_o2, a->cld_frac, a->fluxlwUP_allsky, a->fluxlwDOWN_allsky,
+                         a->fluxlwUP_clrsky, a->fluxlwDOWN_clrsky};
+    for (const void *p : req) if (!need(p)) { seterr(errmsg, errlen, "required array argument is NULL"); return 1; }
+    const bool have_clouds = c->have_lut && a->cld_lwp && a->cld_iwp && a->cld_reliq && a->cld_reice;
+    if (!c->have_lut && a->cld_frac) {
+        // clouds need the LUT; without it the all-sky fluxes are the clear-sky ones
+    }
+    if (a->isubc_lw == 2 && !a->icseed_lw) { seterr(errmsg, errlen, "isubc_lw=2 needs icseed_lw"); return 1; }
+    if ((a->iovr == a->iovr_exp || a->iovr == a->iovr_exprand || a->iovr == a->iovr_dcorr) && !a->cloud_overlap_param) {
+        seterr(errmsg, errlen, "exponential/decorrelation overlap needs cloud_overlap_param"); return 1;
+    }
+    if (a->use_LW_jacobian && !a->fluxlwUP_jac) { seterr(errmsg, errlen, "use_LW_jacobian needs fluxlwUP_jac"); return 1; }
+    const TablesView &T = c->T;
+    const int G = T.ngpt, B = T.nband;
+    const int CH = std::min(nC, c->chunk);
+    const int nblk = nC / bs;
+    HIPCHK(c->w_tau.ensure(sizeof(float) * (size_t)CH * nL * G));
+    HIPCHK(c->w_aux.ensure(sizeof(float) * (size_t)CH * nL * G));
+    HIPCHK(c->w_mask.ensure(sizeof(unsigned) * (size_t)CH * nL * MASK_WORDS));
+    HIPCHK(c->w_ctau.ensure(sizeof(float) * (size_t)CH * nL * B));
+    HIPCHK(c->w_blk.ensure((size_t)nblk + 1));
+    HIPCHK(c->w_colf.ensure((size_t)nC + 1));
+    unsigned char *blk = (unsigned char *)c->w_blk.p;
+    launch_fill_i32(c->err, 1, 0, st);
+    if (have_clouds) launch_block_flags(nC, nL, nC, nullptr, bs, a->cld_frac, blk, (unsigned char *)c->w_colf.p, st);
+    else launch_fill_u8(blk, (size_t)nblk, st);
+    VmrPtrs vm;
+    vm.v[0] = a->vmr_h2o; vm.v[1] = a->vmr_co2; vm.v[2] = a->vmr_o3;
+    vm.v[3] = a->vmr_n2o; vm.v[4] = a->vmr_ch4; vm.v[5] = a->vmr_o2;
+    float D[MAX_ANG], W[MAX_ANG];
+    gauss_table(a->nGauss_angles, D, W);
+    for (int c0 = 0; c0 < nC; c0 += CH) {
+        const int nc = std::min(CH, nC - c0);
+        float *tau = (float *)c->w_tau.p, *pf = (float *)c->w_aux.p;
+        unsigned *mask = (unsigned *)c->w_mask.p;
+        float *ctau = (float *)c->w_ctau.p;
+        if (have_clouds) {
+            CloudArgs ca{};
+            ca.nlay = nL; ca.ncol_ld = nC; ca.col0 = c0; ca.blksz = bs; ca.nband = B; ca.is_lw = 1;
+            ca.col_idx = nullptr; ca.blkflag = blk; ca.cld_frac = a->cld_frac;
+            ca.lwp = a->cld_lwp; ca.iwp = a->cld_iwp; ca.reliq = a->cld_reliq; ca.reice = a->cld_reice;
+            if (a->doGP_sgs_cnv) { ca.cnv[0] = a->cld_cnv_lwp; ca.cnv[1] = a->cld_cnv_iwp; ca.cnv[2] = a->cld_cnv_reliq; ca.cnv[3] = a->cld_cnv_reice; }
+            if (a->doGP_sgs_pbl) { ca.pbl[0] = a->cld_pbl_lwp; ca.pbl[1] = a->cld_pbl_iwp; ca.pbl[2] = a->cld_pbl_reliq; ca.pbl[3] = a->cld_pbl_reice; }
+            ca.rwp = a->cld_rwp; ca.swp = a->cld_swp; ca.resnow = a->cld_resnow;
+            if (!ca.rwp || !ca.swp || !ca.resnow) { seterr(errmsg, errlen, "cld_rwp/cld_swp/cld_resnow required with clouds"); return 1; }
+            ca.ctau = ctau; ca.err = c->err;
+            { KScope k_(c, K_CLOUD, st); launch_cloud_optics(c->L, ca, nc, st); }
+            McicaArgs ma{};
+            ma.nlay = nL; ma.ngpt = G; ma.ncol_ld = nC; ma.col0 = c0; ma.blksz = bs; ma.isubc = a->isubc_lw;
+            ma.iovr = a->iovr; ma.iovr_max = a->iovr_max; ma.iovr_maxrand = a->iovr_maxrand; ma.iovr_rand = a->iovr_rand;
+            ma.iovr_dcorr = a->iovr_dcorr; ma.iovr_exp = a->iovr_exp; ma.iovr_exprand = a->iovr_exprand;
+            ma.col_idx = nullptr; ma.blkflag = blk; ma.cld_frac = a->cld_frac; ma.overlap = a->cloud_overlap_param;
+            ma.icseed = a->icseed_lw; ma.mtpow = c->mtpow; ma.mask = mask;
+            { KScope k_(c, K_MCICA, st); launch_mcica(ma, nc, st); }
+        }
+        { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, tau, pf, c->err, st); }
+        LwSolveArgs s{};
+        s.nlay = nL; s.ncol_ld = nC; s.col0 = c0; s.top_at_1 = a->top_at_1;
+        s.tau = tau; s.pfrac = pf; s.mask = mask; s.ctau = ctau; s.aer_tau = a->aerlw_tau;
+        s.t_lay = a->t_lay; s.t_lev = a->t_lev; s.tsfc = a->tsfg; s.semis = a->semis; s.p_lev = a->p_lev;
+        for (int i = 0; i < MAX_ANG; ++i) { s.D[i] = D[i]; s.W[i] = W[i]; }
+        s.flux_up = a->fluxlwUP_allsky; s.flux_dn = a->fluxlwDOWN_allsky;
+        s.flux_up2 = a->fluxlwUP_radtime; s.flux_dn2 = a->fluxlwDOWN_radtime;
+        s.flux_jac = a->use_LW_jacobian ? a->fluxlwUP_jac : nullptr;
+        s.htr = a->htrlw;
+        { KScope k_(c, K_LWSOLVE, st); launch_lw_solve(T, s, nc, a->nGauss_angles, have_clouds, a->use_LW_jacobian != 0, false, st); }
+        if (a->doLWclrsky) {
+            LwSolveArgs s2 = s;
+            s2.mask = nullptr; s2.ctau = nullptr;
+            s2.flux_up = a->fluxlwUP_clrsky; s2.flux_dn = a->fluxlwDOWN_clrsky;
+            s2.flux_up2 = nullptr; s2.flux_dn2 = nullptr; s2.flux_jac = nullptr;
+            s2.htr = a->htrlwc;
+            KScope k_(c, K_LWSOLVE, st);
+            launch_lw_solve(T, s2, nc, a->nGauss_angles, false, false, a->nGauss_angles == 1, st);
+        }
+    }
+    if (!a->doLWclrsky) {
+        launch_fill_f64(a->fluxlwUP_clrsky, (size_t)nC * (nL + 1), 0.0, st);
+        launch_fill_f64(a->fluxlwDOWN_clrsky, (size_t)nC * (nL + 1), 0.0, st);
+    }
+    HIPCHK(hipGetLastError());
+    if (validate) {
+        int herr = 0;
+        HIPCHK(hipMemcpyAsync(&herr, c->err, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (herr & 1) { seterr(errmsg, errlen, "gas_optics(): p_lay or t_lay outside the table range"); return 3; }
+        if (herr & 2) { seterr(errmsg, errlen, "cloud_optics(): effective radius outside the LUT range"); return 3; }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// SW run, device pointers
+// ---------------------------------------------------------------------------
+extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void *stream, int validate,
+                                  char *errmsg, int errlen)
+{
+    if (errmsg && errlen > 0) errmsg[0] = 0;
+    if (!c || !a) { seterr(errmsg, errlen, "null context or arguments"); return 1; }
+    if (c->is_lw) { seterr(errmsg, errlen, "context is not a SW context"); return 1; }
+    if (!a->doSWrad) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    const int nC = a->nCol, nL = a->nLay, nD = a->nDay, nV = nL + 1;
+    if (nL != c->nlay) { seterr(errmsg, errlen, "nLay=%d differs from init (%d)", nL, c->nlay); return 1; }
+    const void *req[] = {a->fluxswUP_allsky, a->fluxswDOWN_allsky, a->fluxswUP_clrsky, a->fluxswDOWN_clrsky, a->scmpsw, a->cldtausw};
+    for (const void *p : req) if (!need(p)) { seterr(errmsg, errlen, "required output array is NULL"); return 1; }
+    if (nD <= 0) {
+        launch_fill_f64(a->fluxswUP_allsky, (size_t)nC * nV, 0.0, st);
+        launch_fill_f64(a->fluxswDOWN_allsky, (size_t)nC * nV, 0.0, st);
+        launch_fill_f64(a->fluxswUP_clrsky, (size_t)nC * nV, 0.0, st);
+        launch_fill_f64(a->fluxswDOWN_clrsky, (size_t)nC * nV, 0.0, st);
+        launch_fill_f64(a->scmpsw, (size_t)6 * nC, 0.0, st);
+        if (a->htrsw) launch_fill_f64(a->htrsw, (size_t)nC * nL, 0.0, st);
+        if (a->htrswc) launch_fill_f64(a->htrswc, (size_t)nC * nL, 0.0, st);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    const int bs = a->rrtmgp_phys_blksz;
+    if (bs < 1 || nD % bs) { seterr(errmsg, errlen, "nDay=%d must be a multiple of rrtmgp_phys_blksz=%d", nD, bs); return 1; }
+    if (nD > nC) { seterr(errmsg, errlen, "nDay exceeds nCol"); return 1; }
+    const void *req2[] = {a->idx, a->coszen, a->sfc_alb_nir_dir, a->sfc_alb_nir_dif, a->sfc_alb_uvvis_dir,
+                          a->sfc_alb_uvvis_dif, a->p_lay, a->p_lev, a->t_lay, a->vmr_h2o, a->vmr_co2, a->vmr_o3,
+                          a->vmr_n2o, a->vmr_ch4, a->vmr_o2, a->cld_frac};
+    for (const void *p : req2) if (!need(p)) { seterr(errmsg, errlen, "required array argument is NULL"); return 1; }
+    if (a->iSFC < 1 || a->iSFC > nV) { seterr(errmsg, errlen, "iSFC out of range"); return 1; }
+    if (a->isubc_sw == 2 && !a->icseed_sw) { seterr(errmsg, errlen, "isubc_sw=2 needs icseed_sw"); return 1; }
+    if ((a->iovr == a->iovr_exp || a->iovr == a->iovr_exprand || a->iovr == a->iovr_dcorr) && !a->cloud_overlap_param) {
+        seterr(errmsg, errlen, "exponential/decorrelation overlap needs cloud_overlap_param"); return 1;
+    }
+    const bool have_clouds = c->have_lut && a->cld_lwp && a->cld_iwp && a->cld_reliq && a->cld_reice;
+    const TablesView &T = c->T;
+    const int G = T.ngpt, B = T.nband;
+    const int CH = std::min(nD, c->chunk);
+    const int nblk = nD / bs;
+    HIPCHK(c->w_tau.ensure(sizeof(float) * (size_t)CH * nL * G));
+    HIPCHK(c->w_aux.ensure(sizeof(float) * (size_t)CH * nL * G));
+    HIPCHK(c->w_mask.ensure(sizeof(unsigned) * (size_t)CH * nL * MASK_WORDS));
+    HIPCHK(c->w_ctau.ensure(sizeof(float) * (size_t)CH * nL * B));
+    HIPCHK(c->w_cssa.ensure(sizeof(float) * (size_t)CH * nL * B));
+    HIPCHK(c->w_cg.ensure(sizeof(float) * (size_t)CH * nL * B));
+    HIPCHK(c->w_sF.ensure(sizeof(float) * (size_t)CH * nV * G));
+    HIPCHK(c->w_sA.ensure(sizeof(float) * (size_t)CH * nV * G));
+    HIPCHK(c->w_sQ.ensure(sizeof(float) * (size_t)CH * nV * G));
+    HIPCHK(c->w_sb.ensure(sizeof(double) * (size_t)nD * 3 * B));
+    HIPCHK(c->w_blk.ensure((size_t)nblk + 1));
+    HIPCHK(c->w_colf.ensure((size_t)nD + 1));
+    HIPCHK(c->w_day.ensure((size_t)nC + 1));
+    unsigned char *blk = (unsigned char *)c->w_blk.p, *colf = (unsigned char *)c->w_colf.p;
+    launch_fill_i32(c->err, 1, 0, st);
+    launch_fill_i32(c->last_wipe, 1, -1, st);
+    launch_fill_u8(colf, (size_t)nD, st);
+    if (have_clouds) launch_block_flags(nD, nL, nC, a->idx, bs, a->cld_frac, blk, colf, st);
+    else launch_fill_u8(blk, (size_t)nblk, st);
+    launch_toa_norm(c->solar_dev, G, a->solcon, c->toa64, c->toa32, st);
+    // cldtausw: reset by every block of the reference loop, so only the last
+    // block's values survive (rrtmgp_sw_main.F90:308,403)
+    launch_fill_f64(a->cldtausw, (size_t)nC * nL, 0.0, st);
+    if (a->htrsw) launch_fill_f64(a->htrsw, (size_t)nC * nL, 0.0, st);
+    if (a->htrswc) launch_fill_f64(a->htrswc, (size_t)nC * nL, 0.0, st);
+    VmrPtrs vm;
+    vm.v[0] = a->vmr_h2o; vm.v[1] = a->vmr_co2; vm.v[2] = a->vmr_o3;
+    vm.v[3] = a->vmr_n2o; vm.v[4] = a->vmr_ch4; vm.v[5] = a->vmr_o2;
+    const int last_blk_start = (nblk - 1) * bs;
+    for (int c0 = 0; c0 < nD; c0 += CH) {
+        const int nc = std::min(CH, nD - c0);
+        float *tabs = (float *)c->w_tau.p, *tray = (float *)c->w_aux.p;
+        unsigned *mask = (unsigned *)c->w_mask.p;
+        float *ctau = (float *)c->w_ctau.p, *cssa = (float *)c->w_cssa.p, *cg = (float *)c->w_cg.p;
+        if (have_clouds) {
+            CloudArgs ca{};
+            ca.nlay = nL; ca.ncol_ld = nC; ca.col0 = c0; ca.blksz = bs; ca.nband = B; ca.is_lw = 0;
+            ca.col_idx = a->idx; ca.blkflag = blk; ca.cld_frac = a->cld_frac;
+            ca.lwp = a->cld_lwp; ca.iwp = a->cld_iwp; ca.reliq = a->cld_reliq; ca.reice = a->cld_reice;
+            if (a->doGP_sgs_cnv) { ca.cnv[0] = a->cld_cnv_lwp; ca.cnv[1] = a->cld_cnv_iwp; ca.cnv[2] = a->cld_cnv_reliq; ca.cnv[3] = a->cld_cnv_reice; }
+            if (a->doGP_sgs_pbl) { ca.pbl[0] = a->cld_pbl_lwp; ca.pbl[1] = a->cld_pbl_iwp; ca.pbl[2] = a->cld_pbl_reliq; ca.pbl[3] = a->cld_pbl_reice; }
+            ca.rwp = a->cld_rwp; ca.swp = a->cld_swp; ca.resnow = a->cld_resnow;
+            if (!ca.rwp || !ca.swp || !ca.resnow) { seterr(errmsg, errlen, "cld_rwp/cld_swp/cld_resnow required with clouds"); return 1; }
+            ca.ctau = ctau; ca.cssa = cssa; ca.cg = cg; ca.err = c->err;
+            // only the last block's band-11 cloud depth survives the per-block reset
+            ca.cldtau = (c0 + nc > last_blk_start) ? a->cldtausw : nullptr;
+            { KScope k_(c, K_CLOUD, st); launch_cloud_optics(c->L, ca, nc, st); }
+            McicaArgs ma{};
+            ma.nlay = nL; ma.ngpt = G; ma.ncol_ld = nC; ma.col0 = c0; ma.blksz = bs; ma.isubc = a->isubc_sw;
+            ma.iovr = a->iovr; ma.iovr_max = a->iovr_max; ma.iovr_maxrand = a->iovr_maxrand; ma.iovr_rand = a->iovr_rand;
+            ma.iovr_dcorr = a->iovr_dcorr; ma.iovr_exp = a->iovr_exp; ma.iovr_exprand = a->iovr_exprand;
+            ma.col_idx = a->idx; ma.blkflag = blk; ma.cld_frac = a->cld_frac; ma.overlap = a->cloud_overlap_param;
+            ma.icseed = a->icseed_sw; ma.mtpow = c->mtpow; ma.mask = mask;
+            { KScope k_(c, K_MCICA, st); launch_mcica(ma, nc, st); }
+        }
+        { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, a->idx, c0, a->p_lay, a->p_lev, a->t_lay, vm, tabs, tray, c->err, st); }
+        SwSolveArgs s{};
+        s.nlay = nL; s.ncol_ld = nC; s.col0 = c0; s.top_at_1 = a->top_at_1; s.blksz = bs; s.isfc = a->iSFC - 1;
+        s.write_clr_zero = !a->doSWclrsky; s.role = 0;
+        s.col_idx = a->idx; s.tau_abs = tabs; s.tau_ray = tray; s.mask = mask; s.ctau = ctau; s.cssa = cssa; s.cg = cg;
+        s.blkflag = blk; s.aer_tau = a->aersw_tau; s.aer_ssa = a->aersw_ssa; s.aer_g = a->aersw_g;
+        s.coszen = a->coszen; s.alb_nir_dir = a->sfc_alb_nir_dir; s.alb_nir_dif = a->sfc_alb_nir_dif;
+        s.alb_uv_dir = a->sfc_alb_uvvis_dir; s.alb_uv_dif = a->sfc_alb_uvvis_dif; s.p_lev = a->p_lev;
+        s.alb_cat = c->alb_cat; s.sF = (float *)c->w_sF.p; s.sA = (float *)c->w_sA.p; s.sQ = (float *)c->w_sQ.p;
+        s.up_all = a->fluxswUP_allsky; s.dn_all = a->fluxswDOWN_allsky; s.up_clr = a->fluxswUP_clrsky;
+        s.dn_clr = a->fluxswDOWN_clrsky; s.sb = (double *)c->w_sb.p + (size_t)c0 * 3 * B; s.htr = a->htrsw;
+        s.htr_clr = a->doSWclrsky ? a->htrswc : nullptr;
+        { KScope k_(c, K_SWSOLVE, st); launch_sw_solve(T, s, nc, have_clouds, st); }
+        if (a->doSWclrsky && have_clouds) {
+            SwSolveArgs s2 = s;
+            s2.role = 1; s2.write_clr_zero = 0; s2.mask = nullptr; s2.htr = nullptr;
+            KScope k_(c, K_SWSOLVE, st);
+            launch_sw_solve(T, s2, nc, false, st);
+        }
+    }
+    ScmpArgs sc{};
+    sc.ndays = nD; sc.blksz = bs; sc.nband = B; sc.ibd = c->ibd; sc.ibd_uv = c->ibd_uv; sc.do_clr = a->doSWclrsky;
+    sc.col_idx = a->idx; sc.blkflag = blk; sc.colflag = colf; sc.sb = (const double *)c->w_sb.p;
+    sc.last_wipe = c->last_wipe; sc.scmpsw = a->scmpsw;
+    launch_fill_u8((unsigned char *)c->w_day.p, (size_t)nC, st);
+    launch_scmpsw(sc, nC, (unsigned char *)c->w_day.p, c->last_wipe, st);
+    HIPCHK(hipGetLastError());
+    if (validate) {
+        int herr = 0;
+        HIPCHK(hipMemcpyAsync(&herr, c->err, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (herr & 1) { seterr(errmsg, errlen, "gas_optics(): p_lay or t_lay outside the table range"); return 3; }
+        if (herr & 2) { seterr(errmsg, errlen, "cloud_optics(): effective radius outside the LUT range"); return 3; }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// component entries
+// ---------------------------------------------------------------------------
+extern "C" int ccpprad_heating_rate_dev(int nCol, int nLay, const double *flux_up, const double *flux_dn,
+                                        const double *p_lev, double *htr, void *stream, char *errmsg, int errlen)
+{
+    if (errmsg && errlen > 0) errmsg[0] = 0;
+    if (nCol < 0 || nLay < 1) { seterr(errmsg, errlen, "bad dimensions"); return 1; }
+    if (!flux_up || !flux_dn || !p_lev || !htr) { seterr(errmsg, errlen, "NULL array"); return 1; }
+    launch_heating_rate(nCol, nLay, flux_up, flux_dn, p_lev, htr, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int gas_optics_entry(ccpprad_ctx *c, int want_lw, int nCol, int nLay, const double *p_lay,
+                            const double *p_lev, const double *t_lay, const double *vmr, float *tau, float *aux,
+                            void *stream, char *errmsg, int errlen)
+{
+    if (errmsg && errlen > 0) errmsg[0] = 0;
+    if (!c || c->is_lw != want_lw) { seterr(errmsg, errlen, "wrong or null context"); return 1; }
+    if (nCol < 0 || nLay < 1) { seterr(errmsg, errlen, "bad dimensions"); return 1; }
+    if (!p_lay || !p_lev || !t_lay || !vmr || !tau || !aux) { seterr(errmsg, errlen, "NULL array"); return 1; }
+    VmrPtrs vm;
+    for (int g = 0; g < MAX_GAS; ++g) vm.v[g] = vmr + (size_t)std::min(g, c->T.ngas - 1) * nCol * nLay;
+    hipStream_t st = (hipStream_t)stream;
+    launch_fill_i32(c->err, 1, 0, st);
+    launch_gas_optics(c->T, nCol, nLay, nCol, nullptr, 0, p_lay, p_lev, t_lay, vm, tau, aux, c->err, st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ccpprad_lw_gas_optics_dev(ccpprad_ctx *c, int nCol, int nLay, const double *p_lay, const double *p_lev,
+                                         const double *t_lay, const double *vmr, float *tau, float *pfrac,
+                                         void *stream, char *errmsg, int errlen)
+{
+    return gas_optics_entry(c, 1, nCol, nLay, p_lay, p_lev, t_lay, vmr, tau, pfrac, stream, errmsg, errlen);
+}
+
+extern "C" int ccpprad_sw_gas_optics_dev(ccpprad_ctx *c, int nCol, int nLay, const double *p_lay, const double *p_lev,
+                                         const double *t_lay, const double *vmr, float *tau_abs, float *tau_ray,
+                                         void *stream, char *errmsg, int errlen)
+{
+    return gas_optics_entry(c, 0, nCol, nLay, p_lay, p_lev, t_lay, vmr, tau_abs, tau_ray, stream, errmsg, errlen);
+}

@@ -1,0 +1,93 @@
+// Device-side data structures and helpers shared by the gfx950 kernels.
+//
+// Layout in HBM (all device tables FP32 unless stated):
+//   kmajor, pfrac_tab : [ntemp][npres+1][neta][ngpt]     g-point innermost
+//   kminor[r]         : [ntemp][neta][ncontrib_r]        contributor innermost
+//   krayl[r]          : [ntemp][neta][ngpt]
+//   totplnk           : [nband][nplanck]
+//   vmr_ref           : [ntemp][ngas+1][2]  FP64 (feeds the FP64 interpolation)
+// Per-call work buffers, g-point innermost so a wavefront's lanes (one lane
+// per g-point of one column) touch one contiguous row:
+//   tau / pfrac / tau_ray : [col][lay][gpt]
+//   mask                  : [col][lay][MASK_WORDS] u32 bits, bit g of word g/32
+//   cloud band optics     : [col][lay][band] (LW: absorption depth; SW: tau, ssa, g)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ccpprad {
+
+constexpr int MAX_GAS = 6;
+constexpr int MAX_FLAV = 16;
+constexpr int MAX_MINOR = 32;      // minor intervals per regime
+constexpr int MAX_BAND = 32;
+constexpr int MAX_GPT = 128;       // g-points per column handled by one block
+constexpr int MASK_WORDS = 4;      // 128 bits of McICA mask per (col, lay)
+constexpr int MAX_ANG = 4;
+constexpr int MAX_LAY = 1024;
+
+// rte-rrtmgp physical constants (mo_gas_optics_constants)
+constexpr double AVOGAD = 6.02214076e23;
+constexpr double M_DRY = 0.028964;
+constexpr double M_H2O = 0.018016;
+constexpr double GRAV = 9.80665;
+constexpr double CP_DRY = 1004.64;
+
+// Read-only view of the device tables, passed by value as a kernel argument.
+struct TablesView {
+    int is_lw, ngas, nband, ngpt, ntemp, npres, neta, nflav, nplanck;
+    double press_ref_log1, press_ref_log_delta, press_ref_trop_log;
+    double temp_ref_min, temp_ref_delta, totplnk_delta;
+    double press_min, press_max, temp_min, temp_max;
+    const double *temp_ref;    // [ntemp] FP64
+    const float *kmajor;
+    const float *pfrac_tab;
+    const float *kminor[2];
+    const float *krayl[2];
+    const float *totplnk;
+    const float *opt_fit;      // [nband][2]
+    const double *vmr_ref;
+    const int *flavor;         // [nflav][2]
+    const int *gpt_flav;       // [ngpt][2] 0-based flavor per regime
+    const int *g2b;            // [ngpt] band of g-point (0-based)
+    const int *band_lims;      // [nband][2] 0-based first, last
+    int nminor[2];
+    int ncontrib[2];
+    const int *minor_par[2];   // [n][5]: idx_minor, density, idx_scaling, complement, flavor(0-based)
+    const int *minor_gptr[2];  // [ngpt+1] CSR offsets into minor_items
+    const int *minor_items[2]; // [..][2]: interval, contributor index (0-based)
+    const float *toa_norm;     // SW [ngpt] toa source scaled to solcon (FP32), written per call
+    const double *toa_norm64;  // SW [ngpt] FP64 copy
+};
+
+struct LutView {
+    int nband, nsize_liq, nsize_ice;
+    float liq_lwr, liq_step, ice_lwr, ice_step, liq_upr, ice_upr;
+    const float *extliq, *ssaliq, *asyliq;   // [nband][nsize_liq]
+    const float *extice, *ssaice, *asyice;   // [nband][nsize_ice] (roughness selected)
+};
+
+__device__ __forceinline__ int d_min(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int d_max(int a, int b) { return a > b ? a : b; }
+
+// interpolate1D of the band Planck table (FP32), table row for band b.
+__device__ __forceinline__ float planck_band(const TablesView &t, int b, float T)
+{
+    float v0 = (T - (float)t.temp_ref_min) / (float)t.totplnk_delta;
+    int iv = (int)v0;
+    float frac = v0 - (float)iv;
+    int idx = d_min(t.nplanck - 1, d_max(1, iv + 1));
+    const float *tab = t.totplnk + (size_t)b * t.nplanck;
+    float a0 = tab[idx - 1], a1 = tab[idx];
+    return a0 + frac * (a1 - a0);
+}
+
+// FP64 wavefront sum (64 lanes) via cross-lane shuffles.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+}  // namespace ccpprad

@@ -1,0 +1,325 @@
+// Cloud optics (LUT), precipitation optics and McICA sampling on gfx950.
+//
+// cloud_optics_kernel restates ty_cloud_optics%cloud_optics (LUT branch,
+// compute_all_from_table + combine; call sites rrtmgp_lw_main.F90:366-401,
+// rrtmgp_sw_main.F90:396-432), the rain/snow optics of the scheme layer
+// (rrtmgp_lw_main.F90:409-431; rrtmgp_sw_main.F90:436-471, constants
+// rrtmgp_lw_cloud_optics.F90:63-66, rrtmgp_sw_cloud_optics.F90:59-63,369-381)
+// and increment_2stream_by_2stream.  One thread per (column, layer), FP32.
+//
+// mcica_kernel restates the per-column seeding and MT19937 draws of
+// rrtmgp_lw_main.F90:441-466 / rrtmgp_sw_main.F90:479-509
+// (physics/Radiation/mersenne_twister.f:285-372) and sampled_mask
+// (rrtmgp_sampling.F90:84-182).  One wavefront per column: the 624-word MT
+// state lives in LDS, the twist runs 64 words at a time (reads of a chunk
+// complete before its writes, so every word reads its successor's old value
+// and the word 227 places back already updated, exactly as the serial loop
+// does), tempered blocks go to a two-slot LDS ring, and the uniform draws are
+// converted to FP64 exactly as the Fortran does (non-negative draws pass
+// through default REAL).  Masks are bit-identical to the reference's.
+#include "kernels.h"
+
+namespace ccpprad {
+
+
+__device__ __forceinline__ void lut_eval(const float *ext, const float *ssa, const float *asy, int nsz,
+                                         float lwr, float step, float path, float re, float &t, float &ts,
+                                         float &tsg)
+{
+    int idx = min((int)floorf((re - lwr) / step) + 1, nsz - 1);
+    float fint = (re - lwr) / step - (float)(idx - 1);
+    float e0 = ext[idx - 1], e1 = ext[idx];
+    float s0 = ssa[idx - 1], s1 = ssa[idx];
+    float a0 = asy[idx - 1], a1 = asy[idx];
+    t = path * (e0 + fint * (e1 - e0));
+    ts = t * (s0 + fint * (s1 - s0));
+    tsg = ts * (a0 + fint * (a1 - a0));
+}
+
+// increment_2stream_by_2stream: (t2,s2,g2) += (t1,s1,g1), eps = 3*tiny(FP32)
+__device__ __forceinline__ void inc2s(float t1, float s1, float g1, float &t2, float &s2, float &g2)
+{
+    const float eps = 3.0f * 1.17549435e-38f;
+    const float t12 = t1 + t2;
+    const float ts12 = t1 * s1 + t2 * s2;
+    g2 = (t1 * s1 * g1 + t2 * s2 * g2) / fmaxf(eps, ts12);
+    s2 = ts12 / fmaxf(eps, t12);
+    t2 = t12;
+}
+
+__device__ __forceinline__ bool lut_cloud(const LutView &L, int b, float lwp, float iwp, float rl, float ri,
+                                          float &t, float &s, float &g)
+{
+    float lt = 0, lts = 0, ltsg = 0, it = 0, its = 0, itsg = 0;
+    bool bad = false;
+    if (lwp > 0.0f) {
+        bad |= !(rl >= L.liq_lwr && rl <= L.liq_upr);
+        const size_t o = (size_t)b * L.nsize_liq;
+        lut_eval(L.extliq + o, L.ssaliq + o, L.asyliq + o, L.nsize_liq, L.liq_lwr, L.liq_step, lwp,
+                 fminf(fmaxf(rl, L.liq_lwr), L.liq_upr), lt, lts, ltsg);
+    }
+    if (iwp > 0.0f) {
+        bad |= !(ri >= L.ice_lwr && ri <= L.ice_upr);
+        const size_t o = (size_t)b * L.nsize_ice;
+        lut_eval(L.extice + o, L.ssaice + o, L.asyice + o, L.nsize_ice, L.ice_lwr, L.ice_step, iwp,
+                 fminf(fmaxf(ri, L.ice_lwr), L.ice_upr), it, its, itsg);
+    }
+    t = lt + it;
+    const float ts = lts + its, tsg = ltsg + itsg;
+    s = ts / fmaxf(1.1920929e-7f, t);
+    g = tsg / fmaxf(1.1920929e-7f, ts);
+    return bad;
+}
+
+__global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs A, int ncols)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)ncols * A.nlay) return;
+    const int d = (int)(i % ncols), l = (int)(i / ncols);
+    const int od = A.col0 + d;
+    const int c = A.col_idx ? A.col_idx[od] - 1 : od;
+    const int blk = od / A.blksz;
+    const size_t q = (size_t)c + (size_t)A.ncol_ld * l;
+    const int nb = A.nband;
+    float *ot = A.ctau + ((size_t)d * A.nlay + l) * nb;
+    float *os = A.cssa ? A.cssa + ((size_t)d * A.nlay + l) * nb : nullptr;
+    float *og = A.cg ? A.cg + ((size_t)d * A.nlay + l) * nb : nullptr;
+    if (!A.blkflag[blk]) {
+        for (int b = 0; b < nb; ++b) { ot[b] = 0.0f; if (os) { os[b] = 0.0f; og[b] = 0.0f; } }
+        return;
+    }
+    const float lwp = (float)A.lwp[q], iwp = (float)A.iwp[q];
+    const float rl = (float)A.reliq[q], ri = (float)A.reice[q];
+    const double cf = A.cld_frac[q];
+    bool bad = false;
+    // precipitation, band independent parts
+    float tr = 0.0f, tsn = 0.0f, res = 0.0f;
+    bool prec = false;
+    if (A.is_lw) {
+        if (cf > 1.0e-6) {  // rrtmgp_lw_main.F90:413
+            tr = 0.33e-3f * (float)A.rwp[q];
+            const double sw = A.swp[q], rs = A.resnow[q];
+            tsn = (sw > 0.0 && rs > 10.0) ? 1.5f * 1.05756f * (float)sw / (float)rs : 0.0f;
+            prec = true;
+        }
+    } else {
+        if (cf > 1.0e-12) {  // rrtmgp_sw_main.F90:438
+            tr = (float)A.rwp[q] * 3.07e-3f;
+            const double sw = A.swp[q], rs = A.resnow[q];
+            res = (float)rs;
+            tsn = (sw > 0.0 && rs > 10.0) ? (float)sw * 1.09087f * (0.0f + 1.5f / (1.0315f * res)) : 0.0f;
+            prec = true;
+        }
+    }
+    const float b0r[14] = {0.496f, 0.466f, 0.437f, 0.416f, 0.391f, 0.374f, 0.352f, 0.183f, 0.048f, 0.012f, 0.f, 0.f, 0.f, 0.f};
+    const float b0s[14] = {0.460f, 0.460f, 0.460f, 0.460f, 0.460f, 0.460f, 0.460f, 0.460f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const float b1s[14] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 1.62e-5f, 1.62e-5f, 0.f, 0.f, 0.f, 0.f};
+    const float c0r[14] = {0.980f, 0.975f, 0.965f, 0.960f, 0.955f, 0.952f, 0.950f, 0.944f, 0.894f, 0.884f, 0.883f, 0.883f, 0.883f, 0.883f};
+    const float c0s[14] = {0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.700f, 0.700f, 0.700f, 0.700f};
+    for (int b = 0; b < nb; ++b) {
+        float t, s, g;
+        bad |= lut_cloud(L, b, lwp, iwp, rl, ri, t, s, g);
+        if (!A.is_lw && A.cldtau && b == 10) A.cldtau[q] = (double)t;
+        for (int k = 0; k < 2; ++k) {
+            const double *const *src = k == 0 ? A.cnv : A.pbl;
+            if (!src[0]) continue;
+            float t2, s2, g2;
+            bad |= lut_cloud(L, b, (float)src[0][q], (float)src[1][q], (float)src[2][q], (float)src[3][q], t2, s2, g2);
+            inc2s(t2, s2, g2, t, s, g);
+        }
+        float pt = 0.0f, ps = 0.0f, pg = 0.0f;
+        if (A.is_lw) {
+            pt = prec ? tr + tsn : 0.0f;
+        } else if (prec) {
+            const int bb = b < 14 ? b : 13;
+            const float sr = tr * (1.0f - b0r[bb]);
+            const float ar = sr * c0r[bb];
+            const float sn = tsn * (1.0f - (b0s[bb] + b1s[bb] * 1.0315f * res));
+            const float an = sn * c0s[bb];
+            const float tp = fmaxf(1.0e-12f, tr + tsn);
+            const float sp = fmaxf(1.0e-12f, sr + sn);
+            const float ap = fmaxf(1.0e-12f, ar + an);
+            const float asyw = ap / fmaxf(1.0e-12f, sp);
+            const float ssaw = fminf(1.0f - 0.000001f, sp / tp);
+            const float za1 = asyw * asyw, za2 = ssaw * za1;
+            pt = (1.0f - za2) * tp;
+            ps = (ssaw - za2) / (1.0f - za2);
+            pg = asyw / (1.0f + asyw);
+        }
+        inc2s(pt, ps, pg, t, s, g);
+        if (A.is_lw) {
+            ot[b] = t * (1.0f - s);  // 1scl += 2str: absorption depth
+        } else {
+            ot[b] = t; os[b] = s; og[b] = g;
+        }
+    }
+    if (bad) atomicOr(A.err, 2);
+}
+
+// ---------------------------------------------------------------------------
+// McICA
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t mt_temper(uint32_t y)
+{
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+// random_number_t's conversion of a tempered draw to [0, 1]
+__device__ __forceinline__ double mt_uniform(uint32_t y)
+{
+    const int32_t ys = (int32_t)y;
+    if (ys < 0) return ((double)ys + 4294967296.0) / 4294967295.0;
+    return (double)(float)ys / 4294967295.0;
+}
+
+__global__ __launch_bounds__(64) void mcica_kernel(McicaArgs A)
+{
+    extern __shared__ unsigned char smem_raw[];
+    uint32_t *mt = (uint32_t *)smem_raw;              // [624]
+    uint32_t *tmp = mt + 624;                          // [2][624]
+    double *cf = (double *)(tmp + 2 * 624);            // [nlay]
+    const int d = blockIdx.x, lane = threadIdx.x;
+    const int od = A.col0 + d;
+    const int c = A.col_idx ? A.col_idx[od] - 1 : od;
+    const int nlay = A.nlay, ngpt = A.ngpt;
+    unsigned *mout = A.mask + (size_t)d * nlay * MASK_WORDS;
+    if (!A.blkflag[od / A.blksz]) {
+        for (int i = lane; i < nlay * MASK_WORDS; i += 64) mout[i] = 0u;
+        return;
+    }
+    const int iovr = A.iovr;
+    const bool plain = (iovr == A.iovr_maxrand || iovr == A.iovr_rand || iovr == A.iovr_max);
+    const bool expo = (iovr == A.iovr_exp || iovr == A.iovr_exprand);
+    const bool dcorr = (iovr == A.iovr_dcorr);
+    const bool vmax = (iovr == A.iovr_max);
+    for (int l = lane; l < nlay; l += 64) cf[l] = A.cld_frac[(size_t)c + (size_t)A.ncol_ld * l];
+    int seed = 0;
+    if (A.isubc == 1) seed = ngpt + c + 1;
+    else if (A.isubc == 2) seed = A.icseed[c];
+    if (seed == 0) seed = 4357;
+    for (int i = lane; i < 624; i += 64) mt[i] = (uint32_t)seed * A.mtpow[i];
+    __syncthreads();
+    int fst = -1;
+    for (int l = 0; l < nlay; ++l) if (cf[l] > 0.0) { fst = l; break; }
+    int cur = -1;  // index of the newest tempered block
+    double lr[2] = {0.0, 0.0}, pr[2] = {0.0, 0.0};
+    for (int l = 0; l < nlay; ++l) {
+        const int base = vmax ? 0 : l * ngpt;
+        const int need = (base + ngpt - 1) / 624;
+        while (cur < need) {
+            // twist (64 words at a time, reads before writes)
+            for (int k0 = 0; k0 < 624; k0 += 64) {
+                const int kk = k0 + lane;
+                uint32_t nv = 0;
+                if (kk < 624) {
+                    const uint32_t y = (mt[kk] & 0x80000000u) | (mt[kk == 623 ? 0 : kk + 1] & 0x7fffffffu);
+                    nv = mt[kk < 227 ? kk + 397 : kk - 227] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+                }
+                __syncthreads();
+                if (kk < 624) mt[kk] = nv;
+                __syncthreads();
+            }
+            ++cur;
+            uint32_t *slot = tmp + 624 * (cur & 1);
+            for (int i = lane; i < 624; i += 64) slot[i] = mt_temper(mt[i]);
+            __syncthreads();
+        }
+        const double cfl = cf[l];
+        const bool cl = cfl > 0.0;
+        const bool clp = l > 0 && cf[l - 1] > 0.0;
+        const double thr = 1.0 - cfl;
+        unsigned long long bits[2] = {0ull, 0ull};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int g = lane + 64 * h;
+            bool bit = false;
+            if (g < ngpt && (cl || dcorr)) {
+                const int k = base + g;
+                const double r = mt_uniform(tmp[624 * ((k / 624) & 1) + (k % 624)]);
+                if (plain) {
+                    if (cl) {
+                        if (!clp) lr[h] = r;
+                        bit = lr[h] > thr;
+                    }
+                } else if (expo) {
+                    if (cl) {
+                        if (clp) {
+                            const double rho = A.overlap[(size_t)c + (size_t)A.ncol_ld * (l - 1)];
+                            lr[h] = rho * (lr[h] - 0.5) + sqrt(1.0 - rho * rho) * (r - 0.5) + 0.5;
+                        } else {
+                            lr[h] = r;
+                        }
+                        bit = lr[h] > thr;
+                    }
+                } else if (dcorr) {
+                    if (l == fst) bit = r > thr;
+                    else if (fst >= 0 && l > fst) {
+                        const double ov = A.overlap[(size_t)c + (size_t)A.ncol_ld * (l - 1)];
+                        bit = (r <= ov) ? (pr[h] > thr) : (r > thr);
+                    }
+                    pr[h] = r;
+                }
+            }
+            bits[h] = __ballot(bit);
+        }
+        if (lane == 0) {
+            unsigned *w = mout + (size_t)l * MASK_WORDS;
+            w[0] = (unsigned)bits[0]; w[1] = (unsigned)(bits[0] >> 32);
+            w[2] = (unsigned)bits[1]; w[3] = (unsigned)(bits[1] >> 32);
+        }
+    }
+}
+
+// Per scheme block: any column with total cloud cover above eps
+// (rrtmgp_lw_main.F90:353-362, rrtmgp_sw_main.F90:263-273).
+__global__ void block_flag_kernel(int ncols, int nlay, int ncol_ld, const int *col_idx, int blksz,
+                                  const double *cld_frac, unsigned char *blkflag, unsigned char *colflag)
+{
+    const int od = blockIdx.x * blockDim.x + threadIdx.x;
+    if (od >= ncols) return;
+    const int c = col_idx ? col_idx[od] - 1 : od;
+    double z0 = 1.0;
+    for (int l = 0; l < nlay; ++l) z0 = fmin(z0, 1.0 - cld_frac[(size_t)c + (size_t)ncol_ld * l]);
+    if (z0 <= 1.0e-12) z0 = 0.0;
+    if (z0 > 1.0 - 1.0e-6) z0 = 1.0;
+    const double zcf1 = 1.0 - z0;
+    if (colflag) colflag[od] = zcf1 > 1.0e-6;
+    if (zcf1 > 1.0e-6) blkflag[od / blksz] = 1;
+}
+
+__global__ void zero_u8(unsigned char *p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+void launch_block_flags(int ncols, int nlay, int ncol_ld, const int *col_idx, int blksz,
+                        const double *cld_frac, unsigned char *blkflag, unsigned char *colflag, hipStream_t st)
+{
+    const int nblk = (ncols + blksz - 1) / blksz;
+    hipLaunchKernelGGL(zero_u8, dim3((nblk + 255) / 256), dim3(256), 0, st, blkflag, (size_t)nblk);
+    if (ncols > 0)
+        hipLaunchKernelGGL(block_flag_kernel, dim3((ncols + 255) / 256), dim3(256), 0, st, ncols, nlay, ncol_ld,
+                           col_idx, blksz, cld_frac, blkflag, colflag);
+}
+
+void launch_cloud_optics(const LutView &L, const CloudArgs &A, int ncols, hipStream_t st)
+{
+    const size_t n = (size_t)ncols * A.nlay;
+    if (!n) return;
+    hipLaunchKernelGGL(cloud_optics_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, A, ncols);
+}
+
+void launch_mcica(const McicaArgs &A, int ncols, hipStream_t st)
+{
+    if (ncols <= 0) return;
+    const size_t smem = sizeof(uint32_t) * 3 * 624 + sizeof(double) * A.nlay;
+    hipLaunchKernelGGL(mcica_kernel, dim3(ncols), dim3(64), smem, st, A);
+}
+
+}  // namespace ccpprad

@@ -1,0 +1,323 @@
+// Host-pointer scheme entries (the drop-in path the Fortran scheme modules
+// call) and the table-file loader used by *_init_files.
+//
+// ccpprad_lw_run / ccpprad_sw_run copy every non-NULL array of the argument
+// list to device staging buffers (outputs too: the CCPP outputs are
+// intent(inout), rrtmgp_lw_main.F90:183-190 / rrtmgp_sw_main.F90:175-182, and
+// columns the scheme does not touch keep their values), run the device entry
+// on a private stream, copy the outputs back and synchronise.
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <string>
+#include <vector>
+#include <map>
+
+#include "../../include/ccpprad.h"
+
+extern "C" int ccpprad_ctx_stage(ccpprad_ctx *c, int slot, size_t bytes, void **p);
+extern "C" void *ccpprad_ctx_stream(ccpprad_ctx *c);
+
+namespace {
+
+enum Shape { S_COL, S_LAY, S_LEV, S_BAND, S_SCMP, S_DAY };
+struct Field {
+    size_t off;
+    Shape shape;
+    bool is_int;
+    bool out;
+};
+
+#define LWF(name, sh, isint, out) {offsetof(ccpprad_lw_args, name), sh, isint, out}
+#define SWF(name, sh, isint, out) {offsetof(ccpprad_sw_args, name), sh, isint, out}
+
+const Field LW_FIELDS[] = {
+    LWF(icseed_lw, S_COL, true, false), LWF(semis, S_COL, false, false), LWF(tsfg, S_COL, false, false),
+    LWF(p_lay, S_LAY, false, false), LWF(p_lev, S_LEV, false, false), LWF(t_lay, S_LAY, false, false),
+    LWF(t_lev, S_LEV, false, false), LWF(vmr_o2, S_LAY, false, false), LWF(vmr_h2o, S_LAY, false, false),
+    LWF(vmr_o3, S_LAY, false, false), LWF(vmr_ch4, S_LAY, false, false), LWF(vmr_n2o, S_LAY, false, false),
+    LWF(vmr_co2, S_LAY, false, false), LWF(cld_frac, S_LAY, false, false), LWF(cld_lwp, S_LAY, false, false),
+    LWF(cld_reliq, S_LAY, false, false), LWF(cld_iwp, S_LAY, false, false), LWF(cld_reice, S_LAY, false, false),
+    LWF(cld_swp, S_LAY, false, false), LWF(cld_resnow, S_LAY, false, false), LWF(cld_rwp, S_LAY, false, false),
+    LWF(cld_rerain, S_LAY, false, false), LWF(precip_frac, S_LAY, false, false),
+    LWF(cld_cnv_lwp, S_LAY, false, false), LWF(cld_cnv_reliq, S_LAY, false, false),
+    LWF(cld_cnv_iwp, S_LAY, false, false), LWF(cld_cnv_reice, S_LAY, false, false),
+    LWF(cld_pbl_lwp, S_LAY, false, false), LWF(cld_pbl_reliq, S_LAY, false, false),
+    LWF(cld_pbl_iwp, S_LAY, false, false), LWF(cld_pbl_reice, S_LAY, false, false),
+    LWF(cloud_overlap_param, S_LAY, false, false), LWF(aerlw_tau, S_BAND, false, false),
+    LWF(aerlw_ssa, S_BAND, false, false), LWF(aerlw_g, S_BAND, false, false),
+    LWF(fluxlwUP_allsky, S_LEV, false, true), LWF(fluxlwDOWN_allsky, S_LEV, false, true),
+    LWF(fluxlwUP_clrsky, S_LEV, false, true), LWF(fluxlwDOWN_clrsky, S_LEV, false, true),
+    LWF(fluxlwUP_jac, S_LEV, false, true), LWF(fluxlwUP_radtime, S_LEV, false, true),
+    LWF(fluxlwDOWN_radtime, S_LEV, false, true), LWF(htrlw, S_LAY, false, true), LWF(htrlwc, S_LAY, false, true),
+};
+
+const Field SW_FIELDS[] = {
+    SWF(idx, S_DAY, true, false), SWF(icseed_sw, S_COL, true, false),
+    SWF(sfc_alb_nir_dir, S_COL, false, false), SWF(sfc_alb_nir_dif, S_COL, false, false),
+    SWF(sfc_alb_uvvis_dir, S_COL, false, false), SWF(sfc_alb_uvvis_dif, S_COL, false, false),
+    SWF(coszen, S_COL, false, false), SWF(p_lay, S_LAY, false, false), SWF(p_lev, S_LEV, false, false),
+    SWF(t_lay, S_LAY, false, false), SWF(t_lev, S_LEV, false, false), SWF(vmr_o2, S_LAY, false, false),
+    SWF(vmr_h2o, S_LAY, false, false), SWF(vmr_o3, S_LAY, false, false), SWF(vmr_ch4, S_LAY, false, false),
+    SWF(vmr_n2o, S_LAY, false, false), SWF(vmr_co2, S_LAY, false, false), SWF(cld_frac, S_LAY, false, false),
+    SWF(cld_lwp, S_LAY, false, false), SWF(cld_reliq, S_LAY, false, false), SWF(cld_iwp, S_LAY, false, false),
+    SWF(cld_reice, S_LAY, false, false), SWF(cld_swp, S_LAY, false, false), SWF(cld_resnow, S_LAY, false, false),
+    SWF(cld_rwp, S_LAY, false, false), SWF(cld_rerain, S_LAY, false, false), SWF(precip_frac, S_LAY, false, false),
+    SWF(cld_cnv_lwp, S_LAY, false, false), SWF(cld_cnv_reliq, S_LAY, false, false),
+    SWF(cld_cnv_iwp, S_LAY, false, false), SWF(cld_cnv_reice, S_LAY, false, false),
+    SWF(cld_pbl_lwp, S_LAY, false, false), SWF(cld_pbl_reliq, S_LAY, false, false),
+    SWF(cld_pbl_iwp, S_LAY, false, false), SWF(cld_pbl_reice, S_LAY, false, false),
+    SWF(cloud_overlap_param, S_LAY, false, false), SWF(aersw_tau, S_BAND, false, false),
+    SWF(aersw_ssa, S_BAND, false, false), SWF(aersw_g, S_BAND, false, false), SWF(scmpsw, S_SCMP, false, true),
+    SWF(fluxswUP_allsky, S_LEV, false, true), SWF(fluxswDOWN_allsky, S_LEV, false, true),
+    SWF(fluxswUP_clrsky, S_LEV, false, true), SWF(fluxswDOWN_clrsky, S_LEV, false, true),
+    SWF(cldtausw, S_LAY, false, true), SWF(htrsw, S_LAY, false, true), SWF(htrswc, S_LAY, false, true),
+};
+
+size_t field_elems(Shape s, size_t ncol, size_t nlay, size_t nband, size_t nday)
+{
+    switch (s) {
+    case S_COL: return ncol;
+    case S_LAY: return ncol * nlay;
+    case S_LEV: return ncol * (nlay + 1);
+    case S_BAND: return ncol * nlay * nband;
+    case S_SCMP: return 6 * ncol;
+    case S_DAY: return nday;
+    }
+    return 0;
+}
+
+template <typename Args, size_t N>
+int run_host(ccpprad_ctx *c, const Args *a, const Field (&F)[N], int ncol, int nlay, int nday,
+             int (*dev)(ccpprad_ctx *, const Args *, void *, int, char *, int), char *errmsg, int errlen)
+{
+    if (!c || !a) {
+        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "null context or arguments");
+        return 1;
+    }
+    const size_t nband = (size_t)ccpprad_nband(c);
+    Args d = *a;
+    hipStream_t st = (hipStream_t)ccpprad_ctx_stream(c);
+    for (size_t i = 0; i < N; ++i) {
+        void **hp = (void **)((char *)&d + F[i].off);
+        if (!*hp) continue;
+        const size_t n = field_elems(F[i].shape, (size_t)ncol, (size_t)nlay, nband, (size_t)nday);
+        const size_t bytes = n * (F[i].is_int ? sizeof(int) : sizeof(double));
+        void *dp = nullptr;
+        int rc = ccpprad_ctx_stage(c, (int)i, bytes, &dp);
+        if (rc) {
+            if (errmsg && errlen > 0) snprintf(errmsg, errlen, "device staging allocation failed");
+            return 2;
+        }
+        if (bytes && hipMemcpyAsync(dp, *hp, bytes, hipMemcpyHostToDevice, st) != hipSuccess) {
+            if (errmsg && errlen > 0) snprintf(errmsg, errlen, "host to device copy failed");
+            return 2;
+        }
+        *hp = dp;
+    }
+    int rc = dev(c, &d, (void *)st, 1, errmsg, errlen);
+    if (rc) return rc;
+    for (size_t i = 0; i < N; ++i) {
+        if (!F[i].out) continue;
+        void *const *hp = (void *const *)((const char *)a + F[i].off);
+        void *const *dp = (void *const *)((const char *)&d + F[i].off);
+        if (!*hp) continue;
+        const size_t n = field_elems(F[i].shape, (size_t)ncol, (size_t)nlay, nband, (size_t)nday);
+        if (n && hipMemcpyAsync(*hp, *dp, n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) {
+            if (errmsg && errlen > 0) snprintf(errmsg, errlen, "device to host copy failed");
+            return 2;
+        }
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) {
+        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "stream synchronisation failed");
+        return 2;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int ccpprad_lw_run(ccpprad_ctx *c, const ccpprad_lw_args *a, char *errmsg, int errlen)
+{
+    if (a && !a->doLWrad) return 0;
+    return run_host(c, a, LW_FIELDS, a ? a->nCol : 0, a ? a->nLay : 0, 0, ccpprad_lw_run_dev, errmsg, errlen);
+}
+
+extern "C" int ccpprad_sw_run(ccpprad_ctx *c, const ccpprad_sw_args *a, char *errmsg, int errlen)
+{
+    if (a && !a->doSWrad) return 0;
+    return run_host(c, a, SW_FIELDS, a ? a->nCol : 0, a ? a->nLay : 0, a ? (a->nDay > 0 ? a->nDay : 0) : 0,
+                    ccpprad_sw_run_dev, errmsg, errlen);
+}
+
+// ---------------------------------------------------------------------------
+// Table files: "CCPPRADT", int32 version (1), int32 record count, then per
+// record: char name[32], int32 type (1 int32, 2 float64), int32 ndim,
+// int64 dims[4], data in Fortran order.  Written by ccpprad.tables.
+// ---------------------------------------------------------------------------
+namespace {
+struct Rec {
+    int type = 0;
+    std::vector<long long> dims;
+    std::vector<int> i;
+    std::vector<double> d;
+};
+using RecMap = std::map<std::string, Rec>;
+
+bool read_file(const char *path, RecMap &m, std::string &err)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) { err = std::string("cannot open ") + path; return false; }
+    char magic[8];
+    int ver = 0, nrec = 0;
+    bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "CCPPRADT", 8) == 0 && fread(&ver, 4, 1, f) == 1 &&
+              ver == 1 && fread(&nrec, 4, 1, f) == 1 && nrec >= 0 && nrec < 100000;
+    for (int r = 0; ok && r < nrec; ++r) {
+        char name[33] = {0};
+        int type = 0, ndim = 0;
+        long long dims[4];
+        ok = fread(name, 1, 32, f) == 32 && fread(&type, 4, 1, f) == 1 && fread(&ndim, 4, 1, f) == 1 &&
+             fread(dims, 8, 4, f) == 4 && ndim >= 0 && ndim <= 4 && (type == 1 || type == 2);
+        if (!ok) break;
+        long long n = 1;
+        Rec rec;
+        rec.type = type;
+        for (int k = 0; k < ndim; ++k) { if (dims[k] < 0 || dims[k] > (1LL << 32)) ok = false; n *= dims[k]; rec.dims.push_back(dims[k]); }
+        if (!ok || n > (1LL << 31)) { ok = false; break; }
+        if (type == 1) { rec.i.resize((size_t)n); ok = fread(rec.i.data(), 4, (size_t)n, f) == (size_t)n; }
+        else { rec.d.resize((size_t)n); ok = fread(rec.d.data(), 8, (size_t)n, f) == (size_t)n; }
+        m[name] = std::move(rec);
+    }
+    fclose(f);
+    if (!ok) err = std::string("malformed table file ") + path;
+    return ok;
+}
+
+struct TableHold {
+    RecMap k, l;
+    ccpprad_kdist kd{};
+    ccpprad_cloud_lut lut{};
+};
+
+int geti(RecMap &m, const char *n, bool *ok) {
+    auto it = m.find(n);
+    if (it == m.end() || it->second.type != 1 || it->second.i.empty()) { *ok = false; return 0; }
+    return it->second.i[0];
+}
+double getd(RecMap &m, const char *n, bool *ok) {
+    auto it = m.find(n);
+    if (it == m.end() || it->second.type != 2 || it->second.d.empty()) { *ok = false; return 0; }
+    return it->second.d[0];
+}
+const int *pi(RecMap &m, const char *n, bool *ok, size_t need) {
+    auto it = m.find(n);
+    if (it == m.end() || it->second.type != 1 || it->second.i.size() < need) { *ok = false; return nullptr; }
+    return it->second.i.data();
+}
+const double *pd(RecMap &m, const char *n, bool *ok, size_t need, bool optional = false) {
+    auto it = m.find(n);
+    if (it == m.end()) { if (!optional) *ok = false; return nullptr; }
+    if (it->second.type != 2 || it->second.d.size() < need) { *ok = false; return nullptr; }
+    return it->second.d.data();
+}
+
+bool fill_minor(RecMap &m, const char *pre, int neta, int ntemp, ccpprad_minor &mm)
+{
+    bool ok = true;
+    std::string p(pre);
+    mm.n = geti(m, (p + "n").c_str(), &ok);
+    mm.ncontrib = geti(m, (p + "ncontrib").c_str(), &ok);
+    if (!ok || mm.n < 0 || mm.ncontrib < 0) return false;
+    const size_t n = (size_t)mm.n;
+    mm.limits_gpt = pi(m, (p + "limits_gpt").c_str(), &ok, 2 * n);
+    mm.scales_with_density = pi(m, (p + "scales_with_density").c_str(), &ok, n);
+    mm.idx_scaling = pi(m, (p + "idx_scaling").c_str(), &ok, n);
+    mm.scale_by_complement = pi(m, (p + "scale_by_complement").c_str(), &ok, n);
+    mm.idx_minor = pi(m, (p + "idx_minor").c_str(), &ok, n);
+    mm.kminor_start = pi(m, (p + "kminor_start").c_str(), &ok, n);
+    mm.kminor = pd(m, (p + "kminor").c_str(), &ok, (size_t)mm.ncontrib * neta * ntemp);
+    return ok;
+}
+
+int init_files(int lw, const char *gas_file, const char *cloud_file, const char *const *gas_names, int ngases,
+               int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg, int errlen)
+{
+    TableHold h;
+    std::string err;
+    if (!gas_file || !read_file(gas_file, h.k, err)) {
+        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "%s", gas_file ? err.c_str() : "null gas file");
+        return 1;
+    }
+    bool have_lut = cloud_file && cloud_file[0];
+    if (have_lut && !read_file(cloud_file, h.l, err)) {
+        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "%s", err.c_str());
+        return 1;
+    }
+    bool ok = true;
+    ccpprad_kdist &k = h.kd;
+    k.is_lw = geti(h.k, "is_lw", &ok);
+    k.ngas = geti(h.k, "ngas", &ok); k.nband = geti(h.k, "nband", &ok); k.ngpt = geti(h.k, "ngpt", &ok);
+    k.ntemp = geti(h.k, "ntemp", &ok); k.npres = geti(h.k, "npres", &ok); k.neta = geti(h.k, "neta", &ok);
+    k.nflav = geti(h.k, "nflav", &ok); k.nplanck = geti(h.k, "nplanck", &ok);
+    if (ok && (k.ngpt <= 0 || k.nband <= 0 || k.ntemp <= 1 || k.npres <= 1 || k.neta <= 1 || k.nflav <= 0 || k.ngas <= 0))
+        ok = false;
+    if (ok) {
+        const size_t nk = (size_t)k.ngpt * k.neta * (k.npres + 1) * k.ntemp;
+        k.band_lims_gpt = pi(h.k, "band_lims_gpt", &ok, 2 * (size_t)k.nband);
+        k.band_lims_wvn = pd(h.k, "band_lims_wvn", &ok, 2 * (size_t)k.nband);
+        k.flavor = pi(h.k, "flavor", &ok, 2 * (size_t)k.nflav);
+        k.gpoint_flavor = pi(h.k, "gpoint_flavor", &ok, 2 * (size_t)k.ngpt);
+        k.press_ref = pd(h.k, "press_ref", &ok, k.npres);
+        k.temp_ref = pd(h.k, "temp_ref", &ok, k.ntemp);
+        k.press_ref_trop = getd(h.k, "press_ref_trop", &ok);
+        k.vmr_ref = pd(h.k, "vmr_ref", &ok, 2 * (size_t)(k.ngas + 1) * k.ntemp);
+        k.kmajor = pd(h.k, "kmajor", &ok, nk);
+        ok = ok && fill_minor(h.k, "minor_lower_", k.neta, k.ntemp, k.minor_lower) &&
+             fill_minor(h.k, "minor_upper_", k.neta, k.ntemp, k.minor_upper);
+        if (k.is_lw) {
+            k.planck_frac = pd(h.k, "planck_frac", &ok, nk);
+            k.totplnk = pd(h.k, "totplnk", &ok, (size_t)k.nplanck * k.nband);
+            k.optimal_angle_fit = pd(h.k, "optimal_angle_fit", &ok, 2 * (size_t)k.nband, true);
+        } else {
+            const size_t nr = (size_t)k.ngpt * k.neta * k.ntemp;
+            k.krayl_lower = pd(h.k, "krayl_lower", &ok, nr);
+            k.krayl_upper = pd(h.k, "krayl_upper", &ok, nr);
+            k.solar_src = pd(h.k, "solar_src", &ok, k.ngpt);
+        }
+    }
+    if (ok && have_lut) {
+        ccpprad_cloud_lut &L = h.lut;
+        L.nband = geti(h.l, "nband", &ok); L.nsize_liq = geti(h.l, "nsize_liq", &ok);
+        L.nsize_ice = geti(h.l, "nsize_ice", &ok); L.nrghice = geti(h.l, "nrghice", &ok);
+        L.icergh = geti(h.l, "icergh", &ok);
+        L.radliq_lwr = getd(h.l, "radliq_lwr", &ok); L.radliq_upr = getd(h.l, "radliq_upr", &ok);
+        L.radice_lwr = getd(h.l, "radice_lwr", &ok); L.radice_upr = getd(h.l, "radice_upr", &ok);
+        if (ok && (L.nband <= 0 || L.nsize_liq < 2 || L.nsize_ice < 2 || L.nrghice < 1)) ok = false;
+        if (ok) {
+            const size_t nl = (size_t)L.nsize_liq * L.nband, ni = (size_t)L.nsize_ice * L.nband * L.nrghice;
+            L.extliq = pd(h.l, "extliq", &ok, nl); L.ssaliq = pd(h.l, "ssaliq", &ok, nl); L.asyliq = pd(h.l, "asyliq", &ok, nl);
+            L.extice = pd(h.l, "extice", &ok, ni); L.ssaice = pd(h.l, "ssaice", &ok, ni); L.asyice = pd(h.l, "asyice", &ok, ni);
+        }
+    }
+    if (!ok) {
+        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "table file lacks a required record or has a bad size");
+        return 1;
+    }
+    return lw ? ccpprad_lw_init(&h.kd, have_lut ? &h.lut : nullptr, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen)
+              : ccpprad_sw_init(&h.kd, have_lut ? &h.lut : nullptr, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen);
+}
+}  // namespace
+
+extern "C" int ccpprad_lw_init_files(const char *gas_file, const char *cloud_file, const char *const *gas_names,
+                                     int ngases, int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg, int errlen)
+{
+    return init_files(1, gas_file, cloud_file, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen);
+}
+
+extern "C" int ccpprad_sw_init_files(const char *gas_file, const char *cloud_file, const char *const *gas_names,
+                                     int ngases, int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg, int errlen)
+{
+    return init_files(0, gas_file, cloud_file, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen);
+}

@@ -1,0 +1,90 @@
+// Launch interfaces of the gfx950 kernels (argument blocks passed by value).
+#pragma once
+#include "ccpprad_dev.h"
+
+namespace ccpprad {
+
+struct VmrPtrs {
+    const double *v[MAX_GAS];  // gas g (1-based) at v[g-1]; each (ncol_ld, nlay)
+};
+
+void launch_gas_optics(const TablesView &T, int ncols, int nlay, int ncol_ld, const int *col_idx, int col0,
+                       const double *p_lay, const double *p_lev, const double *t_lay, const VmrPtrs &vmr,
+                       float *tau, float *aux, int *err, hipStream_t st);
+
+struct LwSolveArgs {
+    int nlay, ncol_ld, col0, top_at_1;
+    const float *tau;      // [d][l][g]
+    const float *pfrac;    // [d][l][g]
+    const unsigned *mask;  // [d][l][MASK_WORDS] or null (clear sky)
+    const float *ctau;     // [d][l][nband] cloud absorption depth or null
+    const double *aer_tau; // Fortran (ncol_ld, nlay, nband) or null
+    const double *t_lay, *t_lev, *tsfc, *semis, *p_lev;
+    float D[MAX_ANG], W[MAX_ANG];
+    double *flux_up, *flux_dn, *flux_up2, *flux_dn2, *flux_jac, *htr;  // Fortran (ncol_ld, nlev)
+};
+void launch_lw_solve(const TablesView &T, const LwSolveArgs &A, int ncols, int nang, bool clouds, bool jac,
+                     bool optang, hipStream_t st);
+void launch_heating_rate(int ncol, int nlay, const double *up, const double *dn, const double *p_lev,
+                         double *htr, hipStream_t st);
+
+struct CloudArgs {
+    int nlay, ncol_ld, col0, blksz, nband, is_lw;
+    const int *col_idx;            // 1-based gather map (SW daylit) or null
+    const unsigned char *blkflag;  // per scheme block: any cloudy column
+    const double *cld_frac, *lwp, *iwp, *reliq, *reice;
+    const double *cnv[4], *pbl[4];  // lwp, iwp, reliq, reice (null = off)
+    const double *rwp, *swp, *resnow;
+    float *ctau, *cssa, *cg;  // [d][l][b]
+    double *cldtau;           // SW: (ncol_ld, nlay) band-11 cloud tau, or null
+    int *err;
+};
+void launch_cloud_optics(const LutView &L, const CloudArgs &A, int ncols, hipStream_t st);
+
+struct McicaArgs {
+    int nlay, ngpt, ncol_ld, col0, blksz, isubc;
+    int iovr, iovr_max, iovr_maxrand, iovr_rand, iovr_dcorr, iovr_exp, iovr_exprand;
+    const int *col_idx;
+    const unsigned char *blkflag;
+    const double *cld_frac, *overlap;
+    const int *icseed;
+    const uint32_t *mtpow;  // 69069^i mod 2^32, i = 0..623
+    unsigned *mask;         // [d][l][MASK_WORDS]
+};
+void launch_mcica(const McicaArgs &A, int ncols, hipStream_t st);
+void launch_block_flags(int ncols, int nlay, int ncol_ld, const int *col_idx, int blksz, const double *cld_frac,
+                        unsigned char *blkflag, unsigned char *colflag, hipStream_t st);
+
+struct SwSolveArgs {
+    int nlay, ncol_ld, col0, top_at_1, blksz, isfc, write_clr_zero, role;  // role 0 all sky, 1 clear sky
+    const int *col_idx;
+    const float *tau_abs, *tau_ray;        // [d][l][g]
+    const unsigned *mask;                  // [d][l][MASK_WORDS] or null
+    const float *ctau, *cssa, *cg;         // [d][l][nband] or null
+    const unsigned char *blkflag;
+    const double *aer_tau, *aer_ssa, *aer_g;  // (ncol_ld, nlay, nband) or null
+    const double *coszen, *alb_nir_dir, *alb_nir_dif, *alb_uv_dir, *alb_uv_dif, *p_lev;
+    const int *alb_cat;                    // [nband] 0 NIR, 1 average, 2 UV-VIS, 3 unset
+    float *sF, *sA, *sQ;                   // [d][lev][g] scratch
+    double *up_all, *dn_all, *up_clr, *dn_clr;  // (ncol_ld, nlev)
+    double *sb;                            // [d][3][nband]: dn, dn_dir (all sky), dn (clear sky)
+    double *htr, *htr_clr;                 // (ncol_ld, nlay) or null
+};
+void launch_sw_solve(const TablesView &T, const SwSolveArgs &A, int ncols, bool clouds, hipStream_t st);
+
+struct ScmpArgs {
+    int ndays, blksz, nband, ibd, ibd_uv, do_clr;  // ibd, ibd_uv 1-based
+    const int *col_idx;
+    const unsigned char *blkflag, *colflag;
+    const double *sb;        // [d][3][nband]
+    const int *last_wipe;    // block index of the last reset, -1 if none
+    double *scmpsw;          // [nCol][6]
+};
+void launch_scmpsw(const ScmpArgs &A, int ncol, unsigned char *daymap, int *last_wipe, hipStream_t st);
+
+void launch_fill_f64(double *p, size_t n, double v, hipStream_t st);
+void launch_fill_i32(int *p, size_t n, int v, hipStream_t st);
+void launch_fill_u8(unsigned char *p, size_t n, hipStream_t st);
+void launch_toa_norm(const double *solar, int ngpt, double solcon, double *out64, float *out32, hipStream_t st);
+
+}  // namespace ccpprad

@@ -1,0 +1,227 @@
+/*
+ * ccpprad.h -- C-ABI of the MI355X RRTMGP radiation hot path.
+ *
+ * The entry points replace the CCPP RRTMGP scheme calls of ccpp-physics:
+ *   ccpprad_lw_init / ccpprad_lw_run   <- rrtmgp_lw_main_init / _run
+ *        (physics/Radiation/RRTMGP/rrtmgp_lw_main.F90:43-91, :104-613)
+ *   ccpprad_sw_init / ccpprad_sw_run   <- rrtmgp_sw_main_init / _run
+ *        (physics/Radiation/RRTMGP/rrtmgp_sw_main.F90:34-79, :87-683)
+ *   ccpprad_heating_rate               <- compute_heating_rate as called by
+ *        GFS_rrtmgp_post_run (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_post.F90:159-171,251-266)
+ *   ccpprad_*_gas_optics_dev           <- ty_gas_optics_rrtmgp%gas_optics
+ *        (call sites rrtmgp_lw_main.F90:337-345, rrtmgp_sw_main.F90:346-353)
+ *
+ * Conventions
+ *  - Every CCPP field is FP64, Fortran column-major, column index fastest:
+ *    p_lay(nCol,nLay), p_lev(nCol,nLay+1), aer*(nCol,nLay,nBand).
+ *  - Logical flags are int (0 = .false.).  Integer index tables are 1-based.
+ *  - Every entry returns 0 on success or a nonzero status; the message is
+ *    written into errmsg (NUL-terminated, at most errlen bytes).  The Fortran
+ *    scheme modules map a nonzero status to errflg = 1.
+ *  - *_run take HOST pointers and copy in/out; *_run_dev take DEVICE pointers
+ *    (inputs already in HBM) and an explicit hipStream_t (passed as void*).
+ *  - The library holds no mutable global state in the run path: one context
+ *    per scheme, and calls on distinct streams may overlap.
+ */
+#ifndef CCPPRAD_H
+#define CCPPRAD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CCPPRAD_MAX_GAS 6
+
+/* Minor-absorber description of one atmospheric regime.  Mirrors the
+ * minor_* arrays loaded at rrtmgp_lw_gas_optics.F90:29-39,70-74,330-341. */
+typedef struct ccpprad_minor {
+    int n;                           /* number of minor intervals            */
+    int ncontrib;                    /* total contributors (sum of widths)   */
+    const int *limits_gpt;           /* (2, n)  first/last g-point, 1-based  */
+    const int *scales_with_density;  /* (n)                                  */
+    const int *idx_scaling;          /* (n) gas index of scaling gas, 0=none */
+    const int *scale_by_complement;  /* (n)                                  */
+    const int *idx_minor;            /* (n) gas index of the minor absorber  */
+    const int *kminor_start;         /* (n) 1-based start in contributors    */
+    const double *kminor;            /* (ncontrib, neta, ntemp)  cm2/molec   */
+} ccpprad_minor;
+
+/* A k-distribution: the arguments of ty_gas_optics_rrtmgp%load
+ * (rrtmgp_lw_gas_optics.F90:439-448, rrtmgp_sw_gas_optics.F90:487-496).
+ * Gas indices: 0 = dry air, 1..ngas in the order h2o co2 o3 n2o ch4 o2. */
+typedef struct ccpprad_kdist {
+    int is_lw;
+    int ngas, nband, ngpt, ntemp, npres, neta, nflav, nplanck;
+    const int *band_lims_gpt;        /* (2, nband)                          */
+    const double *band_lims_wvn;     /* (2, nband) cm-1                     */
+    const int *flavor;               /* (2, nflav) gas indices              */
+    const int *gpoint_flavor;        /* (2, ngpt) flavor (lower, upper)     */
+    const double *press_ref;         /* (npres) Pa, decreasing              */
+    const double *temp_ref;          /* (ntemp) K, uniform                  */
+    double press_ref_trop;           /* Pa                                  */
+    const double *vmr_ref;           /* (2, 0:ngas, ntemp)                  */
+    const double *kmajor;            /* (ngpt, neta, npres+1, ntemp)        */
+    ccpprad_minor minor_lower, minor_upper;
+    /* LW only (NULL for SW) */
+    const double *planck_frac;       /* (ngpt, neta, npres+1, ntemp)        */
+    const double *totplnk;           /* (nplanck, nband)                    */
+    const double *optimal_angle_fit; /* (2, nband)                          */
+    /* SW only (NULL for LW) */
+    const double *krayl_lower;       /* (ngpt, neta, ntemp)                 */
+    const double *krayl_upper;       /* (ngpt, neta, ntemp)                 */
+    const double *solar_src;         /* (ngpt)                              */
+} ccpprad_kdist;
+
+/* Lookup-table cloud optics: the LUT arguments of ty_cloud_optics%load
+ * (rrtmgp_lw_cloud_optics.F90:184-191,362-367; SW :352-366). */
+typedef struct ccpprad_cloud_lut {
+    int nband, nsize_liq, nsize_ice, nrghice, icergh;
+    double radliq_lwr, radliq_upr, radice_lwr, radice_upr;
+    const double *extliq, *ssaliq, *asyliq;   /* (nsize_liq, nband)          */
+    const double *extice, *ssaice, *asyice;   /* (nsize_ice, nband, nrghice) */
+} ccpprad_cloud_lut;
+
+/* Argument list of rrtmgp_lw_main_run (rrtmgp_lw_main.F90:104-194), in order.
+ * active_gases_array is bound at init.  htrlw/htrlwc are optional extra
+ * outputs (NULL = not computed): the K/s heating rates GFS_rrtmgp_post_run
+ * derives from the all-sky / clear-sky fluxes. */
+typedef struct ccpprad_lw_args {
+    int doLWrad, doLWclrsky, top_at_1, doGP_lwscat, use_LW_jacobian;
+    int doGP_sgs_cnv, doGP_sgs_pbl;
+    int nCol, nLay, nGases, rrtmgp_phys_blksz, nGauss_angles;
+    const int *icseed_lw;                                   /* (nCol) */
+    int iovr, iovr_convcld, iovr_max, iovr_maxrand, iovr_rand, iovr_dcorr;
+    int iovr_exp, iovr_exprand, isubc_lw;
+    const double *semis, *tsfg;                             /* (nCol) */
+    const double *p_lay, *p_lev, *t_lay, *t_lev;
+    const double *vmr_o2, *vmr_h2o, *vmr_o3, *vmr_ch4, *vmr_n2o, *vmr_co2;
+    const double *cld_frac, *cld_lwp, *cld_reliq, *cld_iwp, *cld_reice;
+    const double *cld_swp, *cld_resnow, *cld_rwp, *cld_rerain, *precip_frac;
+    const double *cld_cnv_lwp, *cld_cnv_reliq, *cld_cnv_iwp, *cld_cnv_reice;
+    const double *cld_pbl_lwp, *cld_pbl_reliq, *cld_pbl_iwp, *cld_pbl_reice;
+    const double *cloud_overlap_param;                      /* (nCol,nLay) */
+    const double *aerlw_tau, *aerlw_ssa, *aerlw_g;          /* (nCol,nLay,nBand) or NULL */
+    double *fluxlwUP_allsky, *fluxlwDOWN_allsky;            /* (nCol,nLay+1) */
+    double *fluxlwUP_clrsky, *fluxlwDOWN_clrsky;
+    double *fluxlwUP_jac, *fluxlwUP_radtime, *fluxlwDOWN_radtime;
+    double *htrlw, *htrlwc;                                 /* (nCol,nLay) or NULL */
+} ccpprad_lw_args;
+
+/* Argument list of rrtmgp_sw_main_run (rrtmgp_sw_main.F90:87-189), in order.
+ * scmpsw is cmpfsw_type(nCol) = 6 doubles per column in the order
+ * uvbfc, uvbf0, nirbm, nirdf, visbm, visdf (Radiation/RRTMG/radsw_param.f:111-118).
+ * idx holds the 1-based daylit column indices (nDay of them). */
+typedef struct ccpprad_sw_args {
+    int doSWrad, doSWclrsky, top_at_1, doGP_sgs_cnv, doGP_sgs_pbl;
+    int nCol, nDay, nLay, nGases, rrtmgp_phys_blksz;
+    const int *idx;                                         /* (nDay) */
+    const int *icseed_sw;                                   /* (nCol) */
+    int iovr, iovr_convcld, iovr_max, iovr_maxrand, iovr_rand, iovr_dcorr;
+    int iovr_exp, iovr_exprand, isubc_sw, iSFC;
+    const double *sfc_alb_nir_dir, *sfc_alb_nir_dif;        /* (nCol) */
+    const double *sfc_alb_uvvis_dir, *sfc_alb_uvvis_dif, *coszen;
+    const double *p_lay, *p_lev, *t_lay, *t_lev;
+    const double *vmr_o2, *vmr_h2o, *vmr_o3, *vmr_ch4, *vmr_n2o, *vmr_co2;
+    const double *cld_frac, *cld_lwp, *cld_reliq, *cld_iwp, *cld_reice;
+    const double *cld_swp, *cld_resnow, *cld_rwp, *cld_rerain, *precip_frac;
+    const double *cld_cnv_lwp, *cld_cnv_reliq, *cld_cnv_iwp, *cld_cnv_reice;
+    const double *cld_pbl_lwp, *cld_pbl_reliq, *cld_pbl_iwp, *cld_pbl_reice;
+    const double *cloud_overlap_param;
+    const double *aersw_tau, *aersw_ssa, *aersw_g;          /* (nCol,nLay,nBand) or NULL */
+    double solcon;
+    double *scmpsw;                                         /* (6, nCol) */
+    double *fluxswUP_allsky, *fluxswDOWN_allsky;
+    double *fluxswUP_clrsky, *fluxswDOWN_clrsky;
+    double *cldtausw;                                       /* (nCol,nLay) */
+    double *htrsw, *htrswc;                                 /* (nCol,nLay) or NULL */
+} ccpprad_sw_args;
+
+typedef struct ccpprad_ctx ccpprad_ctx;
+
+/* ---- scheme entry points (host pointers) ---------------------------- */
+
+/* rrtmgp_lw_main_init: device-resident tables for one GPU.  gas_names lists
+ * the ngases active gas names (CCPP active_gases_array); every name must be
+ * one of the k-distribution's gases.  nrghice is the ice roughness category
+ * used when lut->icergh is 0. */
+int ccpprad_lw_init(const ccpprad_kdist *kd, const ccpprad_cloud_lut *lut,
+                    const char *const *gas_names, int ngases, int nLay,
+                    int rrtmgp_phys_blksz, ccpprad_ctx **ctx, char *errmsg, int errlen);
+/* rrtmgp_sw_main_init */
+int ccpprad_sw_init(const ccpprad_kdist *kd, const ccpprad_cloud_lut *lut,
+                    const char *const *gas_names, int ngases, int nLay,
+                    int rrtmgp_phys_blksz, ccpprad_ctx **ctx, char *errmsg, int errlen);
+/* Init from table files written by the package's table writer
+ * (the netCDF files named at rrtmgp_lw_gas_optics.F90:81-450 are replaced by
+ * this binary format). */
+int ccpprad_lw_init_files(const char *gas_file, const char *cloud_file,
+                          const char *const *gas_names, int ngases, int nLay,
+                          int rrtmgp_phys_blksz, ccpprad_ctx **ctx, char *errmsg, int errlen);
+int ccpprad_sw_init_files(const char *gas_file, const char *cloud_file,
+                          const char *const *gas_names, int ngases, int nLay,
+                          int rrtmgp_phys_blksz, ccpprad_ctx **ctx, char *errmsg, int errlen);
+
+/* rrtmgp_lw_main_run / rrtmgp_sw_main_run with host arrays. */
+int ccpprad_lw_run(ccpprad_ctx *ctx, const ccpprad_lw_args *a, char *errmsg, int errlen);
+int ccpprad_sw_run(ccpprad_ctx *ctx, const ccpprad_sw_args *a, char *errmsg, int errlen);
+
+/* Same, every array pointer in the args struct is a DEVICE pointer.  No
+ * host<->device copies, no synchronisation; errors in the inputs that need
+ * a device-side check (table range) are reported only when validate != 0,
+ * which synchronises the stream. */
+int ccpprad_lw_run_dev(ccpprad_ctx *ctx, const ccpprad_lw_args *a, void *stream,
+                       int validate, char *errmsg, int errlen);
+int ccpprad_sw_run_dev(ccpprad_ctx *ctx, const ccpprad_sw_args *a, void *stream,
+                       int validate, char *errmsg, int errlen);
+
+/* Free a context (tables, work buffers).  The reference has no scheme-level
+ * finalize; this plays the role of GFS_rrtmgp_setup_finalize
+ * (GFS_rrtmgp_setup.F90:260-273) for the device state. */
+void ccpprad_finalize(ccpprad_ctx *ctx);
+
+/* ---- component entry points ----------------------------------------- */
+
+/* compute_heating_rate (GFS_rrtmgp_post.F90:167-171): FP64 on the device.
+ * htr(nCol,nLay) [K/s] = (dF_up - dF_dn) * g / (cp * dp).  Device pointers. */
+int ccpprad_heating_rate_dev(int nCol, int nLay, const double *flux_up,
+                             const double *flux_dn, const double *p_lev, double *htr,
+                             void *stream, char *errmsg, int errlen);
+
+/* ty_gas_optics_rrtmgp%gas_optics, LW (rrtmgp_lw_main.F90:337-345).  Device
+ * pointers.  vmr is (nCol, nLay, ngas) in the k-distribution's gas order.
+ * Outputs g-point innermost, FP32: tau(ngpt,nLay,nCol),
+ * pfrac(ngpt,nLay,nCol) (Planck fraction; lay_src = pfrac * B_band(T)). */
+int ccpprad_lw_gas_optics_dev(ccpprad_ctx *ctx, int nCol, int nLay,
+                              const double *p_lay, const double *p_lev,
+                              const double *t_lay, const double *vmr,
+                              float *tau, float *pfrac, void *stream,
+                              char *errmsg, int errlen);
+/* SW variant (rrtmgp_sw_main.F90:346-353): absorption and Rayleigh depths. */
+int ccpprad_sw_gas_optics_dev(ccpprad_ctx *ctx, int nCol, int nLay,
+                              const double *p_lay, const double *p_lev,
+                              const double *t_lay, const double *vmr,
+                              float *tau_abs, float *tau_ray, void *stream,
+                              char *errmsg, int errlen);
+
+/* ---- observability (no counterpart in the reference, which has no timers) */
+
+/* Per-kernel device timing with HIP events recorded on the launch stream.
+ * Kernel ids: 0 gas optics, 1 LW solver, 2 SW solver, 3 cloud optics,
+ * 4 McICA, 5 MLP inference.  enable != 0 starts recording (and clears). */
+int ccpprad_timing_enable(ccpprad_ctx *ctx, int enable);
+/* Synchronises the recorded events; total milliseconds and launch count. */
+int ccpprad_timing_query(ccpprad_ctx *ctx, int kernel_id, double *total_ms, int *count);
+
+/* Number of g-points / bands of a context. */
+int ccpprad_ngpt(const ccpprad_ctx *ctx);
+int ccpprad_nband(const ccpprad_ctx *ctx);
+
+/* Library version string. */
+const char *ccpprad_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CCPPRAD_H */

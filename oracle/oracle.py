@@ -1,0 +1,233 @@
+"""ctypes wrapper of the FP64 oracle (oracle/rrtmgp_oracle.c).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker.  Never imported by the product
+package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_REPO = os.path.dirname(_HERE)
+_PKG = os.path.join(_REPO, "ccpp-physics_amd")
+if _PKG not in sys.path:
+    sys.path.insert(0, _PKG)
+
+from ccpprad import capi, synthetic as syn  # noqa: E402
+
+P_D = C.POINTER(C.c_double)
+P_I = C.POINTER(C.c_int)
+P_U8 = C.POINTER(C.c_ubyte)
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+        L = C.CDLL(path)
+        L.orc_gas_optics.argtypes = [C.c_void_p, C.c_int, C.c_int, P_D, P_D, P_D, P_D, P_D, P_D, P_D]
+        L.orc_lw_sources.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, P_D, P_D, P_D, P_D,
+                                     P_D, P_D, P_D, P_D, P_D]
+        L.orc_lw_solve.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, P_D, P_D, P_D,
+                                   P_D, P_D, P_D, P_D, P_D, P_D, P_D, P_D]
+        L.orc_optimal_angles.argtypes = [C.c_void_p, C.c_int, C.c_int, P_D, P_D]
+        L.orc_gauss.argtypes = [C.c_int, P_D, P_D]
+        L.orc_heating_rate.argtypes = [C.c_int, C.c_int, P_D, P_D, P_D, P_D]
+        L.orc_mt_draws.argtypes = [C.c_int, C.c_int, P_D]
+        L.orc_mcica_mask.argtypes = [C.c_int, C.c_int, C.c_int, P_I, P_D, P_D] + [C.c_int] * 7 + [P_U8]
+        L.orc_cloud_optics.argtypes = [C.c_void_p, C.c_int, C.c_int, P_D, P_D, P_D, P_D, P_D, P_D, P_D]
+        L.orc_cloud_optics.restype = C.c_int
+        L.orc_lw_main_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
+        L.orc_sw_main_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
+        L.orc_sw_solve.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, P_D, P_D, P_D, P_D, P_D,
+                                   P_D, P_D, P_D, P_D, P_D]
+        L.orc_planck_band.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.orc_planck_band.restype = C.c_double
+        L.orc_interp_layer.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, P_D, P_I, P_I,
+                                       P_I, P_D, P_D, P_D, P_I, P_D, P_D]
+        L.orc_nthreads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's own Fortran (mersenne_twister.f, radiation_tools.F90)
+    built by `make -C oracle ref`; None when not built."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(_HERE, "_ref", "libccpp_ref.so")
+        if not os.path.exists(path):
+            return None
+        R = C.CDLL(path)
+        R.ref_mt_draws.argtypes = [C.c_int, C.c_int, P_D]
+        R.ref_mt_draws_chunked.argtypes = [C.c_int, C.c_int, C.c_int, P_D]
+        R.ref_cmp_tlev.argtypes = [C.c_int, C.c_int, C.c_double, P_D, P_D, P_D, P_D, P_D]
+        _ref = R
+    return _ref
+
+
+def _p(a, t=P_D):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def nthreads():
+    return lib().orc_nthreads()
+
+
+def vmr_stack(pr: syn.Profiles):
+    """(ngas, nlay, ncol) numpy == Fortran (ncol, nlay, ngas), k-dist gas order."""
+    return np.ascontiguousarray(np.stack([pr.vmr[g] for g in syn.GAS_NAMES]))
+
+
+def gas_optics(kd, pr):
+    h = capi.kdist_struct(kd)
+    nc, nl, ng = pr.ncol, pr.nlay, kd.ngpt
+    tau = np.zeros((nc, nl, ng))
+    pf = np.zeros((nc, nl, ng)) if kd.is_lw else None
+    ray = None if kd.is_lw else np.zeros((nc, nl, ng))
+    v = vmr_stack(pr)
+    lib().orc_gas_optics(C.byref(h.struct), nc, nl, _p(pr.p_lay), _p(pr.p_lev), _p(pr.t_lay),
+                         _p(v), _p(tau), _p(pf), _p(ray))
+    return (tau, pf) if kd.is_lw else (tau, ray)
+
+
+def lw_sources(kd, pr, pfrac, top_at_1=False):
+    h = capi.kdist_struct(kd)
+    nc, nl, ng = pr.ncol, pr.nlay, kd.ngpt
+    out = [np.zeros((nc, nl, ng)) for _ in range(3)] + [np.zeros((nc, ng)) for _ in range(2)]
+    lib().orc_lw_sources(C.byref(h.struct), nc, nl, int(top_at_1), _p(pr.t_lay), _p(pr.t_lev),
+                         _p(np.ascontiguousarray(pr.tsfc)), _p(np.ascontiguousarray(pfrac)),
+                         *[_p(o) for o in out])
+    return dict(lay_src=out[0], lev_inc=out[1], lev_dec=out[2], sfc_src=out[3], sfc_jac=out[4])
+
+
+def gauss(n):
+    D = np.zeros(max(n, 1))
+    w = np.zeros(max(n, 1))
+    lib().orc_gauss(n, _p(D), _p(w))
+    return D, w
+
+
+def lw_solve(kd, ncol, nlay, tau, src, emis_b, nang=3, Ds=None, top_at_1=False, jac=False):
+    h = capi.kdist_struct(kd)
+    up = np.zeros((nlay + 1, ncol))
+    dn = np.zeros((nlay + 1, ncol))
+    fj = np.zeros((nlay + 1, ncol)) if jac else None
+    c = np.ascontiguousarray
+    lib().orc_lw_solve(C.byref(h.struct), ncol, nlay, int(top_at_1), nang,
+                       _p(None if Ds is None else c(Ds)), _p(c(tau)), _p(c(src["lay_src"])),
+                       _p(c(src["lev_inc"])), _p(c(src["lev_dec"])), _p(c(emis_b)),
+                       _p(c(src["sfc_src"])), _p(c(src["sfc_jac"])), _p(up), _p(dn), _p(fj))
+    return up, dn, fj
+
+
+def optimal_angles(kd, ncol, nlay, tau):
+    h = capi.kdist_struct(kd)
+    Ds = np.zeros((ncol, kd.ngpt))
+    lib().orc_optimal_angles(C.byref(h.struct), ncol, nlay, _p(np.ascontiguousarray(tau)), _p(Ds))
+    return Ds
+
+
+def sw_solve(kd, ncol, nlay, mu0, tau, ssa, g, toa, alb_dir, alb_dif, top_at_1=False):
+    h = capi.kdist_struct(kd)
+    nb, nv = kd.nband, nlay + 1
+    bup, bdn, bdir = (np.zeros((ncol, nv, nb)) for _ in range(3))
+    c = np.ascontiguousarray
+    lib().orc_sw_solve(C.byref(h.struct), ncol, nlay, int(top_at_1), _p(c(mu0)), _p(c(tau)),
+                       _p(c(ssa)), _p(c(g)), _p(c(toa)), _p(c(alb_dir)), _p(c(alb_dif)),
+                       _p(bup), _p(bdn), _p(bdir))
+    return bup, bdn, bdir
+
+
+def heating_rate(up, dn, p_lev):
+    nv, nc = up.shape
+    h = np.zeros((nv - 1, nc))
+    c = np.ascontiguousarray
+    lib().orc_heating_rate(nc, nv - 1, _p(c(up)), _p(c(dn)), _p(c(p_lev)), _p(h))
+    return h
+
+
+def mt_draws(seed, n):
+    out = np.zeros(n)
+    lib().orc_mt_draws(int(seed), int(n), _p(out))
+    return out
+
+
+def mcica_mask(seeds, cf, ngpt, iovr="iovr_maxrand", overlap=None):
+    nl, nc = cf.shape
+    m = np.zeros((nc, ngpt, nl), dtype=np.uint8)
+    io = capi.IOVR
+    lib().orc_mcica_mask(nc, nl, ngpt, _p(np.ascontiguousarray(seeds, dtype=np.int32), P_I),
+                         _p(np.ascontiguousarray(cf)),
+                         _p(None if overlap is None else np.ascontiguousarray(overlap)),
+                         io[iovr], io["iovr_max"], io["iovr_maxrand"], io["iovr_rand"],
+                         io["iovr_dcorr"], io["iovr_exp"], io["iovr_exprand"], _p(m, P_U8))
+    return m  # (ncol, ngpt, nlay) == Fortran (nlay, ngpt, ncol)
+
+
+def cloud_optics(lut, lwp, iwp, reliq, reice):
+    hl = capi.lut_struct(lut)
+    nl, nc = lwp.shape
+    t, s, g = (np.zeros((lut.nband, nl, nc)) for _ in range(3))
+    c = np.ascontiguousarray
+    rc = lib().orc_cloud_optics(C.byref(hl.struct), nc, nl, _p(c(lwp)), _p(c(iwp)), _p(c(reliq)),
+                                _p(c(reice)), _p(t), _p(s), _p(g))
+    if rc:
+        raise ValueError("effective radius outside LUT range")
+    return t, s, g
+
+
+def _run(fn, kd, lut, cls, ints, arrays, out_names, solcon=None):
+    hk = capi.kdist_struct(kd)
+    hl = capi.lut_struct(lut)
+    ha = capi.build_args(cls, ints, arrays, solcon=solcon)
+    buf, n = capi.errbuf()
+    rc = fn(C.byref(hk.struct), C.byref(hl.struct), C.byref(ha.struct), buf, n)
+    capi.check(rc, buf)
+    return {k: arrays[k] for k in out_names if k in arrays}
+
+
+def lw_main_run(kd, lut, ints, arrays):
+    """rrtmgp_lw_main_run on host numpy arrays; outputs are written in place
+    into the arrays dict (which must hold the output buffers)."""
+    return _run(lib().orc_lw_main_run, kd, lut, capi.LWArgs, ints, arrays, capi.LW_OUTPUT_FIELDS)
+
+
+def sw_main_run(kd, lut, ints, arrays, solcon=1361.0):
+    return _run(lib().orc_sw_main_run, kd, lut, capi.SWArgs, ints, arrays, capi.SW_OUTPUT_FIELDS,
+                solcon=solcon)
+
+
+def lw_outputs(ncol, nlay, hr=True):
+    nv = nlay + 1
+    o = {n: np.zeros((nv, ncol)) for n in capi.LW_OUTPUT_FIELDS if n.startswith("flux")}
+    if hr:
+        o["htrlw"] = np.zeros((nlay, ncol))
+        o["htrlwc"] = np.zeros((nlay, ncol))
+    return o
+
+
+def sw_outputs(ncol, nlay, hr=True):
+    nv = nlay + 1
+    o = {n: np.zeros((nv, ncol)) for n in ("fluxswUP_allsky", "fluxswDOWN_allsky",
+                                           "fluxswUP_clrsky", "fluxswDOWN_clrsky")}
+    o["scmpsw"] = np.zeros((ncol, 6))
+    o["cldtausw"] = np.zeros((nlay, ncol))
+    if hr:
+        o["htrsw"] = np.zeros((nlay, ncol))
+        o["htrswc"] = np.zeros((nlay, ncol))
+    return o

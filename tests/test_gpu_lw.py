@@ -1,0 +1,108 @@
+"""GPU parity of the LW path against the FP64 oracle, through the C-ABI.
+
+Tolerances (the plain-path bar): fluxes 0.05 W/m2 and heating rate
+0.01 K/day max-abs against FP64; gas-optics tau relative 2e-5."""
+import numpy as np
+import pytest
+
+from util import report
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+FLUX_TOL = 0.05      # W/m2
+HR_TOL = 0.01        # K/day
+
+
+def _setup(kd, lut, ncol, nlay, clouds, seed=20261203):
+    from ccpprad import synthetic as syn, capi
+    pr = syn.make_profiles(ncol, nlay, seed=seed, clouds=clouds, kd=kd)
+    return pr, capi.profile_fields(pr)
+
+
+def test_lw_gas_optics_parity(kd_lw, lut_lw):
+    from ccpprad import lib, device
+    from oracle import oracle as O
+    ncol, nlay = 96, 127
+    pr, arr = _setup(kd_lw, lut_lw, ncol, nlay, False)
+    tau_o, pf_o = O.gas_optics(kd_lw, pr)
+    s = lib.LWScheme(kd_lw, lut_lw, nlay)
+    v = torch.from_numpy(O.vmr_stack(pr)).cuda()
+    d = device.to_device({k: arr[k] for k in ("p_lay", "p_lev", "t_lay")})
+    tau = torch.zeros((ncol, nlay, kd_lw.ngpt), dtype=torch.float32, device="cuda")
+    pf = torch.zeros_like(tau)
+    s.gas_optics_dev(ncol, nlay, d["p_lay"], d["p_lev"], d["t_lay"], v, tau, pf)
+    torch.cuda.synchronize()
+    tau_g, pf_g = tau.cpu().numpy().astype(np.float64), pf.cpu().numpy().astype(np.float64)
+    rel = np.abs(tau_g - tau_o) / np.maximum(np.abs(tau_o), 1e-30)
+    print("tau rel max", rel.max(), "pfrac abs max", np.abs(pf_g - pf_o).max())
+    assert rel.max() < 2e-5
+    assert np.abs(pf_g - pf_o).max() < 2e-6
+
+
+def _run_both(kd, lut, ncol, nlay, clouds, nangle, clrsky, jac, blksz=None, iovr="iovr_maxrand"):
+    from ccpprad import lib, device, capi
+    from oracle import oracle as O
+    pr, arr = _setup(kd, lut, ncol, nlay, clouds)
+    ints = capi.lw_default_ints(ncol, nlay, blksz=blksz, nangle=nangle, clrsky=clrsky, jacobian=jac, iovr=iovr)
+    ref = dict(arr)
+    ref.update(O.lw_outputs(ncol, nlay))
+    O.lw_main_run(kd, lut, ints, ref)
+    s = lib.LWScheme(kd, lut, nlay, blksz=ints["rrtmgp_phys_blksz"])
+    dev = device.to_device(arr)
+    dev.update(device.lw_outputs(ncol, nlay))
+    s.run_dev(ints, dev)
+    torch.cuda.synchronize()
+    out = device.to_host(dev)
+    return ref, out
+
+
+def _check(ref, out, names, hr_names=("htrlw",)):
+    for n in names:
+        mx, _ = report(n, out[n], ref[n], "W/m2")
+        assert mx < FLUX_TOL, n
+    for n in hr_names:
+        mx, _ = report(n, out[n] * 86400, ref[n] * 86400, "K/day")
+        assert mx < HR_TOL, n
+
+
+def test_lw_clear_sky_one_angle(kd_lw, lut_lw):
+    """128 columns x 64 layers, clear sky, 1 angle (optimal angles for the
+    clear-sky call), Jacobian on."""
+    ref, out = _run_both(kd_lw, lut_lw, 128, 64, clouds=False, nangle=1, clrsky=True, jac=True)
+    _check(ref, out, ["fluxlwUP_allsky", "fluxlwDOWN_allsky", "fluxlwUP_clrsky", "fluxlwDOWN_clrsky",
+                      "fluxlwUP_jac", "fluxlwUP_radtime", "fluxlwDOWN_radtime"], ("htrlw", "htrlwc"))
+
+
+def test_lw_all_sky_three_angles(kd_lw, lut_lw):
+    """All-sky, 127 layers, 3 Gauss angles, McICA max-random overlap."""
+    ref, out = _run_both(kd_lw, lut_lw, 256, 127, clouds=True, nangle=3, clrsky=True, jac=False, blksz=64)
+    _check(ref, out, ["fluxlwUP_allsky", "fluxlwDOWN_allsky", "fluxlwUP_clrsky", "fluxlwDOWN_clrsky"],
+           ("htrlw", "htrlwc"))
+
+
+@pytest.mark.parametrize("iovr", ["iovr_rand", "iovr_max", "iovr_exp", "iovr_dcorr"])
+def test_lw_overlap_variants(kd_lw, lut_lw, iovr):
+    ref, out = _run_both(kd_lw, lut_lw, 64, 127, clouds=True, nangle=2, clrsky=False, jac=False, iovr=iovr)
+    _check(ref, out, ["fluxlwUP_allsky", "fluxlwDOWN_allsky"])
+
+
+def test_lw_host_entry_matches_device_entry(kd_lw, lut_lw):
+    """ccpprad_lw_run (host arrays, the Fortran drop-in path) equals the
+    device entry bit for bit."""
+    from ccpprad import lib, device, capi
+    from oracle import oracle as O
+    ncol, nlay = 64, 127
+    pr, arr = _setup(kd_lw, lut_lw, ncol, nlay, True)
+    ints = capi.lw_default_ints(ncol, nlay, nangle=3)
+    s = lib.LWScheme(kd_lw, lut_lw, nlay, blksz=ncol)
+    host = dict(arr)
+    host.update(O.lw_outputs(ncol, nlay))
+    s.run(ints, host)
+    dev = device.to_device(arr)
+    dev.update(device.lw_outputs(ncol, nlay))
+    s.run_dev(ints, dev)
+    torch.cuda.synchronize()
+    out = device.to_host(dev)
+    for n in ("fluxlwUP_allsky", "fluxlwDOWN_allsky", "htrlw"):
+        assert np.array_equal(out[n], host[n]), n
