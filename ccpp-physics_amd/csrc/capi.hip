@@ -82,6 +82,9 @@ struct ccpprad_ctx {
     std::vector<DevBuf> stage;
     hipStream_t stream = nullptr;
     int chunk = 65536;
+    // learned gas optics (LW): MLP with table fallback, or null = table
+    ccpprad_mlp *mlp = nullptr;
+    DevBuf w_oob, w_colm;
     // optional per-kernel event timing
     int timing = 0;
     std::vector<hipEvent_t> ev_pool;
@@ -369,12 +372,21 @@ extern "C" void ccpprad_finalize(ccpprad_ctx *c)
     if (!c) return;
     for (void *p : c->allocs) (void)hipFree(p);
     DevBuf *bufs[] = {&c->w_tau, &c->w_aux, &c->w_mask, &c->w_ctau, &c->w_cssa, &c->w_cg, &c->w_sF,
-                      &c->w_sA, &c->w_sQ, &c->w_sb, &c->w_blk, &c->w_colf, &c->w_day, &c->w_idx};
+                      &c->w_sA, &c->w_sQ, &c->w_sb, &c->w_blk, &c->w_colf, &c->w_day, &c->w_idx,
+                      &c->w_oob, &c->w_colm};
     for (DevBuf *b : bufs) b->release();
     for (auto &b : c->stage) b.release();
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+extern "C" int ccpprad_lw_set_gas_optics_mlp(ccpprad_ctx *c, ccpprad_mlp *m)
+{
+    if (!c || !c->is_lw) return 1;
+    if (m && c->T.ngpt != 128) return 1;   // the emulator predicts 128 g-points
+    c->mlp = m;
+    return 0;
 }
 
 extern "C" int ccpprad_ngpt(const ccpprad_ctx *c) { return c ? c->T.ngpt : 0; }
@@ -439,6 +451,10 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
     HIPCHK(c->w_ctau.ensure(sizeof(float) * (size_t)CH * nL * B));
     HIPCHK(c->w_blk.ensure((size_t)nblk + 1));
     HIPCHK(c->w_colf.ensure((size_t)nC + 1));
+    if (c->mlp) {
+        HIPCHK(c->w_oob.ensure((size_t)CH * nL));
+        HIPCHK(c->w_colm.ensure((size_t)CH + 1));
+    }
     unsigned char *blk = (unsigned char *)c->w_blk.p;
     launch_fill_i32(c->err, 1, 0, st);
     if (have_clouds) launch_block_flags(nC, nL, nC, nullptr, bs, a->cld_frac, blk, (unsigned char *)c->w_colf.p, st);
@@ -472,7 +488,22 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
             ma.icseed = a->icseed_lw; ma.mtpow = c->mtpow; ma.mask = mask;
             { KScope k_(c, K_MCICA, st); launch_mcica(ma, nc, st); }
         }
-        { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, tau, pf, c->err, st); }
+        if (c->mlp) {
+            MlpInferArgs mi{};
+            mi.nlay = nL; mi.ncol_ld = nC; mi.col0 = c0; mi.ncols = nc;
+            mi.p_lay = a->p_lay; mi.p_lev = a->p_lev; mi.t_lay = a->t_lay;
+            mi.vmr[0] = a->vmr_h2o; mi.vmr[1] = a->vmr_o3; mi.vmr[2] = a->vmr_co2;
+            mi.vmr[3] = a->vmr_ch4; mi.vmr[4] = a->vmr_n2o;
+            mi.tau = tau; mi.pfrac = pf;
+            mi.oob = (unsigned char *)c->w_oob.p; mi.colflag = (unsigned char *)c->w_colm.p;
+            launch_fill_u8(mi.colflag, (size_t)nc, st);
+            { KScope k_(c, K_MLP, st); launch_mlp_infer(c->mlp, mi, st); }
+            // layers outside the training hull: table gas optics
+            { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, tau, pf, c->err, st, mi.oob, mi.colflag); }
+        } else {
+            KScope k_(c, K_GAS, st);
+            launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, tau, pf, c->err, st);
+        }
         LwSolveArgs s{};
         s.nlay = nL; s.ncol_ld = nC; s.col0 = c0; s.top_at_1 = a->top_at_1;
         s.tau = tau; s.pfrac = pf; s.mask = mask; s.ctau = ctau; s.aer_tau = a->aerlw_tau;

@@ -41,12 +41,14 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
     TablesView T, int nlay, int ncol_ld, const int *__restrict__ col_idx, int col0,
     const double *__restrict__ p_lay, const double *__restrict__ p_lev,
     const double *__restrict__ t_lay, VmrPtrs vmr, float *__restrict__ tau,
-    float *__restrict__ aux /* LW pfrac, SW tau_ray */, int *__restrict__ err)
+    float *__restrict__ aux /* LW pfrac, SW tau_ray */, int *__restrict__ err,
+    const unsigned char *__restrict__ row_mask, const unsigned char *__restrict__ col_mask)
 {
     __shared__ GasOpticsSmem s;
     const int d = blockIdx.x;                 // column within this launch
     const int gc = col_idx ? col_idx[col0 + d] - 1 : col0 + d;  // global column
     const int tid = threadIdx.x;
+    if (col_mask && !col_mask[d]) return;   // uniform per block
     const int ngpt = T.ngpt, neta = T.neta, np1 = T.npres + 1, ng1 = T.ngas + 1;
     const double tmin = T.temp_ref_min, dT = T.temp_ref_delta;
 
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
             for (int j = 0; j < nl; ++j) {
                 const int l = l0 + j;
                 const int itr = s.itropo[j];
-                const int f = (g == tid) ? gfl[itr - 1] : T.gpt_flav[2 * g + itr - 1];
+                const int f = (g == tid) ? (itr == 1 ? gfl[0] : gfl[1]) : T.gpt_flav[2 * g + itr - 1];
                 const int jt0 = s.jtemp[j] - 1;
                 const int jp0 = s.jpress[j] + itr - 2;  // 0-based lower pressure row
                 const float fp1 = s.fpress[j], fp0 = 1.0f - fp1;
@@ -171,6 +173,7 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
                     tg += s.ms[j][iv] * kv;
                 }
                 const size_t o = ((size_t)d * nlay + l) * ngpt + g;
+                if (row_mask && !row_mask[(size_t)d * nlay + l]) continue;
                 tau[o] = tg;
                 if (LW) {
                     aux[o] = pf;
@@ -193,15 +196,16 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
 
 void launch_gas_optics(const TablesView &T, int ncols, int nlay, int ncol_ld, const int *col_idx,
                        int col0, const double *p_lay, const double *p_lev, const double *t_lay,
-                       const VmrPtrs &vmr, float *tau, float *aux, int *err, hipStream_t st)
+                       const VmrPtrs &vmr, float *tau, float *aux, int *err, hipStream_t st,
+                       const unsigned char *row_mask, const unsigned char *col_mask)
 {
     if (ncols <= 0) return;
     if (T.is_lw)
         hipLaunchKernelGGL(gas_optics_kernel<true>, dim3(ncols), dim3(128), 0, st, T, nlay, ncol_ld,
-                           col_idx, col0, p_lay, p_lev, t_lay, vmr, tau, aux, err);
+                           col_idx, col0, p_lay, p_lev, t_lay, vmr, tau, aux, err, row_mask, col_mask);
     else
         hipLaunchKernelGGL(gas_optics_kernel<false>, dim3(ncols), dim3(128), 0, st, T, nlay, ncol_ld,
-                           col_idx, col0, p_lay, p_lev, t_lay, vmr, tau, aux, err);
+                           col_idx, col0, p_lay, p_lev, t_lay, vmr, tau, aux, err, row_mask, col_mask);
 }
 
 }  // namespace ccpprad

@@ -2,15 +2,20 @@
 #pragma once
 #include "ccpprad_dev.h"
 
+struct ccpprad_mlp;
+
 namespace ccpprad {
 
 struct VmrPtrs {
     const double *v[MAX_GAS];  // gas g (1-based) at v[g-1]; each (ncol_ld, nlay)
 };
 
+// row_mask / col_mask (optional): only rows with row_mask[d*nlay+l] != 0 of
+// columns with col_mask[d] != 0 are written (the emulator's fallback rows).
 void launch_gas_optics(const TablesView &T, int ncols, int nlay, int ncol_ld, const int *col_idx, int col0,
                        const double *p_lay, const double *p_lev, const double *t_lay, const VmrPtrs &vmr,
-                       float *tau, float *aux, int *err, hipStream_t st);
+                       float *tau, float *aux, int *err, hipStream_t st,
+                       const unsigned char *row_mask = nullptr, const unsigned char *col_mask = nullptr);
 
 struct LwSolveArgs {
     int nlay, ncol_ld, col0, top_at_1;
@@ -81,6 +86,21 @@ struct ScmpArgs {
     double *scmpsw;          // [nCol][6]
 };
 void launch_scmpsw(const ScmpArgs &A, int ncol, unsigned char *daymap, int *last_wipe, hipStream_t st);
+
+// Learned gas optics: fused MLP inference (mlp.hip)
+constexpr int MLP_NIN = 7;
+struct MlpInferArgs {
+    int nlay, ncol_ld, col0, ncols;
+    const double *p_lay, *p_lev, *t_lay;
+    const double *vmr[5];               // h2o, o3, co2, ch4, n2o
+    const unsigned char *image;         // device copy of the LDS image
+    float xmean[MLP_NIN], xistd[MLP_NIN], lo[MLP_NIN], hi[MLP_NIN];
+    const float *ymean, *ystd;          // [256]
+    float *tau, *pfrac;                 // [d][l][g]
+    unsigned char *oob;                 // [d][l] 1 = out of hull
+    unsigned char *colflag;             // [d] any row out of hull
+};
+void launch_mlp_infer(const struct ::ccpprad_mlp *m, MlpInferArgs &a, hipStream_t st);
 
 void launch_fill_f64(double *p, size_t n, double v, hipStream_t st);
 void launch_fill_i32(int *p, size_t n, int v, hipStream_t st);

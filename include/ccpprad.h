@@ -205,6 +205,47 @@ int ccpprad_sw_gas_optics_dev(ccpprad_ctx *ctx, int nCol, int nLay,
                               float *tau_abs, float *tau_ray, void *stream,
                               char *errmsg, int errlen);
 
+/* ---- learned LW gas optics (no counterpart in the reference) -------------
+ * A 7 -> 128 -> 128 -> 128 -> 256 softplus MLP on BF16 MFMA with FP32
+ * accumulation that stands in for the k-distribution lookup of the LW
+ * gas_optics call (rrtmgp_lw_main.F90:337-345).  Inputs per layer: T,
+ * ln p, ln vmr of h2o o3 co2 ch4 n2o (standardised with xmean/xstd); outputs
+ * 128 ln(tau/col_dry) and 128 Planck fractions (standardised with
+ * ymean/ystd).  Flat parameter order: W1(7x128, input-major), b1, W2(128x128),
+ * b2, W3(128x128), b3, W4(128x256), b4; 67072 floats.  Layers whose inputs
+ * leave the [lo, hi] training hull fall back to the table path. */
+typedef struct ccpprad_mlp ccpprad_mlp;
+int ccpprad_mlp_create(ccpprad_mlp **m, char *errmsg, int errlen);
+void ccpprad_mlp_destroy(ccpprad_mlp *m);
+int ccpprad_mlp_nparams(void);
+int ccpprad_mlp_set_params(ccpprad_mlp *m, const float *host_params, char *errmsg, int errlen);
+int ccpprad_mlp_get_params(ccpprad_mlp *m, float *host_params, float *host_grads);
+int ccpprad_mlp_set_norm(ccpprad_mlp *m, const float *xmean, const float *xstd, const float *lo,
+                         const float *hi, const float *ymean, const float *ystd, char *errmsg, int errlen);
+/* Training on standardised batches (device pointers): X rows x 7 FP32 (fed
+ * to the first GEMM as BF16 hi + lo parts), T rows x 256 FP32; loss_out
+ * (device, 1 float) receives the batch MSE.  Adam (0.9, 0.999, 1e-8) on
+ * FP32 master weights with BF16 copies. */
+int ccpprad_mlp_forward_dev(ccpprad_mlp *m, const float *X, int rows, float *Y, void *stream,
+                            char *errmsg, int errlen);
+int ccpprad_mlp_grads_dev(ccpprad_mlp *m, const float *X, const float *T, int rows, float *loss_out,
+                          void *stream, char *errmsg, int errlen);
+int ccpprad_mlp_train_step_dev(ccpprad_mlp *m, const float *X, const float *T, int rows, float lr,
+                               float *loss_out, void *stream, char *errmsg, int errlen);
+/* Rebuild the fused-inference weight image after training or set_params. */
+int ccpprad_mlp_finalize_training(ccpprad_mlp *m, char *errmsg, int errlen);
+/* Fused inference over (nCol, nLay) CCPP-layout inputs -> tau, pfrac
+ * (ngpt, nLay, nCol) FP32; oob (nLay, nCol) and colflag (nCol) mark rows
+ * outside the training hull. */
+int ccpprad_mlp_infer_dev(ccpprad_mlp *m, int nCol, int nLay, const double *p_lay, const double *p_lev,
+                          const double *t_lay, const double *vmr_h2o, const double *vmr_o3,
+                          const double *vmr_co2, const double *vmr_ch4, const double *vmr_n2o, float *tau,
+                          float *pfrac, unsigned char *oob, unsigned char *colflag, void *stream,
+                          char *errmsg, int errlen);
+/* Select the LW gas optics of a context: the MLP (m != NULL) with table
+ * fallback outside the hull, or the k-distribution table (m == NULL). */
+int ccpprad_lw_set_gas_optics_mlp(ccpprad_ctx *ctx, ccpprad_mlp *m);
+
 /* ---- observability (no counterpart in the reference, which has no timers) */
 
 /* Per-kernel device timing with HIP events recorded on the launch stream.

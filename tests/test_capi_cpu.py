@@ -28,3 +28,47 @@ def test_library_exports_every_declared_symbol():
 def test_version_string_without_gpu():
     from ccpprad import lib
     assert b"gfx950" in lib.load().ccpprad_version()
+
+
+def test_fortran_modules_built_with_reference_names():
+    """The Fortran layer exports the reference's scheme entry points."""
+    from ccpprad import build
+    if not os.path.exists(build.F_LIB):
+        pytest.skip("flang not available")
+    out = subprocess.run(["nm", "-D", "--defined-only", build.F_LIB], capture_output=True, text=True).stdout
+    for sym in ("rrtmgp_lw_main_init", "rrtmgp_lw_main_run", "rrtmgp_sw_main_init", "rrtmgp_sw_main_run"):
+        assert sym in out, sym
+
+
+def test_table_file_loader_rejects_malformed_files(tmp_path):
+    """ccpprad_lw_init_files parses on the host: a bad file is an error (no GPU needed)."""
+    import ctypes as C
+    from ccpprad import lib, capi
+    L = lib.load()
+    bad = tmp_path / "bad.tab"
+    bad.write_bytes(b"NOTATABLE" + b"\0" * 64)
+    ctx = C.c_void_p()
+    buf, n = capi.errbuf()
+    rc = L.ccpprad_lw_init_files(str(bad).encode(), b"", None, 0, 127, 1, C.byref(ctx), buf, n)
+    assert rc != 0 and b"malformed" in buf.value
+    rc = L.ccpprad_lw_init_files(str(tmp_path / "missing.tab").encode(), b"", None, 0, 127, 1,
+                                 C.byref(ctx), buf, n)
+    assert rc != 0 and b"cannot open" in buf.value
+
+
+def test_table_file_round_trip_parses(tmp_path):
+    """A written table parses fully: with no GPU the library reaches the
+    device upload and reports that (not a parse error)."""
+    import ctypes as C
+    import torch
+    from ccpprad import lib, capi, tables, synthetic as syn
+    if torch.cuda.is_available():
+        pytest.skip("covered by the GPU tests")
+    L = lib.load()
+    tables.write_kdist(tmp_path / "k.tab", syn.make_kdist(True))
+    tables.write_lut(tmp_path / "c.tab", syn.make_cloud_lut(True))
+    ctx = C.c_void_p()
+    buf, n = capi.errbuf()
+    rc = L.ccpprad_lw_init_files(str(tmp_path / "k.tab").encode(), str(tmp_path / "c.tab").encode(), None, 0,
+                                 127, 1, C.byref(ctx), buf, n)
+    assert rc != 0 and b"malformed" not in buf.value and b"lacks" not in buf.value
