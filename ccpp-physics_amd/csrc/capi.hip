@@ -171,10 +171,11 @@ template <typename T>
 static T *upload(ccpprad_ctx *c, const T *h, size_t n, hipError_t *e)
 {
     T *d = nullptr;
+    if (*e != hipSuccess) return nullptr;   // keep the first error
     *e = hipMalloc(&d, sizeof(T) * (n ? n : 1));
     if (*e != hipSuccess) return nullptr;
     c->allocs.push_back(d);
-    if (n) *e = hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice);
+    if (n && h) *e = hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice);
     return d;
 }
 
