@@ -44,7 +44,7 @@ def _bind(L):
     L.ccpprad_mlp_grads_dev.argtypes = [vp, vp, vp, ci, vp, vp, cp, ci]
     L.ccpprad_mlp_train_step_dev.argtypes = [vp, vp, vp, ci, C.c_float, vp, vp, cp, ci]
     L.ccpprad_mlp_finalize_training.argtypes = [vp, cp, ci]
-    L.ccpprad_mlp_infer_dev.argtypes = [vp, ci, ci] + [vp] * 12 + [cp, ci]
+    L.ccpprad_mlp_infer_dev.argtypes = [vp, ci, ci] + [vp] * 13 + [cp, ci]
     L.ccpprad_lw_set_gas_optics_mlp.argtypes = [vp, vp]
     L._mlp_bound = True
     return L

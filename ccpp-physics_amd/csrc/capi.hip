@@ -84,7 +84,7 @@ struct ccpprad_ctx {
     int chunk = 65536;
     // learned gas optics (LW): MLP with table fallback, or null = table
     ccpprad_mlp *mlp = nullptr;
-    DevBuf w_oob, w_colm;
+    DevBuf w_oob, w_colm, w_sR, w_sT, w_sD;
     // optional per-kernel event timing
     int timing = 0;
     std::vector<hipEvent_t> ev_pool;
@@ -343,6 +343,13 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
         auto d1 = to_f32(lut->extice + io, ni); L.extice = upload(c, d1.data(), ni, &e);
         auto d2 = to_f32(lut->ssaice + io, ni); L.ssaice = upload(c, d2.data(), ni, &e);
         auto d3 = to_f32(lut->asyice + io, ni); L.asyice = upload(c, d3.data(), ni, &e);
+        // single-scattering co-albedo in FP64 before rounding: SW clouds have
+        // ssa within 1e-5 of 1, where 1 - (float)ssa keeps only ~2 digits
+        std::vector<float> cl(nl), ci(ni);
+        for (size_t i = 0; i < nl; ++i) cl[i] = (float)(1.0 - lut->ssaliq[i]);
+        for (size_t i = 0; i < ni; ++i) ci[i] = (float)(1.0 - lut->ssaice[io + i]);
+        L.coaliq = upload(c, cl.data(), nl, &e);
+        L.coaice = upload(c, ci.data(), ni, &e);
         c->have_lut = 1;
     }
     c->err = upload<int>(c, nullptr, 2, &e);
@@ -374,7 +381,7 @@ extern "C" void ccpprad_finalize(ccpprad_ctx *c)
     for (void *p : c->allocs) (void)hipFree(p);
     DevBuf *bufs[] = {&c->w_tau, &c->w_aux, &c->w_mask, &c->w_ctau, &c->w_cssa, &c->w_cg, &c->w_sF,
                       &c->w_sA, &c->w_sQ, &c->w_sb, &c->w_blk, &c->w_colf, &c->w_day, &c->w_idx,
-                      &c->w_oob, &c->w_colm};
+                      &c->w_oob, &c->w_colm, &c->w_sR, &c->w_sT, &c->w_sD};
     for (DevBuf *b : bufs) b->release();
     for (auto &b : c->stage) b.release();
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -592,6 +599,9 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
     HIPCHK(c->w_sF.ensure(sizeof(float) * (size_t)CH * nV * G));
     HIPCHK(c->w_sA.ensure(sizeof(float) * (size_t)CH * nV * G));
     HIPCHK(c->w_sQ.ensure(sizeof(float) * (size_t)CH * nV * G));
+    HIPCHK(c->w_sR.ensure(sizeof(float) * (size_t)CH * nL * G));
+    HIPCHK(c->w_sT.ensure(sizeof(float) * (size_t)CH * nL * G));
+    HIPCHK(c->w_sD.ensure(sizeof(float) * (size_t)CH * nL * G));
     HIPCHK(c->w_sb.ensure(sizeof(double) * (size_t)nD * 3 * B));
     HIPCHK(c->w_blk.ensure((size_t)nblk + 1));
     HIPCHK(c->w_colf.ensure((size_t)nD + 1));
@@ -647,6 +657,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
         s.coszen = a->coszen; s.alb_nir_dir = a->sfc_alb_nir_dir; s.alb_nir_dif = a->sfc_alb_nir_dif;
         s.alb_uv_dir = a->sfc_alb_uvvis_dir; s.alb_uv_dif = a->sfc_alb_uvvis_dif; s.p_lev = a->p_lev;
         s.alb_cat = c->alb_cat; s.sF = (float *)c->w_sF.p; s.sA = (float *)c->w_sA.p; s.sQ = (float *)c->w_sQ.p;
+        s.sR = (float *)c->w_sR.p; s.sT = (float *)c->w_sT.p; s.sD = (float *)c->w_sD.p;
         s.up_all = a->fluxswUP_allsky; s.dn_all = a->fluxswDOWN_allsky; s.up_clr = a->fluxswUP_clrsky;
         s.dn_clr = a->fluxswDOWN_clrsky; s.sb = (double *)c->w_sb.p + (size_t)c0 * 3 * B; s.htr = a->htrsw;
         s.htr_clr = a->doSWclrsky ? a->htrswc : nullptr;

@@ -65,6 +65,7 @@ struct LutView {
     float liq_lwr, liq_step, ice_lwr, ice_step, liq_upr, ice_upr;
     const float *extliq, *ssaliq, *asyliq;   // [nband][nsize_liq]
     const float *extice, *ssaice, *asyice;   // [nband][nsize_ice] (roughness selected)
+    const float *coaliq, *coaice;            // co-albedo 1 - ssa, formed in FP64 at init
 };
 
 __device__ __forceinline__ int d_min(int a, int b) { return a < b ? a : b; }

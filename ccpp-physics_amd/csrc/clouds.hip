@@ -22,52 +22,62 @@
 namespace ccpprad {
 
 
-__device__ __forceinline__ void lut_eval(const float *ext, const float *ssa, const float *asy, int nsz,
-                                         float lwr, float step, float path, float re, float &t, float &ts,
-                                         float &tsg)
+__device__ __forceinline__ void lut_eval(const float *ext, const float *ssa, const float *coa, const float *asy,
+                                         int nsz, float lwr, float step, float path, float re, float &t, float &ts,
+                                         float &ta, float &tsg)
 {
     int idx = min((int)floorf((re - lwr) / step) + 1, nsz - 1);
     float fint = (re - lwr) / step - (float)(idx - 1);
     float e0 = ext[idx - 1], e1 = ext[idx];
     float s0 = ssa[idx - 1], s1 = ssa[idx];
+    float c0 = coa[idx - 1], c1 = coa[idx];
     float a0 = asy[idx - 1], a1 = asy[idx];
     t = path * (e0 + fint * (e1 - e0));
     ts = t * (s0 + fint * (s1 - s0));
+    ta = t * (c0 + fint * (c1 - c0));   // = t - ts, without the cancellation
     tsg = ts * (a0 + fint * (a1 - a0));
 }
 
-// increment_2stream_by_2stream: (t2,s2,g2) += (t1,s1,g1), eps = 3*tiny(FP32)
-__device__ __forceinline__ void inc2s(float t1, float s1, float g1, float &t2, float &s2, float &g2)
+// Two-stream optics carried with the co-albedo: (tau, ssa, 1 - ssa, g).
+struct Opt2 {
+    float t, w, p, g;
+};
+
+// increment_2stream_by_2stream: b += a, eps = 3*tiny(FP32).  The co-albedo
+// combines as the absorption-weighted mean.
+__device__ __forceinline__ void inc2s(const Opt2 &a, Opt2 &b)
 {
     const float eps = 3.0f * 1.17549435e-38f;
-    const float t12 = t1 + t2;
-    const float ts12 = t1 * s1 + t2 * s2;
-    g2 = (t1 * s1 * g1 + t2 * s2 * g2) / fmaxf(eps, ts12);
-    s2 = ts12 / fmaxf(eps, t12);
-    t2 = t12;
+    const float t12 = a.t + b.t;
+    const float ts12 = a.t * a.w + b.t * b.w;
+    const float ta12 = a.t * a.p + b.t * b.p;
+    b.g = (a.t * a.w * a.g + b.t * b.w * b.g) / fmaxf(eps, ts12);
+    b.w = ts12 / fmaxf(eps, t12);
+    b.p = t12 > eps ? ta12 / t12 : 1.0f;
+    b.t = t12;
 }
 
-__device__ __forceinline__ bool lut_cloud(const LutView &L, int b, float lwp, float iwp, float rl, float ri,
-                                          float &t, float &s, float &g)
+__device__ __forceinline__ bool lut_cloud(const LutView &L, int b, float lwp, float iwp, float rl, float ri, Opt2 &o)
 {
-    float lt = 0, lts = 0, ltsg = 0, it = 0, its = 0, itsg = 0;
+    float lt = 0, lts = 0, lta = 0, ltsg = 0, it = 0, its = 0, ita = 0, itsg = 0;
     bool bad = false;
     if (lwp > 0.0f) {
         bad |= !(rl >= L.liq_lwr && rl <= L.liq_upr);
         const size_t o = (size_t)b * L.nsize_liq;
-        lut_eval(L.extliq + o, L.ssaliq + o, L.asyliq + o, L.nsize_liq, L.liq_lwr, L.liq_step, lwp,
-                 fminf(fmaxf(rl, L.liq_lwr), L.liq_upr), lt, lts, ltsg);
+        lut_eval(L.extliq + o, L.ssaliq + o, L.coaliq + o, L.asyliq + o, L.nsize_liq, L.liq_lwr, L.liq_step, lwp,
+                 fminf(fmaxf(rl, L.liq_lwr), L.liq_upr), lt, lts, lta, ltsg);
     }
     if (iwp > 0.0f) {
         bad |= !(ri >= L.ice_lwr && ri <= L.ice_upr);
         const size_t o = (size_t)b * L.nsize_ice;
-        lut_eval(L.extice + o, L.ssaice + o, L.asyice + o, L.nsize_ice, L.ice_lwr, L.ice_step, iwp,
-                 fminf(fmaxf(ri, L.ice_lwr), L.ice_upr), it, its, itsg);
+        lut_eval(L.extice + o, L.ssaice + o, L.coaice + o, L.asyice + o, L.nsize_ice, L.ice_lwr, L.ice_step, iwp,
+                 fminf(fmaxf(ri, L.ice_lwr), L.ice_upr), it, its, ita, itsg);
     }
-    t = lt + it;
-    const float ts = lts + its, tsg = ltsg + itsg;
-    s = ts / fmaxf(1.1920929e-7f, t);
-    g = tsg / fmaxf(1.1920929e-7f, ts);
+    o.t = lt + it;
+    const float ts = lts + its, ta = lta + ita, tsg = ltsg + itsg;
+    o.w = ts / fmaxf(1.1920929e-7f, o.t);
+    o.p = o.t > 0.0f ? ta / fmaxf(1.1920929e-7f, o.t) : 1.0f;
+    o.g = tsg / fmaxf(1.1920929e-7f, ts);
     return bad;
 }
 
@@ -117,19 +127,19 @@ __global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs 
     const float c0r[14] = {0.980f, 0.975f, 0.965f, 0.960f, 0.955f, 0.952f, 0.950f, 0.944f, 0.894f, 0.884f, 0.883f, 0.883f, 0.883f, 0.883f};
     const float c0s[14] = {0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.700f, 0.700f, 0.700f, 0.700f};
     for (int b = 0; b < nb; ++b) {
-        float t, s, g;
-        bad |= lut_cloud(L, b, lwp, iwp, rl, ri, t, s, g);
-        if (!A.is_lw && A.cldtau && b == 10) A.cldtau[q] = (double)t;
+        Opt2 o;
+        bad |= lut_cloud(L, b, lwp, iwp, rl, ri, o);
+        if (!A.is_lw && A.cldtau && b == 10) A.cldtau[q] = (double)o.t;
         for (int k = 0; k < 2; ++k) {
             const double *const *src = k == 0 ? A.cnv : A.pbl;
             if (!src[0]) continue;
-            float t2, s2, g2;
-            bad |= lut_cloud(L, b, (float)src[0][q], (float)src[1][q], (float)src[2][q], (float)src[3][q], t2, s2, g2);
-            inc2s(t2, s2, g2, t, s, g);
+            Opt2 o2;
+            bad |= lut_cloud(L, b, (float)src[0][q], (float)src[1][q], (float)src[2][q], (float)src[3][q], o2);
+            inc2s(o2, o);
         }
-        float pt = 0.0f, ps = 0.0f, pg = 0.0f;
+        Opt2 pr = {0.0f, 0.0f, 1.0f, 0.0f};
         if (A.is_lw) {
-            pt = prec ? tr + tsn : 0.0f;
+            pr.t = prec ? tr + tsn : 0.0f;
         } else if (prec) {
             const int bb = b < 14 ? b : 13;
             const float sr = tr * (1.0f - b0r[bb]);
@@ -141,16 +151,18 @@ __global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs 
             const float ap = fmaxf(1.0e-12f, ar + an);
             const float asyw = ap / fmaxf(1.0e-12f, sp);
             const float ssaw = fminf(1.0f - 0.000001f, sp / tp);
+            const float omw = fmaxf(1.0f - (1.0f - 0.000001f), (tp - sp) / tp);   // 1 - ssaw
             const float za1 = asyw * asyw, za2 = ssaw * za1;
-            pt = (1.0f - za2) * tp;
-            ps = (ssaw - za2) / (1.0f - za2);
-            pg = asyw / (1.0f + asyw);
+            pr.t = (1.0f - za2) * tp;
+            pr.w = (ssaw - za2) / (1.0f - za2);
+            pr.p = omw / (1.0f - za2);
+            pr.g = asyw / (1.0f + asyw);
         }
-        inc2s(pt, ps, pg, t, s, g);
+        inc2s(pr, o);
         if (A.is_lw) {
-            ot[b] = t * (1.0f - s);  // 1scl += 2str: absorption depth
+            ot[b] = o.t * o.p;  // 1scl += 2str: absorption depth tau * (1 - ssa)
         } else {
-            ot[b] = t; os[b] = s; og[b] = g;
+            ot[b] = o.t; os[b] = o.p; og[b] = o.g;   // SW: tau, co-albedo, g
         }
     }
     if (bad) atomicOr(A.err, 2);

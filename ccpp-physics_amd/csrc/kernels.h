@@ -40,7 +40,7 @@ struct CloudArgs {
     const double *cld_frac, *lwp, *iwp, *reliq, *reice;
     const double *cnv[4], *pbl[4];  // lwp, iwp, reliq, reice (null = off)
     const double *rwp, *swp, *resnow;
-    float *ctau, *cssa, *cg;  // [d][l][b]
+    float *ctau, *cssa, *cg;  // [d][l][b]: LW absorption depth; SW tau, co-albedo (1 - ssa), g
     double *cldtau;           // SW: (ncol_ld, nlay) band-11 cloud tau, or null
     int *err;
 };
@@ -65,12 +65,13 @@ struct SwSolveArgs {
     const int *col_idx;
     const float *tau_abs, *tau_ray;        // [d][l][g]
     const unsigned *mask;                  // [d][l][MASK_WORDS] or null
-    const float *ctau, *cssa, *cg;         // [d][l][nband] or null
+    const float *ctau, *cssa, *cg;         // [d][l][nband] tau, co-albedo (1 - ssa), g; or null
     const unsigned char *blkflag;
     const double *aer_tau, *aer_ssa, *aer_g;  // (ncol_ld, nlay, nband) or null
     const double *coszen, *alb_nir_dir, *alb_nir_dif, *alb_uv_dir, *alb_uv_dif, *p_lev;
     const int *alb_cat;                    // [nband] 0 NIR, 1 average, 2 UV-VIS, 3 unset
-    float *sF, *sA, *sQ;                   // [d][lev][g] scratch
+    float *sF, *sA, *sQ;                   // [d][lev][g] scratch: direct beam, adding albedo/source
+    float *sR, *sT, *sD;                   // [d][lay][g] scratch: R_dif, T_dif, direct-beam down source
     double *up_all, *dn_all, *up_clr, *dn_clr;  // (ncol_ld, nlev)
     double *sb;                            // [d][3][nband]: dn, dn_dir (all sky), dn (clear sky)
     double *htr, *htr_clr;                 // (ncol_ld, nlay) or null

@@ -12,62 +12,73 @@
 // per lane:
 //   1 top-down : direct beam F_dir by products of T_noscat (as the reference),
 //                stored per level (FP32 scratch)
-//   2 bottom-up: two-stream layer optics, direct-beam sources, adding
-//                albedo A and source Q per level (FP32 scratch)
-//   3 top-down : layer optics again, diffuse down/up fluxes; F_dn += F_dir
-// The layer optics are recomputed from the 8 B/cell gas optics instead of
-// being stored.  FP32 cancellation in thin layers is avoided with expm1 for
-// 1 - exp(-2k tau) and 1 - T_noscat.
+//   2 bottom-up: layer two-stream coefficients (FP64), direct-beam sources,
+//                adding albedo A and source Q per level; R_dif, T_dif and the
+//                downward direct source are kept per layer (FP32 scratch)
+//   3 top-down : diffuse down/up fluxes from the stored terms; F_dn += F_dir
+// Optical properties carry the co-albedo (1 - ssa) next to ssa, so the
+// absorption of nearly conservative layers is not lost to FP32 rounding.
 #include "kernels.h"
 
 namespace ccpprad {
 
 
-__device__ __forceinline__ void inc2s_f(float t1, float s1, float g1, float &t2, float &s2, float &g2)
+// Two-stream optics carried with the co-albedo p = 1 - w (p = tau_abs / tau).
+struct Opt2s {
+    float t, w, p, g;
+};
+
+// increment_2stream_by_2stream: b += a (eps = 3*tiny(FP32)).
+__device__ __forceinline__ void inc2s_p(float t1, float w1, float p1, float g1, Opt2s &o)
 {
     const float eps = 3.0f * 1.17549435e-38f;
-    const float t12 = t1 + t2;
-    const float ts12 = t1 * s1 + t2 * s2;
-    g2 = (t1 * s1 * g1 + t2 * s2 * g2) / fmaxf(eps, ts12);
-    s2 = ts12 / fmaxf(eps, t12);
-    t2 = t12;
+    const float t12 = t1 + o.t;
+    const float ts12 = t1 * w1 + o.t * o.w;
+    const float ta12 = t1 * p1 + o.t * o.p;
+    o.g = (t1 * w1 * g1 + o.t * o.w * o.g) / fmaxf(eps, ts12);
+    o.w = ts12 / fmaxf(eps, t12);
+    o.p = t12 > eps ? ta12 / t12 : 1.0f;
+    o.t = t12;
 }
 
 struct TwoStream {
-    float Rd, Td, Rdir, Tdir, T0;
+    float Rd, Td, Rdir, Tdir;
 };
 
-__device__ __forceinline__ TwoStream two_stream(float mu0, float mu0_inv, float tau, float w, float g)
+// sw_two_stream for one layer, evaluated in FP64: the direct-beam
+// reflectance/transmittance brackets cancel to O(k tau) while their prefactor
+// grows like 1/k, which FP32 cannot carry to the 0.05 W/m2 bar.  gamma1 -
+// gamma2 = 2 (1 - w) is formed from the co-albedo.  Outputs are rounded to
+// FP32 for the adding recurrences.
+__device__ __forceinline__ TwoStream two_stream64(double mu0, const Opt2s &o)
 {
-    TwoStream o;
-    const float g1 = (8.0f - w * (5.0f + 3.0f * g)) * 0.25f;
-    const float g2 = 3.0f * (w * (1.0f - g)) * 0.25f;
-    const float g3 = (2.0f - 3.0f * mu0 * g) * 0.25f;
-    const float g4 = 1.0f - g3;
-    const float a1 = g1 * g4 + g2 * g3;
-    const float a2 = g1 * g3 + g2 * g4;
-    const float k = sqrtf(fmaxf((g1 - g2) * (g1 + g2), 1.0e-12f));
-    const float e = expf(-tau * k);
-    const float e2 = e * e;
-    const float ome2 = -expm1f(-2.0f * tau * k);  // 1 - e2 without cancellation
-    const float RT = 1.0f / (k * (1.0f + e2) + g1 * ome2);
-    o.Rd = RT * g2 * ome2;
-    o.Td = RT * 2.0f * k * e;
-    const float x0 = tau * mu0_inv;
-    o.T0 = expf(-x0);
-    const float omT0 = -expm1f(-x0);
-    const float kmu = k * mu0, kg3 = k * g3, kg4 = k * g4;
-    const float dd = 1.0f - kmu * kmu;
-    const float RT2 = w * RT / (fabsf(dd) >= 1.1920929e-7f ? dd : 1.1920929e-7f);
-    float Rdir = RT2 * ((1.0f - kmu) * (a2 + kg3) - (1.0f + kmu) * (a2 - kg3) * e2 -
-                        2.0f * (kg3 - a2 * kmu) * e * o.T0);
-    float Tdir = -RT2 * ((1.0f + kmu) * (a1 + kg4) * o.T0 - (1.0f - kmu) * (a1 - kg4) * e2 * o.T0 -
-                         2.0f * (kg4 + a1 * kmu) * e);
-    Rdir = fmaxf(0.0f, fminf(Rdir, omT0));
-    Tdir = fmaxf(0.0f, fminf(Tdir, omT0 - Rdir));
-    o.Rdir = Rdir;
-    o.Tdir = Tdir;
-    return o;
+    const double tau = o.t, w = o.w, p = o.p, g = o.g;
+    const double g1 = (8.0 - w * (5.0 + 3.0 * g)) * 0.25;
+    const double g2 = 3.0 * (w * (1.0 - g)) * 0.25;
+    const double g3 = (2.0 - 3.0 * mu0 * g) * 0.25;
+    const double g4 = 1.0 - g3;
+    const double a1 = g1 * g4 + g2 * g3;
+    const double a2 = g1 * g3 + g2 * g4;
+    const double k = sqrt(fmax(2.0 * p * (g1 + g2), 1.0e-12));
+    const double e = exp(-tau * k);
+    const double e2 = e * e;
+    const double ome2 = -expm1(-2.0 * tau * k);
+    const double RT = 1.0 / (k * (1.0 + e2) + g1 * ome2);
+    const double x0 = tau / mu0;
+    const double T0 = exp(-x0), omT0 = -expm1(-x0);
+    const double kmu = k * mu0, kg3 = k * g3, kg4 = k * g4;
+    const double dd = 1.0 - kmu * kmu;
+    const double RT2 = w * RT / (fabs(dd) >= 2.220446049250313e-16 ? dd : 2.220446049250313e-16);
+    double Rdir = RT2 * ((1.0 - kmu) * (a2 + kg3) - (1.0 + kmu) * (a2 - kg3) * e2 - 2.0 * (kg3 - a2 * kmu) * e * T0);
+    double Tdir = -RT2 * ((1.0 + kmu) * (a1 + kg4) * T0 - (1.0 - kmu) * (a1 - kg4) * e2 * T0 - 2.0 * (kg4 + a1 * kmu) * e);
+    Rdir = fmax(0.0, fmin(Rdir, omT0));
+    Tdir = fmax(0.0, fmin(Tdir, omT0 - Rdir));
+    TwoStream r;
+    r.Rd = (float)(RT * g2 * ome2);
+    r.Td = (float)(RT * 2.0 * k * e);
+    r.Rdir = (float)Rdir;
+    r.Tdir = (float)Tdir;
+    return r;
 }
 
 template <bool CLOUDS>
@@ -88,7 +99,8 @@ __global__ __launch_bounds__(128) void sw_solve_kernel(TablesView T, SwSolveArgs
     double *rup = red + (size_t)wave * 2 * nlev, *rdn = rup + nlev;
     double *rb = red + 4 * (size_t)nlev;  // [wave][3][nband]
 
-    const float mu0 = (float)A.coszen[c];
+    const double mu0d = A.coszen[c];
+    const float mu0 = (float)mu0d;
     const float mu0_inv = 1.0f / mu0;
     const int cat = A.alb_cat[b];
     float adir = 0.0f, adif = 0.0f;
@@ -99,47 +111,50 @@ __global__ __launch_bounds__(128) void sw_solve_kernel(TablesView T, SwSolveArgs
     } else if (cat == 2) { adir = (float)A.alb_uv_dir[c]; adif = (float)A.alb_uv_dif[c]; }
 
     const size_t cl = (size_t)d * nlay;
-    auto optics = [&](int l, float &t, float &w, float &gq) {
+    // combine_and_reorder_2str + aerosol and McICA cloud increments
+    auto optics = [&](int l) -> Opt2s {
         const float ta = A.tau_abs[(cl + l) * ngpt + gg], tr = A.tau_ray[(cl + l) * ngpt + gg];
-        t = ta + tr;
-        if (t > 2.0f * 1.17549435e-38f) w = tr / t; else w = 0.0f;
-        gq = 0.0f;
+        Opt2s o;
+        o.t = ta + tr;
+        if (o.t > 2.0f * 1.17549435e-38f) { o.w = tr / o.t; o.p = ta / o.t; } else { o.w = 0.0f; o.p = 1.0f; }
+        o.g = 0.0f;
         if (A.aer_tau) {
             const size_t q = (size_t)c + (size_t)ld * (l + (size_t)nlay * b);
-            inc2s_f((float)A.aer_tau[q], (float)A.aer_ssa[q], (float)A.aer_g[q], t, w, gq);
+            const float aw = (float)A.aer_ssa[q];
+            inc2s_p((float)A.aer_tau[q], aw, (float)(1.0 - A.aer_ssa[q]), (float)A.aer_g[q], o);
         }
         if (cloudy) {
             const unsigned mw = A.mask[(cl + l) * MASK_WORDS + (gg >> 5)];
             const bool m = (mw >> (gg & 31)) & 1u;
             const size_t q = (cl + l) * nb + b;
-            inc2s_f(m ? A.ctau[q] : 0.0f, m ? A.cssa[q] : 0.0f, m ? A.cg[q] : 0.0f, t, w, gq);
+            const float cp = m ? A.cssa[q] : 1.0f;
+            inc2s_p(m ? A.ctau[q] : 0.0f, 1.0f - cp, cp, m ? A.cg[q] : 0.0f, o);
         }
+        return o;
     };
-    float *sF = A.sF + (size_t)d * nlev * ngpt, *sA = A.sA + (size_t)d * nlev * ngpt;
-    float *sQ = A.sQ + (size_t)d * nlev * ngpt;
+    const size_t sl = (size_t)d * nlev * ngpt, sy = (size_t)d * nlay * ngpt;
+    float *sF = A.sF + sl, *sA = A.sA + sl, *sQ = A.sQ + sl;
+    float *sR = A.sR + sy, *sT = A.sT + sy, *sD = A.sD + sy;
     const float toa = T.toa_norm[gg];
 
-    // ---- sweep 1: direct beam, top-down
+    // ---- sweep 1: direct beam, top-down (products of T_noscat)
     float fdir = toa * mu0;
     sF[(size_t)lev_toa * ngpt + gg] = fdir;
     for (int k = 0; k < nlay; ++k) {
         const int l = top ? k : nlay - 1 - k;
         const int lev_below = top ? l + 1 : l;
-        float t, w, gq;
-        optics(l, t, w, gq);
-        fdir = expf(-t * mu0_inv) * fdir;
+        const Opt2s o = optics(l);
+        fdir = expf(-o.t * mu0_inv) * fdir;
         sF[(size_t)lev_below * ngpt + gg] = fdir;
     }
-    // ---- sweep 2: adding albedo / source, bottom-up
+    // ---- sweep 2: layer two-stream (FP64) and adding albedo / source, bottom-up
     float Ab = adif, Qb = fdir * adir;
     sA[(size_t)lev_sfc * ngpt + gg] = Ab;
     sQ[(size_t)lev_sfc * ngpt + gg] = Qb;
     for (int k = 0; k < nlay; ++k) {
         const int l = top ? nlay - 1 - k : k;
         const int lev_above = top ? l : l + 1;
-        float t, w, gq;
-        optics(l, t, w, gq);
-        const TwoStream o = two_stream(mu0, mu0_inv, t, w, gq);
+        const TwoStream o = two_stream64(mu0d, optics(l));
         const float fa = sF[(size_t)lev_above * ngpt + gg];
         const float su = o.Rdir * fa, sd = o.Tdir * fa;
         const float den = 1.0f / (1.0f - o.Rd * Ab);
@@ -147,36 +162,35 @@ __global__ __launch_bounds__(128) void sw_solve_kernel(TablesView T, SwSolveArgs
         const float Qa = su + o.Td * den * (Qb + Ab * sd);
         sA[(size_t)lev_above * ngpt + gg] = Aa;
         sQ[(size_t)lev_above * ngpt + gg] = Qa;
+        sR[(size_t)l * ngpt + gg] = o.Rd;
+        sT[(size_t)l * ngpt + gg] = o.Td;
+        sD[(size_t)l * ngpt + gg] = sd;
         Ab = Aa; Qb = Qa;
     }
     // ---- sweep 3: fluxes, top-down
     float dnd = 0.0f;                    // diffuse down at TOA
-    fdir = toa * mu0;
     {
-        const float up = dnd * Ab + Qb;  // Ab, Qb now hold TOA values
+        const float up = dnd * Ab + Qb;  // Ab, Qb hold the TOA values
         const double su = wave_sum(act ? (double)up : 0.0);
-        const double sdn = wave_sum(act ? (double)(dnd + fdir) : 0.0);
+        const double sdn = wave_sum(act ? (double)(dnd + toa * mu0) : 0.0);
         if (lane == 0) { rup[lev_toa] = su; rdn[lev_toa] = sdn; }
     }
     for (int k = 0; k < nlay; ++k) {
         const int l = top ? k : nlay - 1 - k;
         const int lev_below = top ? l + 1 : l;
-        float t, w, gq;
-        optics(l, t, w, gq);
-        const TwoStream o = two_stream(mu0, mu0_inv, t, w, gq);
-        const float sd = o.Tdir * fdir;
+        const float Rd = sR[(size_t)l * ngpt + gg], Td = sT[(size_t)l * ngpt + gg], sd = sD[(size_t)l * ngpt + gg];
         const float Abl = sA[(size_t)lev_below * ngpt + gg], Qbl = sQ[(size_t)lev_below * ngpt + gg];
-        const float den = 1.0f / (1.0f - o.Rd * Abl);
-        dnd = (o.Td * dnd + o.Rd * Qbl + sd) * den;
+        const float den = 1.0f / (1.0f - Rd * Abl);
+        dnd = (Td * dnd + Rd * Qbl + sd) * den;
         const float up = dnd * Abl + Qbl;
-        fdir = o.T0 * fdir;
-        const float dn = dnd + fdir;
+        const float fd = sF[(size_t)lev_below * ngpt + gg];
+        const float dn = dnd + fd;
         const double su = wave_sum(act ? (double)up : 0.0);
         const double sdn = wave_sum(act ? (double)dn : 0.0);
         if (lane == 0) { rup[lev_below] = su; rdn[lev_below] = sdn; }
         if (lev_below == A.isfc) {
-            // per-band surface sums (8-lane groups share a band) for scmpsw
-            double vdn = act ? (double)dn : 0.0, vdir = act ? (double)fdir : 0.0;
+            // per-band surface sums for scmpsw
+            const double vdn = act ? (double)dn : 0.0, vdir = act ? (double)fd : 0.0;
             for (int bb = 0; bb < nb; ++bb) {
                 const bool mine = act && b == bb;
                 const double s1 = wave_sum(mine ? vdn : 0.0), s2 = wave_sum(mine ? vdir : 0.0);
