@@ -639,6 +639,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
             ca.ctau = ctau; ca.cssa = cssa; ca.cg = cg; ca.err = c->err;
             // only the last block's band-11 cloud depth survives the per-block reset
             ca.cldtau = (c0 + nc > last_blk_start) ? a->cldtausw : nullptr;
+            ca.cldtau_blk = nblk - 1;
             { KScope k_(c, K_CLOUD, st); launch_cloud_optics(c->L, ca, nc, st); }
             McicaArgs ma{};
             ma.nlay = nL; ma.ngpt = G; ma.ncol_ld = nC; ma.col0 = c0; ma.blksz = bs; ma.isubc = a->isubc_sw;

@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs 
     for (int b = 0; b < nb; ++b) {
         Opt2 o;
         bad |= lut_cloud(L, b, lwp, iwp, rl, ri, o);
-        if (!A.is_lw && A.cldtau && b == 10) A.cldtau[q] = (double)o.t;
+        if (!A.is_lw && A.cldtau && b == 10 && blk == A.cldtau_blk) A.cldtau[q] = (double)o.t;
         for (int k = 0; k < 2; ++k) {
             const double *const *src = k == 0 ? A.cnv : A.pbl;
             if (!src[0]) continue;

@@ -42,6 +42,7 @@ struct CloudArgs {
     const double *rwp, *swp, *resnow;
     float *ctau, *cssa, *cg;  // [d][l][b]: LW absorption depth; SW tau, co-albedo (1 - ssa), g
     double *cldtau;           // SW: (ncol_ld, nlay) band-11 cloud tau, or null
+    int cldtau_blk;           // the one scheme block whose columns write cldtau
     int *err;
 };
 void launch_cloud_optics(const LutView &L, const CloudArgs &A, int ncols, hipStream_t st);
