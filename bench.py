@@ -40,8 +40,11 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--ncol", type=int, default=NCOL)
-    p.add_argument("--cpu-sample", type=int, default=2048)
+    p.add_argument("--cpu-sample", type=int, default=32768)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-probe", action="store_true",
+                   help="after the bench, run one 1 GiB device copy: the reference dispatch that scales "
+                        "FETCH_SIZE / WRITE_SIZE counter readings to bytes")
     return p.parse_args()
 
 
@@ -197,6 +200,13 @@ def main():
         "roofline": roof, "cpu_baseline": cpu, "accuracy_vs_fp64_oracle": acc,
     }
     print(json.dumps(line), flush=True)
+    if a.copy_probe:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("# copy probe: 1073741824 bytes read, 1073741824 bytes written", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":
