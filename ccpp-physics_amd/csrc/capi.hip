@@ -77,14 +77,14 @@ struct ccpprad_ctx {
     int *last_wipe = nullptr;
     int gas_slot[MAX_GAS];              // k-dist gas index (0-based) of each CCPP vmr slot
     // work buffers
-    DevBuf w_tau, w_aux, w_mask, w_ctau, w_cssa, w_cg, w_sF, w_sA, w_sQ, w_sb, w_blk, w_colf, w_day, w_idx;
+    DevBuf w_tau, w_aux, w_mask, w_ctau, w_cssa, w_cg, w_sb, w_blk, w_colf, w_day, w_idx;
     // host-path staging
     std::vector<DevBuf> stage;
     hipStream_t stream = nullptr;
     int chunk = 65536;
     // learned gas optics (LW): MLP with table fallback, or null = table
     ccpprad_mlp *mlp = nullptr;
-    DevBuf w_oob, w_colm, w_sR, w_sT, w_sD;
+    DevBuf w_oob, w_colm;
     // optional per-kernel event timing
     int timing = 0;
     std::vector<hipEvent_t> ev_pool;
@@ -201,6 +201,9 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
     if (kd->minor_lower.n > MAX_MINOR || kd->minor_upper.n > MAX_MINOR) { seterr(errmsg, errlen, "too many minor intervals"); return 1; }
     if (nLay < 2 || nLay > MAX_LAY) { seterr(errmsg, errlen, "nLay=%d outside 2..%d", nLay, MAX_LAY); return 1; }
     if (blksz < 1) { seterr(errmsg, errlen, "rrtmgp_phys_blksz must be positive"); return 1; }
+    if ((want_lw ? lw_solve_smem(nLay, kd->nband) : sw_solve_smem(nLay, kd->nband)) > 64 * 1024) {
+        seterr(errmsg, errlen, "nLay=%d needs more than the solver's 64 KiB of LDS per column", nLay); return 1;
+    }
     if (want_lw && (!kd->planck_frac || !kd->totplnk || kd->nplanck < 2)) { seterr(errmsg, errlen, "LW tables lack Planck data"); return 1; }
     if (!want_lw && (!kd->krayl_lower || !kd->krayl_upper || !kd->solar_src)) { seterr(errmsg, errlen, "SW tables lack Rayleigh/solar data"); return 1; }
     // active gas names (CCPP active_gases_array) must be known to the k-distribution
@@ -379,9 +382,9 @@ extern "C" void ccpprad_finalize(ccpprad_ctx *c)
 {
     if (!c) return;
     for (void *p : c->allocs) (void)hipFree(p);
-    DevBuf *bufs[] = {&c->w_tau, &c->w_aux, &c->w_mask, &c->w_ctau, &c->w_cssa, &c->w_cg, &c->w_sF,
-                      &c->w_sA, &c->w_sQ, &c->w_sb, &c->w_blk, &c->w_colf, &c->w_day, &c->w_idx,
-                      &c->w_oob, &c->w_colm, &c->w_sR, &c->w_sT, &c->w_sD};
+    DevBuf *bufs[] = {&c->w_tau, &c->w_aux, &c->w_mask, &c->w_ctau, &c->w_cssa, &c->w_cg,
+                      &c->w_sb, &c->w_blk, &c->w_colf, &c->w_day, &c->w_idx,
+                      &c->w_oob, &c->w_colm};
     for (DevBuf *b : bufs) b->release();
     for (auto &b : c->stage) b.release();
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -596,12 +599,6 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
     HIPCHK(c->w_ctau.ensure(sizeof(float) * (size_t)CH * nL * B));
     HIPCHK(c->w_cssa.ensure(sizeof(float) * (size_t)CH * nL * B));
     HIPCHK(c->w_cg.ensure(sizeof(float) * (size_t)CH * nL * B));
-    HIPCHK(c->w_sF.ensure(sizeof(float) * (size_t)CH * nV * G));
-    HIPCHK(c->w_sA.ensure(sizeof(float) * (size_t)CH * nV * G));
-    HIPCHK(c->w_sQ.ensure(sizeof(float) * (size_t)CH * nV * G));
-    HIPCHK(c->w_sR.ensure(sizeof(float) * (size_t)CH * nL * G));
-    HIPCHK(c->w_sT.ensure(sizeof(float) * (size_t)CH * nL * G));
-    HIPCHK(c->w_sD.ensure(sizeof(float) * (size_t)CH * nL * G));
     HIPCHK(c->w_sb.ensure(sizeof(double) * (size_t)nD * 3 * B));
     HIPCHK(c->w_blk.ensure((size_t)nblk + 1));
     HIPCHK(c->w_colf.ensure((size_t)nD + 1));
@@ -657,8 +654,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
         s.blkflag = blk; s.aer_tau = a->aersw_tau; s.aer_ssa = a->aersw_ssa; s.aer_g = a->aersw_g;
         s.coszen = a->coszen; s.alb_nir_dir = a->sfc_alb_nir_dir; s.alb_nir_dif = a->sfc_alb_nir_dif;
         s.alb_uv_dir = a->sfc_alb_uvvis_dir; s.alb_uv_dif = a->sfc_alb_uvvis_dif; s.p_lev = a->p_lev;
-        s.alb_cat = c->alb_cat; s.sF = (float *)c->w_sF.p; s.sA = (float *)c->w_sA.p; s.sQ = (float *)c->w_sQ.p;
-        s.sR = (float *)c->w_sR.p; s.sT = (float *)c->w_sT.p; s.sD = (float *)c->w_sD.p;
+        s.alb_cat = c->alb_cat;
         s.up_all = a->fluxswUP_allsky; s.dn_all = a->fluxswDOWN_allsky; s.up_clr = a->fluxswUP_clrsky;
         s.dn_clr = a->fluxswDOWN_clrsky; s.sb = (double *)c->w_sb.p + (size_t)c0 * 3 * B; s.htr = a->htrsw;
         s.htr_clr = a->doSWclrsky ? a->htrswc : nullptr;

@@ -83,12 +83,48 @@ __device__ __forceinline__ float planck_band(const TablesView &t, int b, float T
     return a0 + frac * (a1 - a0);
 }
 
-// FP64 wavefront sum (64 lanes) via cross-lane shuffles.
+// 1/x from v_rcp_f32 (about 1 ulp); x must be a normal number.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// e^{-x} from v_exp_f32 (x >= 0).
+__device__ __forceinline__ float exp_neg(float x) { return __builtin_amdgcn_exp2f(-1.4426950408889634f * x); }
+
+// e = e^{-x} and E = 1 - e^{-x} for x >= 0, E with small relative error: its
+// Taylor series (7 terms, truncation below 2e-9 relative) under x = 0.25,
+// 1 - e^{-x} (no cancellation) above.
+__device__ __forceinline__ void exp_pair(float x, float &e, float &E)
+{
+    e = exp_neg(x);
+    const float s = x * (1.0f - x * (0.5f - x * (1.0f / 6.0f - x * (1.0f / 24.0f - x * (1.0f / 120.0f -
+                    x * (1.0f / 720.0f - x * (1.0f / 5040.0f)))))));
+    E = x < 0.25f ? s : 1.0f - e;
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false),
+                            __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// FP64 sum over the 64 lanes of a wavefront, returned to every lane (all
+// lanes must be active).  Within each row of 16 lanes: quad swaps (lane^1,
+// lane^2), then the half-row and row mirrors; the four row sums are then
+// read out and added in a fixed order, so every lane gets the same value.
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);  // row_half_mirror
+    v += dpp_f64<0x140>(v);  // row_mirror
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
 }  // namespace ccpprad

@@ -5,7 +5,7 @@
 // rrtmgp_sw_main.F90:396-432), the rain/snow optics of the scheme layer
 // (rrtmgp_lw_main.F90:409-431; rrtmgp_sw_main.F90:436-471, constants
 // rrtmgp_lw_cloud_optics.F90:63-66, rrtmgp_sw_cloud_optics.F90:59-63,369-381)
-// and increment_2stream_by_2stream.  One thread per (column, layer), FP32.
+// and increment_2stream_by_2stream.  One thread per (column, layer, band), FP32.
 //
 // mcica_kernel restates the per-column seeding and MT19937 draws of
 // rrtmgp_lw_main.F90:441-466 / rrtmgp_sw_main.F90:479-509
@@ -81,27 +81,36 @@ __device__ __forceinline__ bool lut_cloud(const LutView &L, int b, float lwp, fl
     return bad;
 }
 
+// Rain/snow optics band coefficients (rrtmgp_sw_cloud_optics.F90:59-63);
+// bands beyond 14 use the last value.
+__constant__ float PREC_B0R[14] = {0.496f, 0.466f, 0.437f, 0.416f, 0.391f, 0.374f, 0.352f, 0.183f, 0.048f, 0.012f, 0.f, 0.f, 0.f, 0.f};
+__constant__ float PREC_B0S[14] = {0.460f, 0.460f, 0.460f, 0.460f, 0.460f, 0.460f, 0.460f, 0.460f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+__constant__ float PREC_B1S[14] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 1.62e-5f, 1.62e-5f, 0.f, 0.f, 0.f, 0.f};
+__constant__ float PREC_C0R[14] = {0.980f, 0.975f, 0.965f, 0.960f, 0.955f, 0.952f, 0.950f, 0.944f, 0.894f, 0.884f, 0.883f, 0.883f, 0.883f, 0.883f};
+__constant__ float PREC_C0S[14] = {0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.700f, 0.700f, 0.700f, 0.700f};
+
+// One thread per (column, layer, band), band fastest: the [d][l][b]
+// outputs are written contiguously by each wavefront.
 __global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs A, int ncols)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)ncols * A.nlay) return;
-    const int d = (int)(i % ncols), l = (int)(i / ncols);
+    const int nb = A.nband;
+    if (i >= (size_t)ncols * A.nlay * nb) return;
+    const int b = (int)(i % nb);
+    const size_t dl = i / nb;
+    const int l = (int)(dl % A.nlay), d = (int)(dl / A.nlay);
     const int od = A.col0 + d;
     const int c = A.col_idx ? A.col_idx[od] - 1 : od;
     const int blk = od / A.blksz;
     const size_t q = (size_t)c + (size_t)A.ncol_ld * l;
-    const int nb = A.nband;
-    float *ot = A.ctau + ((size_t)d * A.nlay + l) * nb;
-    float *os = A.cssa ? A.cssa + ((size_t)d * A.nlay + l) * nb : nullptr;
-    float *og = A.cg ? A.cg + ((size_t)d * A.nlay + l) * nb : nullptr;
     if (!A.blkflag[blk]) {
-        for (int b = 0; b < nb; ++b) { ot[b] = 0.0f; if (os) { os[b] = 0.0f; og[b] = 0.0f; } }
+        A.ctau[i] = 0.0f;
+        if (A.cssa) { A.cssa[i] = 0.0f; A.cg[i] = 0.0f; }
         return;
     }
     const float lwp = (float)A.lwp[q], iwp = (float)A.iwp[q];
     const float rl = (float)A.reliq[q], ri = (float)A.reice[q];
     const double cf = A.cld_frac[q];
-    bool bad = false;
     // precipitation, band independent parts
     float tr = 0.0f, tsn = 0.0f, res = 0.0f;
     bool prec = false;
@@ -121,49 +130,42 @@ __global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs 
             prec = true;
         }
     }
-    const float b0r[14] = {0.496f, 0.466f, 0.437f, 0.416f, 0.391f, 0.374f, 0.352f, 0.183f, 0.048f, 0.012f, 0.f, 0.f, 0.f, 0.f};
-    const float b0s[14] = {0.460f, 0.460f, 0.460f, 0.460f, 0.460f, 0.460f, 0.460f, 0.460f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    const float b1s[14] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 1.62e-5f, 1.62e-5f, 0.f, 0.f, 0.f, 0.f};
-    const float c0r[14] = {0.980f, 0.975f, 0.965f, 0.960f, 0.955f, 0.952f, 0.950f, 0.944f, 0.894f, 0.884f, 0.883f, 0.883f, 0.883f, 0.883f};
-    const float c0s[14] = {0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.700f, 0.700f, 0.700f, 0.700f};
-    for (int b = 0; b < nb; ++b) {
-        Opt2 o;
-        bad |= lut_cloud(L, b, lwp, iwp, rl, ri, o);
-        if (!A.is_lw && A.cldtau && b == 10 && blk == A.cldtau_blk) A.cldtau[q] = (double)o.t;
-        for (int k = 0; k < 2; ++k) {
-            const double *const *src = k == 0 ? A.cnv : A.pbl;
-            if (!src[0]) continue;
-            Opt2 o2;
-            bad |= lut_cloud(L, b, (float)src[0][q], (float)src[1][q], (float)src[2][q], (float)src[3][q], o2);
-            inc2s(o2, o);
-        }
-        Opt2 pr = {0.0f, 0.0f, 1.0f, 0.0f};
-        if (A.is_lw) {
-            pr.t = prec ? tr + tsn : 0.0f;
-        } else if (prec) {
-            const int bb = b < 14 ? b : 13;
-            const float sr = tr * (1.0f - b0r[bb]);
-            const float ar = sr * c0r[bb];
-            const float sn = tsn * (1.0f - (b0s[bb] + b1s[bb] * 1.0315f * res));
-            const float an = sn * c0s[bb];
-            const float tp = fmaxf(1.0e-12f, tr + tsn);
-            const float sp = fmaxf(1.0e-12f, sr + sn);
-            const float ap = fmaxf(1.0e-12f, ar + an);
-            const float asyw = ap / fmaxf(1.0e-12f, sp);
-            const float ssaw = fminf(1.0f - 0.000001f, sp / tp);
-            const float omw = fmaxf(1.0f - (1.0f - 0.000001f), (tp - sp) / tp);   // 1 - ssaw
-            const float za1 = asyw * asyw, za2 = ssaw * za1;
-            pr.t = (1.0f - za2) * tp;
-            pr.w = (ssaw - za2) / (1.0f - za2);
-            pr.p = omw / (1.0f - za2);
-            pr.g = asyw / (1.0f + asyw);
-        }
-        inc2s(pr, o);
-        if (A.is_lw) {
-            ot[b] = o.t * o.p;  // 1scl += 2str: absorption depth tau * (1 - ssa)
-        } else {
-            ot[b] = o.t; os[b] = o.p; og[b] = o.g;   // SW: tau, co-albedo, g
-        }
+    Opt2 o;
+    bool bad = lut_cloud(L, b, lwp, iwp, rl, ri, o);
+    if (!A.is_lw && A.cldtau && b == 10 && blk == A.cldtau_blk) A.cldtau[q] = (double)o.t;
+    for (int k = 0; k < 2; ++k) {
+        const double *const *src = k == 0 ? A.cnv : A.pbl;
+        if (!src[0]) continue;
+        Opt2 o2;
+        bad |= lut_cloud(L, b, (float)src[0][q], (float)src[1][q], (float)src[2][q], (float)src[3][q], o2);
+        inc2s(o2, o);
+    }
+    Opt2 pr = {0.0f, 0.0f, 1.0f, 0.0f};
+    if (A.is_lw) {
+        pr.t = prec ? tr + tsn : 0.0f;
+    } else if (prec) {
+        const int bb = b < 14 ? b : 13;
+        const float sr = tr * (1.0f - PREC_B0R[bb]);
+        const float ar = sr * PREC_C0R[bb];
+        const float sn = tsn * (1.0f - (PREC_B0S[bb] + PREC_B1S[bb] * 1.0315f * res));
+        const float an = sn * PREC_C0S[bb];
+        const float tp = fmaxf(1.0e-12f, tr + tsn);
+        const float sp = fmaxf(1.0e-12f, sr + sn);
+        const float ap = fmaxf(1.0e-12f, ar + an);
+        const float asyw = ap / fmaxf(1.0e-12f, sp);
+        const float ssaw = fminf(1.0f - 0.000001f, sp / tp);
+        const float omw = fmaxf(1.0f - (1.0f - 0.000001f), (tp - sp) / tp);   // 1 - ssaw
+        const float za1 = asyw * asyw, za2 = ssaw * za1;
+        pr.t = (1.0f - za2) * tp;
+        pr.w = (ssaw - za2) / (1.0f - za2);
+        pr.p = omw / (1.0f - za2);
+        pr.g = asyw / (1.0f + asyw);
+    }
+    inc2s(pr, o);
+    if (A.is_lw) {
+        A.ctau[i] = o.t * o.p;  // 1scl += 2str: absorption depth tau * (1 - ssa)
+    } else {
+        A.ctau[i] = o.t; A.cssa[i] = o.p; A.cg[i] = o.g;   // SW: tau, co-albedo, g
     }
     if (bad) atomicOr(A.err, 2);
 }
@@ -322,7 +324,7 @@ void launch_block_flags(int ncols, int nlay, int ncol_ld, const int *col_idx, in
 
 void launch_cloud_optics(const LutView &L, const CloudArgs &A, int ncols, hipStream_t st)
 {
-    const size_t n = (size_t)ncols * A.nlay;
+    const size_t n = (size_t)ncols * A.nlay * A.nband;
     if (!n) return;
     hipLaunchKernelGGL(cloud_optics_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, A, ncols);
 }

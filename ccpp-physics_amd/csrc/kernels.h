@@ -30,6 +30,7 @@ struct LwSolveArgs {
 };
 void launch_lw_solve(const TablesView &T, const LwSolveArgs &A, int ncols, int nang, bool clouds, bool jac,
                      bool optang, hipStream_t st);
+size_t lw_solve_smem(int nlay, int nband);
 void launch_heating_rate(int ncol, int nlay, const double *up, const double *dn, const double *p_lev,
                          double *htr, hipStream_t st);
 
@@ -71,13 +72,12 @@ struct SwSolveArgs {
     const double *aer_tau, *aer_ssa, *aer_g;  // (ncol_ld, nlay, nband) or null
     const double *coszen, *alb_nir_dir, *alb_nir_dif, *alb_uv_dir, *alb_uv_dif, *p_lev;
     const int *alb_cat;                    // [nband] 0 NIR, 1 average, 2 UV-VIS, 3 unset
-    float *sF, *sA, *sQ;                   // [d][lev][g] scratch: direct beam, adding albedo/source
-    float *sR, *sT, *sD;                   // [d][lay][g] scratch: R_dif, T_dif, direct-beam down source
     double *up_all, *dn_all, *up_clr, *dn_clr;  // (ncol_ld, nlev)
     double *sb;                            // [d][3][nband]: dn, dn_dir (all sky), dn (clear sky)
     double *htr, *htr_clr;                 // (ncol_ld, nlay) or null
 };
 void launch_sw_solve(const TablesView &T, const SwSolveArgs &A, int ncols, bool clouds, hipStream_t st);
+size_t sw_solve_smem(int nlay, int nband);
 
 struct ScmpArgs {
     int ndays, blksz, nband, ibd, ibd_uv, do_clr;  // ibd, ibd_uv 1-based

@@ -9,11 +9,13 @@
 // compute_heating_rate (GFS_rrtmgp_post.F90:159-171).
 //
 // One block = one column, blockDim = 128 (two wavefronts), lane = g-point.
-// Each lane carries the NANG angle intensities through a top-down sweep
-// (downward intensity) and a bottom-up sweep (upward intensity, Jacobian);
-// the layer optics (tau, Planck fraction, band Planck function) are re-read and
-// recomputed in the second sweep rather than stored, so the only HBM traffic is
-// the two reads of tau/pfrac (8 B per cell each) plus FP64 level fluxes.
+// The block first tabulates the column's band Planck function at every layer,
+// level and the surface in LDS (one interpolation per (row, band) instead of
+// one per g-point).  Each lane then carries the NANG angle intensities through
+// a top-down sweep (downward intensity) and a bottom-up sweep (upward
+// intensity, Jacobian); tau and the Planck fraction are re-read in the second
+// sweep rather than stored, so the HBM traffic is two reads of tau/pfrac (8 B
+// per cell each) plus the FP64 level fluxes.
 // At every level each wavefront sums its lanes' FP32 fluxes in FP64 and
 // parks the partial in LDS; the epilogue adds the two wave partials, writes
 // the (nCol, nLay+1) fluxes and differences them into the heating rate.
@@ -22,43 +24,86 @@
 namespace ccpprad {
 
 
-// Linear-in-tau source weight f(x) = (1-e^-x)/x - e^-x, from e^-x and 1-e^-x.
-// Series below x = 0.1 (relative truncation error < 1e-9): the direct form
-// cancels in FP32 for small x.
-__device__ __forceinline__ float lin_tau_fact(float x, float t, float omt)
+// Linear-in-tau source weight f(x) = (1-e^-x)/x - e^-x from e = e^-x and
+// E = 1 - e^-x.  Series below x = 0.1 (relative truncation error < 1e-9),
+// where the direct form cancels in FP32.
+__device__ __forceinline__ float lin_tau_fact(float x, float e, float E)
 {
-    if (x < 0.1f) {
-        return x * (0.5f - x * (1.0f / 3.0f - x * (0.125f - x * (1.0f / 30.0f - x * (1.0f / 144.0f - x * (1.0f / 840.0f))))));
-    }
-    return omt / x - t;
+    const float s = x * (0.5f - x * (1.0f / 3.0f - x * (0.125f - x * (1.0f / 30.0f - x * (1.0f / 144.0f - x * (1.0f / 840.0f))))));
+    return x < 0.1f ? s : E * frcp(x) - e;
+}
+
+constexpr int LW_BATCH = 8;
+
+// LDS per block: [2 waves][3][nlev] FP64 level sums, then the column's band
+// Planck function at layers [nlay][nband], levels [nlev][nband] and the
+// surface [2][nband] (at T_sfc and T_sfc + 1 K).
+size_t lw_solve_smem(int nlay, int nband)
+{
+    return sizeof(double) * 2 * 3 * (size_t)(nlay + 1) + sizeof(float) * (size_t)nband * (2 * (size_t)nlay + 3);
 }
 
 template <int NANG, bool CLOUDS, bool JAC, bool OPTANG>
 __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs A)
 {
-    extern __shared__ double red[];  // [2 waves][3][nlev]
+    extern __shared__ double red[];
     const int d = blockIdx.x;
     const int c = A.col0 + d;
     const int g = threadIdx.x;
     const int wave = g >> 6, lane = g & 63;
-    const int nlay = A.nlay, nlev = nlay + 1, ld = A.ncol_ld;
-    const bool act = g < T.ngpt;
+    const int nlay = A.nlay, nlev = nlay + 1, ld = A.ncol_ld, ngpt = T.ngpt, nb = T.nband;
+    const bool act = g < ngpt;
     const int gg = act ? g : 0;
     const int b = T.g2b[gg];
     double *rup = red + (size_t)wave * 3 * nlev, *rdn = rup + nlev, *rjc = rdn + nlev;
+    float *Blay = (float *)(red + 6 * (size_t)nlev), *Blev = Blay + (size_t)nlay * nb, *Bsfc = Blev + (size_t)nlev * nb;
+
+    // ---- the column's band Planck function (interpolate1D of totplnk),
+    // shared by the g-points of a band
+    {
+        const float tmin = (float)T.temp_ref_min, rdel = (float)(1.0 / T.totplnk_delta);
+        const int ntot = (2 * nlay + 3) * nb;
+        for (int i = g; i < ntot; i += blockDim.x) {
+            const int row = i / nb, bb = i - row * nb;
+            double tk;
+            if (row < nlay) tk = A.t_lay[(size_t)c + (size_t)ld * row];
+            else if (row < nlay + nlev) tk = A.t_lev[(size_t)c + (size_t)ld * (row - nlay)];
+            else tk = A.tsfc[c] + (row == 2 * nlay + 2 ? 1.0 : 0.0);
+            const float tf = (float)tk;
+            const float v0 = (tf - tmin) * rdel;
+            const int iv = (int)v0;
+            const float frac = v0 - (float)iv;
+            const int idx = d_min(T.nplanck - 1, d_max(1, iv + 1));
+            const float *tab = T.totplnk + (size_t)bb * T.nplanck;
+            const float a0 = tab[idx - 1], a1 = tab[idx];
+            Blay[i] = a0 + frac * (a1 - a0);   // Blay, Blev, Bsfc are consecutive
+        }
+        __syncthreads();
+    }
 
     double se = A.semis[c];
     const float emis = (se > 1.0e-6 && se <= 1.0) ? (float)se : 1.0f;
-    const float *tau_c = A.tau + (size_t)d * nlay * T.ngpt;
-    const float *pf_c = A.pfrac + (size_t)d * nlay * T.ngpt;
+    // column bases (uniform) with 32-bit in-column offsets
+    const float *tau_c = A.tau + (size_t)d * nlay * ngpt;
+    const float *pf_c = A.pfrac + (size_t)d * nlay * ngpt;
+    const unsigned *mk_c = CLOUDS ? A.mask + (size_t)d * nlay * MASK_WORDS : nullptr;
+    const float *ct_c = CLOUDS ? A.ctau + (size_t)d * nlay * nb : nullptr;
+    const unsigned mbit = 1u << (gg & 31), mword = (unsigned)gg >> 5;
 
-    auto layer_tau = [&](int l) -> float {
-        float t = tau_c[(size_t)l * T.ngpt + gg];
-        if (A.aer_tau) t += (float)A.aer_tau[(size_t)c + (size_t)ld * (l + (size_t)nlay * b)];
-        if (CLOUDS) {
-            const unsigned w = A.mask[((size_t)d * nlay + l) * MASK_WORDS + (gg >> 5)];
-            if ((w >> (gg & 31)) & 1u) t += A.ctau[((size_t)d * nlay + l) * T.nband + b];
-        }
+    // Per-layer inputs, loaded for LW_BATCH layers at a time with no load
+    // depending on another (the cloud depth is fetched unconditionally and
+    // selected by the mask bit), so a batch costs one memory latency.
+    struct LayerIn { float tau, pf, ct; unsigned mw; double aer; };
+    auto load_layer = [&](int l, LayerIn &x) {
+        x.tau = tau_c[(unsigned)l * ngpt + gg];
+        x.pf = pf_c[(unsigned)l * ngpt + gg];
+        x.aer = A.aer_tau ? A.aer_tau[(size_t)c + (size_t)ld * (l + (size_t)nlay * b)] : 0.0;
+        if (CLOUDS) { x.mw = mk_c[(unsigned)l * MASK_WORDS + mword]; x.ct = ct_c[(unsigned)l * nb + b]; }
+    };
+    auto layer_tau = [&](const LayerIn &x) -> float {
+        float t = x.tau;
+        if (A.aer_tau) t += (float)x.aer;
+        if (CLOUDS) { if (x.mw & mbit) t += x.ct; }
         return t;
     };
 
@@ -68,7 +113,7 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
     if (OPTANG) {
         // compute_optimal_angles: Ds = fit1 * exp(-column tau) + fit2, weight 0.5
         float s = 0.0f;
-        for (int l = 0; l < nlay; ++l) s += layer_tau(l);
+        for (int l = 0; l < nlay; ++l) { LayerIn x; load_layer(l, x); s += layer_tau(x); }
         D[0] = T.opt_fit[2 * b] * expf(-s) + T.opt_fit[2 * b + 1];
         W2pi[0] = 6.283185307179586f * 0.5f;
     }
@@ -80,34 +125,39 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
 #pragma unroll
     for (int a = 0; a < NANG; ++a) I[a] = 0.0f;
     if (lane == 0) rdn[lev_toa] = 0.0;
-    float B_above = planck_band(T, b, (float)A.t_lev[(size_t)c + (size_t)ld * lev_toa]);
-    for (int k = 0; k < nlay; ++k) {
-        const int l = top ? k : nlay - 1 - k;
-        const int lev_below = top ? l + 1 : l;
-        const float tau = layer_tau(l);
-        const float pf = pf_c[(size_t)l * T.ngpt + gg];
-        const float Bl = planck_band(T, b, (float)A.t_lay[(size_t)c + (size_t)ld * l]);
-        const float Bb = planck_band(T, b, (float)A.t_lev[(size_t)c + (size_t)ld * lev_below]);
-        const float lay = pf * Bl, ldn = pf * Bb;
-        float fsum = 0.0f;
+    for (int k0 = 0; k0 < nlay; k0 += LW_BATCH) {
+        LayerIn x[LW_BATCH];
 #pragma unroll
-        for (int a = 0; a < NANG; ++a) {
-            const float x = tau * D[a];
-            const float t = expf(-x);
-            const float omt = -expm1f(-x);
-            const float f = lin_tau_fact(x, t, omt);
-            I[a] = t * I[a] + (omt * ldn + 2.0f * f * (lay - ldn));
-            fsum += W2pi[a] * I[a];
+        for (int j = 0; j < LW_BATCH; ++j)
+            if (k0 + j < nlay) load_layer(top ? k0 + j : nlay - 1 - (k0 + j), x[j]);
+#pragma unroll
+        for (int j = 0; j < LW_BATCH; ++j) {
+            const int k = k0 + j;
+            if (k >= nlay) break;
+            const int l = top ? k : nlay - 1 - k;
+            const int lev_below = top ? l + 1 : l;
+            const float tau = layer_tau(x[j]);
+            const float pf = x[j].pf;
+            const float lay = pf * Blay[l * nb + b], ldn = pf * Blev[lev_below * nb + b];
+            const float dsrc = lay - ldn;
+            float fsum = 0.0f;
+#pragma unroll
+            for (int a = 0; a < NANG; ++a) {
+                const float xa = tau * D[a];
+                float t, omt;
+                exp_pair(xa, t, omt);
+                const float f = lin_tau_fact(xa, t, omt);
+                I[a] = t * I[a] + (omt * ldn + 2.0f * f * dsrc);
+                fsum += W2pi[a] * I[a];
+            }
+            const double ws = wave_sum(act ? (double)fsum : 0.0);
+            if (lane == 0) rdn[lev_below] = ws;
         }
-        const double ws = wave_sum(act ? (double)fsum : 0.0);
-        if (lane == 0) rdn[lev_below] = ws;
-        B_above = Bb;
     }
     // ---- surface
     const int sfc_lay = top ? nlay - 1 : 0;
-    const float pfs = pf_c[(size_t)sfc_lay * T.ngpt + gg];
-    const float ts = (float)A.tsfc[c];
-    const float B0 = planck_band(T, b, ts);
+    const float pfs = pf_c[(unsigned)sfc_lay * ngpt + gg];
+    const float B0 = Bsfc[b];
     const float sfc_src = pfs * B0;
     float J[NANG];
     {
@@ -117,7 +167,7 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
             I[a] = I[a] * (1.0f - emis) + emis * sfc_src;
             fsum += W2pi[a] * I[a];
             if (JAC) {
-                J[a] = emis * (pfs * (planck_band(T, b, ts + 1.0f) - B0));
+                J[a] = emis * (pfs * (Bsfc[nb + b] - B0));
                 jsum += W2pi[a] * J[a];
             }
         }
@@ -129,35 +179,40 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
         }
     }
     // ---- bottom-up sweep: upward intensity (and Jacobian)
-    float B_below = planck_band(T, b, (float)A.t_lev[(size_t)c + (size_t)ld * lev_sfc]);
-    for (int k = 0; k < nlay; ++k) {
-        const int l = top ? nlay - 1 - k : k;
-        const int lev_above = top ? l : l + 1;
-        const float tau = layer_tau(l);
-        const float pf = pf_c[(size_t)l * T.ngpt + gg];
-        const float Bl = planck_band(T, b, (float)A.t_lay[(size_t)c + (size_t)ld * l]);
-        const float Ba = planck_band(T, b, (float)A.t_lev[(size_t)c + (size_t)ld * lev_above]);
-        const float lay = pf * Bl, lup = pf * Ba;
-        float fsum = 0.0f, jsum = 0.0f;
+    for (int k0 = 0; k0 < nlay; k0 += LW_BATCH) {
+        LayerIn x[LW_BATCH];
 #pragma unroll
-        for (int a = 0; a < NANG; ++a) {
-            const float x = tau * D[a];
-            const float t = expf(-x);
-            const float omt = -expm1f(-x);
-            const float f = lin_tau_fact(x, t, omt);
-            I[a] = t * I[a] + (omt * lup + 2.0f * f * (lay - lup));
-            fsum += W2pi[a] * I[a];
-            if (JAC) { J[a] = t * J[a]; jsum += W2pi[a] * J[a]; }
+        for (int j = 0; j < LW_BATCH; ++j)
+            if (k0 + j < nlay) load_layer(top ? nlay - 1 - (k0 + j) : k0 + j, x[j]);
+#pragma unroll
+        for (int j = 0; j < LW_BATCH; ++j) {
+            const int k = k0 + j;
+            if (k >= nlay) break;
+            const int l = top ? nlay - 1 - k : k;
+            const int lev_above = top ? l : l + 1;
+            const float tau = layer_tau(x[j]);
+            const float pf = x[j].pf;
+            const float lay = pf * Blay[l * nb + b], lup = pf * Blev[lev_above * nb + b];
+            const float dsrc = lay - lup;
+            float fsum = 0.0f, jsum = 0.0f;
+#pragma unroll
+            for (int a = 0; a < NANG; ++a) {
+                const float xa = tau * D[a];
+                float t, omt;
+                exp_pair(xa, t, omt);
+                const float f = lin_tau_fact(xa, t, omt);
+                I[a] = t * I[a] + (omt * lup + 2.0f * f * dsrc);
+                fsum += W2pi[a] * I[a];
+                if (JAC) { J[a] = t * J[a]; jsum += W2pi[a] * J[a]; }
+            }
+            const double ws = wave_sum(act ? (double)fsum : 0.0);
+            if (lane == 0) rup[lev_above] = ws;
+            if (JAC) {
+                const double wj = wave_sum(act ? (double)jsum : 0.0);
+                if (lane == 0) rjc[lev_above] = wj;
+            }
         }
-        const double ws = wave_sum(act ? (double)fsum : 0.0);
-        if (lane == 0) rup[lev_above] = ws;
-        if (JAC) {
-            const double wj = wave_sum(act ? (double)jsum : 0.0);
-            if (lane == 0) rjc[lev_above] = wj;
-        }
-        B_below = Ba;
     }
-    (void)B_above; (void)B_below;
     __syncthreads();
     // ---- epilogue: combine wave partials, write fluxes, heating rate (FP64)
     double *r0 = red, *r1 = red + 3 * (size_t)nlev;
@@ -184,7 +239,7 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
 template <int NANG, bool CLOUDS, bool JAC, bool OPTANG>
 static void launch_one(const TablesView &T, const LwSolveArgs &A, int ncols, hipStream_t st)
 {
-    const size_t smem = sizeof(double) * 2 * 3 * (A.nlay + 1);
+    const size_t smem = lw_solve_smem(A.nlay, T.nband);
     hipLaunchKernelGGL((lw_solve_kernel<NANG, CLOUDS, JAC, OPTANG>), dim3(ncols), dim3(128), smem, st, T, A);
 }
 

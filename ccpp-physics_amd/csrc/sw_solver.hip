@@ -8,14 +8,16 @@
 // ty_fluxes_byband%reduce + the CCPP band sum (rrtmgp_sw_main.F90:558-559,617-618)
 // and compute_heating_rate on daylit columns (GFS_rrtmgp_post.F90:249-266).
 //
-// One block = one daylit column, blockDim 128, lane = g-point.  Three sweeps
-// per lane:
-//   1 top-down : direct beam F_dir by products of T_noscat (as the reference),
-//                stored per level (FP32 scratch)
-//   2 bottom-up: layer two-stream coefficients (FP64), direct-beam sources,
-//                adding albedo A and source Q per level; R_dif, T_dif and the
-//                downward direct source are kept per layer (FP32 scratch)
-//   3 top-down : diffuse down/up fluxes from the stored terms; F_dn += F_dir
+// One block = one daylit column, blockDim 128, lane = g-point.  Three passes
+// per lane, with no per-layer scratch in HBM:
+//   0 top-down : direct beam F_dir by products of T_noscat (as the reference),
+//                kept at segment tops (LDS)
+//   1 bottom-up: layer two-stream coefficients, direct-beam sources and
+//                the adding albedo A and source Q; A, Q kept at segment
+//                boundaries (LDS)
+//   2 top-down : per segment, the layer terms and A, Q below each layer are
+//                rebuilt in registers from the checkpoint at the segment
+//                bottom; diffuse down/up fluxes; F_dn += F_dir
 // Optical properties carry the co-albedo (1 - ssa) next to ssa, so the
 // absorption of nearly conservative layers is not lost to FP32 rounding.
 #include "kernels.h"
@@ -28,28 +30,13 @@ struct Opt2s {
     float t, w, p, g;
 };
 
-// increment_2stream_by_2stream: b += a (eps = 3*tiny(FP32)).
-__device__ __forceinline__ void inc2s_p(float t1, float w1, float p1, float g1, Opt2s &o)
-{
-    const float eps = 3.0f * 1.17549435e-38f;
-    const float t12 = t1 + o.t;
-    const float ts12 = t1 * w1 + o.t * o.w;
-    const float ta12 = t1 * p1 + o.t * o.p;
-    o.g = (t1 * w1 * g1 + o.t * o.w * o.g) / fmaxf(eps, ts12);
-    o.w = ts12 / fmaxf(eps, t12);
-    o.p = t12 > eps ? ta12 / t12 : 1.0f;
-    o.t = t12;
-}
-
 struct TwoStream {
     float Rd, Td, Rdir, Tdir;
 };
 
-// sw_two_stream for one layer, evaluated in FP64: the direct-beam
-// reflectance/transmittance brackets cancel to O(k tau) while their prefactor
-// grows like 1/k, which FP32 cannot carry to the 0.05 W/m2 bar.  gamma1 -
-// gamma2 = 2 (1 - w) is formed from the co-albedo.  Outputs are rounded to
-// FP32 for the adding recurrences.
+// sw_two_stream for one layer, FP64 form: used where 1 - (k mu0)^2 is small,
+// the removable singularity of the direct-beam terms.  One expm1 per
+// exponent gives both e and 1 - e without cancellation.
 __device__ __forceinline__ TwoStream two_stream64(double mu0, const Opt2s &o)
 {
     const double tau = o.t, w = o.w, p = o.p, g = o.g;
@@ -60,48 +47,106 @@ __device__ __forceinline__ TwoStream two_stream64(double mu0, const Opt2s &o)
     const double a1 = g1 * g4 + g2 * g3;
     const double a2 = g1 * g3 + g2 * g4;
     const double k = sqrt(fmax(2.0 * p * (g1 + g2), 1.0e-12));
-    const double e = exp(-tau * k);
-    const double e2 = e * e;
-    const double ome2 = -expm1(-2.0 * tau * k);
-    const double RT = 1.0 / (k * (1.0 + e2) + g1 * ome2);
-    const double x0 = tau / mu0;
-    const double T0 = exp(-x0), omT0 = -expm1(-x0);
+    const double E1 = -expm1(-tau * k);
+    const double e = 1.0 - E1;
+    const double E2 = E1 * (2.0 - E1);
+    const double RT = 1.0 / (k * (1.0 + e * e) + g1 * E2);
+    const double F = -expm1(-tau / mu0);
+    const double T0 = 1.0 - F;
     const double kmu = k * mu0, kg3 = k * g3, kg4 = k * g4;
     const double dd = 1.0 - kmu * kmu;
     const double RT2 = w * RT / (fabs(dd) >= 2.220446049250313e-16 ? dd : 2.220446049250313e-16);
-    double Rdir = RT2 * ((1.0 - kmu) * (a2 + kg3) - (1.0 + kmu) * (a2 - kg3) * e2 - 2.0 * (kg3 - a2 * kmu) * e * T0);
-    double Tdir = -RT2 * ((1.0 + kmu) * (a1 + kg4) * T0 - (1.0 - kmu) * (a1 - kg4) * e2 * T0 - 2.0 * (kg4 + a1 * kmu) * e);
-    Rdir = fmax(0.0, fmin(Rdir, omT0));
-    Tdir = fmax(0.0, fmin(Tdir, omT0 - Rdir));
+    double Rdir = RT2 * ((1.0 - kmu) * (a2 + kg3) - (1.0 + kmu) * (a2 - kg3) * e * e - 2.0 * (kg3 - a2 * kmu) * e * T0);
+    double Tdir = -RT2 * ((1.0 + kmu) * (a1 + kg4) * T0 - (1.0 - kmu) * (a1 - kg4) * e * e * T0 - 2.0 * (kg4 + a1 * kmu) * e);
+    Rdir = fmax(0.0, fmin(Rdir, F));
+    Tdir = fmax(0.0, fmin(Tdir, F - Rdir));
     TwoStream r;
-    r.Rd = (float)(RT * g2 * ome2);
+    r.Rd = (float)(RT * g2 * E2);
     r.Td = (float)(RT * 2.0 * k * e);
     r.Rdir = (float)Rdir;
     r.Tdir = (float)Tdir;
     return r;
 }
 
+// sw_two_stream for one layer in FP32 (Meador & Weaver 1980, Zdunkowski
+// coefficients).  The reference's direct-beam brackets are sums of O(1) terms
+// that cancel to O(k tau) under a 1/k prefactor.  With E1 = 1 - e^{-k tau},
+// E2 = 1 - e^{-2 k tau} and F = 1 - e^{-tau/mu0} (each from expm1, accurate to
+// a few ulp) the O(1) terms cancel exactly in the algebra:
+//   Rdir: (1-k mu)(a2+k g3) - (1+k mu)(a2-k g3) e^2 - 2 (k g3 - a2 k mu) e T0
+//       = (1+k mu)(a2-k g3) E2 + 2 k (g3 - a2 mu)(E1 + F - E1 F)
+//   Tdir: (1+k mu)(a1+k g4) T0 - (1-k mu)(a1-k g4) e^2 T0 - 2 (k g4 + a1 k mu) e
+//       = 2 k (g4 + mu a1)(E1 - F) + (1-k mu)(a1-k g4) E2 (1 - F)
+// so both carry their factor k explicitly and keep FP32 relative accuracy.
+// gamma1 - gamma2 = 2 (1 - w) comes from the co-albedo p.  Near
+// k mu0 = 1 (|1 - (k mu0)^2| < 1e-2) the lane switches to the FP64 form.
+__device__ __forceinline__ TwoStream two_stream32(float mu0, float mu0_inv, double mu0d, const Opt2s &o)
+{
+    const float tau = o.t, w = o.w, p = o.p, g = o.g;
+    const float g1 = (8.0f - w * (5.0f + 3.0f * g)) * 0.25f;
+    const float g2 = 3.0f * (w * (1.0f - g)) * 0.25f;
+    const float g3 = (2.0f - 3.0f * mu0 * g) * 0.25f;
+    const float g4 = 1.0f - g3;
+    const float a1 = g1 * g4 + g2 * g3;
+    const float a2 = g1 * g3 + g2 * g4;
+    const float k = __builtin_amdgcn_sqrtf(fmaxf(2.0f * p * (g1 + g2), 1.0e-12f));
+    const float kmu = k * mu0;
+    const float dd = 1.0f - kmu * kmu;
+    if (fabsf(dd) < 1.0e-2f) return two_stream64(mu0d, o);
+    float e, E1, T0, F;
+    exp_pair(tau * k, e, E1);
+    exp_pair(tau * mu0_inv, T0, F);
+    const float E2 = E1 * (2.0f - E1);
+    const float RT = frcp(k * (1.0f + e * e) + g1 * E2);
+    const float RT2 = w * RT * frcp(dd);
+    const float BR = (1.0f + kmu) * (a2 - k * g3) * E2 + 2.0f * k * (g3 - a2 * mu0) * (E1 + F - E1 * F);
+    const float BT = 2.0f * k * (g4 + mu0 * a1) * (E1 - F) + (1.0f - kmu) * (a1 - k * g4) * E2 * (1.0f - F);
+    TwoStream r;
+    r.Rdir = fmaxf(0.0f, fminf(RT2 * BR, F));
+    r.Tdir = fmaxf(0.0f, fminf(-RT2 * BT, F - r.Rdir));
+    r.Rd = RT * g2 * E2;
+    r.Td = RT * 2.0f * k * e;
+    return r;
+}
+
+// Layers per segment.  The column is walked in segments of SW_SEG layers;
+// per lane only the direct beam at each segment top and the adding albedo /
+// source at each segment boundary are kept (LDS), and a segment's layer
+// terms are recomputed in registers where they are needed.
+constexpr int SW_SEG = 8;
+
+size_t sw_solve_smem(int nlay, int nband)
+{
+    const int nseg = (nlay + SW_SEG - 1) / SW_SEG;
+    return sizeof(double) * (4 * (size_t)(nlay + 1) + 6 * (size_t)nband) + sizeof(float) * 3 * (size_t)(nseg + 1) * 128;
+}
+
 template <bool CLOUDS>
 __global__ __launch_bounds__(128) void sw_solve_kernel(TablesView T, SwSolveArgs A)
 {
-    extern __shared__ double red[];  // [2 waves][2][nlev] + [2][3][nband] band partials
+    // LDS: [2 waves][2][nlev] level sums, [2 waves][3][nband] band partials,
+    // then per lane [3][nseg + 1] checkpoints (direct beam, albedo, source)
+    extern __shared__ double red[];
     const int d = blockIdx.x;
     const int od = A.col0 + d;
     const int c = A.col_idx ? A.col_idx[od] - 1 : od;
     const int g = threadIdx.x, wave = g >> 6, lane = g & 63;
     const int nlay = A.nlay, nlev = nlay + 1, ld = A.ncol_ld, ngpt = T.ngpt, nb = T.nband;
+    const int nseg = (nlay + SW_SEG - 1) / SW_SEG;
     const bool act = g < ngpt;
     const int gg = act ? g : 0;
     const int b = T.g2b[gg];
     const bool cloudy = CLOUDS && A.blkflag[od / A.blksz];
     const int top = A.top_at_1;
-    const int lev_toa = top ? 0 : nlay, lev_sfc = top ? nlay : 0;
+    const int lev_toa = top ? 0 : nlay;
     double *rup = red + (size_t)wave * 2 * nlev, *rdn = rup + nlev;
     double *rb = red + 4 * (size_t)nlev;  // [wave][3][nband]
+    float *cF = (float *)(red + 4 * (size_t)nlev + 6 * (size_t)nb);
+    float *cA = cF + (size_t)(nseg + 1) * 128, *cQ = cA + (size_t)(nseg + 1) * 128;
 
     const double mu0d = A.coszen[c];
     const float mu0 = (float)mu0d;
-    const float mu0_inv = 1.0f / mu0;
+    const float mu0_inv = (float)(1.0 / mu0d);
     const int cat = A.alb_cat[b];
     float adir = 0.0f, adif = 0.0f;
     if (cat == 0) { adir = (float)A.alb_nir_dir[c]; adif = (float)A.alb_nir_dif[c]; }
@@ -110,91 +155,155 @@ __global__ __launch_bounds__(128) void sw_solve_kernel(TablesView T, SwSolveArgs
         adif = (float)(0.5 * (A.alb_nir_dif[c] + A.alb_uv_dif[c]));
     } else if (cat == 2) { adir = (float)A.alb_uv_dir[c]; adif = (float)A.alb_uv_dif[c]; }
 
-    const size_t cl = (size_t)d * nlay;
+    // Column bases (uniform) with 32-bit in-column offsets.
+    const float *ta_c = A.tau_abs + (size_t)d * nlay * ngpt, *tr_c = A.tau_ray + (size_t)d * nlay * ngpt;
+    const unsigned *mk_c = CLOUDS ? A.mask + (size_t)d * nlay * MASK_WORDS : nullptr;
+    const float *ct_c = CLOUDS ? A.ctau + (size_t)d * nlay * nb : nullptr;
+    const float *cp_c = CLOUDS ? A.cssa + (size_t)d * nlay * nb : nullptr;
+    const float *cg_c = CLOUDS ? A.cg + (size_t)d * nlay * nb : nullptr;
+    const unsigned mbit = 1u << (gg & 31), mword = (unsigned)gg >> 5;
+    const float EPS3 = 3.0f * 1.17549435e-38f;
     // combine_and_reorder_2str + aerosol and McICA cloud increments
+    // (increment_2stream_by_2stream), accumulated as extinction, scattering,
+    // absorption and scattering * g depths and normalised once.
     auto optics = [&](int l) -> Opt2s {
-        const float ta = A.tau_abs[(cl + l) * ngpt + gg], tr = A.tau_ray[(cl + l) * ngpt + gg];
-        Opt2s o;
-        o.t = ta + tr;
-        if (o.t > 2.0f * 1.17549435e-38f) { o.w = tr / o.t; o.p = ta / o.t; } else { o.w = 0.0f; o.p = 1.0f; }
-        o.g = 0.0f;
+        const unsigned og = (unsigned)l * ngpt + gg;
+        const float ta = ta_c[og], tr = tr_c[og];
+        float te = ta + tr, ts = tr, tab = ta, tsg = 0.0f;
         if (A.aer_tau) {
             const size_t q = (size_t)c + (size_t)ld * (l + (size_t)nlay * b);
-            const float aw = (float)A.aer_ssa[q];
-            inc2s_p((float)A.aer_tau[q], aw, (float)(1.0 - A.aer_ssa[q]), (float)A.aer_g[q], o);
+            const float t1 = (float)A.aer_tau[q], w1 = (float)A.aer_ssa[q];
+            const float s1 = t1 * w1;
+            te = t1 + te; ts += s1; tab += t1 * (float)(1.0 - A.aer_ssa[q]); tsg += s1 * (float)A.aer_g[q];
         }
         if (cloudy) {
-            const unsigned mw = A.mask[(cl + l) * MASK_WORDS + (gg >> 5)];
-            const bool m = (mw >> (gg & 31)) & 1u;
-            const size_t q = (cl + l) * nb + b;
-            const float cp = m ? A.cssa[q] : 1.0f;
-            inc2s_p(m ? A.ctau[q] : 0.0f, 1.0f - cp, cp, m ? A.cg[q] : 0.0f, o);
+            const unsigned ob = (unsigned)l * nb + b;
+            // unconditional loads, selected by the mask bit: no load waits on another
+            const unsigned mw = mk_c[(unsigned)l * MASK_WORDS + mword];
+            const float ctv = ct_c[ob], cpv = cp_c[ob], cgv = cg_c[ob];
+            const bool m = mw & mbit;
+            const float tc = m ? ctv : 0.0f, pc = m ? cpv : 0.0f, gc = m ? cgv : 0.0f;
+            const float sc = tc - tc * pc;
+            te = tc + te; ts += sc; tab += tc * pc; tsg += sc * gc;
         }
+        Opt2s o;
+        o.t = te;
+        o.w = ts * frcp(fmaxf(EPS3, te));
+        o.p = te > EPS3 ? tab * frcp(te) : 1.0f;
+        o.g = tsg * frcp(fmaxf(EPS3, ts));
         return o;
     };
-    const size_t sl = (size_t)d * nlev * ngpt, sy = (size_t)d * nlay * ngpt;
-    float *sF = A.sF + sl, *sA = A.sA + sl, *sQ = A.sQ + sl;
-    float *sR = A.sR + sy, *sT = A.sT + sy, *sD = A.sD + sy;
+    // the same total depth as optics(l).t, summed in the same order
+    auto depth = [&](int l) -> float {
+        const unsigned og = (unsigned)l * ngpt + gg;
+        float te = ta_c[og] + tr_c[og];
+        if (A.aer_tau) te = (float)A.aer_tau[(size_t)c + (size_t)ld * (l + (size_t)nlay * b)] + te;
+        if (cloudy) {
+            const unsigned mw = mk_c[(unsigned)l * MASK_WORDS + mword];
+            const float ctv = ct_c[(unsigned)l * nb + b];
+            te = ((mw & mbit) ? ctv : 0.0f) + te;
+        }
+        return te;
+    };
+    auto lay = [&](int k) { return top ? k : nlay - 1 - k; };       // k counts layers from the TOA
+    auto lev = [&](int k) { return top ? k : nlay - k; };           // level k is the top of layer k
     const float toa = T.toa_norm[gg];
 
-    // ---- sweep 1: direct beam, top-down (products of T_noscat)
+    // ---- pass 0, top-down: direct beam by products of T_noscat (as the
+    // reference), kept at every segment top and at the surface
     float fdir = toa * mu0;
-    sF[(size_t)lev_toa * ngpt + gg] = fdir;
-    for (int k = 0; k < nlay; ++k) {
-        const int l = top ? k : nlay - 1 - k;
-        const int lev_below = top ? l + 1 : l;
-        const Opt2s o = optics(l);
-        fdir = expf(-o.t * mu0_inv) * fdir;
-        sF[(size_t)lev_below * ngpt + gg] = fdir;
+    for (int s = 0; s < nseg; ++s) {
+        cF[(size_t)s * 128 + g] = fdir;
+#pragma unroll
+        for (int j = 0; j < SW_SEG; ++j) {
+            const int k = s * SW_SEG + j;
+            if (k < nlay) fdir = exp_neg(depth(lay(k)) * mu0_inv) * fdir;
+        }
     }
-    // ---- sweep 2: layer two-stream (FP64) and adding albedo / source, bottom-up
+    cF[(size_t)nseg * 128 + g] = fdir;
+
+    // ---- pass 1, bottom-up: adding albedo A and source Q, kept at segment
+    // boundaries
     float Ab = adif, Qb = fdir * adir;
-    sA[(size_t)lev_sfc * ngpt + gg] = Ab;
-    sQ[(size_t)lev_sfc * ngpt + gg] = Qb;
-    for (int k = 0; k < nlay; ++k) {
-        const int l = top ? nlay - 1 - k : k;
-        const int lev_above = top ? l : l + 1;
-        const TwoStream o = two_stream64(mu0d, optics(l));
-        const float fa = sF[(size_t)lev_above * ngpt + gg];
-        const float su = o.Rdir * fa, sd = o.Tdir * fa;
-        const float den = 1.0f / (1.0f - o.Rd * Ab);
-        const float Aa = o.Rd + o.Td * o.Td * Ab * den;
-        const float Qa = su + o.Td * den * (Qb + Ab * sd);
-        sA[(size_t)lev_above * ngpt + gg] = Aa;
-        sQ[(size_t)lev_above * ngpt + gg] = Qa;
-        sR[(size_t)l * ngpt + gg] = o.Rd;
-        sT[(size_t)l * ngpt + gg] = o.Td;
-        sD[(size_t)l * ngpt + gg] = sd;
-        Ab = Aa; Qb = Qa;
+    cA[(size_t)nseg * 128 + g] = Ab;
+    cQ[(size_t)nseg * 128 + g] = Qb;
+    for (int s = nseg - 1; s >= 0; --s) {
+        Opt2s o[SW_SEG];
+        float f[SW_SEG + 1];
+        f[0] = cF[(size_t)s * 128 + g];
+#pragma unroll
+        for (int j = 0; j < SW_SEG; ++j) {
+            const int k = s * SW_SEG + j;
+            if (k < nlay) { o[j] = optics(lay(k)); f[j + 1] = exp_neg(o[j].t * mu0_inv) * f[j]; }
+        }
+#pragma unroll
+        for (int j = SW_SEG - 1; j >= 0; --j) {
+            const int k = s * SW_SEG + j;
+            if (k >= nlay) continue;
+            const TwoStream ts = two_stream32(mu0, mu0_inv, mu0d, o[j]);
+            const float su = ts.Rdir * f[j], sd = ts.Tdir * f[j];
+            const float den = frcp(1.0f - ts.Rd * Ab);
+            const float Aa = ts.Rd + ts.Td * ts.Td * Ab * den;
+            const float Qa = su + ts.Td * den * (Qb + Ab * sd);
+            Ab = Aa; Qb = Qa;
+        }
+        cA[(size_t)s * 128 + g] = Ab;
+        cQ[(size_t)s * 128 + g] = Qb;
     }
-    // ---- sweep 3: fluxes, top-down
-    float dnd = 0.0f;                    // diffuse down at TOA
+
+    // ---- pass 2, top-down: fluxes.  Per segment the layer terms and the
+    // albedo / source below each layer are rebuilt from the checkpoint at the
+    // segment bottom.
+    float dnd = 0.0f;  // diffuse down at TOA
     {
-        const float up = dnd * Ab + Qb;  // Ab, Qb hold the TOA values
+        const float up = dnd * cA[g] + cQ[g];
         const double su = wave_sum(act ? (double)up : 0.0);
         const double sdn = wave_sum(act ? (double)(dnd + toa * mu0) : 0.0);
         if (lane == 0) { rup[lev_toa] = su; rdn[lev_toa] = sdn; }
     }
-    for (int k = 0; k < nlay; ++k) {
-        const int l = top ? k : nlay - 1 - k;
-        const int lev_below = top ? l + 1 : l;
-        const float Rd = sR[(size_t)l * ngpt + gg], Td = sT[(size_t)l * ngpt + gg], sd = sD[(size_t)l * ngpt + gg];
-        const float Abl = sA[(size_t)lev_below * ngpt + gg], Qbl = sQ[(size_t)lev_below * ngpt + gg];
-        const float den = 1.0f / (1.0f - Rd * Abl);
-        dnd = (Td * dnd + Rd * Qbl + sd) * den;
-        const float up = dnd * Abl + Qbl;
-        const float fd = sF[(size_t)lev_below * ngpt + gg];
-        const float dn = dnd + fd;
-        const double su = wave_sum(act ? (double)up : 0.0);
-        const double sdn = wave_sum(act ? (double)dn : 0.0);
-        if (lane == 0) { rup[lev_below] = su; rdn[lev_below] = sdn; }
-        if (lev_below == A.isfc) {
-            // per-band surface sums for scmpsw
-            const double vdn = act ? (double)dn : 0.0, vdir = act ? (double)fd : 0.0;
-            for (int bb = 0; bb < nb; ++bb) {
-                const bool mine = act && b == bb;
-                const double s1 = wave_sum(mine ? vdn : 0.0), s2 = wave_sum(mine ? vdir : 0.0);
-                if (lane == 0) { rb[(size_t)wave * 3 * nb + bb] = s1; rb[(size_t)wave * 3 * nb + nb + bb] = s2; }
+    for (int s = 0; s < nseg; ++s) {
+        Opt2s o[SW_SEG];
+        float f[SW_SEG + 1];
+        float Rd[SW_SEG], Td[SW_SEG], sdv[SW_SEG], Abl[SW_SEG], Qbl[SW_SEG], dnv[SW_SEG];
+        f[0] = cF[(size_t)s * 128 + g];
+#pragma unroll
+        for (int j = 0; j < SW_SEG; ++j) {
+            const int k = s * SW_SEG + j;
+            if (k < nlay) { o[j] = optics(lay(k)); f[j + 1] = exp_neg(o[j].t * mu0_inv) * f[j]; }
+        }
+        float a = cA[(size_t)(s + 1) * 128 + g], q = cQ[(size_t)(s + 1) * 128 + g];
+#pragma unroll
+        for (int j = SW_SEG - 1; j >= 0; --j) {
+            const int k = s * SW_SEG + j;
+            if (k >= nlay) continue;
+            const TwoStream ts = two_stream32(mu0, mu0_inv, mu0d, o[j]);
+            const float su = ts.Rdir * f[j], sd = ts.Tdir * f[j];
+            const float den = frcp(1.0f - ts.Rd * a);
+            Rd[j] = ts.Rd; Td[j] = ts.Td; sdv[j] = sd; Abl[j] = a; Qbl[j] = q; dnv[j] = den;
+            const float Aa = ts.Rd + ts.Td * ts.Td * a * den;
+            const float Qa = su + ts.Td * den * (q + a * sd);
+            a = Aa; q = Qa;
+        }
+#pragma unroll
+        for (int j = 0; j < SW_SEG; ++j) {
+            const int k = s * SW_SEG + j;
+            if (k >= nlay) continue;
+            const int lev_below = lev(k + 1);
+            dnd = (Td[j] * dnd + Rd[j] * Qbl[j] + sdv[j]) * dnv[j];
+            const float up = dnd * Abl[j] + Qbl[j];
+            const float fd = f[j + 1];
+            const float dn = dnd + fd;
+            const double su = wave_sum(act ? (double)up : 0.0);
+            const double sdn = wave_sum(act ? (double)dn : 0.0);
+            if (lane == 0) { rup[lev_below] = su; rdn[lev_below] = sdn; }
+            if (lev_below == A.isfc) {
+                // per-band surface sums for scmpsw
+                const double vdn = act ? (double)dn : 0.0, vdir = act ? (double)fd : 0.0;
+                for (int bb = 0; bb < nb; ++bb) {
+                    const bool mine = act && b == bb;
+                    const double s1 = wave_sum(mine ? vdn : 0.0), s2 = wave_sum(mine ? vdir : 0.0);
+                    if (lane == 0) { rb[(size_t)wave * 3 * nb + bb] = s1; rb[(size_t)wave * 3 * nb + nb + bb] = s2; }
+                }
             }
         }
     }
@@ -222,14 +331,15 @@ __global__ __launch_bounds__(128) void sw_solve_kernel(TablesView T, SwSolveArgs
         if (A.role == 0) { o[bb] = sdn; o[nb + bb] = sdir; }
         if (clr_too) o[2 * nb + bb] = sdn;
     }
-    double *hout[2] = {A.role == 0 ? A.htr : nullptr, (clr_too || A.role == 1) ? A.htr_clr : nullptr};
-    for (int h = 0; h < 2; ++h) {
-        if (!hout[h]) continue;
+    double *h_all = A.role == 0 ? A.htr : nullptr, *h_clr = (clr_too || A.role == 1) ? A.htr_clr : nullptr;
+    if (h_all || h_clr) {
         for (int l = g; l < nlay; l += blockDim.x) {
             const double u0 = r0[l] + r1[l], u1 = r0[l + 1] + r1[l + 1];
             const double d0 = r0[nlev + l] + r1[nlev + l], d1 = r0[nlev + l + 1] + r1[nlev + l + 1];
             const double p0 = A.p_lev[(size_t)c + (size_t)ld * l], p1 = A.p_lev[(size_t)c + (size_t)ld * (l + 1)];
-            hout[h][(size_t)c + (size_t)ld * l] = (u1 - u0 - d1 + d0) * GRAV / (CP_DRY * (p1 - p0));
+            const double hr = (u1 - u0 - d1 + d0) * GRAV / (CP_DRY * (p1 - p0));
+            if (h_all) h_all[(size_t)c + (size_t)ld * l] = hr;
+            if (h_clr) h_clr[(size_t)c + (size_t)ld * l] = hr;
         }
     }
 }
@@ -237,7 +347,7 @@ __global__ __launch_bounds__(128) void sw_solve_kernel(TablesView T, SwSolveArgs
 void launch_sw_solve(const TablesView &T, const SwSolveArgs &A, int ncols, bool clouds, hipStream_t st)
 {
     if (ncols <= 0) return;
-    const size_t smem = sizeof(double) * (4 * (size_t)(A.nlay + 1) + 6 * (size_t)T.nband);
+    const size_t smem = sw_solve_smem(A.nlay, T.nband);
     if (clouds)
         hipLaunchKernelGGL(sw_solve_kernel<true>, dim3(ncols), dim3(128), smem, st, T, A);
     else
