@@ -258,6 +258,20 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
     std::vector<int> flav(2 * kd->nflav);
     for (int i = 0; i < 2 * kd->nflav; ++i) flav[i] = kd->flavor[i];
     T.flavor = upload(c, flav.data(), flav.size(), &e);
+    {
+        // vmr_ref ratio of each flavor's gas pair per (temperature, regime),
+        // the same FP64 division the interpolation would do per layer
+        const int ng1 = kd->ngas + 1;
+        std::vector<double> ratio((size_t)kd->ntemp * kd->nflav * 2);
+        for (int jt = 0; jt < kd->ntemp; ++jt)
+            for (int f = 0; f < kd->nflav; ++f)
+                for (int r = 0; r < 2; ++r) {
+                    const int g1 = kd->flavor[2 * f], g2 = kd->flavor[2 * f + 1];
+                    ratio[((size_t)jt * kd->nflav + f) * 2 + r] =
+                        kd->vmr_ref[((size_t)jt * ng1 + g1) * 2 + r] / kd->vmr_ref[((size_t)jt * ng1 + g2) * 2 + r];
+                }
+        T.flav_ratio = upload(c, ratio.data(), ratio.size(), &e);
+    }
     std::vector<int> gf(2 * kd->ngpt), g2b(kd->ngpt, 0), bl(2 * kd->nband);
     for (int g = 0; g < kd->ngpt; ++g) { gf[2 * g] = kd->gpoint_flavor[2 * g] - 1; gf[2 * g + 1] = kd->gpoint_flavor[2 * g + 1] - 1; }
     for (int b = 0; b < kd->nband; ++b) {
@@ -292,6 +306,11 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
         }
         std::vector<int> ptr(kd->ngpt + 1, 0), items;
         for (int g = 0; g < kd->ngpt; ++g) {
+            if ((int)per[g].size() > MAX_MINOR_G) {
+                delete c;
+                seterr(errmsg, errlen, "g-point %d has %d minor absorbers (limit %d)", g + 1, (int)per[g].size(), MAX_MINOR_G);
+                return 1;
+            }
             ptr[g + 1] = ptr[g] + (int)per[g].size();
             for (auto &p : per[g]) { items.push_back(p.first); items.push_back(p.second); }
         }

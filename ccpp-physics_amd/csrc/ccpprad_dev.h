@@ -20,6 +20,7 @@ namespace ccpprad {
 constexpr int MAX_GAS = 6;
 constexpr int MAX_FLAV = 16;
 constexpr int MAX_MINOR = 32;      // minor intervals per regime
+constexpr int MAX_MINOR_G = 8;     // minor intervals covering one g-point, per regime
 constexpr int MAX_BAND = 32;
 constexpr int MAX_GPT = 128;       // g-points per column handled by one block
 constexpr int MASK_WORDS = 4;      // 128 bits of McICA mask per (col, lay)
@@ -47,6 +48,7 @@ struct TablesView {
     const float *totplnk;
     const float *opt_fit;      // [nband][2]
     const double *vmr_ref;
+    const double *flav_ratio;  // [ntemp][nflav][2] vmr_ref(gas1)/vmr_ref(gas2) per regime
     const int *flavor;         // [nflav][2]
     const int *gpt_flav;       // [ngpt][2] 0-based flavor per regime
     const int *g2b;            // [ngpt] band of g-point (0-based)

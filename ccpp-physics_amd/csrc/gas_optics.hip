@@ -7,7 +7,8 @@
 // (interpolate2D), compute_Planck_source's Planck-fraction interpolation,
 // and for SW compute_tau_rayleigh.
 //
-// One block = one column, 128 threads.  Layers go in chunks of LCHUNK:
+// One block = LCHUNK layers of one column, 128 threads; the layer chunk is the
+// slowest grid index so that co-resident blocks share k-table rows in L2:
 //   phase A0 (FP64): per layer  col_dry, col_gas, jtemp/ftemp, jpress/fpress, tropo
 //   phase A1 (FP64): per (layer, flavor, T-corner) col_mix, jeta, fminor;
 //                    per (layer, minor interval) the absorber scaling
@@ -21,7 +22,7 @@
 
 namespace ccpprad {
 
-constexpr int LCHUNK = 8;
+constexpr int LCHUNK = 16;
 
 struct GasOpticsSmem {
     double col[LCHUNK][MAX_GAS + 1];
@@ -36,6 +37,12 @@ struct GasOpticsSmem {
     float ms[LCHUNK][MAX_MINOR];
 };
 
+// A minor absorber item of one g-point: contributor column (16 bits),
+// interval (8 bits), flavor (8 bits).
+__device__ __forceinline__ int mi_ci(unsigned v) { return (int)(v & 0xFFFFu); }
+__device__ __forceinline__ int mi_iv(unsigned v) { return (int)((v >> 16) & 0xFFu); }
+__device__ __forceinline__ int mi_fv(unsigned v) { return (int)(v >> 24); }
+
 template <bool LW>
 __global__ __launch_bounds__(128) void gas_optics_kernel(
     TablesView T, int nlay, int ncol_ld, const int *__restrict__ col_idx, int col0,
@@ -45,18 +52,43 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
     const unsigned char *__restrict__ row_mask, const unsigned char *__restrict__ col_mask)
 {
     __shared__ GasOpticsSmem s;
-    const int d = blockIdx.x;                 // column within this launch
+    // Block = (layer chunk, column), chunk slowest: blocks resident at the same
+    // time work on the same pressure range and share that slice of the
+    // k-tables in L2.
+    const int ncols = gridDim.x / ((nlay + LCHUNK - 1) / LCHUNK);
+    const int chunk = blockIdx.x / ncols;
+    const int d = blockIdx.x - chunk * ncols;  // column within this launch
     const int gc = col_idx ? col_idx[col0 + d] - 1 : col0 + d;  // global column
     const int tid = threadIdx.x;
     if (col_mask && !col_mask[d]) return;   // uniform per block
-    const int ngpt = T.ngpt, neta = T.neta, np1 = T.npres + 1, ng1 = T.ngas + 1;
+    const int ngpt = T.ngpt, neta = T.neta, np1 = T.npres + 1, nflav = T.nflav;
     const double tmin = T.temp_ref_min, dT = T.temp_ref_delta;
 
-    // per-lane constants
-    int gfl[2] = {0, 0};
-    if (tid < ngpt) { gfl[0] = T.gpt_flav[2 * tid]; gfl[1] = T.gpt_flav[2 * tid + 1]; }
+    // ---- per-lane constants: flavors per regime and the minor-absorber
+    // items covering this g-point (the lane's g-point is tid)
+    const bool act = tid < ngpt;
+    const int g = act ? tid : 0;
+    const int gfl0 = T.gpt_flav[2 * g], gfl1 = T.gpt_flav[2 * g + 1];
+    unsigned mit[2][MAX_MINOR_G];
+    int nmi[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int i0 = T.minor_gptr[r][g], i1 = T.minor_gptr[r][g + 1];
+        nmi[r] = i1 - i0;
+#pragma unroll
+        for (int ii = 0; ii < MAX_MINOR_G; ++ii) {
+            unsigned v = 0;
+            if (ii < nmi[r]) {
+                const int iv = T.minor_items[r][2 * (i0 + ii)], ci = T.minor_items[r][2 * (i0 + ii) + 1];
+                v = (unsigned)ci | ((unsigned)iv << 16) | ((unsigned)T.minor_par[r][5 * iv + 4] << 24);
+            }
+            mit[r][ii] = v;
+        }
+    }
+    const unsigned dp_ = (unsigned)(neta * ngpt), de = (unsigned)ngpt;
 
-    for (int l0 = 0; l0 < nlay; l0 += LCHUNK) {
+    {
+        const int l0 = chunk * LCHUNK;
         const int nl = min(LCHUNK, nlay - l0);
         // ---- phase A0: per layer scalars (FP64)
         if (tid < nl) {
@@ -71,7 +103,7 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
             double m_air = (M_DRY + M_H2O * vh) * fact;
             double col_dry = 10.0 * dp * AVOGAD / (1000.0 * m_air * 100.0 * GRAV) * fact;
             s.col[tid][0] = col_dry;
-            for (int g = 1; g <= T.ngas; ++g) s.col[tid][g] = vmr.v[g - 1][q] * col_dry;
+            for (int gs = 1; gs <= T.ngas; ++gs) s.col[tid][gs] = vmr.v[gs - 1][q] * col_dry;
             int jt = (int)((tl - (tmin - dT)) / dT);
             jt = min(T.ntemp - 1, max(1, jt));
             s.jtemp[tid] = jt;
@@ -89,28 +121,28 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
         }
         __syncthreads();
         // ---- phase A1: flavor interpolation items and minor scalings (FP64)
-        const int nflav_items = nl * T.nflav * 2;
-        const int nitems = nflav_items + nl * MAX_MINOR;
+        const int nflav_items = nl * nflav * 2;
+        const int nmin_max = max(T.nminor[0], T.nminor[1]);
+        const int nitems = nflav_items + nl * nmin_max;
         for (int it = tid; it < nitems; it += blockDim.x) {
             if (it < nflav_items) {
-                int j = it / (T.nflav * 2), r = it % (T.nflav * 2), f = r >> 1, t = r & 1;
-                int g1 = T.flavor[2 * f], g2 = T.flavor[2 * f + 1];
-                int itr = s.itropo[j], jt = s.jtemp[j];
-                double ratio = T.vmr_ref[((size_t)(jt - 1 + t) * ng1 + g1) * 2 + (itr - 1)] /
-                               T.vmr_ref[((size_t)(jt - 1 + t) * ng1 + g2) * 2 + (itr - 1)];
-                double cm = s.col[j][g1] + ratio * s.col[j][g2];
-                double eta = (cm > 2.0 * 2.2250738585072014e-308) ? s.col[j][g1] / cm : 0.5;
-                double loceta = eta * (double)(neta - 1);
-                int jeta = min((int)loceta + 1, neta - 1);
-                double feta = fmod(loceta, 1.0);
-                double ftt = t == 0 ? (1.0 - s.ftemp[j]) : s.ftemp[j];
+                const int j = it / (nflav * 2), r = it - j * (nflav * 2), f = r >> 1, t = r & 1;
+                const int g1 = T.flavor[2 * f], g2 = T.flavor[2 * f + 1];
+                const int itr = s.itropo[j], jt = s.jtemp[j];
+                const double ratio = T.flav_ratio[((size_t)(jt - 1 + t) * nflav + f) * 2 + (itr - 1)];
+                const double cm = s.col[j][g1] + ratio * s.col[j][g2];
+                const double eta = (cm > 2.0 * 2.2250738585072014e-308) ? s.col[j][g1] / cm : 0.5;
+                const double loceta = eta * (double)(neta - 1);
+                const int jeta = min((int)loceta + 1, neta - 1);
+                const double feta = loceta - floor(loceta);   // fmod(loceta, 1) for loceta >= 0
+                const double ftt = t == 0 ? (1.0 - s.ftemp[j]) : s.ftemp[j];
                 s.cm[j][f][t] = (float)cm;
                 s.je[j][f][t] = jeta;
                 s.fm[j][f][t][0] = (float)((1.0 - feta) * ftt);
                 s.fm[j][f][t][1] = (float)(feta * ftt);
             } else {
-                int r = it - nflav_items, j = r / MAX_MINOR, i = r % MAX_MINOR;
-                int reg = s.itropo[j] - 1;
+                const int r = it - nflav_items, j = r / nmin_max, i = r - j * nmin_max;
+                const int reg = s.itropo[j] - 1;
                 if (i < T.nminor[reg]) {
                     const int *mp = T.minor_par[reg] + 5 * i;
                     double sc = s.col[j][mp[0]];
@@ -128,69 +160,92 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
             }
         }
         __syncthreads();
-        // ---- phase B: g-point gathers (FP32)
-        for (int g = tid; g < ngpt; g += blockDim.x) {
-            for (int j = 0; j < nl; ++j) {
-                const int l = l0 + j;
-                const int itr = s.itropo[j];
-                const int f = (g == tid) ? (itr == 1 ? gfl[0] : gfl[1]) : T.gpt_flav[2 * g + itr - 1];
-                const int jt0 = s.jtemp[j] - 1;
-                const int jp0 = s.jpress[j] + itr - 2;  // 0-based lower pressure row
-                const float fp1 = s.fpress[j], fp0 = 1.0f - fp1;
-                float tg = 0.0f, pf = 0.0f;
+        // ---- phase B: g-point gathers (FP32), two layers at a time: the major
+        // (and Planck-fraction / Rayleigh) table loads of both layers are
+        // issued before any arithmetic, then the minor items of both layers
+        if (act) {
+            for (int j0 = 0; j0 < nl; j0 += 2) {
+                const bool two = j0 + 1 < nl;
+                float k[2][2][4], q[2][2][4], kr[2][2][2];
+                float w[2][2][2], cmv[2][2], fp1v[2];
+                int regv[2], jt0v[2];
 #pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    const int je0 = s.je[j][f][t] - 1;
-                    const float fe0 = s.fm[j][f][t][0], fe1 = s.fm[j][f][t][1];
-                    const size_t base = (((size_t)(jt0 + t) * np1 + jp0) * neta + je0) * ngpt + g;
-                    const size_t dp_ = (size_t)neta * ngpt, de = (size_t)ngpt;
-                    float k00 = T.kmajor[base], k10 = T.kmajor[base + dp_];
-                    float k01 = T.kmajor[base + de], k11 = T.kmajor[base + dp_ + de];
-                    float st = fp0 * fe0 * k00 + fp1 * fe0 * k10 + fp0 * fe1 * k01 + fp1 * fe1 * k11;
-                    tg += s.cm[j][f][t] * st;
-                    if (LW) {
-                        float q00 = T.pfrac_tab[base], q10 = T.pfrac_tab[base + dp_];
-                        float q01 = T.pfrac_tab[base + de], q11 = T.pfrac_tab[base + dp_ + de];
-                        pf += fp0 * fe0 * q00 + fp1 * fe0 * q10 + fp0 * fe1 * q01 + fp1 * fe1 * q11;
-                    }
-                }
-                // minor absorbers of this regime covering g
-                const int reg = itr - 1;
-                const int *gp = T.minor_gptr[reg];
-                const int i0 = gp[g], i1 = gp[g + 1];
-                const int nc = T.ncontrib[reg];
-                const float *km = T.kminor[reg];
-                for (int ii = i0; ii < i1; ++ii) {
-                    const int iv = T.minor_items[reg][2 * ii], ci = T.minor_items[reg][2 * ii + 1];
-                    const int fv = T.minor_par[reg][5 * iv + 4];
-                    float kv = 0.0f;
-#pragma unroll
-                    for (int t = 0; t < 2; ++t) {
-                        const int je0 = s.je[j][fv][t] - 1;
-                        const size_t b0 = ((size_t)(jt0 + t) * neta + je0) * nc + ci;
-                        kv += s.fm[j][fv][t][0] * km[b0] + s.fm[j][fv][t][1] * km[b0 + nc];
-                    }
-                    tg += s.ms[j][iv] * kv;
-                }
-                const size_t o = ((size_t)d * nlay + l) * ngpt + g;
-                if (row_mask && !row_mask[(size_t)d * nlay + l]) continue;
-                tau[o] = tg;
-                if (LW) {
-                    aux[o] = pf;
-                } else {
-                    const float *kr = T.krayl[reg];
-                    float kv = 0.0f;
+                for (int u = 0; u < 2; ++u) {
+                    if (u == 1 && !two) break;
+                    const int j = j0 + u;
+                    const int itr = s.itropo[j], reg = itr - 1;
+                    regv[u] = reg;
+                    const int f = reg == 0 ? gfl0 : gfl1;
+                    const unsigned jt0 = (unsigned)(s.jtemp[j] - 1);
+                    jt0v[u] = (int)jt0;
+                    const unsigned jp0 = (unsigned)(s.jpress[j] + itr - 2);  // 0-based lower pressure row
+                    fp1v[u] = s.fpress[j];
 #pragma unroll
                     for (int t = 0; t < 2; ++t) {
                         const int je0 = s.je[j][f][t] - 1;
-                        const size_t b0 = ((size_t)(jt0 + t) * neta + je0) * ngpt + g;
-                        kv += s.fm[j][f][t][0] * kr[b0] + s.fm[j][f][t][1] * kr[b0 + ngpt];
+                        w[u][t][0] = s.fm[j][f][t][0];
+                        w[u][t][1] = s.fm[j][f][t][1];
+                        cmv[u][t] = s.cm[j][f][t];
+                        const unsigned base = (((jt0 + t) * np1 + jp0) * neta + je0) * ngpt + g;
+                        k[u][t][0] = T.kmajor[base]; k[u][t][1] = T.kmajor[base + dp_];
+                        k[u][t][2] = T.kmajor[base + de]; k[u][t][3] = T.kmajor[base + dp_ + de];
+                        if (LW) {
+                            q[u][t][0] = T.pfrac_tab[base]; q[u][t][1] = T.pfrac_tab[base + dp_];
+                            q[u][t][2] = T.pfrac_tab[base + de]; q[u][t][3] = T.pfrac_tab[base + dp_ + de];
+                        } else {
+                            const unsigned b0 = ((jt0 + t) * neta + je0) * ngpt + g;
+                            kr[u][t][0] = T.krayl[reg][b0]; kr[u][t][1] = T.krayl[reg][b0 + ngpt];
+                        }
                     }
-                    aux[o] = kv * s.cray[j];
+                }
+                // minor absorbers of each layer's regime covering g
+                float tm[2] = {0.0f, 0.0f};
+#pragma unroll
+                for (int ii = 0; ii < MAX_MINOR_G; ++ii) {
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        if (u == 1 && !two) break;
+                        const int reg = regv[u];
+                        if (ii < nmi[reg]) {
+                            const int j = j0 + u;
+                            const unsigned v = reg == 0 ? mit[0][ii] : mit[1][ii];
+                            const int fv = mi_fv(v), ci = mi_ci(v), nc = T.ncontrib[reg];
+                            const float *kmt = T.kminor[reg];
+                            float kv = 0.0f;
+#pragma unroll
+                            for (int t = 0; t < 2; ++t) {
+                                const int je0 = s.je[j][fv][t] - 1;
+                                const unsigned b0 = (((unsigned)jt0v[u] + t) * neta + je0) * nc + ci;
+                                kv += s.fm[j][fv][t][0] * kmt[b0] + s.fm[j][fv][t][1] * kmt[b0 + nc];
+                            }
+                            tm[u] += s.ms[j][mi_iv(v)] * kv;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    if (u == 1 && !two) break;
+                    const int j = j0 + u, l = l0 + j;
+                    const float fp1 = fp1v[u], fp0 = 1.0f - fp1;
+                    float tg = 0.0f, pf = 0.0f, kv = 0.0f;
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) {
+                        const float fe0 = w[u][t][0], fe1 = w[u][t][1];
+                        const float st = fp0 * fe0 * k[u][t][0] + fp1 * fe0 * k[u][t][1] + fp0 * fe1 * k[u][t][2] +
+                                         fp1 * fe1 * k[u][t][3];
+                        tg += cmv[u][t] * st;
+                        if (LW) pf += fp0 * fe0 * q[u][t][0] + fp1 * fe0 * q[u][t][1] + fp0 * fe1 * q[u][t][2] +
+                                      fp1 * fe1 * q[u][t][3];
+                        else kv += fe0 * kr[u][t][0] + fe1 * kr[u][t][1];
+                    }
+                    tg += tm[u];
+                    if (row_mask && !row_mask[(size_t)d * nlay + l]) continue;
+                    const size_t o = ((size_t)d * nlay + l) * ngpt + g;
+                    tau[o] = tg;
+                    aux[o] = LW ? pf : kv * s.cray[j];
                 }
             }
         }
-        __syncthreads();
     }
 }
 
@@ -199,12 +254,13 @@ void launch_gas_optics(const TablesView &T, int ncols, int nlay, int ncol_ld, co
                        const VmrPtrs &vmr, float *tau, float *aux, int *err, hipStream_t st,
                        const unsigned char *row_mask, const unsigned char *col_mask)
 {
-    if (ncols <= 0) return;
+    if (ncols <= 0 || nlay <= 0) return;
+    const int nchunk = (nlay + LCHUNK - 1) / LCHUNK;
     if (T.is_lw)
-        hipLaunchKernelGGL(gas_optics_kernel<true>, dim3(ncols), dim3(128), 0, st, T, nlay, ncol_ld,
+        hipLaunchKernelGGL(gas_optics_kernel<true>, dim3(ncols * nchunk), dim3(128), 0, st, T, nlay, ncol_ld,
                            col_idx, col0, p_lay, p_lev, t_lay, vmr, tau, aux, err, row_mask, col_mask);
     else
-        hipLaunchKernelGGL(gas_optics_kernel<false>, dim3(ncols), dim3(128), 0, st, T, nlay, ncol_ld,
+        hipLaunchKernelGGL(gas_optics_kernel<false>, dim3(ncols * nchunk), dim3(128), 0, st, T, nlay, ncol_ld,
                            col_idx, col0, p_lay, p_lev, t_lay, vmr, tau, aux, err, row_mask, col_mask);
 }
 
