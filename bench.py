@@ -42,6 +42,9 @@ def parse():
     p.add_argument("--ncol", type=int, default=NCOL)
     p.add_argument("--cpu-sample", type=int, default=32768)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--mlp-weights", default=None,
+                   help="trained emulator weights (.npz); default: the packaged ones")
+    p.add_argument("--no-nn", action="store_true", help="skip the learned LW gas-optics leg")
     p.add_argument("--copy-probe", action="store_true",
                    help="after the bench, run one 1 GiB device copy: the reference dispatch that scales "
                         "FETCH_SIZE / WRITE_SIZE counter readings to bytes")
@@ -149,6 +152,75 @@ def main():
             "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_launch, "columns_per_launch": chunk,
             "kernel_ms_per_step": {k: v[0] for k, v in kt.items()}}
 
+    # ---- learned LW gas optics: the same step with the MLP emulator in place
+    # of the LW table lookup (out-of-hull layers fall back to the table)
+    nn = None
+    nn_out = None
+    if not a.no_nn:
+        from ccpprad import mlp
+        wpath = a.mlp_weights or mlp.DEFAULT_WEIGHTS
+        if os.path.exists(wpath):
+            emu = mlp.LWEmulator.load(wpath)
+            lw_nn = lib.LWScheme(kd_lw, lut_lw, NLAY, blksz=blksz)
+            emu.attach(lw_nn)
+            nn_out = device.lw_outputs(ncol, NLAY)
+            nn_args = dict(dev)
+            nn_args.update(nn_out)
+
+            def step_nn(validate=False):
+                lw_nn.run_dev(lw_ints, nn_args, stream=sp, validate=validate)
+                sw.run_dev(sw_ints, sw_args, stream=sp, validate=validate)
+
+            step_nn(validate=True)
+            for _ in range(a.warmup):
+                step_nn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(a.steps):
+                step_nn()
+            torch.cuda.synchronize()
+            ms_nn = (time.perf_counter() - t0) * 1e3 / a.steps
+            # LW alone, table vs NN
+            def lw_only(s, args):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(a.steps):
+                    s.run_dev(lw_ints, args, stream=sp, validate=False)
+                torch.cuda.synchronize()
+                return (time.perf_counter() - t0) * 1e3 / a.steps
+            ms_lw_table, ms_lw_nn = lw_only(lw, lw_args), lw_only(lw_nn, nn_args)
+            L.ccpprad_timing_enable(lw_nn.ctx, 1)
+            lw_nn.run_dev(lw_ints, nn_args, stream=sp, validate=False)
+            nkt = {}
+            for kid, nm in {0: "gas_optics_fallback", 1: "lw_solve", 3: "cloud_optics", 4: "mcica", 5: "mlp_infer"}.items():
+                tot, cnt = C.c_double(), C.c_int()
+                L.ccpprad_timing_query(lw_nn.ctx, kid, C.byref(tot), C.byref(cnt))
+                if cnt.value:
+                    nkt[nm] = (tot.value, cnt.value)
+            L.ccpprad_timing_enable(lw_nn.ctx, 0)
+            # MFMA roofline of the fused inference kernel: 2 x (14x128 + 128x128
+            # + 128x128 + 128x256) FLOP per layer row (layer 1 takes the inputs
+            # as hi/lo BF16 pairs, K = 14)
+            flop_row = 2 * (14 * 128 + 128 * 128 + 128 * 128 + 128 * 256)
+            mt, mc = nkt.get("mlp_infer", (0.0, 0))
+            mlp_avg = mt / mc if mc else None
+            mlp_tf = flop_row * chunk * NLAY / (mlp_avg / 1e3) / 1e12 if mlp_avg else None
+            # rows outside the training hull (table fallback), from the
+            # features of the distinct columns
+            F = mlp.features(pr)
+            lo, hi = emu.norm["lo"].astype(np.float64), emu.norm["hi"].astype(np.float64)
+            out_row = ((F < lo) | (F > hi)).any(axis=1)
+            nn = {"value": ncol / (ms_nn / 1e3), "unit": "columns/s", "ms_per_step": ms_nn,
+                  "out_of_hull": {"row_fraction": float(out_row.mean()),
+                                  "column_fraction": float(out_row.reshape(-1, NLAY).any(axis=1).mean())},
+                  "lw_only_columns_per_s": {"table": ncol / (ms_lw_table / 1e3), "nn": ncol / (ms_lw_nn / 1e3)},
+                  "roofline": {"kernel": "mlp_infer", "bound": "mfma", "achieved": mlp_tf, "peak": 2500.0,
+                               "unit": "TFLOP/s", "frac": (mlp_tf / 2500.0) if mlp_tf else None,
+                               "avg_launch_ms": mlp_avg, "flop_per_launch": flop_row * chunk * NLAY},
+                  "kernel_ms_per_step": {k: v[0] for k, v in nkt.items()},
+                  "weights": os.path.relpath(wpath, REPO)}
+            emu.close()
+
     # ---- CPU baseline: FP64 oracle on a bounded sample, and GPU error on it
     cpu = None
     acc = None
@@ -180,6 +252,14 @@ def main():
             d = x - y
             return float(np.abs(d).max()), float(np.sqrt(np.mean(d * d)))
 
+        if nn_out is not None:
+            gl = {k: v[..., :ns].cpu().numpy() for k, v in
+                  {"u": nn_out["fluxlwUP_allsky"], "d": nn_out["fluxlwDOWN_allsky"], "h": nn_out["htrlw"]}.items()}
+            hu = e(gl["h"] * 86400, lw_ref["htrlw"] * 86400)
+            nn["accuracy_vs_fp64_oracle"] = {
+                "lw_flux_maxabs_Wm2": max(e(gl["u"], lw_ref["fluxlwUP_allsky"])[0], e(gl["d"], lw_ref["fluxlwDOWN_allsky"])[0]),
+                "lw_flux_rmse_Wm2": max(e(gl["u"], lw_ref["fluxlwUP_allsky"])[1], e(gl["d"], lw_ref["fluxlwDOWN_allsky"])[1]),
+                "lw_hr_maxabs_Kday": hu[0], "lw_hr_rmse_Kday": hu[1], "columns": ns}
         acc = {
             "lw_flux_maxabs_Wm2": max(e(g["lwu"], lw_ref["fluxlwUP_allsky"])[0], e(g["lwd"], lw_ref["fluxlwDOWN_allsky"])[0]),
             "sw_flux_maxabs_Wm2": max(e(g["swu"], sw_ref["fluxswUP_allsky"])[0], e(g["swd"], sw_ref["fluxswDOWN_allsky"])[0]),
@@ -197,7 +277,7 @@ def main():
                                f"(LW 16 bands/128 g, 3 Gauss angles; SW 14 bands/112 g; McICA max-random; "
                                f"FP32 optics/transport, FP64 reductions and heating rate)",
                    "ncol": ncol, "nlay": NLAY, "path": "table", "blksz": blksz},
-        "roofline": roof, "cpu_baseline": cpu, "accuracy_vs_fp64_oracle": acc,
+        "roofline": roof, "cpu_baseline": cpu, "accuracy_vs_fp64_oracle": acc, "nn": nn,
     }
     print(json.dumps(line), flush=True)
     if a.copy_probe:

@@ -195,7 +195,11 @@ def make_training_set(kd, nsamples=4_000_000, nlay=127, seed=20261204, torch=Non
     return X, Y
 
 
-def fit_norm(X, Y, margin=0.0):
+def fit_norm(X, Y, margin=1e-3):
+    """Standardisation of features/targets and the training hull: per-feature
+    min/max widened by margin x span, which absorbs the FP32 rounding of the
+    bounds and of the on-device feature computation (a feature pinned at its
+    extreme, like water vapour at its 1e-6 floor aloft, stays inside)."""
     xm = X.mean(0)
     xs = X.std(0).clamp_min(1e-6)
     ym = Y.mean(0)

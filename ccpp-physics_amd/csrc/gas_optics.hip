@@ -61,6 +61,12 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
     const int gc = col_idx ? col_idx[col0 + d] - 1 : col0 + d;  // global column
     const int tid = threadIdx.x;
     if (col_mask && !col_mask[d]) return;   // uniform per block
+    if (row_mask) {
+        // fallback launch: only chunks holding a flagged layer do any work
+        bool any = false;
+        for (int l = chunk * LCHUNK; l < min(nlay, (chunk + 1) * LCHUNK); ++l) any |= row_mask[(size_t)d * nlay + l] != 0;
+        if (!any) return;                    // uniform per block
+    }
     const int ngpt = T.ngpt, neta = T.neta, np1 = T.npres + 1, nflav = T.nflav;
     const double tmin = T.temp_ref_min, dT = T.temp_ref_delta;
 
