@@ -32,6 +32,12 @@ NCOL = 262_144
 NLAY = 127
 UNIQUE = 32_768        # distinct synthetic columns, tiled to NCOL on the device
 METRIC = "LW+SW columns/s at 127 layers (table vs NN) @ HR RMSE<=0.1 K/day vs FP64"
+# HBM bytes per 65,536-column launch measured with rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE (separate runs of `bench.py --steps 1 --warmup 0 --no-cpu
+# --no-nn --copy-probe`), each scaled by the reading of a 1 GiB device copy
+# (FETCH x 2.0, WRITE x 1.0).  Counters cannot be read from inside this
+# process, so the last measurement of the current kernels is recorded here.
+TRAFFIC_GB = {"sw_sw_solve": 28.077, "lw_lw_solve": 21.987, "lw_gas_optics": 13.363, "sw_gas_optics": 12.372}
 
 
 def parse():
@@ -148,7 +154,9 @@ def main():
     bytes_launch = per_col.get(dname, 0) * chunk
     achieved = bytes_launch / (avg_ms / 1e3) / 1e9 if bytes_launch else None
     roof = {"kernel": dname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": TRAFFIC_GB[dname] * 1e9 if dname in TRAFFIC_GB and chunk == 65536 else None,
+            "traffic_unit": "bytes per launch",
             "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_launch, "columns_per_launch": chunk,
             "kernel_ms_per_step": {k: v[0] for k, v in kt.items()}}
 
