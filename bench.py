@@ -134,19 +134,20 @@ def main():
             if cnt.value:
                 kt[f"{tag}_{nm}"] = (tot.value, cnt.value)
         L.ccpprad_timing_enable(s.ctx, 0)
-    # algorithmic bytes per launch (column chunk of 65,536 at most)
-    G_LW, G_SW, B_LW, B_SW, V = kd_lw.ngpt, kd_sw.ngpt, kd_lw.nband, kd_sw.nband, NLAY + 1
+    # Algorithmic bytes per column of each stage: the compulsory I/O of the
+    # reference's own data flow (every array it materialises, counted once;
+    # FP32 optics, FP64 fluxes / heating rate), at 127 layers, LW 128 g /
+    # 16 bands, SW 112 g / 14 bands.  The solvers include the fused
+    # heating-rate stage (3,072 B in, 1,016 B out).  One launch processes a
+    # chunk of 65,536 columns at most.
     chunk = min(ncol, 65536)
     per_col = {
-        # tau + pfrac read once (FP32), cloud absorption depth per band, mask
-        # bits, T/p inputs (FP64), fluxes x4 + HR out (FP64)
-        "lw_lw_solve": NLAY * G_LW * 8 + NLAY * B_LW * 4 + NLAY * 16 + (NLAY + 2 * V + 2) * 8 + (4 * V + NLAY) * 8,
-        # tau_abs + tau_ray (FP32), cloud tau/ssa/g per band, mask bits,
-        # fluxes + HR out
-        "sw_sw_solve": NLAY * G_SW * 8 + NLAY * B_SW * 12 + NLAY * 16 + (5 + V) * 8 + (4 * V + NLAY) * 8,
-        # p, T, 6 vmr, p_lev inputs (FP64) -> tau + pfrac (FP32)
-        "lw_gas_optics": (8 * NLAY + V) * 8 + NLAY * G_LW * 8,
-        "sw_gas_optics": (8 * NLAY + V) * 8 + NLAY * G_SW * 8,
+        "lw_lw_solve": 269_308 + 2_048 + 3_072 + 1_016,
+        "sw_sw_solve": 193_000 + 3_120 + 3_072 + 1_016,
+        "lw_gas_optics": 4_584 + 261_120,
+        "sw_gas_optics": 4_576 + 171_136,
+        "lw_cloud_optics": 2_540 + 8_128,
+        "sw_cloud_optics": 2_540 + 21_336,
     }
     dom = max(kt.items(), key=lambda kv: kv[1][0])
     dname, (dtot, dcnt) = dom
@@ -157,7 +158,8 @@ def main():
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": TRAFFIC_GB[dname] * 1e9 if dname in TRAFFIC_GB and chunk == 65536 else None,
             "traffic_unit": "bytes per launch",
-            "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_launch, "columns_per_launch": chunk,
+            "avg_launch_ms": avg_ms, "bytes_per_column": per_col.get(dname), "bytes_per_launch": bytes_launch,
+            "columns_per_launch": chunk,
             "kernel_ms_per_step": {k: v[0] for k, v in kt.items()}}
 
     # ---- learned LW gas optics: the same step with the MLP emulator in place
