@@ -30,9 +30,10 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(HIP_LIB):
-        raise RuntimeError(f"{HIP_LIB} is not built: run __graft_entry__.build()")
-    L = C.CDLL(HIP_LIB)
+    path = os.environ.get("CCPPRAD_LIB", HIP_LIB)   # an alternative build of the same library
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run __graft_entry__.build()")
+    L = C.CDLL(path)
     vp = C.c_void_p
     L.ccpprad_lw_init.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp), C.c_char_p, C.c_int]
     L.ccpprad_sw_init.argtypes = L.ccpprad_lw_init.argtypes

@@ -11,13 +11,14 @@
 // One block = one daylit column, blockDim 128, lane = g-point.  Three passes
 // per lane, with no per-layer scratch in HBM:
 //   0 top-down : direct beam F_dir by products of T_noscat (as the reference),
-//                kept at segment tops (LDS)
+//                kept at segment tops (LDS) for pass 1
 //   1 bottom-up: layer two-stream coefficients, direct-beam sources and
 //                the adding albedo A and source Q; A, Q kept at segment
 //                boundaries (LDS)
-//   2 top-down : per segment, the layer terms and A, Q below each layer are
-//                rebuilt in registers from the checkpoint at the segment
-//                bottom; diffuse down/up fluxes; F_dn += F_dir
+//   2 top-down : per segment, the direct beam (carried down from the segment
+//                above), the layer terms and A, Q below each layer are rebuilt
+//                in registers from the checkpoint at the segment bottom;
+//                diffuse down/up fluxes; F_dn += F_dir
 // Optical properties carry the co-albedo (1 - ssa) next to ssa, so the
 // absorption of nearly conservative layers is not lost to FP32 rounding.
 #include "kernels.h"
@@ -34,53 +35,37 @@ struct TwoStream {
     float Rd, Td, Rdir, Tdir;
 };
 
-// sw_two_stream for one layer, FP64 form: used where 1 - (k mu0)^2 is small,
-// the removable singularity of the direct-beam terms.  One expm1 per
-// exponent gives both e and 1 - e without cancellation.
-__device__ __forceinline__ TwoStream two_stream64(double mu0, const Opt2s &o)
+// (e^{-a} - e^{-b}) / d for b - a = d * s, s > 0, without cancellation:
+// e^{-a} s phi(d s) with phi(x) = (1 - e^{-x}) / x from its series where
+// |d s| < 0.25 (truncation below 2e-9 relative), the direct difference
+// (whose terms then differ by a factor e^{0.25} or more) elsewhere.
+__device__ __forceinline__ float exp_diff_ratio(float a, float b, float d, float s)
 {
-    const double tau = o.t, w = o.w, p = o.p, g = o.g;
-    const double g1 = (8.0 - w * (5.0 + 3.0 * g)) * 0.25;
-    const double g2 = 3.0 * (w * (1.0 - g)) * 0.25;
-    const double g3 = (2.0 - 3.0 * mu0 * g) * 0.25;
-    const double g4 = 1.0 - g3;
-    const double a1 = g1 * g4 + g2 * g3;
-    const double a2 = g1 * g3 + g2 * g4;
-    const double k = sqrt(fmax(2.0 * p * (g1 + g2), 1.0e-12));
-    const double E1 = -expm1(-tau * k);
-    const double e = 1.0 - E1;
-    const double E2 = E1 * (2.0 - E1);
-    const double RT = 1.0 / (k * (1.0 + e * e) + g1 * E2);
-    const double F = -expm1(-tau / mu0);
-    const double T0 = 1.0 - F;
-    const double kmu = k * mu0, kg3 = k * g3, kg4 = k * g4;
-    const double dd = 1.0 - kmu * kmu;
-    const double RT2 = w * RT / (fabs(dd) >= 2.220446049250313e-16 ? dd : 2.220446049250313e-16);
-    double Rdir = RT2 * ((1.0 - kmu) * (a2 + kg3) - (1.0 + kmu) * (a2 - kg3) * e * e - 2.0 * (kg3 - a2 * kmu) * e * T0);
-    double Tdir = -RT2 * ((1.0 + kmu) * (a1 + kg4) * T0 - (1.0 - kmu) * (a1 - kg4) * e * e * T0 - 2.0 * (kg4 + a1 * kmu) * e);
-    Rdir = fmax(0.0, fmin(Rdir, F));
-    Tdir = fmax(0.0, fmin(Tdir, F - Rdir));
-    TwoStream r;
-    r.Rd = (float)(RT * g2 * E2);
-    r.Td = (float)(RT * 2.0 * k * e);
-    r.Rdir = (float)Rdir;
-    r.Tdir = (float)Tdir;
-    return r;
+    const float x = d * s;
+    const float ea = exp_neg(a);
+    const float phi = 1.0f - x * (0.5f - x * (1.0f / 6.0f - x * (1.0f / 24.0f - x * (1.0f / 120.0f -
+                      x * (1.0f / 720.0f - x * (1.0f / 5040.0f))))));
+    return fabsf(x) < 0.25f ? ea * s * phi : (ea - exp_neg(b)) * frcp(d);
 }
 
 // sw_two_stream for one layer in FP32 (Meador & Weaver 1980, Zdunkowski
 // coefficients).  The reference's direct-beam brackets are sums of O(1) terms
-// that cancel to O(k tau) under a 1/k prefactor.  With E1 = 1 - e^{-k tau},
-// E2 = 1 - e^{-2 k tau} and F = 1 - e^{-tau/mu0} (each from expm1, accurate to
-// a few ulp) the O(1) terms cancel exactly in the algebra:
-//   Rdir: (1-k mu)(a2+k g3) - (1+k mu)(a2-k g3) e^2 - 2 (k g3 - a2 k mu) e T0
-//       = (1+k mu)(a2-k g3) E2 + 2 k (g3 - a2 mu)(E1 + F - E1 F)
-//   Tdir: (1+k mu)(a1+k g4) T0 - (1-k mu)(a1-k g4) e^2 T0 - 2 (k g4 + a1 k mu) e
-//       = 2 k (g4 + mu a1)(E1 - F) + (1-k mu)(a1-k g4) E2 (1 - F)
-// so both carry their factor k explicitly and keep FP32 relative accuracy.
-// gamma1 - gamma2 = 2 (1 - w) comes from the co-albedo p.  Near
-// k mu0 = 1 (|1 - (k mu0)^2| < 1e-2) the lane switches to the FP64 form.
-__device__ __forceinline__ TwoStream two_stream32(float mu0, float mu0_inv, double mu0d, const Opt2s &o)
+// that cancel to O(k tau) under a 1/k prefactor, with a removable singularity
+// at k mu0 = 1.  With E1 = 1 - e^{-k tau}, E2 = 1 - e^{-2 k tau},
+// F = 1 - e^{-tau/mu0} and G = 1 - e^{-(k + 1/mu0) tau} (each accurate to a
+// few ulp) the brackets are rewritten two ways, both exact in the algebra:
+//  * away from k mu0 = 1 the O(1) terms cancel explicitly,
+//      Rdir: (1+k mu)(a2-k g3) E2 + 2 k (g3 - a2 mu) G
+//      Tdir: 2 k (g4 + mu a1)(E1 - F) + (1-k mu)(a1-k g4) E2 (1 - F)
+//    so both carry their factor k and keep FP32 relative accuracy;
+//  * within 1 % of k mu0 = 1 the factor delta = 1 - k mu0 is divided out of
+//    both brackets and of 1 - (k mu0)^2, with
+//      T_R = (e^{-2k tau} - e^{-(k+1/mu0) tau}) / delta,
+//      T_T = (e^{-k tau} - e^{-tau/mu0}) / delta,
+//    Rdir = w RT / (1+k mu) [a2 (2G - E2 - 2 T_R) + k g3 (E2 + 2 T_R)]
+//    Tdir = -w RT / (1+k mu) [(a1 - k g4) E2 (1 - F) - 2 k (g4 + mu a1) T_T].
+// gamma1 - gamma2 = 2 (1 - w) comes from the co-albedo p.
+__device__ __forceinline__ TwoStream two_stream32(float mu0, float mu0_inv, const Opt2s &o)
 {
     const float tau = o.t, w = o.w, p = o.p, g = o.g;
     const float g1 = (8.0f - w * (5.0f + 3.0f * g)) * 0.25f;
@@ -92,18 +77,29 @@ __device__ __forceinline__ TwoStream two_stream32(float mu0, float mu0_inv, doub
     const float k = __builtin_amdgcn_sqrtf(fmaxf(2.0f * p * (g1 + g2), 1.0e-12f));
     const float kmu = k * mu0;
     const float dd = 1.0f - kmu * kmu;
-    if (fabsf(dd) < 1.0e-2f) return two_stream64(mu0d, o);
+    const float s = tau * mu0_inv;
     float e, E1, T0, F;
     exp_pair(tau * k, e, E1);
-    exp_pair(tau * mu0_inv, T0, F);
+    exp_pair(s, T0, F);
     const float E2 = E1 * (2.0f - E1);
+    const float G = E1 + F - E1 * F;
     const float RT = frcp(k * (1.0f + e * e) + g1 * E2);
-    const float RT2 = w * RT * frcp(dd);
-    const float BR = (1.0f + kmu) * (a2 - k * g3) * E2 + 2.0f * k * (g3 - a2 * mu0) * (E1 + F - E1 * F);
-    const float BT = 2.0f * k * (g4 + mu0 * a1) * (E1 - F) + (1.0f - kmu) * (a1 - k * g4) * E2 * (1.0f - F);
+    float Rdir, Tdir;
+    if (fabsf(dd) >= 1.0e-2f) {
+        const float RT2 = w * RT * frcp(dd);
+        Rdir = RT2 * ((1.0f + kmu) * (a2 - k * g3) * E2 + 2.0f * k * (g3 - a2 * mu0) * G);
+        Tdir = -RT2 * (2.0f * k * (g4 + mu0 * a1) * (E1 - F) + (1.0f - kmu) * (a1 - k * g4) * E2 * (1.0f - F));
+    } else {
+        const float delta = 1.0f - kmu;
+        const float TR = exp_diff_ratio(2.0f * k * tau, k * tau + s, delta, s);
+        const float TT = exp_diff_ratio(k * tau, s, delta, s);
+        const float pre = w * RT * frcp(1.0f + kmu);
+        Rdir = pre * (a2 * (2.0f * G - E2 - 2.0f * TR) + k * g3 * (E2 + 2.0f * TR));
+        Tdir = -pre * ((a1 - k * g4) * E2 * (1.0f - F) - 2.0f * k * (g4 + mu0 * a1) * TT);
+    }
     TwoStream r;
-    r.Rdir = fmaxf(0.0f, fminf(RT2 * BR, F));
-    r.Tdir = fmaxf(0.0f, fminf(-RT2 * BT, F - r.Rdir));
+    r.Rdir = fmaxf(0.0f, fminf(Rdir, F));
+    r.Tdir = fmaxf(0.0f, fminf(Tdir, F - r.Rdir));
     r.Rd = RT * g2 * E2;
     r.Td = RT * 2.0f * k * e;
     return r;
@@ -118,14 +114,16 @@ constexpr int SW_SEG = 8;
 size_t sw_solve_smem(int nlay, int nband)
 {
     const int nseg = (nlay + SW_SEG - 1) / SW_SEG;
-    return sizeof(double) * (4 * (size_t)(nlay + 1) + 6 * (size_t)nband) + sizeof(float) * 3 * (size_t)(nseg + 1) * 128;
+    return sizeof(double) * (4 * (size_t)(nlay + 1) + 6 * (size_t)nband) + sizeof(float) * 2 * (size_t)(nseg + 1) * 128;
 }
 
 template <bool CLOUDS>
-__global__ __launch_bounds__(128) void sw_solve_kernel(TablesView T, SwSolveArgs A)
+__global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveArgs A)
 {
     // LDS: [2 waves][2][nlev] level sums, [2 waves][3][nband] band partials,
-    // then per lane [3][nseg + 1] checkpoints (direct beam, albedo, source)
+    // then per lane [2][nseg + 1] checkpoints: slot X holds the direct beam at
+    // a segment top until pass 1 has read it, then the adding albedo there;
+    // slot Q holds the adding source
     extern __shared__ double red[];
     const int d = blockIdx.x;
     const int od = A.col0 + d;
@@ -141,8 +139,8 @@ __global__ __launch_bounds__(128) void sw_solve_kernel(TablesView T, SwSolveArgs
     const int lev_toa = top ? 0 : nlay;
     double *rup = red + (size_t)wave * 2 * nlev, *rdn = rup + nlev;
     double *rb = red + 4 * (size_t)nlev;  // [wave][3][nband]
-    float *cF = (float *)(red + 4 * (size_t)nlev + 6 * (size_t)nb);
-    float *cA = cF + (size_t)(nseg + 1) * 128, *cQ = cA + (size_t)(nseg + 1) * 128;
+    float *cX = (float *)(red + 4 * (size_t)nlev + 6 * (size_t)nb);
+    float *cQ = cX + (size_t)(nseg + 1) * 128;
 
     const double mu0d = A.coszen[c];
     const float mu0 = (float)mu0d;
@@ -213,24 +211,23 @@ __global__ __launch_bounds__(128) void sw_solve_kernel(TablesView T, SwSolveArgs
     // reference), kept at every segment top and at the surface
     float fdir = toa * mu0;
     for (int s = 0; s < nseg; ++s) {
-        cF[(size_t)s * 128 + g] = fdir;
+        cX[(size_t)s * 128 + g] = fdir;
 #pragma unroll
         for (int j = 0; j < SW_SEG; ++j) {
             const int k = s * SW_SEG + j;
             if (k < nlay) fdir = exp_neg(depth(lay(k)) * mu0_inv) * fdir;
         }
     }
-    cF[(size_t)nseg * 128 + g] = fdir;
 
     // ---- pass 1, bottom-up: adding albedo A and source Q, kept at segment
     // boundaries
     float Ab = adif, Qb = fdir * adir;
-    cA[(size_t)nseg * 128 + g] = Ab;
+    cX[(size_t)nseg * 128 + g] = Ab;
     cQ[(size_t)nseg * 128 + g] = Qb;
     for (int s = nseg - 1; s >= 0; --s) {
         Opt2s o[SW_SEG];
         float f[SW_SEG + 1];
-        f[0] = cF[(size_t)s * 128 + g];
+        f[0] = cX[(size_t)s * 128 + g];
 #pragma unroll
         for (int j = 0; j < SW_SEG; ++j) {
             const int k = s * SW_SEG + j;
@@ -240,14 +237,14 @@ __global__ __launch_bounds__(128) void sw_solve_kernel(TablesView T, SwSolveArgs
         for (int j = SW_SEG - 1; j >= 0; --j) {
             const int k = s * SW_SEG + j;
             if (k >= nlay) continue;
-            const TwoStream ts = two_stream32(mu0, mu0_inv, mu0d, o[j]);
+            const TwoStream ts = two_stream32(mu0, mu0_inv, o[j]);
             const float su = ts.Rdir * f[j], sd = ts.Tdir * f[j];
             const float den = frcp(1.0f - ts.Rd * Ab);
             const float Aa = ts.Rd + ts.Td * ts.Td * Ab * den;
             const float Qa = su + ts.Td * den * (Qb + Ab * sd);
             Ab = Aa; Qb = Qa;
         }
-        cA[(size_t)s * 128 + g] = Ab;
+        cX[(size_t)s * 128 + g] = Ab;
         cQ[(size_t)s * 128 + g] = Qb;
     }
 
@@ -255,8 +252,9 @@ __global__ __launch_bounds__(128) void sw_solve_kernel(TablesView T, SwSolveArgs
     // albedo / source below each layer are rebuilt from the checkpoint at the
     // segment bottom.
     float dnd = 0.0f;  // diffuse down at TOA
+    float ftop = toa * mu0;  // direct beam at the current segment top
     {
-        const float up = dnd * cA[g] + cQ[g];
+        const float up = dnd * cX[g] + cQ[g];
         const double su = wave_sum(act ? (double)up : 0.0);
         const double sdn = wave_sum(act ? (double)(dnd + toa * mu0) : 0.0);
         if (lane == 0) { rup[lev_toa] = su; rdn[lev_toa] = sdn; }
@@ -265,18 +263,19 @@ __global__ __launch_bounds__(128) void sw_solve_kernel(TablesView T, SwSolveArgs
         Opt2s o[SW_SEG];
         float f[SW_SEG + 1];
         float Rd[SW_SEG], Td[SW_SEG], sdv[SW_SEG], Abl[SW_SEG], Qbl[SW_SEG], dnv[SW_SEG];
-        f[0] = cF[(size_t)s * 128 + g];
+        f[0] = ftop;
 #pragma unroll
         for (int j = 0; j < SW_SEG; ++j) {
             const int k = s * SW_SEG + j;
             if (k < nlay) { o[j] = optics(lay(k)); f[j + 1] = exp_neg(o[j].t * mu0_inv) * f[j]; }
         }
-        float a = cA[(size_t)(s + 1) * 128 + g], q = cQ[(size_t)(s + 1) * 128 + g];
+        ftop = f[SW_SEG];
+        float a = cX[(size_t)(s + 1) * 128 + g], q = cQ[(size_t)(s + 1) * 128 + g];
 #pragma unroll
         for (int j = SW_SEG - 1; j >= 0; --j) {
             const int k = s * SW_SEG + j;
             if (k >= nlay) continue;
-            const TwoStream ts = two_stream32(mu0, mu0_inv, mu0d, o[j]);
+            const TwoStream ts = two_stream32(mu0, mu0_inv, o[j]);
             const float su = ts.Rdir * f[j], sd = ts.Tdir * f[j];
             const float den = frcp(1.0f - ts.Rd * a);
             Rd[j] = ts.Rd; Td[j] = ts.Td; sdv[j] = sd; Abl[j] = a; Qbl[j] = q; dnv[j] = den;
