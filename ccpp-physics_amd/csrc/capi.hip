@@ -222,6 +222,7 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
     if (ch && atoi(ch) > 0) c->chunk = atoi(ch);
     hipError_t e = hipSuccess;
     TablesView &T = c->T;
+    T.maxminor_g = 0;
     T.is_lw = want_lw; T.ngas = kd->ngas; T.nband = kd->nband; T.ngpt = kd->ngpt; T.ntemp = kd->ntemp;
     T.npres = kd->npres; T.neta = kd->neta; T.nflav = kd->nflav; T.nplanck = kd->nplanck;
     T.press_ref_log1 = log(kd->press_ref[0]);
@@ -306,6 +307,7 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
         }
         std::vector<int> ptr(kd->ngpt + 1, 0), items;
         for (int g = 0; g < kd->ngpt; ++g) {
+            T.maxminor_g = std::max(T.maxminor_g, (int)per[g].size());
             if ((int)per[g].size() > MAX_MINOR_G) {
                 delete c;
                 seterr(errmsg, errlen, "g-point %d has %d minor absorbers (limit %d)", g + 1, (int)per[g].size(), MAX_MINOR_G);

@@ -55,6 +55,7 @@ struct TablesView {
     const int *band_lims;      // [nband][2] 0-based first, last
     int nminor[2];
     int ncontrib[2];
+    int maxminor_g;            // most minor intervals covering one g-point in either regime
     const int *minor_par[2];   // [n][5]: idx_minor, density, idx_scaling, complement, flavor(0-based)
     const int *minor_gptr[2];  // [ngpt+1] CSR offsets into minor_items
     const int *minor_items[2]; // [..][2]: interval, contributor index (0-based)

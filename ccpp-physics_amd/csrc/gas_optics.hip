@@ -34,7 +34,7 @@ struct GasOpticsSmem {
     float cm[LCHUNK][MAX_FLAV][2];
     int je[LCHUNK][MAX_FLAV][2];
     float fm[LCHUNK][MAX_FLAV][2][2];
-    float ms[LCHUNK][MAX_MINOR];
+    float ms[LCHUNK][MAX_MINOR + 1];   // [MAX_MINOR] = 0: scale of unused item slots
 };
 
 // A minor absorber item of one g-point: contributor column (16 bits),
@@ -43,7 +43,13 @@ __device__ __forceinline__ int mi_ci(unsigned v) { return (int)(v & 0xFFFFu); }
 __device__ __forceinline__ int mi_iv(unsigned v) { return (int)((v >> 16) & 0xFFu); }
 __device__ __forceinline__ int mi_fv(unsigned v) { return (int)(v >> 24); }
 
-template <bool LW>
+// Gathers through a uniform table base and a 32-bit byte offset.
+__device__ __forceinline__ float ldt(const float *base, unsigned idx)
+{
+    return *(const float *)((const char *)base + (idx << 2));
+}
+
+template <bool LW, int NM>
 __global__ __launch_bounds__(128) void gas_optics_kernel(
     TablesView T, int nlay, int ncol_ld, const int *__restrict__ col_idx, int col0,
     const double *__restrict__ p_lay, const double *__restrict__ p_lev,
@@ -75,22 +81,25 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
     const bool act = tid < ngpt;
     const int g = act ? tid : 0;
     const int gfl0 = T.gpt_flav[2 * g], gfl1 = T.gpt_flav[2 * g + 1];
-    unsigned mit[2][MAX_MINOR_G];
-    int nmi[2];
+    // Per-lane minor items: NM slots per regime (NM = the table's most items
+    // covering one g-point); unused slots point at interval MAX_MINOR, whose
+    // scale is 0, and contributor 0, so every load is unconditional.
+    unsigned mit[2][NM > 0 ? NM : 1];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
-        const int i0 = T.minor_gptr[r][g], i1 = T.minor_gptr[r][g + 1];
-        nmi[r] = i1 - i0;
+        const int i0 = T.minor_gptr[r][g], n = T.minor_gptr[r][g + 1] - i0;
 #pragma unroll
-        for (int ii = 0; ii < MAX_MINOR_G; ++ii) {
-            unsigned v = 0;
-            if (ii < nmi[r]) {
+        for (int ii = 0; ii < NM; ++ii) {
+            unsigned v = (unsigned)MAX_MINOR << 16;
+            if (ii < n) {
                 const int iv = T.minor_items[r][2 * (i0 + ii)], ci = T.minor_items[r][2 * (i0 + ii) + 1];
                 v = (unsigned)ci | ((unsigned)iv << 16) | ((unsigned)T.minor_par[r][5 * iv + 4] << 24);
             }
             mit[r][ii] = v;
         }
     }
+    const float *km_r[2] = {T.kminor[0], T.kminor[1]};
+    const unsigned nc_r[2] = {(unsigned)T.ncontrib[0], (unsigned)T.ncontrib[1]};
     const unsigned dp_ = (unsigned)(neta * ngpt), de = (unsigned)ngpt;
 
     {
@@ -124,6 +133,7 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
             s.play[tid] = pl;
             s.tlay[tid] = tl;
             s.cray[tid] = (float)(s.col[tid][1] + col_dry);
+            s.ms[tid][MAX_MINOR] = 0.0f;
         }
         __syncthreads();
         // ---- phase A1: flavor interpolation items and minor scalings (FP64)
@@ -166,25 +176,22 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
             }
         }
         __syncthreads();
-        // ---- phase B: g-point gathers (FP32), two layers at a time: the major
-        // (and Planck-fraction / Rayleigh) table loads of both layers are
-        // issued before any arithmetic, then the minor items of both layers
+        // ---- phase B: g-point gathers (FP32), two layers at a time: every
+        // table load of both layers (major, Planck fraction or Rayleigh, minor)
+        // is issued before any arithmetic; layer-uniform indices are scalar
         if (act) {
             for (int j0 = 0; j0 < nl; j0 += 2) {
                 const bool two = j0 + 1 < nl;
-                float k[2][2][4], q[2][2][4], kr[2][2][2];
+                float k[2][2][4], q[2][2][4], kr[2][2][2], km[2][NM > 0 ? NM : 1][2][2];
                 float w[2][2][2], cmv[2][2], fp1v[2];
-                int regv[2], jt0v[2];
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
                     if (u == 1 && !two) break;
                     const int j = j0 + u;
-                    const int itr = s.itropo[j], reg = itr - 1;
-                    regv[u] = reg;
+                    const int itr = __builtin_amdgcn_readfirstlane(s.itropo[j]), reg = itr - 1;
                     const int f = reg == 0 ? gfl0 : gfl1;
-                    const unsigned jt0 = (unsigned)(s.jtemp[j] - 1);
-                    jt0v[u] = (int)jt0;
-                    const unsigned jp0 = (unsigned)(s.jpress[j] + itr - 2);  // 0-based lower pressure row
+                    const unsigned jt0 = (unsigned)__builtin_amdgcn_readfirstlane(s.jtemp[j] - 1);
+                    const unsigned jp0 = (unsigned)__builtin_amdgcn_readfirstlane(s.jpress[j] + itr - 2);
                     fp1v[u] = s.fpress[j];
 #pragma unroll
                     for (int t = 0; t < 2; ++t) {
@@ -193,38 +200,28 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
                         w[u][t][1] = s.fm[j][f][t][1];
                         cmv[u][t] = s.cm[j][f][t];
                         const unsigned base = (((jt0 + t) * np1 + jp0) * neta + je0) * ngpt + g;
-                        k[u][t][0] = T.kmajor[base]; k[u][t][1] = T.kmajor[base + dp_];
-                        k[u][t][2] = T.kmajor[base + de]; k[u][t][3] = T.kmajor[base + dp_ + de];
+                        k[u][t][0] = ldt(T.kmajor, base); k[u][t][1] = ldt(T.kmajor, base + dp_);
+                        k[u][t][2] = ldt(T.kmajor, base + de); k[u][t][3] = ldt(T.kmajor, base + dp_ + de);
                         if (LW) {
-                            q[u][t][0] = T.pfrac_tab[base]; q[u][t][1] = T.pfrac_tab[base + dp_];
-                            q[u][t][2] = T.pfrac_tab[base + de]; q[u][t][3] = T.pfrac_tab[base + dp_ + de];
+                            q[u][t][0] = ldt(T.pfrac_tab, base); q[u][t][1] = ldt(T.pfrac_tab, base + dp_);
+                            q[u][t][2] = ldt(T.pfrac_tab, base + de); q[u][t][3] = ldt(T.pfrac_tab, base + dp_ + de);
                         } else {
                             const unsigned b0 = ((jt0 + t) * neta + je0) * ngpt + g;
-                            kr[u][t][0] = T.krayl[reg][b0]; kr[u][t][1] = T.krayl[reg][b0 + ngpt];
+                            kr[u][t][0] = ldt(T.krayl[reg], b0); kr[u][t][1] = ldt(T.krayl[reg], b0 + ngpt);
                         }
                     }
-                }
-                // minor absorbers of each layer's regime covering g
-                float tm[2] = {0.0f, 0.0f};
+                    const unsigned nc = nc_r[reg];
+                    const float *kmt = km_r[reg];
 #pragma unroll
-                for (int ii = 0; ii < MAX_MINOR_G; ++ii) {
+                    for (int ii = 0; ii < NM; ++ii) {
+                        const unsigned v = reg == 0 ? mit[0][ii] : mit[1][ii];
+                        const int fv = mi_fv(v);
+                        const unsigned ci = (unsigned)mi_ci(v);
 #pragma unroll
-                    for (int u = 0; u < 2; ++u) {
-                        if (u == 1 && !two) break;
-                        const int reg = regv[u];
-                        if (ii < nmi[reg]) {
-                            const int j = j0 + u;
-                            const unsigned v = reg == 0 ? mit[0][ii] : mit[1][ii];
-                            const int fv = mi_fv(v), ci = mi_ci(v), nc = T.ncontrib[reg];
-                            const float *kmt = T.kminor[reg];
-                            float kv = 0.0f;
-#pragma unroll
-                            for (int t = 0; t < 2; ++t) {
-                                const int je0 = s.je[j][fv][t] - 1;
-                                const unsigned b0 = (((unsigned)jt0v[u] + t) * neta + je0) * nc + ci;
-                                kv += s.fm[j][fv][t][0] * kmt[b0] + s.fm[j][fv][t][1] * kmt[b0 + nc];
-                            }
-                            tm[u] += s.ms[j][mi_iv(v)] * kv;
+                        for (int t = 0; t < 2; ++t) {
+                            const unsigned b0 = ((jt0 + t) * neta + (unsigned)(s.je[j][fv][t] - 1)) * nc + ci;
+                            km[u][ii][t][0] = ldt(kmt, b0);
+                            km[u][ii][t][1] = ldt(kmt, b0 + nc);
                         }
                     }
                 }
@@ -232,6 +229,7 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
                 for (int u = 0; u < 2; ++u) {
                     if (u == 1 && !two) break;
                     const int j = j0 + u, l = l0 + j;
+                    const int reg = __builtin_amdgcn_readfirstlane(s.itropo[j]) - 1;
                     const float fp1 = fp1v[u], fp0 = 1.0f - fp1;
                     float tg = 0.0f, pf = 0.0f, kv = 0.0f;
 #pragma unroll
@@ -244,7 +242,17 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
                                       fp1 * fe1 * q[u][t][3];
                         else kv += fe0 * kr[u][t][0] + fe1 * kr[u][t][1];
                     }
-                    tg += tm[u];
+                    // minor absorbers of this layer's regime covering g
+#pragma unroll
+                    for (int ii = 0; ii < NM; ++ii) {
+                        const unsigned v = reg == 0 ? mit[0][ii] : mit[1][ii];
+                        const int fv = mi_fv(v);
+                        float kvm = 0.0f;
+#pragma unroll
+                        for (int t = 0; t < 2; ++t)
+                            kvm += s.fm[j][fv][t][0] * km[u][ii][t][0] + s.fm[j][fv][t][1] * km[u][ii][t][1];
+                        tg += s.ms[j][mi_iv(v)] * kvm;
+                    }
                     if (row_mask && !row_mask[(size_t)d * nlay + l]) continue;
                     const size_t o = ((size_t)d * nlay + l) * ngpt + g;
                     tau[o] = tg;
@@ -255,19 +263,37 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
     }
 }
 
+template <bool LW>
+static void launch_nm(const TablesView &T, int nblocks, int nlay, int ncol_ld, const int *col_idx, int col0,
+                      const double *p_lay, const double *p_lev, const double *t_lay, const VmrPtrs &vmr,
+                      float *tau, float *aux, int *err, hipStream_t st, const unsigned char *row_mask,
+                      const unsigned char *col_mask)
+{
+#define GO_LAUNCH(NMV)                                                                                        \
+    hipLaunchKernelGGL((gas_optics_kernel<LW, NMV>), dim3(nblocks), dim3(128), 0, st, T, nlay, ncol_ld, col_idx, \
+                       col0, p_lay, p_lev, t_lay, vmr, tau, aux, err, row_mask, col_mask)
+    const int nm = T.maxminor_g;
+    if (nm <= 0) GO_LAUNCH(0);
+    else if (nm == 1) GO_LAUNCH(1);
+    else if (nm == 2) GO_LAUNCH(2);
+    else if (nm <= 4) GO_LAUNCH(4);
+    else GO_LAUNCH(MAX_MINOR_G);
+#undef GO_LAUNCH
+}
+
 void launch_gas_optics(const TablesView &T, int ncols, int nlay, int ncol_ld, const int *col_idx,
                        int col0, const double *p_lay, const double *p_lev, const double *t_lay,
                        const VmrPtrs &vmr, float *tau, float *aux, int *err, hipStream_t st,
                        const unsigned char *row_mask, const unsigned char *col_mask)
 {
     if (ncols <= 0 || nlay <= 0) return;
-    const int nchunk = (nlay + LCHUNK - 1) / LCHUNK;
+    const int nblocks = ncols * ((nlay + LCHUNK - 1) / LCHUNK);
     if (T.is_lw)
-        hipLaunchKernelGGL(gas_optics_kernel<true>, dim3(ncols * nchunk), dim3(128), 0, st, T, nlay, ncol_ld,
-                           col_idx, col0, p_lay, p_lev, t_lay, vmr, tau, aux, err, row_mask, col_mask);
+        launch_nm<true>(T, nblocks, nlay, ncol_ld, col_idx, col0, p_lay, p_lev, t_lay, vmr, tau, aux, err, st,
+                        row_mask, col_mask);
     else
-        hipLaunchKernelGGL(gas_optics_kernel<false>, dim3(ncols * nchunk), dim3(128), 0, st, T, nlay, ncol_ld,
-                           col_idx, col0, p_lay, p_lev, t_lay, vmr, tau, aux, err, row_mask, col_mask);
+        launch_nm<false>(T, nblocks, nlay, ncol_ld, col_idx, col0, p_lay, p_lev, t_lay, vmr, tau, aux, err, st,
+                         row_mask, col_mask);
 }
 
 }  // namespace ccpprad
