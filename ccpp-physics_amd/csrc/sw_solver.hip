@@ -8,17 +8,15 @@
 // ty_fluxes_byband%reduce + the CCPP band sum (rrtmgp_sw_main.F90:558-559,617-618)
 // and compute_heating_rate on daylit columns (GFS_rrtmgp_post.F90:249-266).
 //
-// One block = one daylit column, blockDim 128, lane = g-point.  Three passes
-// per lane, with no per-layer scratch in HBM:
-//   0 top-down : direct beam F_dir by products of T_noscat (as the reference),
-//                kept at segment tops (LDS) for pass 1
-//   1 bottom-up: layer two-stream coefficients, direct-beam sources and
-//                the adding albedo A and source Q; A, Q kept at segment
-//                boundaries (LDS)
-//   2 top-down : per segment, the direct beam (carried down from the segment
-//                above), the layer terms and A, Q below each layer are rebuilt
-//                in registers from the checkpoint at the segment bottom;
-//                diffuse down/up fluxes; F_dn += F_dir
+// One block = one daylit column, blockDim 128, lane = g-point.  Two passes
+// per lane, with no per-layer scratch:
+//   1 bottom-up: layer two-stream coefficients and the adding albedo A and
+//                source q = Q / F_dir (normalised by the direct beam, so this
+//                pass needs none); A, q kept every 8 layers (LDS)
+//   2 top-down : direct beam F_dir by products of T_noscat (as the
+//                reference); per segment the layer terms and A, q below each
+//                layer rebuilt in registers from the checkpoint at the segment
+//                bottom; diffuse down/up fluxes; F_dn += F_dir
 // Optical properties carry the co-albedo (1 - ssa) next to ssa, so the
 // absorption of nearly conservative layers is not lost to FP32 rounding.
 #include "kernels.h"
@@ -32,7 +30,7 @@ struct Opt2s {
 };
 
 struct TwoStream {
-    float Rd, Td, Rdir, Tdir;
+    float Rd, Td, Rdir, Tdir, T0;   // T0 = exp(-tau/mu0), the direct transmittance
 };
 
 // (e^{-a} - e^{-b}) / d for b - a = d * s, s > 0, without cancellation:
@@ -102,6 +100,7 @@ __device__ __forceinline__ TwoStream two_stream32(float mu0, float mu0_inv, cons
     r.Tdir = fmaxf(0.0f, fminf(Tdir, F - r.Rdir));
     r.Rd = RT * g2 * E2;
     r.Td = RT * 2.0f * k * e;
+    r.T0 = T0;
     return r;
 }
 
@@ -121,9 +120,8 @@ template <bool CLOUDS>
 __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveArgs A)
 {
     // LDS: [2 waves][2][nlev] level sums, [2 waves][3][nband] band partials,
-    // then per lane [2][nseg + 1] checkpoints: slot X holds the direct beam at
-    // a segment top until pass 1 has read it, then the adding albedo there;
-    // slot Q holds the adding source
+    // then per lane [2][nseg + 1] checkpoints: the adding albedo A (slot X)
+    // and the normalised source q (slot Q) at every segment boundary
     extern __shared__ double red[];
     const int d = blockIdx.x;
     const int od = A.col0 + d;
@@ -191,107 +189,83 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
         o.g = tsg * frcp(fmaxf(EPS3, ts));
         return o;
     };
-    // the same total depth as optics(l).t, summed in the same order
-    auto depth = [&](int l) -> float {
-        const unsigned og = (unsigned)l * ngpt + gg;
-        float te = ta_c[og] + tr_c[og];
-        if (A.aer_tau) te = (float)A.aer_tau[(size_t)c + (size_t)ld * (l + (size_t)nlay * b)] + te;
-        if (cloudy) {
-            const unsigned mw = mk_c[(unsigned)l * MASK_WORDS + mword];
-            const float ctv = ct_c[(unsigned)l * nb + b];
-            te = ((mw & mbit) ? ctv : 0.0f) + te;
-        }
-        return te;
-    };
     auto lay = [&](int k) { return top ? k : nlay - 1 - k; };       // k counts layers from the TOA
     auto lev = [&](int k) { return top ? k : nlay - k; };           // level k is the top of layer k
     const float toa = T.toa_norm[gg];
 
-    // ---- pass 0, top-down: direct beam by products of T_noscat (as the
-    // reference), kept at every segment top and at the surface
-    float fdir = toa * mu0;
-    for (int s = 0; s < nseg; ++s) {
-        cX[(size_t)s * 128 + g] = fdir;
-#pragma unroll
-        for (int j = 0; j < SW_SEG; ++j) {
-            const int k = s * SW_SEG + j;
-            if (k < nlay) fdir = exp_neg(depth(lay(k)) * mu0_inv) * fdir;
-        }
-    }
-
-    // ---- pass 1, bottom-up: adding albedo A and source Q, kept at segment
-    // boundaries
-    float Ab = adif, Qb = fdir * adir;
+    // ---- pass 1, bottom-up: adding albedo A and the direct-beam source
+    // normalised by the direct beam at its level, q = Q / F_dir.  Q is linear
+    // in F_dir, so q follows a local recurrence,
+    //   q_k = Rdir_k + Td_k den_k (T0_k q_{k+1} + A_{k+1} Tdir_k),
+    // bounded by 1 (the upwelling a direct beam can feed), and this pass
+    // needs no direct beam.  A and q are kept at segment boundaries.
+    float Ab = adif, qb = adir;
     cX[(size_t)nseg * 128 + g] = Ab;
-    cQ[(size_t)nseg * 128 + g] = Qb;
+    cQ[(size_t)nseg * 128 + g] = qb;
     for (int s = nseg - 1; s >= 0; --s) {
         Opt2s o[SW_SEG];
-        float f[SW_SEG + 1];
-        f[0] = cX[(size_t)s * 128 + g];
 #pragma unroll
         for (int j = 0; j < SW_SEG; ++j) {
             const int k = s * SW_SEG + j;
-            if (k < nlay) { o[j] = optics(lay(k)); f[j + 1] = exp_neg(o[j].t * mu0_inv) * f[j]; }
+            if (k < nlay) o[j] = optics(lay(k));
         }
 #pragma unroll
         for (int j = SW_SEG - 1; j >= 0; --j) {
             const int k = s * SW_SEG + j;
             if (k >= nlay) continue;
             const TwoStream ts = two_stream32(mu0, mu0_inv, o[j]);
-            const float su = ts.Rdir * f[j], sd = ts.Tdir * f[j];
             const float den = frcp(1.0f - ts.Rd * Ab);
             const float Aa = ts.Rd + ts.Td * ts.Td * Ab * den;
-            const float Qa = su + ts.Td * den * (Qb + Ab * sd);
-            Ab = Aa; Qb = Qa;
+            const float qa = ts.Rdir + ts.Td * den * (qb * ts.T0 + Ab * ts.Tdir);
+            Ab = Aa; qb = qa;
         }
         cX[(size_t)s * 128 + g] = Ab;
-        cQ[(size_t)s * 128 + g] = Qb;
+        cQ[(size_t)s * 128 + g] = qb;
     }
 
-    // ---- pass 2, top-down: fluxes.  Per segment the layer terms and the
-    // albedo / source below each layer are rebuilt from the checkpoint at the
-    // segment bottom.
-    float dnd = 0.0f;  // diffuse down at TOA
-    float ftop = toa * mu0;  // direct beam at the current segment top
+    // ---- pass 2, top-down: direct beam by products of T_noscat (as the
+    // reference) and fluxes.  Per segment the layer terms and A, q below each
+    // layer are rebuilt from the checkpoint at the segment bottom.
+    float dnd = 0.0f;        // diffuse down at TOA
+    float fdir = toa * mu0;  // direct beam at the current level
     {
-        const float up = dnd * cX[g] + cQ[g];
+        const float up = dnd * cX[g] + cQ[g] * fdir;
         const double su = wave_sum(act ? (double)up : 0.0);
-        const double sdn = wave_sum(act ? (double)(dnd + toa * mu0) : 0.0);
+        const double sdn = wave_sum(act ? (double)(dnd + fdir) : 0.0);
         if (lane == 0) { rup[lev_toa] = su; rdn[lev_toa] = sdn; }
     }
     for (int s = 0; s < nseg; ++s) {
         Opt2s o[SW_SEG];
-        float f[SW_SEG + 1];
-        float Rd[SW_SEG], Td[SW_SEG], sdv[SW_SEG], Abl[SW_SEG], Qbl[SW_SEG], dnv[SW_SEG];
-        f[0] = ftop;
+        float Rd[SW_SEG], Td[SW_SEG], Tdr[SW_SEG], T0[SW_SEG], Abl[SW_SEG], qbl[SW_SEG], dnv[SW_SEG];
 #pragma unroll
         for (int j = 0; j < SW_SEG; ++j) {
             const int k = s * SW_SEG + j;
-            if (k < nlay) { o[j] = optics(lay(k)); f[j + 1] = exp_neg(o[j].t * mu0_inv) * f[j]; }
+            if (k < nlay) o[j] = optics(lay(k));
         }
-        ftop = f[SW_SEG];
         float a = cX[(size_t)(s + 1) * 128 + g], q = cQ[(size_t)(s + 1) * 128 + g];
 #pragma unroll
         for (int j = SW_SEG - 1; j >= 0; --j) {
             const int k = s * SW_SEG + j;
             if (k >= nlay) continue;
             const TwoStream ts = two_stream32(mu0, mu0_inv, o[j]);
-            const float su = ts.Rdir * f[j], sd = ts.Tdir * f[j];
             const float den = frcp(1.0f - ts.Rd * a);
-            Rd[j] = ts.Rd; Td[j] = ts.Td; sdv[j] = sd; Abl[j] = a; Qbl[j] = q; dnv[j] = den;
-            const float Aa = ts.Rd + ts.Td * ts.Td * a * den;
-            const float Qa = su + ts.Td * den * (q + a * sd);
-            a = Aa; q = Qa;
+            Rd[j] = ts.Rd; Td[j] = ts.Td; Tdr[j] = ts.Tdir; T0[j] = ts.T0; Abl[j] = a; qbl[j] = q; dnv[j] = den;
+            const float qa = ts.Rdir + ts.Td * den * (q * ts.T0 + a * ts.Tdir);
+            a = ts.Rd + ts.Td * ts.Td * a * den;
+            q = qa;
         }
 #pragma unroll
         for (int j = 0; j < SW_SEG; ++j) {
             const int k = s * SW_SEG + j;
             if (k >= nlay) continue;
             const int lev_below = lev(k + 1);
-            dnd = (Td[j] * dnd + Rd[j] * Qbl[j] + sdv[j]) * dnv[j];
-            const float up = dnd * Abl[j] + Qbl[j];
-            const float fd = f[j + 1];
+            const float sd = Tdr[j] * fdir;           // direct-beam source into the layer below
+            const float fd = T0[j] * fdir;            // direct beam at the level below
+            const float Qb = qbl[j] * fd;
+            dnd = (Td[j] * dnd + Rd[j] * Qb + sd) * dnv[j];
+            const float up = dnd * Abl[j] + Qb;
             const float dn = dnd + fd;
+            fdir = fd;
             const double su = wave_sum(act ? (double)up : 0.0);
             const double sdn = wave_sum(act ? (double)dn : 0.0);
             if (lane == 0) { rup[lev_below] = su; rdn[lev_below] = sdn; }
