@@ -84,7 +84,7 @@ struct ccpprad_ctx {
     int chunk = 65536;
     // learned gas optics (LW): MLP with table fallback, or null = table
     ccpprad_mlp *mlp = nullptr;
-    DevBuf w_oob, w_colm;
+    DevBuf w_oob, w_colm, w_rec;
     // optional per-kernel event timing
     int timing = 0;
     std::vector<hipEvent_t> ev_pool;
@@ -405,7 +405,7 @@ extern "C" void ccpprad_finalize(ccpprad_ctx *c)
     for (void *p : c->allocs) (void)hipFree(p);
     DevBuf *bufs[] = {&c->w_tau, &c->w_aux, &c->w_mask, &c->w_ctau, &c->w_cssa, &c->w_cg,
                       &c->w_sb, &c->w_blk, &c->w_colf, &c->w_day, &c->w_idx,
-                      &c->w_oob, &c->w_colm};
+                      &c->w_oob, &c->w_colm, &c->w_rec};
     for (DevBuf *b : bufs) b->release();
     for (auto &b : c->stage) b.release();
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -479,6 +479,7 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
     const int nblk = nC / bs;
     HIPCHK(c->w_tau.ensure(sizeof(float) * (size_t)CH * nL * G));
     HIPCHK(c->w_aux.ensure(sizeof(float) * (size_t)CH * nL * G));
+    HIPCHK(c->w_rec.ensure(sizeof(unsigned) * (size_t)CH * nL * gas_record_words(T)));
     HIPCHK(c->w_mask.ensure(sizeof(unsigned) * (size_t)CH * nL * MASK_WORDS));
     HIPCHK(c->w_ctau.ensure(sizeof(float) * (size_t)CH * nL * B));
     HIPCHK(c->w_blk.ensure((size_t)nblk + 1));
@@ -531,10 +532,10 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
             launch_fill_u8(mi.colflag, (size_t)nc, st);
             { KScope k_(c, K_MLP, st); launch_mlp_infer(c->mlp, mi, st); }
             // layers outside the training hull: table gas optics
-            { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, tau, pf, c->err, st, mi.oob, mi.colflag); }
+            { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)c->w_rec.p, tau, pf, c->err, st, mi.oob, mi.colflag); }
         } else {
             KScope k_(c, K_GAS, st);
-            launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, tau, pf, c->err, st);
+            launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)c->w_rec.p, tau, pf, c->err, st);
         }
         LwSolveArgs s{};
         s.nlay = nL; s.ncol_ld = nC; s.col0 = c0; s.top_at_1 = a->top_at_1;
@@ -616,6 +617,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
     const int nblk = nD / bs;
     HIPCHK(c->w_tau.ensure(sizeof(float) * (size_t)CH * nL * G));
     HIPCHK(c->w_aux.ensure(sizeof(float) * (size_t)CH * nL * G));
+    HIPCHK(c->w_rec.ensure(sizeof(unsigned) * (size_t)CH * nL * gas_record_words(T)));
     HIPCHK(c->w_mask.ensure(sizeof(unsigned) * (size_t)CH * nL * MASK_WORDS));
     HIPCHK(c->w_ctau.ensure(sizeof(float) * (size_t)CH * nL * B));
     HIPCHK(c->w_cssa.ensure(sizeof(float) * (size_t)CH * nL * B));
@@ -667,7 +669,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
             ma.icseed = a->icseed_sw; ma.mtpow = c->mtpow; ma.mask = mask;
             { KScope k_(c, K_MCICA, st); launch_mcica(ma, nc, st); }
         }
-        { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, a->idx, c0, a->p_lay, a->p_lev, a->t_lay, vm, tabs, tray, c->err, st); }
+        { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, a->idx, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)c->w_rec.p, tabs, tray, c->err, st); }
         SwSolveArgs s{};
         s.nlay = nL; s.ncol_ld = nC; s.col0 = c0; s.top_at_1 = a->top_at_1; s.blksz = bs; s.isfc = a->iSFC - 1;
         s.write_clr_zero = !a->doSWclrsky; s.role = 0;
@@ -730,7 +732,8 @@ static int gas_optics_entry(ccpprad_ctx *c, int want_lw, int nCol, int nLay, con
     for (int g = 0; g < MAX_GAS; ++g) vm.v[g] = vmr + (size_t)std::min(g, c->T.ngas - 1) * nCol * nLay;
     hipStream_t st = (hipStream_t)stream;
     launch_fill_i32(c->err, 1, 0, st);
-    launch_gas_optics(c->T, nCol, nLay, nCol, nullptr, 0, p_lay, p_lev, t_lay, vm, tau, aux, c->err, st);
+    HIPCHK(c->w_rec.ensure(sizeof(unsigned) * (size_t)nCol * nLay * gas_record_words(c->T)));
+    launch_gas_optics(c->T, nCol, nLay, nCol, nullptr, 0, p_lay, p_lev, t_lay, vm, (unsigned *)c->w_rec.p, tau, aux, c->err, st);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -12,10 +12,13 @@ struct VmrPtrs {
 
 // row_mask / col_mask (optional): only rows with row_mask[d*nlay+l] != 0 of
 // columns with col_mask[d] != 0 are written (the emulator's fallback rows).
+// rec: work buffer of ncols * nlay * gas_record_words(T) words (per-layer
+// interpolation records).
 void launch_gas_optics(const TablesView &T, int ncols, int nlay, int ncol_ld, const int *col_idx, int col0,
                        const double *p_lay, const double *p_lev, const double *t_lay, const VmrPtrs &vmr,
-                       float *tau, float *aux, int *err, hipStream_t st,
+                       unsigned *rec, float *tau, float *aux, int *err, hipStream_t st,
                        const unsigned char *row_mask = nullptr, const unsigned char *col_mask = nullptr);
+int gas_record_words(const TablesView &T);
 
 struct LwSolveArgs {
     int nlay, ncol_ld, col0, top_at_1;
