@@ -10,6 +10,8 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <mutex>
+#include <thread>
 
 #include "../../include/ccpprad.h"
 #include "ccpprad_dev.h"
@@ -60,6 +62,32 @@ struct DevBuf {
     }
 };
 
+// Per-call mutable state.  One workspace serves one stream: calls on that
+// stream reuse it in stream order, calls on other streams get their own, so
+// concurrent scheme calls from several host threads do not share buffers.
+struct Workspace {
+    hipStream_t stream = nullptr;  // the stream it serves
+    bool own_stream = false;       // created for a host thread's staging path
+    std::thread::id owner;         // that thread
+    DevBuf w_tau, w_aux, w_mask, w_ctau, w_cssa, w_cg, w_sb, w_blk, w_colf, w_day, w_idx;
+    DevBuf w_oob, w_colm, w_rec;
+    std::vector<DevBuf> stage;     // host-path staging
+    int *err = nullptr;            // device error word, then the SW last-wipe block
+    double *toa64 = nullptr;       // SW TOA source scaled to solcon
+    float *toa32 = nullptr;
+    void release()
+    {
+        DevBuf *bufs[] = {&w_tau, &w_aux, &w_mask, &w_ctau, &w_cssa, &w_cg, &w_sb, &w_blk, &w_colf, &w_day, &w_idx,
+                          &w_oob, &w_colm, &w_rec};
+        for (DevBuf *b : bufs) b->release();
+        for (auto &b : stage) b.release();
+        if (err) (void)hipFree(err);
+        if (toa64) (void)hipFree(toa64);
+        if (toa32) (void)hipFree(toa32);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+};
+
 struct ccpprad_ctx {
     int is_lw = 0;
     int nlay = 0, blksz = 0, device = 0;
@@ -70,27 +98,39 @@ struct ccpprad_ctx {
     uint32_t *mtpow = nullptr;
     int *alb_cat = nullptr;
     int ibd = 0, ibd_uv = 0;
-    double *toa64 = nullptr;
-    float *toa32 = nullptr;
     const double *solar_dev = nullptr;
-    int *err = nullptr;                 // device error word
-    int *last_wipe = nullptr;
-    int gas_slot[MAX_GAS];              // k-dist gas index (0-based) of each CCPP vmr slot
-    // work buffers
-    DevBuf w_tau, w_aux, w_mask, w_ctau, w_cssa, w_cg, w_sb, w_blk, w_colf, w_day, w_idx;
-    // host-path staging
-    std::vector<DevBuf> stage;
-    hipStream_t stream = nullptr;
     int chunk = 65536;
     // learned gas optics (LW): MLP with table fallback, or null = table
     ccpprad_mlp *mlp = nullptr;
-    DevBuf w_oob, w_colm, w_rec;
+    // workspaces and the lock for them and for the timing records; held while
+    // a call enqueues its work, not while the GPU runs it
+    std::mutex mu;
+    std::vector<Workspace *> ws;
     // optional per-kernel event timing
     int timing = 0;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_rec;
 };
+
+// The workspace of a stream (caller holds c->mu).
+static Workspace *ws_for_stream(ccpprad_ctx *c, hipStream_t st)
+{
+    for (Workspace *w : c->ws)
+        if (w->stream == st) return w;
+    Workspace *w = new Workspace();
+    w->stream = st;
+    const int G = c->T.ngpt > 0 ? c->T.ngpt : 1;
+    if (hipMalloc(&w->err, 2 * sizeof(int)) != hipSuccess ||
+        hipMalloc(&w->toa64, sizeof(double) * G) != hipSuccess ||
+        hipMalloc(&w->toa32, sizeof(float) * G) != hipSuccess) {
+        w->release();
+        delete w;
+        return nullptr;
+    }
+    c->ws.push_back(w);
+    return w;
+}
 
 enum { K_GAS = 0, K_LWSOLVE = 1, K_SWSOLVE = 2, K_CLOUD = 3, K_MCICA = 4, K_MLP = 5, K_N = 6 };
 
@@ -151,20 +191,33 @@ extern "C" int ccpprad_timing_query(ccpprad_ctx *c, int kernel_id, double *total
     return 0;
 }
 
-// Internal to the library (hidden): staging buffers and the private stream
-// of the host-pointer entries.
-extern "C" __attribute__((visibility("hidden"))) int ccpprad_ctx_stage(ccpprad_ctx *c, int slot, size_t bytes, void **p)
+// Internal to the library (hidden): the calling thread's workspace for the
+// host-pointer entries (a private stream per thread), and its staging buffers.
+extern "C" __attribute__((visibility("hidden"))) void *ccpprad_ctx_host_stream(ccpprad_ctx *c)
 {
-    if ((int)c->stage.size() <= slot) c->stage.resize(slot + 1);
-    if (c->stage[slot].ensure(bytes ? bytes : 8) != hipSuccess) return 1;
-    *p = c->stage[slot].p;
-    return 0;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const std::thread::id me = std::this_thread::get_id();
+    for (Workspace *w : c->ws)
+        if (w->own_stream && w->owner == me) return (void *)w->stream;
+    hipStream_t st = nullptr;
+    if (hipStreamCreate(&st) != hipSuccess) return nullptr;
+    Workspace *w = ws_for_stream(c, st);
+    if (!w) { (void)hipStreamDestroy(st); return nullptr; }
+    w->own_stream = true;
+    w->owner = me;
+    return (void *)st;
 }
 
-extern "C" __attribute__((visibility("hidden"))) void *ccpprad_ctx_stream(ccpprad_ctx *c)
+extern "C" __attribute__((visibility("hidden"))) int ccpprad_ctx_stage(ccpprad_ctx *c, void *stream, int slot,
+                                                                       size_t bytes, void **p)
 {
-    if (!c->stream) (void)hipStreamCreate(&c->stream);
-    return (void *)c->stream;
+    std::lock_guard<std::mutex> lk(c->mu);
+    Workspace *w = ws_for_stream(c, (hipStream_t)stream);
+    if (!w) return 1;
+    if ((int)w->stage.size() <= slot) w->stage.resize(slot + 1);
+    if (w->stage[slot].ensure(bytes ? bytes : 8) != hipSuccess) return 1;
+    *p = w->stage[slot].p;
+    return 0;
 }
 
 template <typename T>
@@ -250,10 +303,6 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
         auto a = to_f32(kd->krayl_lower, nr); T.krayl[0] = upload(c, a.data(), nr, &e);
         auto b = to_f32(kd->krayl_upper, nr); T.krayl[1] = upload(c, b.data(), nr, &e);
         c->solar_dev = upload(c, kd->solar_src, kd->ngpt, &e);
-        c->toa64 = upload<double>(c, nullptr, kd->ngpt, &e);
-        c->toa32 = upload<float>(c, nullptr, kd->ngpt, &e);
-        T.toa_norm = c->toa32;
-        T.toa_norm64 = c->toa64;
     }
     T.vmr_ref = upload(c, kd->vmr_ref, (size_t)2 * (kd->ngas + 1) * kd->ntemp, &e);
     std::vector<int> flav(2 * kd->nflav);
@@ -376,8 +425,6 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
         L.coaice = upload(c, ci.data(), ni, &e);
         c->have_lut = 1;
     }
-    c->err = upload<int>(c, nullptr, 2, &e);
-    c->last_wipe = c->err + 1;
     if (e != hipSuccess) {
         seterr(errmsg, errlen, "device table upload failed: %s", hipGetErrorString(e));
         ccpprad_finalize(c);
@@ -402,14 +449,10 @@ extern "C" int ccpprad_sw_init(const ccpprad_kdist *kd, const ccpprad_cloud_lut 
 extern "C" void ccpprad_finalize(ccpprad_ctx *c)
 {
     if (!c) return;
+    if (c->device >= 0) (void)hipDeviceSynchronize();
     for (void *p : c->allocs) (void)hipFree(p);
-    DevBuf *bufs[] = {&c->w_tau, &c->w_aux, &c->w_mask, &c->w_ctau, &c->w_cssa, &c->w_cg,
-                      &c->w_sb, &c->w_blk, &c->w_colf, &c->w_day, &c->w_idx,
-                      &c->w_oob, &c->w_colm, &c->w_rec};
-    for (DevBuf *b : bufs) b->release();
-    for (auto &b : c->stage) b.release();
+    for (Workspace *w : c->ws) { w->release(); delete w; }
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -477,20 +520,23 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
     const int G = T.ngpt, B = T.nband;
     const int CH = std::min(nC, c->chunk);
     const int nblk = nC / bs;
-    HIPCHK(c->w_tau.ensure(sizeof(float) * (size_t)CH * nL * G));
-    HIPCHK(c->w_aux.ensure(sizeof(float) * (size_t)CH * nL * G));
-    HIPCHK(c->w_rec.ensure(sizeof(unsigned) * (size_t)CH * nL * gas_record_words(T)));
-    HIPCHK(c->w_mask.ensure(sizeof(unsigned) * (size_t)CH * nL * MASK_WORDS));
-    HIPCHK(c->w_ctau.ensure(sizeof(float) * (size_t)CH * nL * B));
-    HIPCHK(c->w_blk.ensure((size_t)nblk + 1));
-    HIPCHK(c->w_colf.ensure((size_t)nC + 1));
+    std::unique_lock<std::mutex> lk(c->mu);
+    Workspace *w = ws_for_stream(c, st);
+    if (!w) { seterr(errmsg, errlen, "workspace allocation failed"); return 2; }
+    HIPCHK(w->w_tau.ensure(sizeof(float) * (size_t)CH * nL * G));
+    HIPCHK(w->w_aux.ensure(sizeof(float) * (size_t)CH * nL * G));
+    HIPCHK(w->w_rec.ensure(sizeof(unsigned) * (size_t)CH * nL * gas_record_words(T)));
+    HIPCHK(w->w_mask.ensure(sizeof(unsigned) * (size_t)CH * nL * MASK_WORDS));
+    HIPCHK(w->w_ctau.ensure(sizeof(float) * (size_t)CH * nL * B));
+    HIPCHK(w->w_blk.ensure((size_t)nblk + 1));
+    HIPCHK(w->w_colf.ensure((size_t)nC + 1));
     if (c->mlp) {
-        HIPCHK(c->w_oob.ensure((size_t)CH * nL));
-        HIPCHK(c->w_colm.ensure((size_t)CH + 1));
+        HIPCHK(w->w_oob.ensure((size_t)CH * nL));
+        HIPCHK(w->w_colm.ensure((size_t)CH + 1));
     }
-    unsigned char *blk = (unsigned char *)c->w_blk.p;
-    launch_fill_i32(c->err, 1, 0, st);
-    if (have_clouds) launch_block_flags(nC, nL, nC, nullptr, bs, a->cld_frac, blk, (unsigned char *)c->w_colf.p, st);
+    unsigned char *blk = (unsigned char *)w->w_blk.p;
+    launch_fill_i32(w->err, 1, 0, st);
+    if (have_clouds) launch_block_flags(nC, nL, nC, nullptr, bs, a->cld_frac, blk, (unsigned char *)w->w_colf.p, st);
     else launch_fill_u8(blk, (size_t)nblk, st);
     VmrPtrs vm;
     vm.v[0] = a->vmr_h2o; vm.v[1] = a->vmr_co2; vm.v[2] = a->vmr_o3;
@@ -499,9 +545,9 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
     gauss_table(a->nGauss_angles, D, W);
     for (int c0 = 0; c0 < nC; c0 += CH) {
         const int nc = std::min(CH, nC - c0);
-        float *tau = (float *)c->w_tau.p, *pf = (float *)c->w_aux.p;
-        unsigned *mask = (unsigned *)c->w_mask.p;
-        float *ctau = (float *)c->w_ctau.p;
+        float *tau = (float *)w->w_tau.p, *pf = (float *)w->w_aux.p;
+        unsigned *mask = (unsigned *)w->w_mask.p;
+        float *ctau = (float *)w->w_ctau.p;
         if (have_clouds) {
             CloudArgs ca{};
             ca.nlay = nL; ca.ncol_ld = nC; ca.col0 = c0; ca.blksz = bs; ca.nband = B; ca.is_lw = 1;
@@ -511,7 +557,7 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
             if (a->doGP_sgs_pbl) { ca.pbl[0] = a->cld_pbl_lwp; ca.pbl[1] = a->cld_pbl_iwp; ca.pbl[2] = a->cld_pbl_reliq; ca.pbl[3] = a->cld_pbl_reice; }
             ca.rwp = a->cld_rwp; ca.swp = a->cld_swp; ca.resnow = a->cld_resnow;
             if (!ca.rwp || !ca.swp || !ca.resnow) { seterr(errmsg, errlen, "cld_rwp/cld_swp/cld_resnow required with clouds"); return 1; }
-            ca.ctau = ctau; ca.err = c->err;
+            ca.ctau = ctau; ca.err = w->err;
             { KScope k_(c, K_CLOUD, st); launch_cloud_optics(c->L, ca, nc, st); }
             McicaArgs ma{};
             ma.nlay = nL; ma.ngpt = G; ma.ncol_ld = nC; ma.col0 = c0; ma.blksz = bs; ma.isubc = a->isubc_lw;
@@ -528,14 +574,14 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
             mi.vmr[0] = a->vmr_h2o; mi.vmr[1] = a->vmr_o3; mi.vmr[2] = a->vmr_co2;
             mi.vmr[3] = a->vmr_ch4; mi.vmr[4] = a->vmr_n2o;
             mi.tau = tau; mi.pfrac = pf;
-            mi.oob = (unsigned char *)c->w_oob.p; mi.colflag = (unsigned char *)c->w_colm.p;
+            mi.oob = (unsigned char *)w->w_oob.p; mi.colflag = (unsigned char *)w->w_colm.p;
             launch_fill_u8(mi.colflag, (size_t)nc, st);
             { KScope k_(c, K_MLP, st); launch_mlp_infer(c->mlp, mi, st); }
             // layers outside the training hull: table gas optics
-            { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)c->w_rec.p, tau, pf, c->err, st, mi.oob, mi.colflag); }
+            { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)w->w_rec.p, tau, pf, w->err, st, mi.oob, mi.colflag); }
         } else {
             KScope k_(c, K_GAS, st);
-            launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)c->w_rec.p, tau, pf, c->err, st);
+            launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)w->w_rec.p, tau, pf, w->err, st);
         }
         LwSolveArgs s{};
         s.nlay = nL; s.ncol_ld = nC; s.col0 = c0; s.top_at_1 = a->top_at_1;
@@ -564,7 +610,8 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
     HIPCHK(hipGetLastError());
     if (validate) {
         int herr = 0;
-        HIPCHK(hipMemcpyAsync(&herr, c->err, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&herr, w->err, sizeof(int), hipMemcpyDeviceToHost, st));
+        lk.unlock();   // the copy is ordered after this call's kernels
         HIPCHK(hipStreamSynchronize(st));
         if (herr & 1) { seterr(errmsg, errlen, "gas_optics(): p_lay or t_lay outside the table range"); return 3; }
         if (herr & 2) { seterr(errmsg, errlen, "cloud_optics(): effective radius outside the LUT range"); return 3; }
@@ -611,28 +658,33 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
         seterr(errmsg, errlen, "exponential/decorrelation overlap needs cloud_overlap_param"); return 1;
     }
     const bool have_clouds = c->have_lut && a->cld_lwp && a->cld_iwp && a->cld_reliq && a->cld_reice;
-    const TablesView &T = c->T;
+    std::unique_lock<std::mutex> lk(c->mu);
+    Workspace *w = ws_for_stream(c, st);
+    if (!w) { seterr(errmsg, errlen, "workspace allocation failed"); return 2; }
+    TablesView T = c->T;   // with this call's TOA source
+    T.toa_norm = w->toa32;
+    T.toa_norm64 = w->toa64;
     const int G = T.ngpt, B = T.nband;
     const int CH = std::min(nD, c->chunk);
     const int nblk = nD / bs;
-    HIPCHK(c->w_tau.ensure(sizeof(float) * (size_t)CH * nL * G));
-    HIPCHK(c->w_aux.ensure(sizeof(float) * (size_t)CH * nL * G));
-    HIPCHK(c->w_rec.ensure(sizeof(unsigned) * (size_t)CH * nL * gas_record_words(T)));
-    HIPCHK(c->w_mask.ensure(sizeof(unsigned) * (size_t)CH * nL * MASK_WORDS));
-    HIPCHK(c->w_ctau.ensure(sizeof(float) * (size_t)CH * nL * B));
-    HIPCHK(c->w_cssa.ensure(sizeof(float) * (size_t)CH * nL * B));
-    HIPCHK(c->w_cg.ensure(sizeof(float) * (size_t)CH * nL * B));
-    HIPCHK(c->w_sb.ensure(sizeof(double) * (size_t)nD * 3 * B));
-    HIPCHK(c->w_blk.ensure((size_t)nblk + 1));
-    HIPCHK(c->w_colf.ensure((size_t)nD + 1));
-    HIPCHK(c->w_day.ensure((size_t)nC + 1));
-    unsigned char *blk = (unsigned char *)c->w_blk.p, *colf = (unsigned char *)c->w_colf.p;
-    launch_fill_i32(c->err, 1, 0, st);
-    launch_fill_i32(c->last_wipe, 1, -1, st);
+    HIPCHK(w->w_tau.ensure(sizeof(float) * (size_t)CH * nL * G));
+    HIPCHK(w->w_aux.ensure(sizeof(float) * (size_t)CH * nL * G));
+    HIPCHK(w->w_rec.ensure(sizeof(unsigned) * (size_t)CH * nL * gas_record_words(T)));
+    HIPCHK(w->w_mask.ensure(sizeof(unsigned) * (size_t)CH * nL * MASK_WORDS));
+    HIPCHK(w->w_ctau.ensure(sizeof(float) * (size_t)CH * nL * B));
+    HIPCHK(w->w_cssa.ensure(sizeof(float) * (size_t)CH * nL * B));
+    HIPCHK(w->w_cg.ensure(sizeof(float) * (size_t)CH * nL * B));
+    HIPCHK(w->w_sb.ensure(sizeof(double) * (size_t)nD * 3 * B));
+    HIPCHK(w->w_blk.ensure((size_t)nblk + 1));
+    HIPCHK(w->w_colf.ensure((size_t)nD + 1));
+    HIPCHK(w->w_day.ensure((size_t)nC + 1));
+    unsigned char *blk = (unsigned char *)w->w_blk.p, *colf = (unsigned char *)w->w_colf.p;
+    launch_fill_i32(w->err, 1, 0, st);
+    launch_fill_i32((w->err + 1), 1, -1, st);
     launch_fill_u8(colf, (size_t)nD, st);
     if (have_clouds) launch_block_flags(nD, nL, nC, a->idx, bs, a->cld_frac, blk, colf, st);
     else launch_fill_u8(blk, (size_t)nblk, st);
-    launch_toa_norm(c->solar_dev, G, a->solcon, c->toa64, c->toa32, st);
+    launch_toa_norm(c->solar_dev, G, a->solcon, w->toa64, w->toa32, st);
     // cldtausw: reset by every block of the reference loop, so only the last
     // block's values survive (rrtmgp_sw_main.F90:308,403)
     launch_fill_f64(a->cldtausw, (size_t)nC * nL, 0.0, st);
@@ -644,9 +696,9 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
     const int last_blk_start = (nblk - 1) * bs;
     for (int c0 = 0; c0 < nD; c0 += CH) {
         const int nc = std::min(CH, nD - c0);
-        float *tabs = (float *)c->w_tau.p, *tray = (float *)c->w_aux.p;
-        unsigned *mask = (unsigned *)c->w_mask.p;
-        float *ctau = (float *)c->w_ctau.p, *cssa = (float *)c->w_cssa.p, *cg = (float *)c->w_cg.p;
+        float *tabs = (float *)w->w_tau.p, *tray = (float *)w->w_aux.p;
+        unsigned *mask = (unsigned *)w->w_mask.p;
+        float *ctau = (float *)w->w_ctau.p, *cssa = (float *)w->w_cssa.p, *cg = (float *)w->w_cg.p;
         if (have_clouds) {
             CloudArgs ca{};
             ca.nlay = nL; ca.ncol_ld = nC; ca.col0 = c0; ca.blksz = bs; ca.nband = B; ca.is_lw = 0;
@@ -656,7 +708,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
             if (a->doGP_sgs_pbl) { ca.pbl[0] = a->cld_pbl_lwp; ca.pbl[1] = a->cld_pbl_iwp; ca.pbl[2] = a->cld_pbl_reliq; ca.pbl[3] = a->cld_pbl_reice; }
             ca.rwp = a->cld_rwp; ca.swp = a->cld_swp; ca.resnow = a->cld_resnow;
             if (!ca.rwp || !ca.swp || !ca.resnow) { seterr(errmsg, errlen, "cld_rwp/cld_swp/cld_resnow required with clouds"); return 1; }
-            ca.ctau = ctau; ca.cssa = cssa; ca.cg = cg; ca.err = c->err;
+            ca.ctau = ctau; ca.cssa = cssa; ca.cg = cg; ca.err = w->err;
             // only the last block's band-11 cloud depth survives the per-block reset
             ca.cldtau = (c0 + nc > last_blk_start) ? a->cldtausw : nullptr;
             ca.cldtau_blk = nblk - 1;
@@ -669,7 +721,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
             ma.icseed = a->icseed_sw; ma.mtpow = c->mtpow; ma.mask = mask;
             { KScope k_(c, K_MCICA, st); launch_mcica(ma, nc, st); }
         }
-        { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, a->idx, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)c->w_rec.p, tabs, tray, c->err, st); }
+        { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, a->idx, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)w->w_rec.p, tabs, tray, w->err, st); }
         SwSolveArgs s{};
         s.nlay = nL; s.ncol_ld = nC; s.col0 = c0; s.top_at_1 = a->top_at_1; s.blksz = bs; s.isfc = a->iSFC - 1;
         s.write_clr_zero = !a->doSWclrsky; s.role = 0;
@@ -679,7 +731,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
         s.alb_uv_dir = a->sfc_alb_uvvis_dir; s.alb_uv_dif = a->sfc_alb_uvvis_dif; s.p_lev = a->p_lev;
         s.alb_cat = c->alb_cat;
         s.up_all = a->fluxswUP_allsky; s.dn_all = a->fluxswDOWN_allsky; s.up_clr = a->fluxswUP_clrsky;
-        s.dn_clr = a->fluxswDOWN_clrsky; s.sb = (double *)c->w_sb.p + (size_t)c0 * 3 * B; s.htr = a->htrsw;
+        s.dn_clr = a->fluxswDOWN_clrsky; s.sb = (double *)w->w_sb.p + (size_t)c0 * 3 * B; s.htr = a->htrsw;
         s.htr_clr = a->doSWclrsky ? a->htrswc : nullptr;
         { KScope k_(c, K_SWSOLVE, st); launch_sw_solve(T, s, nc, have_clouds, st); }
         if (a->doSWclrsky && have_clouds) {
@@ -691,14 +743,15 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
     }
     ScmpArgs sc{};
     sc.ndays = nD; sc.blksz = bs; sc.nband = B; sc.ibd = c->ibd; sc.ibd_uv = c->ibd_uv; sc.do_clr = a->doSWclrsky;
-    sc.col_idx = a->idx; sc.blkflag = blk; sc.colflag = colf; sc.sb = (const double *)c->w_sb.p;
-    sc.last_wipe = c->last_wipe; sc.scmpsw = a->scmpsw;
-    launch_fill_u8((unsigned char *)c->w_day.p, (size_t)nC, st);
-    launch_scmpsw(sc, nC, (unsigned char *)c->w_day.p, c->last_wipe, st);
+    sc.col_idx = a->idx; sc.blkflag = blk; sc.colflag = colf; sc.sb = (const double *)w->w_sb.p;
+    sc.last_wipe = (w->err + 1); sc.scmpsw = a->scmpsw;
+    launch_fill_u8((unsigned char *)w->w_day.p, (size_t)nC, st);
+    launch_scmpsw(sc, nC, (unsigned char *)w->w_day.p, (w->err + 1), st);
     HIPCHK(hipGetLastError());
     if (validate) {
         int herr = 0;
-        HIPCHK(hipMemcpyAsync(&herr, c->err, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&herr, w->err, sizeof(int), hipMemcpyDeviceToHost, st));
+        lk.unlock();   // the copy is ordered after this call's kernels
         HIPCHK(hipStreamSynchronize(st));
         if (herr & 1) { seterr(errmsg, errlen, "gas_optics(): p_lay or t_lay outside the table range"); return 3; }
         if (herr & 2) { seterr(errmsg, errlen, "cloud_optics(): effective radius outside the LUT range"); return 3; }
@@ -731,9 +784,12 @@ static int gas_optics_entry(ccpprad_ctx *c, int want_lw, int nCol, int nLay, con
     VmrPtrs vm;
     for (int g = 0; g < MAX_GAS; ++g) vm.v[g] = vmr + (size_t)std::min(g, c->T.ngas - 1) * nCol * nLay;
     hipStream_t st = (hipStream_t)stream;
-    launch_fill_i32(c->err, 1, 0, st);
-    HIPCHK(c->w_rec.ensure(sizeof(unsigned) * (size_t)nCol * nLay * gas_record_words(c->T)));
-    launch_gas_optics(c->T, nCol, nLay, nCol, nullptr, 0, p_lay, p_lev, t_lay, vm, (unsigned *)c->w_rec.p, tau, aux, c->err, st);
+    std::lock_guard<std::mutex> lk(c->mu);
+    Workspace *w = ws_for_stream(c, st);
+    if (!w) { seterr(errmsg, errlen, "workspace allocation failed"); return 2; }
+    launch_fill_i32(w->err, 1, 0, st);
+    HIPCHK(w->w_rec.ensure(sizeof(unsigned) * (size_t)nCol * nLay * gas_record_words(c->T)));
+    launch_gas_optics(c->T, nCol, nLay, nCol, nullptr, 0, p_lay, p_lev, t_lay, vm, (unsigned *)w->w_rec.p, tau, aux, w->err, st);
     HIPCHK(hipGetLastError());
     return 0;
 }

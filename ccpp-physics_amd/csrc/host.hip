@@ -17,8 +17,8 @@
 
 #include "../../include/ccpprad.h"
 
-extern "C" int ccpprad_ctx_stage(ccpprad_ctx *c, int slot, size_t bytes, void **p);
-extern "C" void *ccpprad_ctx_stream(ccpprad_ctx *c);
+extern "C" int ccpprad_ctx_stage(ccpprad_ctx *c, void *stream, int slot, size_t bytes, void **p);
+extern "C" void *ccpprad_ctx_host_stream(ccpprad_ctx *c);
 
 namespace {
 
@@ -99,14 +99,18 @@ int run_host(ccpprad_ctx *c, const Args *a, const Field (&F)[N], int ncol, int n
     }
     const size_t nband = (size_t)ccpprad_nband(c);
     Args d = *a;
-    hipStream_t st = (hipStream_t)ccpprad_ctx_stream(c);
+    hipStream_t st = (hipStream_t)ccpprad_ctx_host_stream(c);
+    if (!st) {
+        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "stream creation failed");
+        return 2;
+    }
     for (size_t i = 0; i < N; ++i) {
         void **hp = (void **)((char *)&d + F[i].off);
         if (!*hp) continue;
         const size_t n = field_elems(F[i].shape, (size_t)ncol, (size_t)nlay, nband, (size_t)nday);
         const size_t bytes = n * (F[i].is_int ? sizeof(int) : sizeof(double));
         void *dp = nullptr;
-        int rc = ccpprad_ctx_stage(c, (int)i, bytes, &dp);
+        int rc = ccpprad_ctx_stage(c, (void *)st, (int)i, bytes, &dp);
         if (rc) {
             if (errmsg && errlen > 0) snprintf(errmsg, errlen, "device staging allocation failed");
             return 2;
