@@ -14,7 +14,8 @@
 //       interval the absorber scaling.  The discrete decisions (indices,
 //       tropopause) are taken in FP64 as in the FP64 reference.  The result
 //       is one compact record per (column, layer) (layout below), stored
-//       layer-major so a wavefront's records are contiguous.
+//       layer-major ([layer][column]) so a wavefront's 64 records are one
+//       contiguous block, assembled in LDS and written with 16-byte stores.
 //   gas_optics_kernel (FP32), block = (16-layer chunk, column), chunk slowest
 //       so co-resident blocks share k-table rows in L2, lane = g-point: the
 //       chunk's records are staged into LDS, then 8 major-table corners (+8
@@ -57,29 +58,58 @@ int gas_record_words(const TablesView &T) { return rec_geom(T.nflav, T.nminor[0]
 // ---------------------------------------------------------------------------
 // interpolation: thread per (column, layer), FP64
 // ---------------------------------------------------------------------------
+// v: the lane's gas columns in LDS (index 0 = dry air); i outside 1..n-1
+// reads the dry-air column
 __device__ __forceinline__ double pick(const double *v, int n, int i)
 {
-    double r = v[0];
-#pragma unroll
-    for (int k = 1; k <= MAX_GAS; ++k)
-        if (k == i && k < n) r = v[k];
-    return r;
+    return v[(i >= 1 && i < n) ? i : 0];
 }
 
-__global__ __launch_bounds__(256) void gas_interp_kernel(
+// One wavefront of 64 (column, layer) rows: each lane builds its record in
+// LDS, then the wavefront writes the 64 consecutive records as one
+// contiguous block with 16-byte stores.
+constexpr int INTERP_ROWS = 64;
+
+__device__ __forceinline__ void interp_row(const TablesView &T, const RecGeom &G, size_t i, int ncols, int nlay,
+                                           int ncol_ld, const int *col_idx, int col0, const double *p_lay,
+                                           const double *p_lev, const double *t_lay, const VmrPtrs &vmr,
+                                           unsigned *r, double *col, int *err);
+
+__global__ __launch_bounds__(INTERP_ROWS) void gas_interp_kernel(
     TablesView T, int nlay, int ncol_ld, const int *__restrict__ col_idx, int col0, int ncols,
     const double *__restrict__ p_lay, const double *__restrict__ p_lev, const double *__restrict__ t_lay,
     VmrPtrs vmr, unsigned *__restrict__ rec, int *__restrict__ err, const unsigned char *__restrict__ col_mask)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)ncols * nlay) return;
-    const int d = (int)(i % ncols), l = (int)(i / ncols);   // column fastest: coalesced CCPP reads
+    extern __shared__ unsigned srec[];   // [INTERP_ROWS][words]
+    __shared__ double scol[INTERP_ROWS][MAX_GAS + 1];
+    const RecGeom G = rec_geom(T.nflav, T.nminor[0], T.nminor[1]);
+    const size_t nrow = (size_t)ncols * nlay;
+    const size_t i0 = (size_t)blockIdx.x * INTERP_ROWS, i = i0 + threadIdx.x;
+    // fallback launch: blocks without a flagged column write nothing
+    if (col_mask && !__any(i < nrow && col_mask[i % ncols])) return;   // one wavefront: uniform
     // every row of a flagged column gets a record: the gather kernel stages
-    // whole chunks, so no record it reads may be left unwritten
-    if (col_mask && !col_mask[d]) return;
+    // whole chunks, so no record it reads may be left unwritten (rows of
+    // unflagged columns are written with whatever the slot holds; nothing
+    // reads them)
+    if (i < nrow && (!col_mask || col_mask[i % ncols]))
+        interp_row(T, G, i, ncols, nlay, ncol_ld, col_idx, col0, p_lay, p_lev, t_lay, vmr,
+                   srec + threadIdx.x * G.words, scol[threadIdx.x], err);
+    __syncthreads();
+    const size_t nr = nrow - i0 < (size_t)INTERP_ROWS ? nrow - i0 : (size_t)INTERP_ROWS;
+    const int n4 = (int)nr * G.words / 4;
+    uint4 *dst = (uint4 *)(rec + i0 * G.words);
+    const uint4 *src = (const uint4 *)srec;
+    for (int k = threadIdx.x; k < n4; k += INTERP_ROWS) dst[k] = src[k];
+}
+
+__device__ __forceinline__ void interp_row(const TablesView &T, const RecGeom &G, size_t i, int ncols, int nlay,
+                                           int ncol_ld, const int *col_idx, int col0, const double *p_lay,
+                                           const double *p_lev, const double *t_lay, const VmrPtrs &vmr,
+                                           unsigned *r, double *col, int *err)
+{
+    const int d = (int)(i % ncols), l = (int)(i / ncols);   // column fastest: coalesced CCPP reads
     const int gc = col_idx ? col_idx[col0 + d] - 1 : col0 + d;
     const size_t q = (size_t)gc + (size_t)ncol_ld * l;
-    const RecGeom G = rec_geom(T.nflav, T.nminor[0], T.nminor[1]);
     const double tmin = T.temp_ref_min, dT = T.temp_ref_delta;
 
     const double pl = p_lay[q], tl = t_lay[q];
@@ -89,7 +119,6 @@ __global__ __launch_bounds__(256) void gas_interp_kernel(
     const double fact = 1.0 / (1.0 + vh);
     const double m_air = (M_DRY + M_H2O * vh) * fact;
     const double col_dry = 10.0 * dp * AVOGAD / (1000.0 * m_air * 100.0 * GRAV) * fact;
-    double col[MAX_GAS + 1];
     col[0] = col_dry;
 #pragma unroll
     for (int gs = 1; gs <= MAX_GAS; ++gs) col[gs] = gs <= T.ngas ? vmr.v[gs - 1][q] * col_dry : 0.0;
@@ -103,7 +132,6 @@ __global__ __launch_bounds__(256) void gas_interp_kernel(
     const int itr = (lpl > T.press_ref_trop_log) ? 1 : 2;
     const int ng1 = T.ngas + 1;
 
-    unsigned *r = rec + i * G.words;   // records layer-major: [l][d]
     r[0] = (unsigned)jt | ((unsigned)jp << 8) | ((unsigned)itr << 16);
     r[1] = __float_as_uint((float)(locp - (double)jp));
     r[2] = __float_as_uint((float)(col[1] + col_dry));
@@ -341,8 +369,9 @@ void launch_gas_optics(const TablesView &T, int ncols, int nlay, int ncol_ld, co
 {
     if (ncols <= 0 || nlay <= 0) return;
     const size_t nrow = (size_t)ncols * nlay;
-    hipLaunchKernelGGL(gas_interp_kernel, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, st, T, nlay, ncol_ld,
-                       col_idx, col0, ncols, p_lay, p_lev, t_lay, vmr, rec, err, col_mask);
+    const size_t smem = sizeof(unsigned) * INTERP_ROWS * gas_record_words(T);
+    hipLaunchKernelGGL(gas_interp_kernel, dim3((unsigned)((nrow + INTERP_ROWS - 1) / INTERP_ROWS)), dim3(INTERP_ROWS),
+                       smem, st, T, nlay, ncol_ld, col_idx, col0, ncols, p_lay, p_lev, t_lay, vmr, rec, err, col_mask);
     const int nblocks = ncols * ((nlay + LCHUNK - 1) / LCHUNK);
     if (T.is_lw) launch_nm<true>(T, nblocks, nlay, ncols, rec, tau, aux, st, row_mask, col_mask);
     else launch_nm<false>(T, nblocks, nlay, ncols, rec, tau, aux, st, row_mask, col_mask);

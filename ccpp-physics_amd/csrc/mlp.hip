@@ -225,7 +225,9 @@ __global__ void to_bf16_kernel(const float *x, __bf16 *y, long n)
 // W3^T (4 x 8), W4^T (8 x 8): 1 KB each; then biases (FP32).
 constexpr int FRAG_BYTES = 64 * 16;
 constexpr int NFRAG = 4 * 1 + 4 * 8 + 4 * 8 + 8 * 8;              // 132
-constexpr int MLP_LDS_BYTES = NFRAG * FRAG_BYTES + (3 * 128 + 256) * 4;
+// fragments, then biases (3 x 128 + 256) and the output de-standardisation
+// (ystd, ymean: 2 x 256), all FP32
+constexpr int MLP_LDS_BYTES = NFRAG * FRAG_BYTES + (3 * 128 + 256 + 2 * 256) * 4;
 
 
 __global__ __launch_bounds__(512) void mlp_infer_kernel(MlpInferArgs A)
@@ -322,7 +324,7 @@ __global__ __launch_bounds__(512) void mlp_infer_kernel(MlpInferArgs A)
             for (int kc = 0; kc < 8; ++kc) h[kc] = hn[kc];
         }
         // layer 4 (linear), two tiles at a time, fused output transform
-        const float *b4 = bias + 384;
+        const float *b4 = bias + 384, *ysd = bias + 640, *ymn = bias + 896;
         float *tau_row = A.tau + row * MLP_G;
         float *pf_row = A.pfrac + row * MLP_G;
 #pragma unroll 1
@@ -345,7 +347,7 @@ __global__ __launch_bounds__(512) void mlp_infer_kernel(MlpInferArgs A)
 #pragma unroll
                         for (int i = 0; i < 4; ++i) {
                             const int n = n0 + i;
-                            const float y = (acc[u][4 * r4 + i] + b4[n]) * A.ystd[n] + A.ymean[n];
+                            const float y = (acc[u][4 * r4 + i] + b4[n]) * ysd[n] + ymn[n];
                             op[i] = nt < 4 ? __expf(y) * col_dry : fmaxf(y, 0.0f);
                         }
                         if (nt < 4) *(float4 *)(tau_row + n0) = o;
@@ -379,6 +381,7 @@ struct ccpprad_mlp {
     __bf16 *p16 = nullptr;                                          // BF16 copies
     unsigned char *image = nullptr;                                 // inference LDS image
     float *ymean = nullptr, *ystd = nullptr;
+    float hymean[MLP_OUT] = {}, hystd[MLP_OUT] = {};   // host copies for the inference image
     float xmean[MLP_IN], xistd[MLP_IN], lo[MLP_IN], hi[MLP_IN];
     int step = 0;
     // training work buffers (rows capacity)
@@ -441,7 +444,7 @@ extern "C" void ccpprad_mlp_destroy(ccpprad_mlp *m)
 }
 
 // Build the inference LDS image from the FP32 parameters (host side).
-static void build_image(const std::vector<float> &P, std::vector<unsigned char> &img)
+static void build_image(const std::vector<float> &P, const float *ystd, const float *ymean, std::vector<unsigned char> &img)
 {
     img.assign(MLP_LDS_BYTES, 0);
     __bf16 *fr = (__bf16 *)img.data();
@@ -473,6 +476,7 @@ static void build_image(const std::vector<float> &P, std::vector<unsigned char> 
     for (int i = 0; i < 3; ++i)
         for (int n = 0; n < 128; ++n) b[128 * i + n] = P[boff(i) + n];
     for (int n = 0; n < 256; ++n) b[384 + n] = P[boff(3) + n];
+    for (int n = 0; n < 256; ++n) { b[640 + n] = ystd[n]; b[896 + n] = ymean[n]; }
 }
 
 static int refresh(ccpprad_mlp *m)
@@ -480,7 +484,7 @@ static int refresh(ccpprad_mlp *m)
     std::vector<float> P(MLP_NPARAM);
     if (hipMemcpy(P.data(), m->p, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost) != hipSuccess) return 2;
     std::vector<unsigned char> img;
-    build_image(P, img);
+    build_image(P, m->hystd, m->hymean, img);
     if (hipMemcpy(m->image, img.data(), img.size(), hipMemcpyHostToDevice) != hipSuccess) return 2;
     hipLaunchKernelGGL(to_bf16_kernel, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, 0, m->p, m->p16, (long)MLP_NPARAM);
     return hipDeviceSynchronize() == hipSuccess ? 0 : 2;
@@ -515,7 +519,8 @@ extern "C" int ccpprad_mlp_set_norm(ccpprad_mlp *m, const float *xmean, const fl
     if (hipMemcpy(m->ymean, ymean, sizeof(float) * MLP_OUT, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(m->ystd, ystd, sizeof(float) * MLP_OUT, hipMemcpyHostToDevice) != hipSuccess)
         return mlp_seterr(errmsg, errlen, "copy failed");
-    return 0;
+    for (int n = 0; n < MLP_OUT; ++n) { m->hymean[n] = ymean[n]; m->hystd[n] = ystd[n]; }
+    return refresh(m) ? mlp_seterr(errmsg, errlen, "refresh failed") : 0;
 }
 
 static int ensure_cap(ccpprad_mlp *m, int rows)
