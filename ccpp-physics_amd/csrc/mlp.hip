@@ -253,8 +253,11 @@ __global__ __launch_bounds__(512) void mlp_infer_kernel(MlpInferArgs A)
         const long row = tile * 32 + rl;
         const bool valid = row < nrows;
         const long rr = valid ? row : nrows - 1;
-        const int d = (int)(rr / A.nlay), lay = (int)(rr % A.nlay);
+        // rows are ordered column-fastest within a layer, so the 32 rows of a
+        // tile read consecutive columns of the (nCol, nLay) inputs
+        const int d = (int)(rr % A.ncols), lay = (int)(rr / A.ncols);
         const long q = (long)(A.col0 + d) + (long)A.ncol_ld * lay;
+        const long orow = (long)d * A.nlay + lay;   // output row, [col][lay] order
         // features (standardised, split into BF16 hi + lo) and dry-air column
         const double vh = A.vmr[0][q];
         float f[MLP_NIN];
@@ -325,8 +328,8 @@ __global__ __launch_bounds__(512) void mlp_infer_kernel(MlpInferArgs A)
         }
         // layer 4 (linear), two tiles at a time, fused output transform
         const float *b4 = bias + 384, *ysd = bias + 640, *ymn = bias + 896;
-        float *tau_row = A.tau + row * MLP_G;
-        float *pf_row = A.pfrac + row * MLP_G;
+        float *tau_row = A.tau + orow * MLP_G;
+        float *pf_row = A.pfrac + orow * MLP_G;
 #pragma unroll 1
         for (int np = 0; np < 4; ++np) {
             f32x16 acc[2] = {{}, {}};
@@ -357,7 +360,7 @@ __global__ __launch_bounds__(512) void mlp_infer_kernel(MlpInferArgs A)
             }
         }
         if (valid && half == 0) {
-            A.oob[row] = out ? 1 : 0;
+            A.oob[orow] = out ? 1 : 0;
             if (out) A.colflag[d] = 1;
         }
     }
