@@ -24,13 +24,20 @@
 namespace ccpprad {
 
 
-// Linear-in-tau source weight f(x) = (1-e^-x)/x - e^-x from e = e^-x and
-// E = 1 - e^-x.  Series below x = 0.1 (relative truncation error < 1e-9),
-// where the direct form cancels in FP32.
-__device__ __forceinline__ float lin_tau_fact(float x, float e, float E)
+// Transmittance e = e^{-x}, absorptance E = 1 - e^{-x} and the
+// linear-in-tau source weight f = E/x - e for x >= 0.  Below x = 0.25, f comes
+// from its Taylor series sum_{n>=1} (-1)^{n+1} n x^n / (n+1)! (7 terms,
+// truncation below 3e-9 relative; the direct form cancels in FP32 there) and
+// E = x (f + e) with no cancellation; above, E = 1 - e and f = E/x - e.
+__device__ __forceinline__ void lw_trans(float x, float &e, float &E, float &f)
 {
-    const float s = x * (0.5f - x * (1.0f / 3.0f - x * (0.125f - x * (1.0f / 30.0f - x * (1.0f / 144.0f - x * (1.0f / 840.0f))))));
-    return x < 0.1f ? s : E * frcp(x) - e;
+    e = exp_neg(x);
+    const float fs = x * (0.5f - x * (1.0f / 3.0f - x * (0.125f - x * (1.0f / 30.0f - x * (1.0f / 144.0f -
+                     x * (1.0f / 840.0f - x * (1.0f / 5760.0f)))))));
+    const bool small = x < 0.25f;
+    const float Eb = 1.0f - e;
+    f = small ? fs : Eb * frcp(x) - e;
+    E = small ? x * (fs + e) : Eb;
 }
 
 constexpr int LW_BATCH = 8;
@@ -144,9 +151,8 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
 #pragma unroll
             for (int a = 0; a < NANG; ++a) {
                 const float xa = tau * D[a];
-                float t, omt;
-                exp_pair(xa, t, omt);
-                const float f = lin_tau_fact(xa, t, omt);
+                float t, omt, f;
+                lw_trans(xa, t, omt, f);
                 I[a] = t * I[a] + (omt * ldn + 2.0f * f * dsrc);
                 fsum += W2pi[a] * I[a];
             }
@@ -198,9 +204,8 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
 #pragma unroll
             for (int a = 0; a < NANG; ++a) {
                 const float xa = tau * D[a];
-                float t, omt;
-                exp_pair(xa, t, omt);
-                const float f = lin_tau_fact(xa, t, omt);
+                float t, omt, f;
+                lw_trans(xa, t, omt, f);
                 I[a] = t * I[a] + (omt * lup + 2.0f * f * dsrc);
                 fsum += W2pi[a] * I[a];
                 if (JAC) { J[a] = t * J[a]; jsum += W2pi[a] * J[a]; }
