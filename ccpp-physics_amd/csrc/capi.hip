@@ -69,7 +69,7 @@ struct Workspace {
     hipStream_t stream = nullptr;  // the stream it serves
     bool own_stream = false;       // created for a host thread's staging path
     std::thread::id owner;         // that thread
-    DevBuf w_tau, w_aux, w_mask, w_ctau, w_cssa, w_cg, w_sb, w_blk, w_colf, w_day, w_idx;
+    DevBuf w_tau, w_aux, w_mask, w_ctau, w_sb, w_blk, w_colf, w_day, w_idx;
     DevBuf w_oob, w_colm, w_rec;
     std::vector<DevBuf> stage;     // host-path staging
     int *err = nullptr;            // device error word, then the SW last-wipe block
@@ -77,7 +77,7 @@ struct Workspace {
     float *toa32 = nullptr;
     void release()
     {
-        DevBuf *bufs[] = {&w_tau, &w_aux, &w_mask, &w_ctau, &w_cssa, &w_cg, &w_sb, &w_blk, &w_colf, &w_day, &w_idx,
+        DevBuf *bufs[] = {&w_tau, &w_aux, &w_mask, &w_ctau, &w_sb, &w_blk, &w_colf, &w_day, &w_idx,
                           &w_oob, &w_colm, &w_rec};
         for (DevBuf *b : bufs) b->release();
         for (auto &b : stage) b.release();
@@ -671,9 +671,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
     HIPCHK(w->w_aux.ensure(sizeof(float) * (size_t)CH * nL * G));
     HIPCHK(w->w_rec.ensure(sizeof(unsigned) * (size_t)CH * nL * gas_record_words(T)));
     HIPCHK(w->w_mask.ensure(sizeof(unsigned) * (size_t)CH * nL * MASK_WORDS));
-    HIPCHK(w->w_ctau.ensure(sizeof(float) * (size_t)CH * nL * B));
-    HIPCHK(w->w_cssa.ensure(sizeof(float) * (size_t)CH * nL * B));
-    HIPCHK(w->w_cg.ensure(sizeof(float) * (size_t)CH * nL * B));
+    HIPCHK(w->w_ctau.ensure(sizeof(float3) * (size_t)CH * nL * B));
     HIPCHK(w->w_sb.ensure(sizeof(double) * (size_t)nD * 3 * B));
     HIPCHK(w->w_blk.ensure((size_t)nblk + 1));
     HIPCHK(w->w_colf.ensure((size_t)nD + 1));
@@ -698,7 +696,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
         const int nc = std::min(CH, nD - c0);
         float *tabs = (float *)w->w_tau.p, *tray = (float *)w->w_aux.p;
         unsigned *mask = (unsigned *)w->w_mask.p;
-        float *ctau = (float *)w->w_ctau.p, *cssa = (float *)w->w_cssa.p, *cg = (float *)w->w_cg.p;
+        float3 *copt = (float3 *)w->w_ctau.p;
         if (have_clouds) {
             CloudArgs ca{};
             ca.nlay = nL; ca.ncol_ld = nC; ca.col0 = c0; ca.blksz = bs; ca.nband = B; ca.is_lw = 0;
@@ -708,7 +706,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
             if (a->doGP_sgs_pbl) { ca.pbl[0] = a->cld_pbl_lwp; ca.pbl[1] = a->cld_pbl_iwp; ca.pbl[2] = a->cld_pbl_reliq; ca.pbl[3] = a->cld_pbl_reice; }
             ca.rwp = a->cld_rwp; ca.swp = a->cld_swp; ca.resnow = a->cld_resnow;
             if (!ca.rwp || !ca.swp || !ca.resnow) { seterr(errmsg, errlen, "cld_rwp/cld_swp/cld_resnow required with clouds"); return 1; }
-            ca.ctau = ctau; ca.cssa = cssa; ca.cg = cg; ca.err = w->err;
+            ca.copt = copt; ca.err = w->err;
             // only the last block's band-11 cloud depth survives the per-block reset
             ca.cldtau = (c0 + nc > last_blk_start) ? a->cldtausw : nullptr;
             ca.cldtau_blk = nblk - 1;
@@ -725,7 +723,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
         SwSolveArgs s{};
         s.nlay = nL; s.ncol_ld = nC; s.col0 = c0; s.top_at_1 = a->top_at_1; s.blksz = bs; s.isfc = a->iSFC - 1;
         s.write_clr_zero = !a->doSWclrsky; s.role = 0;
-        s.col_idx = a->idx; s.tau_abs = tabs; s.tau_ray = tray; s.mask = mask; s.ctau = ctau; s.cssa = cssa; s.cg = cg;
+        s.col_idx = a->idx; s.tau_abs = tabs; s.tau_ray = tray; s.mask = mask; s.copt = copt;
         s.blkflag = blk; s.aer_tau = a->aersw_tau; s.aer_ssa = a->aersw_ssa; s.aer_g = a->aersw_g;
         s.coszen = a->coszen; s.alb_nir_dir = a->sfc_alb_nir_dir; s.alb_nir_dif = a->sfc_alb_nir_dif;
         s.alb_uv_dir = a->sfc_alb_uvvis_dir; s.alb_uv_dif = a->sfc_alb_uvvis_dif; s.p_lev = a->p_lev;

@@ -104,8 +104,8 @@ __global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs 
     const int blk = od / A.blksz;
     const size_t q = (size_t)c + (size_t)A.ncol_ld * l;
     if (!A.blkflag[blk]) {
-        A.ctau[i] = 0.0f;
-        if (A.cssa) { A.cssa[i] = 0.0f; A.cg[i] = 0.0f; }
+        if (A.is_lw) A.ctau[i] = 0.0f;
+        else A.copt[i] = make_float3(0.0f, 0.0f, 0.0f);
         return;
     }
     const float lwp = (float)A.lwp[q], iwp = (float)A.iwp[q];
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs 
     if (A.is_lw) {
         A.ctau[i] = o.t * o.p;  // 1scl += 2str: absorption depth tau * (1 - ssa)
     } else {
-        A.ctau[i] = o.t; A.cssa[i] = o.p; A.cg[i] = o.g;   // SW: tau, co-albedo, g
+        A.copt[i] = make_float3(o.t, o.p, o.g);   // SW: tau, co-albedo, g in one 12-byte record
     }
     if (bad) atomicOr(A.err, 2);
 }

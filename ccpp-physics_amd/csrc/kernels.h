@@ -44,7 +44,8 @@ struct CloudArgs {
     const double *cld_frac, *lwp, *iwp, *reliq, *reice;
     const double *cnv[4], *pbl[4];  // lwp, iwp, reliq, reice (null = off)
     const double *rwp, *swp, *resnow;
-    float *ctau, *cssa, *cg;  // [d][l][b]: LW absorption depth; SW tau, co-albedo (1 - ssa), g
+    float *ctau;              // LW: [d][l][b] absorption depth
+    float3 *copt;             // SW: [d][l][b] (tau, co-albedo 1 - ssa, g)
     double *cldtau;           // SW: (ncol_ld, nlay) band-11 cloud tau, or null
     int cldtau_blk;           // the one scheme block whose columns write cldtau
     int *err;
@@ -70,7 +71,7 @@ struct SwSolveArgs {
     const int *col_idx;
     const float *tau_abs, *tau_ray;        // [d][l][g]
     const unsigned *mask;                  // [d][l][MASK_WORDS] or null
-    const float *ctau, *cssa, *cg;         // [d][l][nband] tau, co-albedo (1 - ssa), g; or null
+    const float3 *copt;                    // [d][l][nband] (tau, co-albedo 1 - ssa, g) or null
     const unsigned char *blkflag;
     const double *aer_tau, *aer_ssa, *aer_g;  // (ncol_ld, nlay, nband) or null
     const double *coszen, *alb_nir_dir, *alb_nir_dif, *alb_uv_dir, *alb_uv_dif, *p_lev;

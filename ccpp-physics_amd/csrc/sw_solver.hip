@@ -21,6 +21,8 @@
 // absorption of nearly conservative layers is not lost to FP32 rounding.
 #include "kernels.h"
 
+#include <type_traits>
+
 namespace ccpprad {
 
 
@@ -113,10 +115,11 @@ constexpr int SW_SEG = 8;
 size_t sw_solve_smem(int nlay, int nband)
 {
     const int nseg = (nlay + SW_SEG - 1) / SW_SEG;
-    return sizeof(double) * (4 * (size_t)(nlay + 1) + 6 * (size_t)nband) + sizeof(float) * 2 * (size_t)(nseg + 1) * 128;
+    return sizeof(double) * (4 * (size_t)(nlay + 1) + 6 * (size_t)nband) + sizeof(float) * 2 * (size_t)(nseg + 1) * 128 +
+           2 * sizeof(unsigned long long);
 }
 
-template <bool CLOUDS>
+template <bool CLOUDS, bool AER>
 __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveArgs A)
 {
     // LDS: [2 waves][2][nlev] level sums, [2 waves][3][nband] band partials,
@@ -139,6 +142,7 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     double *rb = red + 4 * (size_t)nlev;  // [wave][3][nband]
     float *cX = (float *)(red + 4 * (size_t)nlev + 6 * (size_t)nb);
     float *cQ = cX + (size_t)(nseg + 1) * 128;
+    unsigned long long *sflag = (unsigned long long *)(cQ + (size_t)(nseg + 1) * 128);  // [2 waves]
 
     const double mu0d = A.coszen[c];
     const float mu0 = (float)mu0d;
@@ -151,46 +155,92 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
         adif = (float)(0.5 * (A.alb_nir_dif[c] + A.alb_uv_dif[c]));
     } else if (cat == 2) { adir = (float)A.alb_uv_dir[c]; adif = (float)A.alb_uv_dif[c]; }
 
+    auto lay = [&](int k) { return top ? k : nlay - 1 - k; };       // k counts layers from the TOA
+    auto lev = [&](int k) { return top ? k : nlay - k; };           // level k is the top of layer k
+
     // Column bases (uniform) with 32-bit in-column offsets.
     const float *ta_c = A.tau_abs + (size_t)d * nlay * ngpt, *tr_c = A.tau_ray + (size_t)d * nlay * ngpt;
     const unsigned *mk_c = CLOUDS ? A.mask + (size_t)d * nlay * MASK_WORDS : nullptr;
-    const float *ct_c = CLOUDS ? A.ctau + (size_t)d * nlay * nb : nullptr;
-    const float *cp_c = CLOUDS ? A.cssa + (size_t)d * nlay * nb : nullptr;
-    const float *cg_c = CLOUDS ? A.cg + (size_t)d * nlay * nb : nullptr;
+    const float3 *co_c = CLOUDS ? A.copt + (size_t)d * nlay * nb : nullptr;
     const unsigned mbit = 1u << (gg & 31), mword = (unsigned)gg >> 5;
+
+    // Segments with a cloudy g-point in any layer: bit s of segc for segment
+    // s (segments from 64 on count as cloudy).  Clear segments skip the mask
+    // and cloud loads; both passes walk the same segments.
+    unsigned long long segc = 0;
+    if (CLOUDS && cloudy) {
+        unsigned long long mine = 0;
+        for (int k0 = wave * 64; k0 < nlay; k0 += 128) {
+            const int k = k0 + lane;
+            bool f = false;
+            if (k < nlay) {
+                const uint4 m = *(const uint4 *)(mk_c + (unsigned)lay(k) * MASK_WORDS);
+                f = (m.x | m.y | m.z | m.w) != 0u;
+            }
+            const unsigned long long bal = __ballot(f);
+            for (int i = 0; i < 64 / SW_SEG; ++i) {
+                const int sg = k0 / SW_SEG + i;
+                if (sg < 64 && ((bal >> (SW_SEG * i)) & ((1ull << SW_SEG) - 1))) mine |= 1ull << sg;
+            }
+        }
+        if (lane == 0) sflag[wave] = mine;
+        __syncthreads();
+        const unsigned long long both = sflag[0] | sflag[1];
+        segc = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(both >> 32)) << 32) |
+               (unsigned)__builtin_amdgcn_readfirstlane((unsigned)both);
+    }
+    auto seg_cloudy = [&](int sg) { return CLOUDS && cloudy && (sg >= 64 || ((segc >> sg) & 1ull)); };
+
     const float EPS3 = 3.0f * 1.17549435e-38f;
     // combine_and_reorder_2str + aerosol and McICA cloud increments
     // (increment_2stream_by_2stream), accumulated as extinction, scattering,
     // absorption and scattering * g depths and normalised once.
-    auto optics = [&](int l) -> Opt2s {
-        const unsigned og = (unsigned)l * ngpt + gg;
-        const float ta = ta_c[og], tr = tr_c[og];
-        float te = ta + tr, ts = tr, tab = ta, tsg = 0.0f;
-        if (A.aer_tau) {
-            const size_t q = (size_t)c + (size_t)ld * (l + (size_t)nlay * b);
-            const float t1 = (float)A.aer_tau[q], w1 = (float)A.aer_ssa[q];
-            const float s1 = t1 * w1;
-            te = t1 + te; ts += s1; tab += t1 * (float)(1.0 - A.aer_ssa[q]); tsg += s1 * (float)A.aer_g[q];
+    // One segment's optics.  Every load of the segment is issued before the
+    // first use, with no branch between them: layers past the column end
+    // read the last layer again (their results are not used), and cloud
+    // values are selected by the mask bit rather than loaded under it.
+    auto load_seg = [&](int sg, Opt2s *o, auto cl) {
+        float ta[SW_SEG], tr[SW_SEG], at[SW_SEG], aw[SW_SEG], ap[SW_SEG], ag[SW_SEG];
+        unsigned mw[SW_SEG];
+        float3 cv[SW_SEG];
+#pragma unroll
+        for (int j = 0; j < SW_SEG; ++j) {
+            const int l = lay(min(sg * SW_SEG + j, nlay - 1));
+            const unsigned og = (unsigned)l * ngpt + gg;
+            ta[j] = ta_c[og]; tr[j] = tr_c[og];
+            if (AER) {
+                const size_t q = (size_t)c + (size_t)ld * (l + (size_t)nlay * b);
+                const double sd = A.aer_ssa[q];
+                at[j] = (float)A.aer_tau[q]; aw[j] = (float)sd; ap[j] = (float)(1.0 - sd); ag[j] = (float)A.aer_g[q];
+            }
+            if (cl.value) {
+                mw[j] = mk_c[(unsigned)l * MASK_WORDS + mword];
+                cv[j] = co_c[(unsigned)l * nb + b];
+            }
         }
-        if (cloudy) {
-            const unsigned ob = (unsigned)l * nb + b;
-            // unconditional loads, selected by the mask bit: no load waits on another
-            const unsigned mw = mk_c[(unsigned)l * MASK_WORDS + mword];
-            const float ctv = ct_c[ob], cpv = cp_c[ob], cgv = cg_c[ob];
-            const bool m = mw & mbit;
-            const float tc = m ? ctv : 0.0f, pc = m ? cpv : 0.0f, gc = m ? cgv : 0.0f;
-            const float sc = tc - tc * pc;
-            te = tc + te; ts += sc; tab += tc * pc; tsg += sc * gc;
+#pragma unroll
+        for (int j = 0; j < SW_SEG; ++j) {
+            float te = ta[j] + tr[j], ts = tr[j], tab = ta[j], tsg = 0.0f;
+            if (AER) {
+                const float s1 = at[j] * aw[j];
+                te = at[j] + te; ts += s1; tab += at[j] * ap[j]; tsg += s1 * ag[j];
+            }
+            if (cl.value) {
+                const bool m = mw[j] & mbit;
+                const float tc = m ? cv[j].x : 0.0f, pc = m ? cv[j].y : 0.0f, gc = m ? cv[j].z : 0.0f;
+                const float sc = tc - tc * pc;
+                te = tc + te; ts += sc; tab += tc * pc; tsg += sc * gc;
+            }
+            o[j].t = te;
+            o[j].w = ts * frcp(fmaxf(EPS3, te));
+            o[j].p = te > EPS3 ? tab * frcp(te) : 1.0f;
+            o[j].g = tsg * frcp(fmaxf(EPS3, ts));
         }
-        Opt2s o;
-        o.t = te;
-        o.w = ts * frcp(fmaxf(EPS3, te));
-        o.p = te > EPS3 ? tab * frcp(te) : 1.0f;
-        o.g = tsg * frcp(fmaxf(EPS3, ts));
-        return o;
     };
-    auto lay = [&](int k) { return top ? k : nlay - 1 - k; };       // k counts layers from the TOA
-    auto lev = [&](int k) { return top ? k : nlay - k; };           // level k is the top of layer k
+    auto optics_seg = [&](int sg, Opt2s *o) {
+        if (seg_cloudy(sg)) load_seg(sg, o, std::true_type{});
+        else load_seg(sg, o, std::false_type{});
+    };
     const float toa = T.toa_norm[gg];
 
     // ---- pass 1, bottom-up: adding albedo A and the direct-beam source
@@ -204,11 +254,7 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     cQ[(size_t)nseg * 128 + g] = qb;
     for (int s = nseg - 1; s >= 0; --s) {
         Opt2s o[SW_SEG];
-#pragma unroll
-        for (int j = 0; j < SW_SEG; ++j) {
-            const int k = s * SW_SEG + j;
-            if (k < nlay) o[j] = optics(lay(k));
-        }
+        optics_seg(s, o);
 #pragma unroll
         for (int j = SW_SEG - 1; j >= 0; --j) {
             const int k = s * SW_SEG + j;
@@ -234,14 +280,15 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
         const double sdn = wave_sum(act ? (double)(dnd + fdir) : 0.0);
         if (lane == 0) { rup[lev_toa] = su; rdn[lev_toa] = sdn; }
     }
+    // direct and total down flux at the surface level (TOA values if the
+    // surface level is the TOA)
+    float sfc_dn = fdir, sfc_dir = fdir;
     for (int s = 0; s < nseg; ++s) {
         Opt2s o[SW_SEG];
-        float Rd[SW_SEG], Td[SW_SEG], Tdr[SW_SEG], T0[SW_SEG], Abl[SW_SEG], qbl[SW_SEG], dnv[SW_SEG];
-#pragma unroll
-        for (int j = 0; j < SW_SEG; ++j) {
-            const int k = s * SW_SEG + j;
-            if (k < nlay) o[j] = optics(lay(k));
-        }
+        // per layer: diffuse down below = al * above + be * F_dir above,
+        // direct transmittance T0, and A, q below the layer
+        float al[SW_SEG], be[SW_SEG], T0[SW_SEG], Abl[SW_SEG], qbl[SW_SEG];
+        optics_seg(s, o);
         float a = cX[(size_t)(s + 1) * 128 + g], q = cQ[(size_t)(s + 1) * 128 + g];
 #pragma unroll
         for (int j = SW_SEG - 1; j >= 0; --j) {
@@ -249,7 +296,10 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
             if (k >= nlay) continue;
             const TwoStream ts = two_stream32(mu0, mu0_inv, o[j]);
             const float den = frcp(1.0f - ts.Rd * a);
-            Rd[j] = ts.Rd; Td[j] = ts.Td; Tdr[j] = ts.Tdir; T0[j] = ts.T0; Abl[j] = a; qbl[j] = q; dnv[j] = den;
+            // dn_dif below = (Td dn_dif + Rd Q + Tdir F_dir) den with Q = q T0 F_dir
+            al[j] = ts.Td * den;
+            be[j] = (ts.Rd * q * ts.T0 + ts.Tdir) * den;
+            T0[j] = ts.T0; Abl[j] = a; qbl[j] = q;
             const float qa = ts.Rdir + ts.Td * den * (q * ts.T0 + a * ts.Tdir);
             a = ts.Rd + ts.Td * ts.Td * a * den;
             q = qa;
@@ -259,31 +309,23 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
             const int k = s * SW_SEG + j;
             if (k >= nlay) continue;
             const int lev_below = lev(k + 1);
-            const float sd = Tdr[j] * fdir;           // direct-beam source into the layer below
             const float fd = T0[j] * fdir;            // direct beam at the level below
-            const float Qb = qbl[j] * fd;
-            dnd = (Td[j] * dnd + Rd[j] * Qb + sd) * dnv[j];
-            const float up = dnd * Abl[j] + Qb;
+            dnd = al[j] * dnd + be[j] * fdir;
+            const float up = dnd * Abl[j] + qbl[j] * fd;
             const float dn = dnd + fd;
             fdir = fd;
             const double su = wave_sum(act ? (double)up : 0.0);
             const double sdn = wave_sum(act ? (double)dn : 0.0);
             if (lane == 0) { rup[lev_below] = su; rdn[lev_below] = sdn; }
-            if (lev_below == A.isfc) {
-                // per-band surface sums for scmpsw
-                const double vdn = act ? (double)dn : 0.0, vdir = act ? (double)fd : 0.0;
-                for (int bb = 0; bb < nb; ++bb) {
-                    const bool mine = act && b == bb;
-                    const double s1 = wave_sum(mine ? vdn : 0.0), s2 = wave_sum(mine ? vdir : 0.0);
-                    if (lane == 0) { rb[(size_t)wave * 3 * nb + bb] = s1; rb[(size_t)wave * 3 * nb + nb + bb] = s2; }
-                }
-            }
+            if (lev_below == A.isfc) { sfc_dn = dn; sfc_dir = fd; }
         }
     }
-    if (A.isfc == lev_toa) {
+    {
+        // per-band surface sums for scmpsw
+        const double vdn = act ? (double)sfc_dn : 0.0, vdir = act ? (double)sfc_dir : 0.0;
         for (int bb = 0; bb < nb; ++bb) {
             const bool mine = act && b == bb;
-            const double s1 = wave_sum(mine ? (double)(toa * mu0) : 0.0), s2 = s1;
+            const double s1 = wave_sum(mine ? vdn : 0.0), s2 = wave_sum(mine ? vdir : 0.0);
             if (lane == 0) { rb[(size_t)wave * 3 * nb + bb] = s1; rb[(size_t)wave * 3 * nb + nb + bb] = s2; }
         }
     }
@@ -321,10 +363,15 @@ void launch_sw_solve(const TablesView &T, const SwSolveArgs &A, int ncols, bool 
 {
     if (ncols <= 0) return;
     const size_t smem = sw_solve_smem(A.nlay, T.nband);
-    if (clouds)
-        hipLaunchKernelGGL(sw_solve_kernel<true>, dim3(ncols), dim3(128), smem, st, T, A);
+    const bool aer = A.aer_tau != nullptr;
+    if (clouds && aer)
+        hipLaunchKernelGGL((sw_solve_kernel<true, true>), dim3(ncols), dim3(128), smem, st, T, A);
+    else if (clouds)
+        hipLaunchKernelGGL((sw_solve_kernel<true, false>), dim3(ncols), dim3(128), smem, st, T, A);
+    else if (aer)
+        hipLaunchKernelGGL((sw_solve_kernel<false, true>), dim3(ncols), dim3(128), smem, st, T, A);
     else
-        hipLaunchKernelGGL(sw_solve_kernel<false>, dim3(ncols), dim3(128), smem, st, T, A);
+        hipLaunchKernelGGL((sw_solve_kernel<false, false>), dim3(ncols), dim3(128), smem, st, T, A);
 }
 
 // scmpsw (cmpfsw_type) from the surface band fluxes, reproducing
