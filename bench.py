@@ -131,7 +131,7 @@ def main():
         L.ccpprad_timing_enable(s.ctx, 1)
     step(serial=True)
     kt = {}
-    names = {0: "gas_optics", 1: "lw_solve", 2: "sw_solve", 3: "cloud_optics", 4: "mcica"}
+    names = {6: "gas_interp", 0: "gas_optics", 1: "lw_solve", 2: "sw_solve", 3: "cloud_optics", 4: "mcica"}
     for tag, s in (("lw", lw), ("sw", sw)):
         for kid, nm in names.items():
             tot, cnt = C.c_double(), C.c_int()
@@ -149,8 +149,10 @@ def main():
     per_col = {
         "lw_lw_solve": 269_308 + 2_048 + 3_072 + 1_016,
         "sw_sw_solve": 193_000 + 3_120 + 3_072 + 1_016,
-        "lw_gas_optics": 4_584 + 261_120,
-        "sw_gas_optics": 4_576 + 171_136,
+        "lw_gas_interp": 4_584,          # gas optics inputs (interpolation stage)
+        "lw_gas_optics": 261_120,        # tau + Planck fraction out (g-point gathers)
+        "sw_gas_interp": 4_576,
+        "sw_gas_optics": 171_136,        # tau_abs + tau_Rayleigh out
         "lw_cloud_optics": 2_540 + 8_128,
         "sw_cloud_optics": 2_540 + 21_336,
     }
@@ -207,7 +209,8 @@ def main():
             L.ccpprad_timing_enable(lw_nn.ctx, 1)
             lw_nn.run_dev(lw_ints, nn_args, stream=sp, validate=False)
             nkt = {}
-            for kid, nm in {0: "gas_optics_fallback", 1: "lw_solve", 3: "cloud_optics", 4: "mcica", 5: "mlp_infer"}.items():
+            for kid, nm in {6: "gas_interp_fallback", 0: "gas_optics_fallback", 1: "lw_solve", 3: "cloud_optics",
+                            4: "mcica", 5: "mlp_infer"}.items():
                 tot, cnt = C.c_double(), C.c_int()
                 L.ccpprad_timing_query(lw_nn.ctx, kid, C.byref(tot), C.byref(cnt))
                 if cnt.value:

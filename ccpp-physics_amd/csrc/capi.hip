@@ -132,7 +132,7 @@ static Workspace *ws_for_stream(ccpprad_ctx *c, hipStream_t st)
     return w;
 }
 
-enum { K_GAS = 0, K_LWSOLVE = 1, K_SWSOLVE = 2, K_CLOUD = 3, K_MCICA = 4, K_MLP = 5, K_N = 6 };
+enum { K_GAS = 0, K_LWSOLVE = 1, K_SWSOLVE = 2, K_CLOUD = 3, K_MCICA = 4, K_MLP = 5, K_INTERP = 6, K_N = 7 };
 
 static hipEvent_t ev_take(ccpprad_ctx *c)
 {
@@ -256,6 +256,9 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
     if (blksz < 1) { seterr(errmsg, errlen, "rrtmgp_phys_blksz must be positive"); return 1; }
     if ((want_lw ? lw_solve_smem(nLay, kd->nband) : sw_solve_smem(nLay, kd->nband)) > 64 * 1024) {
         seterr(errmsg, errlen, "nLay=%d needs more than the solver's 64 KiB of LDS per column", nLay); return 1;
+    }
+    if (gas_gather_smem(want_lw, kd->nflav, kd->minor_lower.n, kd->minor_upper.n, kd->ngpt, kd->neta) > 64 * 1024) {
+        seterr(errmsg, errlen, "table dimensions need more than the gas optics' 64 KiB of LDS per block"); return 1;
     }
     if (want_lw && (!kd->planck_frac || !kd->totplnk || kd->nplanck < 2)) { seterr(errmsg, errlen, "LW tables lack Planck data"); return 1; }
     if (!want_lw && (!kd->krayl_lower || !kd->krayl_upper || !kd->solar_src)) { seterr(errmsg, errlen, "SW tables lack Rayleigh/solar data"); return 1; }
@@ -578,10 +581,11 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
             launch_fill_u8(mi.colflag, (size_t)nc, st);
             { KScope k_(c, K_MLP, st); launch_mlp_infer(c->mlp, mi, st); }
             // layers outside the training hull: table gas optics
-            { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)w->w_rec.p, tau, pf, w->err, st, mi.oob, mi.colflag); }
+            { KScope k_(c, K_INTERP, st); launch_gas_interp(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)w->w_rec.p, w->err, st, mi.colflag); }
+            { KScope k_(c, K_GAS, st); launch_gas_gather(T, nc, nL, (const unsigned *)w->w_rec.p, tau, pf, st, mi.oob, mi.colflag); }
         } else {
-            KScope k_(c, K_GAS, st);
-            launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)w->w_rec.p, tau, pf, w->err, st);
+            { KScope k_(c, K_INTERP, st); launch_gas_interp(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)w->w_rec.p, w->err, st); }
+            { KScope k_(c, K_GAS, st); launch_gas_gather(T, nc, nL, (const unsigned *)w->w_rec.p, tau, pf, st); }
         }
         LwSolveArgs s{};
         s.nlay = nL; s.ncol_ld = nC; s.col0 = c0; s.top_at_1 = a->top_at_1;
@@ -719,7 +723,8 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
             ma.icseed = a->icseed_sw; ma.mtpow = c->mtpow; ma.mask = mask;
             { KScope k_(c, K_MCICA, st); launch_mcica(ma, nc, st); }
         }
-        { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, a->idx, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)w->w_rec.p, tabs, tray, w->err, st); }
+        { KScope k_(c, K_INTERP, st); launch_gas_interp(T, nc, nL, nC, a->idx, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)w->w_rec.p, w->err, st); }
+        { KScope k_(c, K_GAS, st); launch_gas_gather(T, nc, nL, (const unsigned *)w->w_rec.p, tabs, tray, st); }
         SwSolveArgs s{};
         s.nlay = nL; s.ncol_ld = nC; s.col0 = c0; s.top_at_1 = a->top_at_1; s.blksz = bs; s.isfc = a->iSFC - 1;
         s.write_clr_zero = !a->doSWclrsky; s.role = 0;

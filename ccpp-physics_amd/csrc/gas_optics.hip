@@ -18,8 +18,9 @@
 //       contiguous block, assembled in LDS and written with 16-byte stores.
 //   gas_optics_kernel (FP32), block = (16-layer chunk, column), chunk slowest
 //       so co-resident blocks share k-table rows in L2, lane = g-point: the
-//       chunk's records are staged into LDS, then 8 major-table corners (+8
-//       Planck-fraction corners LW, +4 Rayleigh corners SW) and the minor
+//       chunk's records are staged into LDS with the per-(layer, item) table
+//       offsets and weights built once per block, then 8 major-table corners
+//       (+8 Planck-fraction corners LW, +4 Rayleigh corners SW) and the minor
 //       intervals covering the g-point, two layers' gathers in flight.
 // Output tau / pfrac / tau_ray : [col][lay][gpt] FP32.
 //
@@ -36,7 +37,6 @@
 namespace ccpprad {
 
 constexpr int LCHUNK = 16;
-constexpr int REC_MAX = 4 * ((3 + 6 * MAX_FLAV + (2 * MAX_FLAV + 3) / 4 + MAX_MINOR + 1 + 3) / 4);
 
 // record geometry shared by both kernels
 struct RecGeom {
@@ -189,10 +189,32 @@ __device__ __forceinline__ int mi_ci(unsigned v) { return (int)(v & 0xFFFFu); }
 __device__ __forceinline__ int mi_iv(unsigned v) { return (int)((v >> 16) & 0xFFu); }
 __device__ __forceinline__ int mi_fv(unsigned v) { return (int)(v >> 24); }
 
-// Gathers through a uniform table base and a 32-bit byte offset.
+// Gathers through a uniform table base and a 32-bit byte offset, as global
+// loads: a generic (flat) load would also count against the LDS counter, so
+// every LDS read of the record would wait for the table loads in flight.
+typedef const __attribute__((address_space(1))) char gchar;
+typedef const __attribute__((address_space(1))) float gfloat;
 __device__ __forceinline__ float ldt(const float *base, unsigned idx)
 {
-    return *(const float *)((const char *)base + (idx << 2));
+    return *(const gfloat *)((gchar *)base + (idx << 2));
+}
+
+// Per (layer, item) constants of a chunk, built once per block so that the
+// lanes of a flavor do not each redo the index arithmetic:
+//   quad 0: major-table element offset of the (jtemp + t, jpress, jeta)
+//           corner without the g-point, fe0, fe1 (eta weights times the
+//           temperature weight), col_mix
+//   quad 1: (jtemp + t) * neta + jeta - 1 (row of the minor and Rayleigh
+//           tables), fe0, fe1, 0
+__host__ __device__ inline size_t gather_smem(const RecGeom &G)
+{
+    return (size_t)LCHUNK * (sizeof(unsigned) * G.words + 2 * sizeof(uint4) * G.nf2);
+}
+
+size_t gas_gather_smem(int is_lw, int nflav, int nminor0, int nminor1, int ngpt, int neta)
+{
+    (void)is_lw; (void)ngpt; (void)neta;
+    return gather_smem(rec_geom(nflav, nminor0, nminor1));
 }
 
 template <bool LW, int NM>
@@ -201,7 +223,7 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
     float *__restrict__ aux /* LW pfrac, SW tau_ray */, const unsigned char *__restrict__ row_mask,
     const unsigned char *__restrict__ col_mask)
 {
-    __shared__ unsigned srec[LCHUNK * REC_MAX];
+    extern __shared__ uint4 smem_q[];
     const int chunk = blockIdx.x / ncols;
     const int d = blockIdx.x - chunk * ncols;  // column within this launch
     const int tid = threadIdx.x;
@@ -216,6 +238,8 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
     }
     const RecGeom G = rec_geom(T.nflav, T.nminor[0], T.nminor[1]);
     const int ngpt = T.ngpt, neta = T.neta, np1 = T.npres + 1;
+    uint4 *sitem = smem_q;                                                   // [LCHUNK][nf2][2]
+    unsigned *srec = (unsigned *)(smem_q + (size_t)LCHUNK * G.nf2 * 2);    // [LCHUNK][words]
 
     // stage the chunk's records (layer-major, one contiguous record per
     // layer) into LDS, 16 B per thread
@@ -249,54 +273,59 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
             mit[r][ii] = v;
         }
     }
-    const float *km_r[2] = {T.kminor[0], T.kminor[1]};
-    const unsigned nc_r[2] = {(unsigned)T.ncontrib[0], (unsigned)T.ncontrib[1]};
+    const unsigned nc0 = (unsigned)T.ncontrib[0], nc1 = (unsigned)T.ncontrib[1];
     const unsigned dp_ = (unsigned)(neta * ngpt), de = (unsigned)ngpt;
+    __syncthreads();
+
+    // item constants of every (layer, item) of the chunk
+    for (int k = tid; k < nl * G.nf2; k += blockDim.x) {
+        const int j = k / G.nf2, it = k - j * G.nf2;
+        const unsigned *r = srec + j * G.words;
+        const unsigned w0 = r[0];
+        const unsigned itr = w0 >> 16;
+        const unsigned jt = (w0 & 0xFFu) - 1u + (unsigned)(it & 1);
+        const unsigned jp = ((w0 >> 8) & 0xFFu) + itr - 2u;   // 0-based lower pressure row
+        const unsigned je = ((r[G.je + (it >> 2)] >> (8 * (it & 3))) & 0xFFu) - 1u;
+        const unsigned row = __umul24(jt, (unsigned)neta) + je;
+        const unsigned ko = __umul24(__umul24(__umul24(jt, (unsigned)np1) + jp, (unsigned)neta) + je, (unsigned)ngpt);
+        const unsigned fe0 = r[3 + 3 * it], fe1 = r[4 + 3 * it], cm = r[5 + 3 * it];
+        sitem[2 * k] = make_uint4(ko, fe0, fe1, cm);
+        sitem[2 * k + 1] = make_uint4(row, fe0, fe1, 0u);
+    }
     __syncthreads();
     if (!act) return;
 
-    auto R = [&](int j) { return srec + j * G.words; };
-    auto fw = [&](const unsigned *r, int it, int k) { return __uint_as_float(r[3 + 3 * it + k]); };
-    auto je_of = [&](const unsigned *r, int it) { return (int)((r[G.je + (it >> 2)] >> (8 * (it & 3))) & 0xFFu); };
-
     // two layers at a time: every table load of both layers (major, Planck
-    // fraction or Rayleigh, minor) is issued before any arithmetic
+    // fraction or Rayleigh, minor) is issued before any arithmetic; an odd
+    // last layer is computed twice and stored once
     for (int j0 = 0; j0 < nl; j0 += 2) {
-        const bool two = j0 + 1 < nl;
         float k[2][2][4], q[2][2][4], kr[2][2][2], km[2][NM > 0 ? NM : 1][2][2];
-        float w[2][2][2], cmv[2][2], fp1v[2];
         int regv[2];
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-            if (u == 1 && !two) break;
-            const unsigned *r = R(j0 + u);
-            const unsigned w0 = __builtin_amdgcn_readfirstlane(r[0]);
-            const int itr = (int)(w0 >> 16), reg = itr - 1;
+            const int j = min(j0 + u, nl - 1);
+            const unsigned *r = srec + j * G.words;
+            const int reg = (int)(__builtin_amdgcn_readfirstlane(r[0]) >> 16) - 1;
             regv[u] = reg;
             const int f = reg == 0 ? gfl0 : gfl1;
-            const unsigned jt0 = (w0 & 0xFFu) - 1u;
-            const unsigned jp0 = ((w0 >> 8) & 0xFFu) + (unsigned)itr - 2u;  // 0-based lower pressure row
-            fp1v[u] = __uint_as_float(r[1]);
+            const uint4 *Ij = sitem + (size_t)j * G.nf2 * 2;
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
-                const int it = 2 * f + t;
-                const unsigned je0 = (unsigned)je_of(r, it) - 1u;
-                w[u][t][0] = fw(r, it, 0);
-                w[u][t][1] = fw(r, it, 1);
-                cmv[u][t] = fw(r, it, 2);
-                const unsigned base = (((jt0 + t) * np1 + jp0) * neta + je0) * ngpt + g;
+                const uint4 a = Ij[(2 * f + t) * 2];
+                const unsigned base = a.x + (unsigned)g;
                 k[u][t][0] = ldt(T.kmajor, base); k[u][t][1] = ldt(T.kmajor, base + dp_);
                 k[u][t][2] = ldt(T.kmajor, base + de); k[u][t][3] = ldt(T.kmajor, base + dp_ + de);
                 if (LW) {
                     q[u][t][0] = ldt(T.pfrac_tab, base); q[u][t][1] = ldt(T.pfrac_tab, base + dp_);
                     q[u][t][2] = ldt(T.pfrac_tab, base + de); q[u][t][3] = ldt(T.pfrac_tab, base + dp_ + de);
                 } else {
-                    const unsigned b0 = ((jt0 + t) * neta + je0) * ngpt + g;
-                    kr[u][t][0] = ldt(T.krayl[reg], b0); kr[u][t][1] = ldt(T.krayl[reg], b0 + ngpt);
+                    const unsigned b0 = __umul24(Ij[(2 * f + t) * 2 + 1].x, (unsigned)ngpt) + (unsigned)g;
+                    const float *kt = reg == 0 ? T.krayl[0] : T.krayl[1];
+                    kr[u][t][0] = ldt(kt, b0); kr[u][t][1] = ldt(kt, b0 + de);
                 }
             }
-            const unsigned nc = nc_r[reg];
-            const float *kmt = km_r[reg];
+            const unsigned nc = reg == 0 ? nc0 : nc1;
+            const float *kmt = reg == 0 ? T.kminor[0] : T.kminor[1];
 #pragma unroll
             for (int ii = 0; ii < NM; ++ii) {
                 const unsigned v = reg == 0 ? mit[0][ii] : mit[1][ii];
@@ -304,7 +333,7 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
                 const unsigned ci = (unsigned)mi_ci(v);
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
-                    const unsigned b0 = ((jt0 + t) * neta + (unsigned)(je_of(r, 2 * fv + t) - 1)) * nc + ci;
+                    const unsigned b0 = __umul24(Ij[(2 * fv + t) * 2 + 1].x, nc) + ci;
                     km[u][ii][t][0] = ldt(kmt, b0);
                     km[u][ii][t][1] = ldt(kmt, b0 + nc);
                 }
@@ -312,19 +341,19 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
         }
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-            if (u == 1 && !two) break;
-            const int j = j0 + u, l = l0 + j, reg = regv[u];
-            const unsigned *r = R(j);
-            const float fp1 = fp1v[u], fp0 = 1.0f - fp1;
+            const int j = min(j0 + u, nl - 1), l = l0 + j, reg = regv[u];
+            const unsigned *r = srec + j * G.words;
+            const uint4 *Ij = sitem + (size_t)j * G.nf2 * 2;
+            const int f = reg == 0 ? gfl0 : gfl1;
+            const float fp1 = __uint_as_float(r[1]), fp0 = 1.0f - fp1;
             float tg = 0.0f, pf = 0.0f, kv = 0.0f;
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
-                const float fe0 = w[u][t][0], fe1 = w[u][t][1];
-                const float st = fp0 * fe0 * k[u][t][0] + fp1 * fe0 * k[u][t][1] + fp0 * fe1 * k[u][t][2] +
-                                 fp1 * fe1 * k[u][t][3];
-                tg += cmv[u][t] * st;
-                if (LW) pf += fp0 * fe0 * q[u][t][0] + fp1 * fe0 * q[u][t][1] + fp0 * fe1 * q[u][t][2] +
-                              fp1 * fe1 * q[u][t][3];
+                const uint4 a = Ij[(2 * f + t) * 2];
+                const float fe0 = __uint_as_float(a.y), fe1 = __uint_as_float(a.z);
+                const float st = fp0 * (fe0 * k[u][t][0] + fe1 * k[u][t][2]) + fp1 * (fe0 * k[u][t][1] + fe1 * k[u][t][3]);
+                tg += __uint_as_float(a.w) * st;
+                if (LW) pf += fp0 * (fe0 * q[u][t][0] + fe1 * q[u][t][2]) + fp1 * (fe0 * q[u][t][1] + fe1 * q[u][t][3]);
                 else kv += fe0 * kr[u][t][0] + fe1 * kr[u][t][1];
             }
             // minor absorbers of this layer's regime covering g
@@ -334,10 +363,13 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
                 const int fv = mi_fv(v);
                 float kvm = 0.0f;
 #pragma unroll
-                for (int t = 0; t < 2; ++t)
-                    kvm += fw(r, 2 * fv + t, 0) * km[u][ii][t][0] + fw(r, 2 * fv + t, 1) * km[u][ii][t][1];
+                for (int t = 0; t < 2; ++t) {
+                    const uint4 m = Ij[(2 * fv + t) * 2 + 1];
+                    kvm += __uint_as_float(m.y) * km[u][ii][t][0] + __uint_as_float(m.z) * km[u][ii][t][1];
+                }
                 tg += __uint_as_float(r[G.ms + mi_iv(v)]) * kvm;
             }
+            if (j0 + u >= nl) continue;
             if (row_mask && !row_mask[(size_t)d * nlay + l]) continue;
             const size_t o = ((size_t)d * nlay + l) * ngpt + g;
             tau[o] = tg;
@@ -350,9 +382,10 @@ template <bool LW>
 static void launch_nm(const TablesView &T, int nblocks, int nlay, int ncols, const unsigned *rec, float *tau,
                       float *aux, hipStream_t st, const unsigned char *row_mask, const unsigned char *col_mask)
 {
+    const size_t smem = gather_smem(rec_geom(T.nflav, T.nminor[0], T.nminor[1]));
 #define GO_LAUNCH(NMV)                                                                                          \
-    hipLaunchKernelGGL((gas_optics_kernel<LW, NMV>), dim3(nblocks), dim3(128), 0, st, T, nlay, ncols, rec, tau, \
-                       aux, row_mask, col_mask)
+    hipLaunchKernelGGL((gas_optics_kernel<LW, NMV>), dim3(nblocks), dim3(128), smem, st, T, nlay, ncols, rec, \
+                       tau, aux, row_mask, col_mask)
     const int nm = T.maxminor_g;
     if (nm <= 0) GO_LAUNCH(0);
     else if (nm == 1) GO_LAUNCH(1);
@@ -362,19 +395,33 @@ static void launch_nm(const TablesView &T, int nblocks, int nlay, int ncols, con
 #undef GO_LAUNCH
 }
 
-void launch_gas_optics(const TablesView &T, int ncols, int nlay, int ncol_ld, const int *col_idx,
-                       int col0, const double *p_lay, const double *p_lev, const double *t_lay,
-                       const VmrPtrs &vmr, unsigned *rec, float *tau, float *aux, int *err, hipStream_t st,
-                       const unsigned char *row_mask, const unsigned char *col_mask)
+void launch_gas_interp(const TablesView &T, int ncols, int nlay, int ncol_ld, const int *col_idx, int col0,
+                       const double *p_lay, const double *p_lev, const double *t_lay, const VmrPtrs &vmr,
+                       unsigned *rec, int *err, hipStream_t st, const unsigned char *col_mask)
 {
     if (ncols <= 0 || nlay <= 0) return;
     const size_t nrow = (size_t)ncols * nlay;
     const size_t smem = sizeof(unsigned) * INTERP_ROWS * gas_record_words(T);
     hipLaunchKernelGGL(gas_interp_kernel, dim3((unsigned)((nrow + INTERP_ROWS - 1) / INTERP_ROWS)), dim3(INTERP_ROWS),
                        smem, st, T, nlay, ncol_ld, col_idx, col0, ncols, p_lay, p_lev, t_lay, vmr, rec, err, col_mask);
+}
+
+void launch_gas_gather(const TablesView &T, int ncols, int nlay, const unsigned *rec, float *tau, float *aux,
+                       hipStream_t st, const unsigned char *row_mask, const unsigned char *col_mask)
+{
+    if (ncols <= 0 || nlay <= 0) return;
     const int nblocks = ncols * ((nlay + LCHUNK - 1) / LCHUNK);
     if (T.is_lw) launch_nm<true>(T, nblocks, nlay, ncols, rec, tau, aux, st, row_mask, col_mask);
     else launch_nm<false>(T, nblocks, nlay, ncols, rec, tau, aux, st, row_mask, col_mask);
+}
+
+void launch_gas_optics(const TablesView &T, int ncols, int nlay, int ncol_ld, const int *col_idx,
+                       int col0, const double *p_lay, const double *p_lev, const double *t_lay,
+                       const VmrPtrs &vmr, unsigned *rec, float *tau, float *aux, int *err, hipStream_t st,
+                       const unsigned char *row_mask, const unsigned char *col_mask)
+{
+    launch_gas_interp(T, ncols, nlay, ncol_ld, col_idx, col0, p_lay, p_lev, t_lay, vmr, rec, err, st, col_mask);
+    launch_gas_gather(T, ncols, nlay, rec, tau, aux, st, row_mask, col_mask);
 }
 
 }  // namespace ccpprad

@@ -19,6 +19,15 @@ void launch_gas_optics(const TablesView &T, int ncols, int nlay, int ncol_ld, co
                        unsigned *rec, float *tau, float *aux, int *err, hipStream_t st,
                        const unsigned char *row_mask = nullptr, const unsigned char *col_mask = nullptr);
 int gas_record_words(const TablesView &T);
+// LDS per block of the g-point gather kernel
+size_t gas_gather_smem(int is_lw, int nflav, int nminor0, int nminor1, int ngpt, int neta);
+// the two stages of launch_gas_optics: interpolation records, then g-point gathers
+void launch_gas_interp(const TablesView &T, int ncols, int nlay, int ncol_ld, const int *col_idx, int col0,
+                       const double *p_lay, const double *p_lev, const double *t_lay, const VmrPtrs &vmr,
+                       unsigned *rec, int *err, hipStream_t st, const unsigned char *col_mask = nullptr);
+void launch_gas_gather(const TablesView &T, int ncols, int nlay, const unsigned *rec, float *tau, float *aux,
+                       hipStream_t st, const unsigned char *row_mask = nullptr,
+                       const unsigned char *col_mask = nullptr);
 
 struct LwSolveArgs {
     int nlay, ncol_ld, col0, top_at_1;

@@ -249,8 +249,9 @@ int ccpprad_lw_set_gas_optics_mlp(ccpprad_ctx *ctx, ccpprad_mlp *m);
 /* ---- observability (no counterpart in the reference, which has no timers) */
 
 /* Per-kernel device timing with HIP events recorded on the launch stream.
- * Kernel ids: 0 gas optics, 1 LW solver, 2 SW solver, 3 cloud optics,
- * 4 McICA, 5 MLP inference.  enable != 0 starts recording (and clears). */
+ * Kernel ids: 0 gas optics g-point gathers, 1 LW solver, 2 SW solver,
+ * 3 cloud optics, 4 McICA, 5 MLP inference, 6 gas optics interpolation.
+ * enable != 0 starts recording (and clears). */
 int ccpprad_timing_enable(ccpprad_ctx *ctx, int enable);
 /* Synchronises the recorded events; total milliseconds and launch count. */
 int ccpprad_timing_query(ccpprad_ctx *ctx, int kernel_id, double *total_ms, int *count);

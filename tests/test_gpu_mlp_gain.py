@@ -46,7 +46,7 @@ def test_emulator_beats_table_within_hr_budget(kd_lw, lut_lw):
     emu = mlp.LWEmulator.load()
     emu.attach(nn)
     outs, stage = {}, {}
-    for name, s, ids in (("table", table, (0,)), ("nn", nn, (0, 5))):
+    for name, s, ids in (("table", table, (0, 6)), ("nn", nn, (0, 5, 6))):
         d = device.to_device(arr)
         d.update(device.lw_outputs(NCOL, NLAY))
         s.run_dev(ints, d)                      # warm-up and validation
