@@ -130,4 +130,75 @@ __device__ __forceinline__ double wave_sum(double v)
     return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
+// Sum of lane pairs by the gfx950 half-wave / row-pair swaps: lanes whose
+// lane bit (bit 5 for H32, bit 4 otherwise) is 0 get a(L) + a(L ^ m), the
+// others b(L ^ m) + b(L).  v_permlane32_swap exchanges the upper 32 lanes of
+// its first operand with the lower 32 of its second; v_permlane16_swap does
+// the same for the odd rows of the first and the even rows of the second.
+template <bool H32>
+__device__ __forceinline__ double swap_sum(double a, double b)
+{
+    const unsigned ah = (unsigned)__double2hiint(a), al = (unsigned)__double2loint(a);
+    const unsigned bh = (unsigned)__double2hiint(b), bl = (unsigned)__double2loint(b);
+    const auto h = H32 ? __builtin_amdgcn_permlane32_swap(ah, bh, false, false)
+                       : __builtin_amdgcn_permlane16_swap(ah, bh, false, false);
+    const auto l = H32 ? __builtin_amdgcn_permlane32_swap(al, bl, false, false)
+                       : __builtin_amdgcn_permlane16_swap(al, bl, false, false);
+    return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+}
+
+// Quad-level pair sum: lanes whose bit is 0 get a(L) + a(partner), the
+// others b(L) + b(partner); CTRL is the quad_perm swapping that bit.
+template <int CTRL>
+__device__ __forceinline__ double xsel_sum(double a, double b, bool bit)
+{
+    const double keep = bit ? b : a, send = bit ? a : b;
+    return keep + dpp_f64<CTRL>(send);
+}
+
+// FP64 sums over the 64 lanes of N = 8 or 16 per-lane FP32 values at once
+// (all lanes active).  Each exchange step halves the number of values a lane
+// carries, so the N sums cost about what two single sums do.  Lane L gets
+// the sum of value
+//   N = 16: 8 bit1 + 4 bit0 + 2 bit4 + bit5,   N = 8: 4 bit0 + 2 bit4 + bit5
+// of L (lanes differing only in the other bits hold the same, bit-identical
+// sum); wave_sum_idx gives that index.
+template <int N>
+__device__ __forceinline__ double wave_sum_n(const float (&v)[N])
+{
+    static_assert(N == 8 || N == 16, "8 or 16 values");
+    const int lane = __lane_id();
+    double a[N / 2];
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) a[i] = swap_sum<true>((double)v[2 * i], (double)v[2 * i + 1]);
+    double b[N / 4];
+#pragma unroll
+    for (int i = 0; i < N / 4; ++i) b[i] = swap_sum<false>(a[2 * i], a[2 * i + 1]);
+    double d;
+    if (N == 16) {
+        const double c0 = xsel_sum<0xB1>(b[0], b[1], lane & 1), c1 = xsel_sum<0xB1>(b[2], b[3], lane & 1);
+        d = xsel_sum<0x4E>(c0, c1, (lane >> 1) & 1);
+    } else {
+        d = xsel_sum<0xB1>(b[0], b[1], lane & 1);
+        d += dpp_f64<0x4E>(d);    // quad_perm [2,3,0,1]: lanes L, L ^ 2
+    }
+    d += dpp_f64<0x128>(d);       // row_ror:8: lanes L, L ^ 8
+    d += dpp_f64<0x124>(d);       // row_ror:4: lanes L, L + 4, L + 8, L + 12 of the row
+    return d;
+}
+
+template <int N>
+__device__ __forceinline__ int wave_sum_idx(int lane)
+{
+    return N == 16 ? 8 * ((lane >> 1) & 1) + 4 * (lane & 1) + 2 * ((lane >> 4) & 1) + ((lane >> 5) & 1)
+                   : 4 * (lane & 1) + 2 * ((lane >> 4) & 1) + ((lane >> 5) & 1);
+}
+
+// the lanes holding each of the N sums once (bits 2-3 zero, bit 1 too for N = 8)
+template <int N>
+__device__ __forceinline__ bool wave_sum_writer(int lane)
+{
+    return (lane & (N == 16 ? 12 : 14)) == 0;
+}
+
 }  // namespace ccpprad

@@ -40,7 +40,7 @@ __device__ __forceinline__ void lw_trans(float x, float &e, float &E, float &f)
     E = small ? x * (fs + e) : Eb;
 }
 
-constexpr int LW_BATCH = 8;
+constexpr int LW_BATCH = 8;   // also the number of level sums reduced together
 
 // LDS per block: [2 waves][3][nlev] FP64 level sums, then the column's band
 // Planck function at layers [nlay][nband], levels [nlev][nband] and the
@@ -137,10 +137,12 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
 #pragma unroll
         for (int j = 0; j < LW_BATCH; ++j)
             if (k0 + j < nlay) load_layer(top ? k0 + j : nlay - 1 - (k0 + j), x[j]);
+        float fl[LW_BATCH];   // g-point fluxes at the batch's lower levels, summed together
 #pragma unroll
         for (int j = 0; j < LW_BATCH; ++j) {
+            fl[j] = 0.0f;
             const int k = k0 + j;
-            if (k >= nlay) break;
+            if (k >= nlay) continue;
             const int l = top ? k : nlay - 1 - k;
             const int lev_below = top ? l + 1 : l;
             const float tau = layer_tau(x[j]);
@@ -156,9 +158,11 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
                 I[a] = t * I[a] + (omt * ldn + 2.0f * f * dsrc);
                 fsum += W2pi[a] * I[a];
             }
-            const double ws = wave_sum(act ? (double)fsum : 0.0);
-            if (lane == 0) rdn[lev_below] = ws;
+            if (act) fl[j] = fsum;
         }
+        const double sum = wave_sum_n<LW_BATCH>(fl);
+        const int k = k0 + wave_sum_idx<LW_BATCH>(lane);
+        if (wave_sum_writer<LW_BATCH>(lane) && k < nlay) rdn[top ? k + 1 : nlay - 1 - k] = sum;
     }
     // ---- surface
     const int sfc_lay = top ? nlay - 1 : 0;
@@ -190,10 +194,12 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
 #pragma unroll
         for (int j = 0; j < LW_BATCH; ++j)
             if (k0 + j < nlay) load_layer(top ? nlay - 1 - (k0 + j) : k0 + j, x[j]);
+        float fl[2 * LW_BATCH];   // g-point fluxes (and Jacobians) at the batch's upper levels
 #pragma unroll
         for (int j = 0; j < LW_BATCH; ++j) {
+            fl[j] = 0.0f; fl[LW_BATCH + j] = 0.0f;
             const int k = k0 + j;
-            if (k >= nlay) break;
+            if (k >= nlay) continue;
             const int l = top ? nlay - 1 - k : k;
             const int lev_above = top ? l : l + 1;
             const float tau = layer_tau(x[j]);
@@ -210,12 +216,16 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
                 fsum += W2pi[a] * I[a];
                 if (JAC) { J[a] = t * J[a]; jsum += W2pi[a] * J[a]; }
             }
-            const double ws = wave_sum(act ? (double)fsum : 0.0);
-            if (lane == 0) rup[lev_above] = ws;
-            if (JAC) {
-                const double wj = wave_sum(act ? (double)jsum : 0.0);
-                if (lane == 0) rjc[lev_above] = wj;
-            }
+            if (act) { fl[j] = fsum; fl[LW_BATCH + j] = jsum; }
+        }
+        if (JAC) {
+            const double sum = wave_sum_n<2 * LW_BATCH>(fl);
+            const int idx = wave_sum_idx<2 * LW_BATCH>(lane), k = k0 + (idx & (LW_BATCH - 1));
+            if (wave_sum_writer<2 * LW_BATCH>(lane) && k < nlay) (idx < LW_BATCH ? rup : rjc)[top ? nlay - 1 - k : k + 1] = sum;
+        } else {
+            const double sum = wave_sum_n<LW_BATCH>(*(const float(*)[LW_BATCH])fl);
+            const int k = k0 + wave_sum_idx<LW_BATCH>(lane);
+            if (wave_sum_writer<LW_BATCH>(lane) && k < nlay) rup[top ? nlay - 1 - k : k + 1] = sum;
         }
     }
     __syncthreads();

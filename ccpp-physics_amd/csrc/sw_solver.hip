@@ -110,7 +110,7 @@ __device__ __forceinline__ TwoStream two_stream32(float mu0, float mu0_inv, cons
 // per lane only the direct beam at each segment top and the adding albedo /
 // source at each segment boundary are kept (LDS), and a segment's layer
 // terms are recomputed in registers where they are needed.
-constexpr int SW_SEG = 8;
+constexpr int SW_SEG = 8;   // 2 SW_SEG = 16 level sums per segment, reduced together
 
 size_t sw_solve_smem(int nlay, int nband)
 {
@@ -304,21 +304,25 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
             a = ts.Rd + ts.Td * ts.Td * a * den;
             q = qa;
         }
+        // the segment's fluxes at the levels below its layers: up in
+        // fl[0..7], down in fl[8..15], summed over the g-points together
+        float fl[2 * SW_SEG];
 #pragma unroll
         for (int j = 0; j < SW_SEG; ++j) {
+            fl[j] = 0.0f; fl[SW_SEG + j] = 0.0f;
             const int k = s * SW_SEG + j;
             if (k >= nlay) continue;
-            const int lev_below = lev(k + 1);
             const float fd = T0[j] * fdir;            // direct beam at the level below
             dnd = al[j] * dnd + be[j] * fdir;
             const float up = dnd * Abl[j] + qbl[j] * fd;
             const float dn = dnd + fd;
             fdir = fd;
-            const double su = wave_sum(act ? (double)up : 0.0);
-            const double sdn = wave_sum(act ? (double)dn : 0.0);
-            if (lane == 0) { rup[lev_below] = su; rdn[lev_below] = sdn; }
-            if (lev_below == A.isfc) { sfc_dn = dn; sfc_dir = fd; }
+            if (act) { fl[j] = up; fl[SW_SEG + j] = dn; }
+            if (lev(k + 1) == A.isfc) { sfc_dn = dn; sfc_dir = fd; }
         }
+        const double sum = wave_sum_n<2 * SW_SEG>(fl);
+        const int idx = wave_sum_idx<2 * SW_SEG>(lane), k = s * SW_SEG + (idx & (SW_SEG - 1));
+        if (wave_sum_writer<2 * SW_SEG>(lane) && k < nlay) (idx < SW_SEG ? rup : rdn)[lev(k + 1)] = sum;
     }
     {
         // per-band surface sums for scmpsw
