@@ -293,18 +293,34 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
     T.temp_max = kd->temp_ref[kd->ntemp - 1];
     T.temp_ref = upload(c, kd->temp_ref, kd->ntemp, &e);
     const size_t nk = (size_t)kd->ngpt * kd->neta * (kd->npres + 1) * kd->ntemp;
-    { auto v = to_f32(kd->kmajor, nk); T.kmajor = upload(c, v.data(), nk, &e); }
+    // eta pairs: element (.., eta, g) with its eta + 1 neighbour (0 past the last row)
+    auto eta_pair = [&](const double *t, size_t i, int stride) -> std::pair<float, float> {
+        const int e = (int)((i / (size_t)stride) % (size_t)kd->neta);
+        return {(float)t[i], e + 1 < kd->neta ? (float)t[i + stride] : 0.0f};
+    };
     if (want_lw) {
-        auto v = to_f32(kd->planck_frac, nk); T.pfrac_tab = upload(c, v.data(), nk, &e);
+        std::vector<float4> q(nk);
+        for (size_t i = 0; i < nk; ++i) {
+            const auto k = eta_pair(kd->kmajor, i, kd->ngpt), f = eta_pair(kd->planck_frac, i, kd->ngpt);
+            q[i] = make_float4(k.first, k.second, f.first, f.second);
+        }
+        T.kq = upload(c, q.data(), nk, &e);
         auto p = to_f32(kd->totplnk, (size_t)kd->nplanck * kd->nband); T.totplnk = upload(c, p.data(), p.size(), &e);
         std::vector<float> fit(2 * kd->nband, 0.0f);
         if (kd->optimal_angle_fit) for (int i = 0; i < 2 * kd->nband; ++i) fit[i] = (float)kd->optimal_angle_fit[i];
         else for (int b = 0; b < kd->nband; ++b) fit[2 * b + 1] = 1.66f;
         T.opt_fit = upload(c, fit.data(), fit.size(), &e);
     } else {
+        std::vector<float2> k(nk);
+        for (size_t i = 0; i < nk; ++i) { const auto v = eta_pair(kd->kmajor, i, kd->ngpt); k[i] = make_float2(v.first, v.second); }
+        T.k2 = upload(c, k.data(), nk, &e);
         const size_t nr = (size_t)kd->ngpt * kd->neta * kd->ntemp;
-        auto a = to_f32(kd->krayl_lower, nr); T.krayl[0] = upload(c, a.data(), nr, &e);
-        auto b = to_f32(kd->krayl_upper, nr); T.krayl[1] = upload(c, b.data(), nr, &e);
+        for (int r = 0; r < 2; ++r) {
+            const double *src = r == 0 ? kd->krayl_lower : kd->krayl_upper;
+            std::vector<float2> v(nr);
+            for (size_t i = 0; i < nr; ++i) { const auto p = eta_pair(src, i, kd->ngpt); v[i] = make_float2(p.first, p.second); }
+            T.kr2[r] = upload(c, v.data(), nr, &e);
+        }
         c->solar_dev = upload(c, kd->solar_src, kd->ngpt, &e);
     }
     T.vmr_ref = upload(c, kd->vmr_ref, (size_t)2 * (kd->ngas + 1) * kd->ntemp, &e);
@@ -373,9 +389,10 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
         T.minor_gptr[r] = upload(c, ptr.data(), ptr.size(), &e);
         T.minor_items[r] = upload(c, items.data(), items.size(), &e);
         const size_t nkm = (size_t)std::max(1, m.ncontrib) * kd->neta * kd->ntemp;
-        std::vector<float> km(nkm, 0.0f);
-        if (m.ncontrib) for (size_t i = 0; i < nkm; ++i) km[i] = (float)m.kminor[i];
-        T.kminor[r] = upload(c, km.data(), km.size(), &e);
+        std::vector<float2> km(nkm, make_float2(0.0f, 0.0f));
+        if (m.ncontrib)
+            for (size_t i = 0; i < nkm; ++i) { const auto p = eta_pair(m.kminor, i, m.ncontrib); km[i] = make_float2(p.first, p.second); }
+        T.km2[r] = upload(c, km.data(), km.size(), &e);
     }
     // McICA: 69069^i mod 2^32
     {

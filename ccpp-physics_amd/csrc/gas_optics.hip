@@ -189,14 +189,19 @@ __device__ __forceinline__ int mi_ci(unsigned v) { return (int)(v & 0xFFFFu); }
 __device__ __forceinline__ int mi_iv(unsigned v) { return (int)((v >> 16) & 0xFFu); }
 __device__ __forceinline__ int mi_fv(unsigned v) { return (int)(v >> 24); }
 
-// Gathers through a uniform table base and a 32-bit byte offset, as global
+// Gathers through a uniform table base and a 32-bit element offset, as global
 // loads: a generic (flat) load would also count against the LDS counter, so
 // every LDS read of the record would wait for the table loads in flight.
 typedef const __attribute__((address_space(1))) char gchar;
-typedef const __attribute__((address_space(1))) float gfloat;
-__device__ __forceinline__ float ldt(const float *base, unsigned idx)
+typedef const __attribute__((address_space(1))) float2 gfloat2;
+typedef const __attribute__((address_space(1))) float4 gfloat4;
+__device__ __forceinline__ float2 ld2(const float2 *base, unsigned idx)
 {
-    return *(const gfloat *)((gchar *)base + (idx << 2));
+    return *(const gfloat2 *)((gchar *)base + (idx << 3));
+}
+__device__ __forceinline__ float4 ld4(const float4 *base, unsigned idx)
+{
+    return *(const gfloat4 *)((gchar *)base + (idx << 4));
 }
 
 // Per (layer, item) constants of a chunk, built once per block so that the
@@ -274,7 +279,7 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
         }
     }
     const unsigned nc0 = (unsigned)T.ncontrib[0], nc1 = (unsigned)T.ncontrib[1];
-    const unsigned dp_ = (unsigned)(neta * ngpt), de = (unsigned)ngpt;
+    const unsigned dp_ = (unsigned)(neta * ngpt);   // jpress stride of the major table
     __syncthreads();
 
     // item constants of every (layer, item) of the chunk
@@ -311,21 +316,23 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
             const uint4 *Ij = sitem + (size_t)j * G.nf2 * 2;
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
+                // corners (jpress, eta), (jpress + 1, eta) and their eta + 1 neighbours
                 const uint4 a = Ij[(2 * f + t) * 2];
                 const unsigned base = a.x + (unsigned)g;
-                k[u][t][0] = ldt(T.kmajor, base); k[u][t][1] = ldt(T.kmajor, base + dp_);
-                k[u][t][2] = ldt(T.kmajor, base + de); k[u][t][3] = ldt(T.kmajor, base + dp_ + de);
                 if (LW) {
-                    q[u][t][0] = ldt(T.pfrac_tab, base); q[u][t][1] = ldt(T.pfrac_tab, base + dp_);
-                    q[u][t][2] = ldt(T.pfrac_tab, base + de); q[u][t][3] = ldt(T.pfrac_tab, base + dp_ + de);
+                    const float4 v0 = ld4(T.kq, base), v1 = ld4(T.kq, base + dp_);
+                    k[u][t][0] = v0.x; k[u][t][2] = v0.y; q[u][t][0] = v0.z; q[u][t][2] = v0.w;
+                    k[u][t][1] = v1.x; k[u][t][3] = v1.y; q[u][t][1] = v1.z; q[u][t][3] = v1.w;
                 } else {
+                    const float2 v0 = ld2(T.k2, base), v1 = ld2(T.k2, base + dp_);
+                    k[u][t][0] = v0.x; k[u][t][2] = v0.y; k[u][t][1] = v1.x; k[u][t][3] = v1.y;
                     const unsigned b0 = __umul24(Ij[(2 * f + t) * 2 + 1].x, (unsigned)ngpt) + (unsigned)g;
-                    const float *kt = reg == 0 ? T.krayl[0] : T.krayl[1];
-                    kr[u][t][0] = ldt(kt, b0); kr[u][t][1] = ldt(kt, b0 + de);
+                    const float2 r2 = ld2(reg == 0 ? T.kr2[0] : T.kr2[1], b0);
+                    kr[u][t][0] = r2.x; kr[u][t][1] = r2.y;
                 }
             }
             const unsigned nc = reg == 0 ? nc0 : nc1;
-            const float *kmt = reg == 0 ? T.kminor[0] : T.kminor[1];
+            const float2 *kmt = reg == 0 ? T.km2[0] : T.km2[1];
 #pragma unroll
             for (int ii = 0; ii < NM; ++ii) {
                 const unsigned v = reg == 0 ? mit[0][ii] : mit[1][ii];
@@ -333,9 +340,9 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
                 const unsigned ci = (unsigned)mi_ci(v);
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
-                    const unsigned b0 = __umul24(Ij[(2 * fv + t) * 2 + 1].x, nc) + ci;
-                    km[u][ii][t][0] = ldt(kmt, b0);
-                    km[u][ii][t][1] = ldt(kmt, b0 + nc);
+                    const float2 m2 = ld2(kmt, __umul24(Ij[(2 * fv + t) * 2 + 1].x, nc) + ci);
+                    km[u][ii][t][0] = m2.x;
+                    km[u][ii][t][1] = m2.y;
                 }
             }
         }
