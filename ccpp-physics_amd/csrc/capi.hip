@@ -428,6 +428,8 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
         L.ice_lwr = (float)lut->radice_lwr; L.ice_upr = (float)lut->radice_upr;
         L.liq_step = (float)((lut->radliq_upr - lut->radliq_lwr) / (lut->nsize_liq - 1));
         L.ice_step = (float)((lut->radice_upr - lut->radice_lwr) / (lut->nsize_ice - 1));
+        L.liq_istep = (float)((lut->nsize_liq - 1) / (lut->radliq_upr - lut->radliq_lwr));
+        L.ice_istep = (float)((lut->nsize_ice - 1) / (lut->radice_upr - lut->radice_lwr));
         const size_t nl = (size_t)lut->nsize_liq * lut->nband, ni = (size_t)lut->nsize_ice * lut->nband;
         auto a = to_f32(lut->extliq, nl); L.extliq = upload(c, a.data(), nl, &e);
         auto b = to_f32(lut->ssaliq, nl); L.ssaliq = upload(c, b.data(), nl, &e);
@@ -443,6 +445,11 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
         for (size_t i = 0; i < ni; ++i) ci[i] = (float)(1.0 - lut->ssaice[io + i]);
         L.coaliq = upload(c, cl.data(), nl, &e);
         L.coaice = upload(c, ci.data(), ni, &e);
+        if (sizeof(float) * cloud_lut_floats(L) > 48 * 1024) {
+            delete c;
+            seterr(errmsg, errlen, "cloud LUT larger than the 48 KiB the cloud kernel stages in LDS");
+            return 1;
+        }
         c->have_lut = 1;
     }
     if (e != hipSuccess) {

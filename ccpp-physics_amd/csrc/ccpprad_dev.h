@@ -14,7 +14,7 @@
 // per g-point of one column) touch one contiguous row:
 //   tau / pfrac / tau_ray : [col][lay][gpt]
 //   mask                  : [col][lay][MASK_WORDS] u32 bits, bit g of word g/32
-//   cloud band optics     : [col][lay][band] (LW: absorption depth; SW: tau, ssa, g)
+//   cloud band optics     : [lay][col][band] (LW: absorption depth; SW: tau, co-albedo, g)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -70,6 +70,7 @@ struct TablesView {
 struct LutView {
     int nband, nsize_liq, nsize_ice;
     float liq_lwr, liq_step, ice_lwr, ice_step, liq_upr, ice_upr;
+    float liq_istep, ice_istep;               // 1 / step, formed in FP64
     const float *extliq, *ssaliq, *asyliq;   // [nband][nsize_liq]
     const float *extice, *ssaice, *asyice;   // [nband][nsize_ice] (roughness selected)
     const float *coaliq, *coaice;            // co-albedo 1 - ssa, formed in FP64 at init

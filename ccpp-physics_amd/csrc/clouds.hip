@@ -22,12 +22,16 @@
 namespace ccpprad {
 
 
+// Divisions below go through v_rcp_f32 (about 1 ulp): the results are FP32
+// optics, and the table position's floor decides only which of two equal
+// neighbouring interpolants is taken on an exact grid point.
 __device__ __forceinline__ void lut_eval(const float *ext, const float *ssa, const float *coa, const float *asy,
-                                         int nsz, float lwr, float step, float path, float re, float &t, float &ts,
+                                         int nsz, float lwr, float istep, float path, float re, float &t, float &ts,
                                          float &ta, float &tsg)
 {
-    int idx = min((int)floorf((re - lwr) / step) + 1, nsz - 1);
-    float fint = (re - lwr) / step - (float)(idx - 1);
+    const float x = (re - lwr) * istep;
+    int idx = min((int)floorf(x) + 1, nsz - 1);
+    float fint = x - (float)(idx - 1);
     float e0 = ext[idx - 1], e1 = ext[idx];
     float s0 = ssa[idx - 1], s1 = ssa[idx];
     float c0 = coa[idx - 1], c1 = coa[idx];
@@ -51,9 +55,10 @@ __device__ __forceinline__ void inc2s(const Opt2 &a, Opt2 &b)
     const float t12 = a.t + b.t;
     const float ts12 = a.t * a.w + b.t * b.w;
     const float ta12 = a.t * a.p + b.t * b.p;
-    b.g = (a.t * a.w * a.g + b.t * b.w * b.g) / fmaxf(eps, ts12);
-    b.w = ts12 / fmaxf(eps, t12);
-    b.p = t12 > eps ? ta12 / t12 : 1.0f;
+    const float rt = frcp(fmaxf(eps, t12));
+    b.g = (a.t * a.w * a.g + b.t * b.w * b.g) * frcp(fmaxf(eps, ts12));
+    b.w = ts12 * rt;
+    b.p = t12 > eps ? ta12 * rt : 1.0f;
     b.t = t12;
 }
 
@@ -64,20 +69,21 @@ __device__ __forceinline__ bool lut_cloud(const LutView &L, int b, float lwp, fl
     if (lwp > 0.0f) {
         bad |= !(rl >= L.liq_lwr && rl <= L.liq_upr);
         const size_t o = (size_t)b * L.nsize_liq;
-        lut_eval(L.extliq + o, L.ssaliq + o, L.coaliq + o, L.asyliq + o, L.nsize_liq, L.liq_lwr, L.liq_step, lwp,
+        lut_eval(L.extliq + o, L.ssaliq + o, L.coaliq + o, L.asyliq + o, L.nsize_liq, L.liq_lwr, L.liq_istep, lwp,
                  fminf(fmaxf(rl, L.liq_lwr), L.liq_upr), lt, lts, lta, ltsg);
     }
     if (iwp > 0.0f) {
         bad |= !(ri >= L.ice_lwr && ri <= L.ice_upr);
         const size_t o = (size_t)b * L.nsize_ice;
-        lut_eval(L.extice + o, L.ssaice + o, L.coaice + o, L.asyice + o, L.nsize_ice, L.ice_lwr, L.ice_step, iwp,
+        lut_eval(L.extice + o, L.ssaice + o, L.coaice + o, L.asyice + o, L.nsize_ice, L.ice_lwr, L.ice_istep, iwp,
                  fminf(fmaxf(ri, L.ice_lwr), L.ice_upr), it, its, ita, itsg);
     }
     o.t = lt + it;
     const float ts = lts + its, ta = lta + ita, tsg = ltsg + itsg;
-    o.w = ts / fmaxf(1.1920929e-7f, o.t);
-    o.p = o.t > 0.0f ? ta / fmaxf(1.1920929e-7f, o.t) : 1.0f;
-    o.g = tsg / fmaxf(1.1920929e-7f, ts);
+    const float rt = frcp(fmaxf(1.1920929e-7f, o.t));
+    o.w = ts * rt;
+    o.p = o.t > 0.0f ? ta * rt : 1.0f;
+    o.g = tsg * frcp(fmaxf(1.1920929e-7f, ts));
     return bad;
 }
 
@@ -89,83 +95,125 @@ __constant__ float PREC_B1S[14] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 1.62e
 __constant__ float PREC_C0R[14] = {0.980f, 0.975f, 0.965f, 0.960f, 0.955f, 0.952f, 0.950f, 0.944f, 0.894f, 0.884f, 0.883f, 0.883f, 0.883f, 0.883f};
 __constant__ float PREC_C0S[14] = {0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.700f, 0.700f, 0.700f, 0.700f};
 
-// One thread per (column, layer, band), band fastest: the [d][l][b]
-// outputs are written contiguously by each wavefront.
-__global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs A, int ncols)
+// Floats of the LUT a block stages in LDS: ext, ssa, co-albedo, asy for
+// liquid then ice, [band][size] each.
+size_t cloud_lut_floats(const LutView &L)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    return 4 * (size_t)L.nband * ((size_t)L.nsize_liq + L.nsize_ice);
+}
+
+__device__ __forceinline__ const float *lut_table(const LutView &L, int t)
+{
+    return t == 0 ? L.extliq : t == 1 ? L.ssaliq : t == 2 ? L.coaliq : t == 3 ? L.asyliq
+         : t == 4 ? L.extice : t == 5 ? L.ssaice : t == 6 ? L.coaice : L.asyice;
+}
+
+// One thread per (layer, column) item, column fastest, all bands in a loop:
+// the CCPP inputs of a wavefront's items are consecutive, they are read once
+// for all bands, and its [l][d][b] outputs form one contiguous block.  The
+// LUT is staged in LDS once per block, so an item waits on one round of
+// global loads.  Items are strided over a grid of a few blocks per CU.
+__global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs A, int ncols, unsigned n)
+{
+    extern __shared__ float slut[];
+    const int nl = L.nband * L.nsize_liq, ni = L.nband * L.nsize_ice, ntot = 4 * (nl + ni);
+    {
+        constexpr int NLD = 4;   // staging loads in flight per thread
+        for (int k0 = threadIdx.x; k0 < ntot; k0 += NLD * blockDim.x) {
+            float v[NLD];
+#pragma unroll
+            for (int u = 0; u < NLD; ++u) {
+                const int k = min(k0 + u * (int)blockDim.x, ntot - 1);
+                const int t = k < 4 * nl ? k / nl : 4 + (k - 4 * nl) / ni;
+                const int o = k < 4 * nl ? k - t * nl : k - 4 * nl - (t - 4) * ni;
+                v[u] = lut_table(L, t)[o];
+            }
+#pragma unroll
+            for (int u = 0; u < NLD; ++u)
+                if (k0 + u * (int)blockDim.x < ntot) slut[k0 + u * blockDim.x] = v[u];
+        }
+    }
+    LutView S = L;
+    S.extliq = slut; S.ssaliq = slut + nl; S.coaliq = slut + 2 * nl; S.asyliq = slut + 3 * nl;
+    S.extice = slut + 4 * nl; S.ssaice = S.extice + ni; S.coaice = S.extice + 2 * ni; S.asyice = S.extice + 3 * ni;
+    __syncthreads();
+
     const int nb = A.nband;
-    if (i >= (size_t)ncols * A.nlay * nb) return;
-    const int b = (int)(i % nb);
-    const size_t dl = i / nb;
-    const int l = (int)(dl % A.nlay), d = (int)(dl / A.nlay);
-    const int od = A.col0 + d;
-    const int c = A.col_idx ? A.col_idx[od] - 1 : od;
-    const int blk = od / A.blksz;
-    const size_t q = (size_t)c + (size_t)A.ncol_ld * l;
-    if (!A.blkflag[blk]) {
-        if (A.is_lw) A.ctau[i] = 0.0f;
-        else A.copt[i] = make_float3(0.0f, 0.0f, 0.0f);
-        return;
-    }
-    const float lwp = (float)A.lwp[q], iwp = (float)A.iwp[q];
-    const float rl = (float)A.reliq[q], ri = (float)A.reice[q];
-    const double cf = A.cld_frac[q];
-    // precipitation, band independent parts
-    float tr = 0.0f, tsn = 0.0f, res = 0.0f;
-    bool prec = false;
-    if (A.is_lw) {
-        if (cf > 1.0e-6) {  // rrtmgp_lw_main.F90:413
-            tr = 0.33e-3f * (float)A.rwp[q];
-            const double sw = A.swp[q], rs = A.resnow[q];
-            tsn = (sw > 0.0 && rs > 10.0) ? 1.5f * 1.05756f * (float)sw / (float)rs : 0.0f;
-            prec = true;
+    bool bad = false;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int l = (int)(i / (unsigned)ncols), d = (int)(i - (unsigned)l * (unsigned)ncols);
+        const int od = A.col0 + d;
+        const int c = A.col_idx ? A.col_idx[od] - 1 : od;
+        const int blk = od / A.blksz;
+        const size_t q = (size_t)c + (size_t)A.ncol_ld * l;
+        const size_t o0 = (size_t)i * nb;
+        if (!A.blkflag[blk]) {
+            for (int b = 0; b < nb; ++b) {
+                if (A.is_lw) A.ctau[o0 + b] = 0.0f;
+                else A.copt[o0 + b] = make_float3(0.0f, 0.0f, 0.0f);
+            }
+            continue;
         }
-    } else {
-        if (cf > 1.0e-12) {  // rrtmgp_sw_main.F90:438
-            tr = (float)A.rwp[q] * 3.07e-3f;
-            const double sw = A.swp[q], rs = A.resnow[q];
-            res = (float)rs;
-            tsn = (sw > 0.0 && rs > 10.0) ? (float)sw * 1.09087f * (0.0f + 1.5f / (1.0315f * res)) : 0.0f;
-            prec = true;
+        const float lwp = (float)A.lwp[q], iwp = (float)A.iwp[q];
+        const float rl = (float)A.reliq[q], ri = (float)A.reice[q];
+        const double cf = A.cld_frac[q];
+        // precipitation, band independent parts
+        float tr = 0.0f, tsn = 0.0f, res = 0.0f;
+        bool prec = false;
+        if (A.is_lw) {
+            if (cf > 1.0e-6) {  // rrtmgp_lw_main.F90:413
+                tr = 0.33e-3f * (float)A.rwp[q];
+                const double sw = A.swp[q], rs = A.resnow[q];
+                tsn = (sw > 0.0 && rs > 10.0) ? 1.5f * 1.05756f * (float)sw / (float)rs : 0.0f;
+                prec = true;
+            }
+        } else {
+            if (cf > 1.0e-12) {  // rrtmgp_sw_main.F90:438
+                tr = (float)A.rwp[q] * 3.07e-3f;
+                const double sw = A.swp[q], rs = A.resnow[q];
+                res = (float)rs;
+                tsn = (sw > 0.0 && rs > 10.0) ? (float)sw * 1.09087f * (0.0f + 1.5f / (1.0315f * res)) : 0.0f;
+                prec = true;
+            }
         }
-    }
-    Opt2 o;
-    bool bad = lut_cloud(L, b, lwp, iwp, rl, ri, o);
-    if (!A.is_lw && A.cldtau && b == 10 && blk == A.cldtau_blk) A.cldtau[q] = (double)o.t;
-    for (int k = 0; k < 2; ++k) {
-        const double *const *src = k == 0 ? A.cnv : A.pbl;
-        if (!src[0]) continue;
-        Opt2 o2;
-        bad |= lut_cloud(L, b, (float)src[0][q], (float)src[1][q], (float)src[2][q], (float)src[3][q], o2);
-        inc2s(o2, o);
-    }
-    Opt2 pr = {0.0f, 0.0f, 1.0f, 0.0f};
-    if (A.is_lw) {
-        pr.t = prec ? tr + tsn : 0.0f;
-    } else if (prec) {
-        const int bb = b < 14 ? b : 13;
-        const float sr = tr * (1.0f - PREC_B0R[bb]);
-        const float ar = sr * PREC_C0R[bb];
-        const float sn = tsn * (1.0f - (PREC_B0S[bb] + PREC_B1S[bb] * 1.0315f * res));
-        const float an = sn * PREC_C0S[bb];
-        const float tp = fmaxf(1.0e-12f, tr + tsn);
-        const float sp = fmaxf(1.0e-12f, sr + sn);
-        const float ap = fmaxf(1.0e-12f, ar + an);
-        const float asyw = ap / fmaxf(1.0e-12f, sp);
-        const float ssaw = fminf(1.0f - 0.000001f, sp / tp);
-        const float omw = fmaxf(1.0f - (1.0f - 0.000001f), (tp - sp) / tp);   // 1 - ssaw
-        const float za1 = asyw * asyw, za2 = ssaw * za1;
-        pr.t = (1.0f - za2) * tp;
-        pr.w = (ssaw - za2) / (1.0f - za2);
-        pr.p = omw / (1.0f - za2);
-        pr.g = asyw / (1.0f + asyw);
-    }
-    inc2s(pr, o);
-    if (A.is_lw) {
-        A.ctau[i] = o.t * o.p;  // 1scl += 2str: absorption depth tau * (1 - ssa)
-    } else {
-        A.copt[i] = make_float3(o.t, o.p, o.g);   // SW: tau, co-albedo, g in one 12-byte record
+        for (int b = 0; b < nb; ++b) {
+            Opt2 o;
+            bad |= lut_cloud(S, b, lwp, iwp, rl, ri, o);
+            if (!A.is_lw && A.cldtau && b == 10 && blk == A.cldtau_blk) A.cldtau[q] = (double)o.t;
+            for (int k = 0; k < 2; ++k) {
+                const double *const *src = k == 0 ? A.cnv : A.pbl;
+                if (!src[0]) continue;
+                Opt2 o2;
+                bad |= lut_cloud(S, b, (float)src[0][q], (float)src[1][q], (float)src[2][q], (float)src[3][q], o2);
+                inc2s(o2, o);
+            }
+            Opt2 pr = {0.0f, 0.0f, 1.0f, 0.0f};
+            if (A.is_lw) {
+                pr.t = prec ? tr + tsn : 0.0f;
+            } else if (prec) {
+                const int bb = b < 14 ? b : 13;
+                const float sr = tr * (1.0f - PREC_B0R[bb]);
+                const float ar = sr * PREC_C0R[bb];
+                const float sn = tsn * (1.0f - (PREC_B0S[bb] + PREC_B1S[bb] * 1.0315f * res));
+                const float an = sn * PREC_C0S[bb];
+                const float tp = fmaxf(1.0e-12f, tr + tsn);
+                const float sp = fmaxf(1.0e-12f, sr + sn);
+                const float ap = fmaxf(1.0e-12f, ar + an);
+                const float asyw = ap * frcp(fmaxf(1.0e-12f, sp));   // < 1: ar, an are below sr, sn
+                const float rtp = frcp(tp);
+                const float ssaw = fminf(1.0f - 0.000001f, sp * rtp);
+                const float omw = fmaxf(1.0f - (1.0f - 0.000001f), (tp - sp) * rtp);   // 1 - ssaw
+                const float za1 = asyw * asyw, za2 = ssaw * za1;
+                const float rz = frcp(1.0f - za2);
+                pr.t = (1.0f - za2) * tp;
+                pr.w = (ssaw - za2) * rz;
+                pr.p = omw * rz;
+                pr.g = asyw * frcp(1.0f + asyw);
+            }
+            inc2s(pr, o);
+            if (A.is_lw) A.ctau[o0 + b] = o.t * o.p;   // 1scl += 2str: absorption depth tau * (1 - ssa)
+            else A.copt[o0 + b] = make_float3(o.t, o.p, o.g);   // SW: tau, co-albedo, g in one 12-byte record
+        }
     }
     if (bad) atomicOr(A.err, 2);
 }
@@ -324,9 +372,11 @@ void launch_block_flags(int ncols, int nlay, int ncol_ld, const int *col_idx, in
 
 void launch_cloud_optics(const LutView &L, const CloudArgs &A, int ncols, hipStream_t st)
 {
-    const size_t n = (size_t)ncols * A.nlay * A.nband;
+    const size_t n = (size_t)ncols * A.nlay;
     if (!n) return;
-    hipLaunchKernelGGL(cloud_optics_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, A, ncols);
+    const size_t nblk = std::min<size_t>((n + 255) / 256, 2048);
+    hipLaunchKernelGGL(cloud_optics_kernel, dim3((unsigned)nblk), dim3(256), sizeof(float) * cloud_lut_floats(L), st,
+                       L, A, ncols, (unsigned)n);
 }
 
 void launch_mcica(const McicaArgs &A, int ncols, hipStream_t st)

@@ -193,15 +193,17 @@ __device__ __forceinline__ int mi_fv(unsigned v) { return (int)(v >> 24); }
 // loads: a generic (flat) load would also count against the LDS counter, so
 // every LDS read of the record would wait for the table loads in flight.
 typedef const __attribute__((address_space(1))) char gchar;
-typedef const __attribute__((address_space(1))) float2 gfloat2;
-typedef const __attribute__((address_space(1))) float4 gfloat4;
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float2 ld2(const float2 *base, unsigned idx)
 {
-    return *(const gfloat2 *)((gchar *)base + (idx << 3));
+    const v2f v = *(const __attribute__((address_space(1))) v2f *)((gchar *)base + (idx << 3));
+    return make_float2(v.x, v.y);
 }
 __device__ __forceinline__ float4 ld4(const float4 *base, unsigned idx)
 {
-    return *(const gfloat4 *)((gchar *)base + (idx << 4));
+    const v4f v = *(const __attribute__((address_space(1))) v4f *)((gchar *)base + (idx << 4));
+    return make_float4(v.x, v.y, v.z, v.w);
 }
 
 // Per (layer, item) constants of a chunk, built once per block so that the

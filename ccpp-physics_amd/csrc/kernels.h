@@ -34,7 +34,7 @@ struct LwSolveArgs {
     const float *tau;      // [d][l][g]
     const float *pfrac;    // [d][l][g]
     const unsigned *mask;  // [d][l][MASK_WORDS] or null (clear sky)
-    const float *ctau;     // [d][l][nband] cloud absorption depth or null
+    const float *ctau;     // [l][d][nband] cloud absorption depth (d = column of the launch) or null
     const double *aer_tau; // Fortran (ncol_ld, nlay, nband) or null
     const double *t_lay, *t_lev, *tsfc, *semis, *p_lev;
     float D[MAX_ANG], W[MAX_ANG];
@@ -53,13 +53,14 @@ struct CloudArgs {
     const double *cld_frac, *lwp, *iwp, *reliq, *reice;
     const double *cnv[4], *pbl[4];  // lwp, iwp, reliq, reice (null = off)
     const double *rwp, *swp, *resnow;
-    float *ctau;              // LW: [d][l][b] absorption depth
-    float3 *copt;             // SW: [d][l][b] (tau, co-albedo 1 - ssa, g)
+    float *ctau;              // LW: [l][d][b] absorption depth (d = column of the launch)
+    float3 *copt;             // SW: [l][d][b] (tau, co-albedo 1 - ssa, g)
     double *cldtau;           // SW: (ncol_ld, nlay) band-11 cloud tau, or null
     int cldtau_blk;           // the one scheme block whose columns write cldtau
     int *err;
 };
 void launch_cloud_optics(const LutView &L, const CloudArgs &A, int ncols, hipStream_t st);
+size_t cloud_lut_floats(const LutView &L);   // LUT floats the cloud kernel stages in LDS
 
 struct McicaArgs {
     int nlay, ngpt, ncol_ld, col0, blksz, isubc;
@@ -80,7 +81,7 @@ struct SwSolveArgs {
     const int *col_idx;
     const float *tau_abs, *tau_ray;        // [d][l][g]
     const unsigned *mask;                  // [d][l][MASK_WORDS] or null
-    const float3 *copt;                    // [d][l][nband] (tau, co-albedo 1 - ssa, g) or null
+    const float3 *copt;                    // [l][d][nband] (tau, co-albedo 1 - ssa, g) or null
     const unsigned char *blkflag;
     const double *aer_tau, *aer_ssa, *aer_g;  // (ncol_ld, nlay, nband) or null
     const double *coszen, *alb_nir_dir, *alb_nir_dif, *alb_uv_dir, *alb_uv_dif, *p_lev;

@@ -94,7 +94,9 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
     const float *tau_c = A.tau + (size_t)d * nlay * ngpt;
     const float *pf_c = A.pfrac + (size_t)d * nlay * ngpt;
     const unsigned *mk_c = CLOUDS ? A.mask + (size_t)d * nlay * MASK_WORDS : nullptr;
-    const float *ct_c = CLOUDS ? A.ctau + (size_t)d * nlay * nb : nullptr;
+    // cloud depths [layer][column][band]; the grid has one block per column
+    const float *ct_c = CLOUDS ? A.ctau + (size_t)d * nb : nullptr;
+    const unsigned ct_ls = gridDim.x * (unsigned)nb;
     const unsigned mbit = 1u << (gg & 31), mword = (unsigned)gg >> 5;
 
     // Per-layer inputs, loaded for LW_BATCH layers at a time with no load
@@ -105,7 +107,7 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
         x.tau = tau_c[(unsigned)l * ngpt + gg];
         x.pf = pf_c[(unsigned)l * ngpt + gg];
         x.aer = A.aer_tau ? A.aer_tau[(size_t)c + (size_t)ld * (l + (size_t)nlay * b)] : 0.0;
-        if (CLOUDS) { x.mw = mk_c[(unsigned)l * MASK_WORDS + mword]; x.ct = ct_c[(unsigned)l * nb + b]; }
+        if (CLOUDS) { x.mw = mk_c[(unsigned)l * MASK_WORDS + mword]; x.ct = ct_c[(unsigned)l * ct_ls + b]; }
     };
     auto layer_tau = [&](const LayerIn &x) -> float {
         float t = x.tau;

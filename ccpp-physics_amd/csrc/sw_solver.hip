@@ -161,7 +161,9 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     // Column bases (uniform) with 32-bit in-column offsets.
     const float *ta_c = A.tau_abs + (size_t)d * nlay * ngpt, *tr_c = A.tau_ray + (size_t)d * nlay * ngpt;
     const unsigned *mk_c = CLOUDS ? A.mask + (size_t)d * nlay * MASK_WORDS : nullptr;
-    const float3 *co_c = CLOUDS ? A.copt + (size_t)d * nlay * nb : nullptr;
+    // cloud records [layer][column][band]; the grid has one block per column
+    const float3 *co_c = CLOUDS ? A.copt + (size_t)d * nb : nullptr;
+    const unsigned co_ls = gridDim.x * (unsigned)nb;
     const unsigned mbit = 1u << (gg & 31), mword = (unsigned)gg >> 5;
 
     // Segments with a cloudy g-point in any layer: bit s of segc for segment
@@ -215,7 +217,7 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
             }
             if (cl.value) {
                 mw[j] = mk_c[(unsigned)l * MASK_WORDS + mword];
-                cv[j] = co_c[(unsigned)l * nb + b];
+                cv[j] = co_c[(unsigned)l * co_ls + b];
             }
         }
 #pragma unroll
