@@ -311,16 +311,10 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
         else for (int b = 0; b < kd->nband; ++b) fit[2 * b + 1] = 1.66f;
         T.opt_fit = upload(c, fit.data(), fit.size(), &e);
     } else {
-        std::vector<float2> k(nk);
-        for (size_t i = 0; i < nk; ++i) { const auto v = eta_pair(kd->kmajor, i, kd->ngpt); k[i] = make_float2(v.first, v.second); }
-        T.k2 = upload(c, k.data(), nk, &e);
+        { auto v = to_f32(kd->kmajor, nk); T.kmaj = upload(c, v.data(), nk, &e); }
         const size_t nr = (size_t)kd->ngpt * kd->neta * kd->ntemp;
-        for (int r = 0; r < 2; ++r) {
-            const double *src = r == 0 ? kd->krayl_lower : kd->krayl_upper;
-            std::vector<float2> v(nr);
-            for (size_t i = 0; i < nr; ++i) { const auto p = eta_pair(src, i, kd->ngpt); v[i] = make_float2(p.first, p.second); }
-            T.kr2[r] = upload(c, v.data(), nr, &e);
-        }
+        auto a = to_f32(kd->krayl_lower, nr); T.kray[0] = upload(c, a.data(), nr, &e);
+        auto b = to_f32(kd->krayl_upper, nr); T.kray[1] = upload(c, b.data(), nr, &e);
         c->solar_dev = upload(c, kd->solar_src, kd->ngpt, &e);
     }
     T.vmr_ref = upload(c, kd->vmr_ref, (size_t)2 * (kd->ngas + 1) * kd->ntemp, &e);
@@ -389,10 +383,16 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
         T.minor_gptr[r] = upload(c, ptr.data(), ptr.size(), &e);
         T.minor_items[r] = upload(c, items.data(), items.size(), &e);
         const size_t nkm = (size_t)std::max(1, m.ncontrib) * kd->neta * kd->ntemp;
-        std::vector<float2> km(nkm, make_float2(0.0f, 0.0f));
-        if (m.ncontrib)
-            for (size_t i = 0; i < nkm; ++i) { const auto p = eta_pair(m.kminor, i, m.ncontrib); km[i] = make_float2(p.first, p.second); }
-        T.km2[r] = upload(c, km.data(), km.size(), &e);
+        if (want_lw) {
+            std::vector<float2> km(nkm, make_float2(0.0f, 0.0f));
+            if (m.ncontrib)
+                for (size_t i = 0; i < nkm; ++i) { const auto p = eta_pair(m.kminor, i, m.ncontrib); km[i] = make_float2(p.first, p.second); }
+            T.km2[r] = upload(c, km.data(), km.size(), &e);
+        } else {
+            std::vector<float> km(nkm, 0.0f);
+            if (m.ncontrib) for (size_t i = 0; i < nkm; ++i) km[i] = (float)m.kminor[i];
+            T.kmin[r] = upload(c, km.data(), km.size(), &e);
+        }
     }
     // McICA: 69069^i mod 2^32
     {

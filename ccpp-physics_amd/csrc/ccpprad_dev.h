@@ -1,13 +1,15 @@
 // Device-side data structures and helpers shared by the gfx950 kernels.
 //
 // Layout in HBM (all device tables FP32 unless stated):
-//   kq (LW)           : [ntemp][npres+1][neta][ngpt] float4 {k(eta), k(eta+1),
+//   LW, eta pairs (eta + 1 past the last eta row holds 0, never read):
+//   kq                : [ntemp][npres+1][neta][ngpt] float4 {k(eta), k(eta+1),
 //                       Planck fraction(eta), (eta+1)}: the two eta corners of
 //                       both tables in one 16-byte load, g-point innermost
-//   k2 (SW)           : [ntemp][npres+1][neta][ngpt] float2 {k(eta), k(eta+1)}
 //   km2[r]            : [ntemp][neta][ncontrib_r] float2, contributor innermost
-//   kr2[r] (SW)       : [ntemp][neta][ngpt] float2 Rayleigh
-//   (eta + 1 past the last eta row holds 0; the interpolation never reads it)
+//   SW, plain:
+//   kmaj              : [ntemp][npres+1][neta][ngpt]
+//   kmin[r]           : [ntemp][neta][ncontrib_r]
+//   kray[r]           : [ntemp][neta][ngpt] Rayleigh
 //   totplnk           : [nband][nplanck]
 //   vmr_ref           : [ntemp][ngas+1][2]  FP64 (feeds the FP64 interpolation)
 // Per-call work buffers, g-point innermost so a wavefront's lanes (one lane
@@ -46,9 +48,10 @@ struct TablesView {
     double press_min, press_max, temp_min, temp_max;
     const double *temp_ref;    // [ntemp] FP64
     const float4 *kq;          // LW major absorption and Planck fraction, eta pairs
-    const float2 *k2;          // SW major absorption, eta pairs
-    const float2 *km2[2];      // minor absorption per regime, eta pairs
-    const float2 *kr2[2];      // SW Rayleigh per regime, eta pairs
+    const float2 *km2[2];      // LW minor absorption per regime, eta pairs
+    const float *kmaj;         // SW major absorption
+    const float *kmin[2];      // SW minor absorption per regime
+    const float *kray[2];      // SW Rayleigh per regime
     const float *totplnk;
     const float *opt_fit;      // [nband][2]
     const double *vmr_ref;

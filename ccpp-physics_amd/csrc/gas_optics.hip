@@ -193,6 +193,10 @@ __device__ __forceinline__ int mi_fv(unsigned v) { return (int)(v >> 24); }
 // loads: a generic (flat) load would also count against the LDS counter, so
 // every LDS read of the record would wait for the table loads in flight.
 typedef const __attribute__((address_space(1))) char gchar;
+__device__ __forceinline__ float ld1(const float *base, unsigned idx)
+{
+    return *(const __attribute__((address_space(1))) float *)((gchar *)base + (idx << 2));
+}
 typedef float v2f __attribute__((ext_vector_type(2)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float2 ld2(const float2 *base, unsigned idx)
@@ -281,7 +285,7 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
         }
     }
     const unsigned nc0 = (unsigned)T.ncontrib[0], nc1 = (unsigned)T.ncontrib[1];
-    const unsigned dp_ = (unsigned)(neta * ngpt);   // jpress stride of the major table
+    const unsigned dp_ = (unsigned)(neta * ngpt), de = (unsigned)ngpt;   // jpress and eta strides of the major table
     __syncthreads();
 
     // item constants of every (layer, item) of the chunk
@@ -326,15 +330,14 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
                     k[u][t][0] = v0.x; k[u][t][2] = v0.y; q[u][t][0] = v0.z; q[u][t][2] = v0.w;
                     k[u][t][1] = v1.x; k[u][t][3] = v1.y; q[u][t][1] = v1.z; q[u][t][3] = v1.w;
                 } else {
-                    const float2 v0 = ld2(T.k2, base), v1 = ld2(T.k2, base + dp_);
-                    k[u][t][0] = v0.x; k[u][t][2] = v0.y; k[u][t][1] = v1.x; k[u][t][3] = v1.y;
+                    k[u][t][0] = ld1(T.kmaj, base); k[u][t][1] = ld1(T.kmaj, base + dp_);
+                    k[u][t][2] = ld1(T.kmaj, base + de); k[u][t][3] = ld1(T.kmaj, base + dp_ + de);
                     const unsigned b0 = __umul24(Ij[(2 * f + t) * 2 + 1].x, (unsigned)ngpt) + (unsigned)g;
-                    const float2 r2 = ld2(reg == 0 ? T.kr2[0] : T.kr2[1], b0);
-                    kr[u][t][0] = r2.x; kr[u][t][1] = r2.y;
+                    const float *kt = reg == 0 ? T.kray[0] : T.kray[1];
+                    kr[u][t][0] = ld1(kt, b0); kr[u][t][1] = ld1(kt, b0 + de);
                 }
             }
             const unsigned nc = reg == 0 ? nc0 : nc1;
-            const float2 *kmt = reg == 0 ? T.km2[0] : T.km2[1];
 #pragma unroll
             for (int ii = 0; ii < NM; ++ii) {
                 const unsigned v = reg == 0 ? mit[0][ii] : mit[1][ii];
@@ -342,9 +345,16 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
                 const unsigned ci = (unsigned)mi_ci(v);
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
-                    const float2 m2 = ld2(kmt, __umul24(Ij[(2 * fv + t) * 2 + 1].x, nc) + ci);
-                    km[u][ii][t][0] = m2.x;
-                    km[u][ii][t][1] = m2.y;
+                    const unsigned b0 = __umul24(Ij[(2 * fv + t) * 2 + 1].x, nc) + ci;
+                    if (LW) {
+                        const float2 m2 = ld2(reg == 0 ? T.km2[0] : T.km2[1], b0);
+                        km[u][ii][t][0] = m2.x;
+                        km[u][ii][t][1] = m2.y;
+                    } else {
+                        const float *kmt = reg == 0 ? T.kmin[0] : T.kmin[1];
+                        km[u][ii][t][0] = ld1(kmt, b0);
+                        km[u][ii][t][1] = ld1(kmt, b0 + nc);
+                    }
                 }
             }
         }
