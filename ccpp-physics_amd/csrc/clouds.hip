@@ -231,12 +231,26 @@ __device__ __forceinline__ uint32_t mt_temper(uint32_t y)
     return y;
 }
 
-// random_number_t's conversion of a tempered draw to [0, 1]
+// x / 4294967295 correctly rounded, for integral 0 <= x < 2^32, without a
+// division: q0 = x 2^-32 (1 + 2^-32) is within an ulp of the quotient, the
+// residual x - q0 D is exact under the FMA, and one correction step rounds
+// it.  Equal to the IEEE quotient for every such x (checked exhaustively:
+// tests/test_mt_uniform.py).
+__device__ __forceinline__ double div_2p32m1(double x)
+{
+    const double S = 2.3283064365386963e-10;   // 2^-32
+    const double v = x * S;                    // exact
+    const double q0 = fma(v, S, v);
+    const double rem = fma(-q0, 4294967295.0, x);
+    return fma(rem, 1.0 / 4294967295.0, q0);
+}
+
+// random_number_t's conversion of a tempered draw to [0, 1]: non-negative
+// draws pass through default REAL first
 __device__ __forceinline__ double mt_uniform(uint32_t y)
 {
     const int32_t ys = (int32_t)y;
-    if (ys < 0) return ((double)ys + 4294967296.0) / 4294967295.0;
-    return (double)(float)ys / 4294967295.0;
+    return div_2p32m1(ys < 0 ? (double)ys + 4294967296.0 : (double)(float)ys);
 }
 
 __global__ __launch_bounds__(64) void mcica_kernel(McicaArgs A)
