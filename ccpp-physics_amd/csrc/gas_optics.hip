@@ -158,16 +158,19 @@ __device__ __forceinline__ void interp_row(const TablesView &T, const RecGeom &G
     }
     if (G.nf2 & 3) r[G.je + G.nf2 / 4] = jeword;
     const int reg = itr - 1;
+    // per-layer factors of the minor scalings (the same FP64 expressions the
+    // interval loop would form each time)
+    const double dens = 0.01 * pl / tl;
+    const double vf = 1.0 / col_dry;
+    const double df = 1.0 / (1.0 + col[1] * vf);
     for (int m = 0; m < G.nmn; ++m) {
         double sc = 0.0;
         if (m < T.nminor[reg]) {
             const int *mp = T.minor_par[reg] + 5 * m;
             sc = pick(col, ng1, mp[0]);
             if (mp[1]) {
-                sc *= 0.01 * pl / tl;
+                sc *= dens;
                 if (mp[2] > 0) {
-                    const double vf = 1.0 / col_dry;
-                    const double df = 1.0 / (1.0 + col[1] * vf);
                     const double cs = pick(col, ng1, mp[2]);
                     if (mp[3]) sc *= (1.0 - cs * vf * df);
                     else sc *= cs * vf * df;

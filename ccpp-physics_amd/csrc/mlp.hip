@@ -22,6 +22,8 @@
 // match, by storing its weights accordingly.  One wavefront = 32 rows.
 #include "kernels.h"
 
+#include <type_traits>
+
 namespace ccpprad {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -249,57 +251,70 @@ __global__ __launch_bounds__(512) void mlp_infer_kernel(MlpInferArgs A)
     const int half = l >> 5, rl = l & 31;
     const long nrows = (long)A.ncols * A.nlay;
     const long ntiles = (nrows + 31) / 32;
-    for (long tile = (long)blockIdx.x * 8 + w; tile < ntiles; tile += (long)gridDim.x * 8) {
-        const long row = tile * 32 + rl;
-        const bool valid = row < nrows;
-        const long rr = valid ? row : nrows - 1;
-        // rows are ordered column-fastest within a layer, so the 32 rows of a
-        // tile read consecutive columns of the (nCol, nLay) inputs
-        const int d = (int)(rr % A.ncols), lay = (int)(rr / A.ncols);
-        const long q = (long)(A.col0 + d) + (long)A.ncol_ld * lay;
-        const long orow = (long)d * A.nlay + lay;   // output row, [col][lay] order
-        // features (standardised, split into BF16 hi + lo) and dry-air column
-        const double vh = A.vmr[0][q];
-        float f[MLP_NIN];
-        f[0] = (float)A.t_lay[q];
-        f[1] = __logf((float)A.p_lay[q]);
-        f[2] = __logf((float)vh);
+    const long stride = (long)gridDim.x * 8;
+    // Each wavefront carries two 32-row tiles through the network together:
+    // the two tiles' MFMA chains and softplus work are independent, so one
+    // tile's arithmetic runs while the other's matrix products are in flight.
+    for (long tile0 = (long)blockIdx.x * 8 + w; tile0 < ntiles; tile0 += 2 * stride) {
+        long orow[2];
+        int dcol[2];
+        bool valid[2], out[2];
+        float col_dry[2];
+        bf16x8 h[2][8];   // B operands of the next layer (8 K-chunks of 16), per tile
 #pragma unroll
-        for (int i = 1; i < 5; ++i) f[2 + i] = __logf((float)A.vmr[i][q]);
-        bool out = false;
+        for (int s = 0; s < 2; ++s) {
+            const long row = (tile0 + s * stride) * 32 + rl;
+            valid[s] = row < nrows;
+            const long rr = valid[s] ? row : nrows - 1;
+            // rows are ordered column-fastest within a layer, so the 32 rows of
+            // a tile read consecutive columns of the (nCol, nLay) inputs
+            const int d = (int)(rr % A.ncols), lay = (int)(rr / A.ncols);
+            const long q = (long)(A.col0 + d) + (long)A.ncol_ld * lay;
+            orow[s] = (long)d * A.nlay + lay;   // output row, [col][lay] order
+            dcol[s] = d;
+            // features (standardised, split into BF16 hi + lo) and dry-air column
+            const double vh = A.vmr[0][q];
+            float f[MLP_NIN];
+            f[0] = (float)A.t_lay[q];
+            f[1] = __logf((float)A.p_lay[q]);
+            f[2] = __logf((float)vh);
 #pragma unroll
-        for (int i = 0; i < MLP_NIN; ++i) {
-            out |= !(f[i] >= A.lo[i] && f[i] <= A.hi[i]);
-            f[i] = (f[i] - A.xmean[i]) * A.xistd[i];
-        }
-        const double dp = fabs(A.p_lev[q] - A.p_lev[q + A.ncol_ld]);
-        const double fact = 1.0 / (1.0 + vh);
-        const float col_dry = (float)(10.0 * dp * AVOGAD / (1000.0 * ((M_DRY + M_H2O * vh) * fact) * 100.0 * GRAV) * fact);
-        // layer 1 B operand: slot k = hi_k (k < 7), lo_{k-7} (7 <= k < 14), 0
-        bf16x8 b1;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int k = 8 * half + j;
-            float v = 0.0f;
+            for (int i = 1; i < 5; ++i) f[2 + i] = __logf((float)A.vmr[i][q]);
+            bool o = false;
 #pragma unroll
             for (int i = 0; i < MLP_NIN; ++i) {
-                const float hi = (float)f2bf(f[i]);
-                if (k == i) v = hi;
-                if (k == i + MLP_NIN) v = (float)f2bf(f[i] - hi);
+                o |= !(f[i] >= A.lo[i] && f[i] <= A.hi[i]);
+                f[i] = (f[i] - A.xmean[i]) * A.xistd[i];
             }
-            b1[j] = f2bf(v);
-        }
-        bf16x8 h[8];   // B operands of the next layer (8 K-chunks of 16)
+            out[s] = o;
+            const double dp = fabs(A.p_lev[q] - A.p_lev[q + A.ncol_ld]);
+            const double fact = 1.0 / (1.0 + vh);
+            col_dry[s] = (float)(10.0 * dp * AVOGAD / (1000.0 * ((M_DRY + M_H2O * vh) * fact) * 100.0 * GRAV) * fact);
+            // layer 1 B operand: slot k = hi_k (k < 7), lo_{k-7} (7 <= k < 14), 0
+            bf16x8 b1;
 #pragma unroll
-        for (int np = 0; np < 2; ++np) {
-            f32x16 a0 = {}, a1 = {};
-            a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(F1[(2 * np) * 64 + l], b1, a0, 0, 0, 0);
-            a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(F1[(2 * np + 1) * 64 + l], b1, a1, 0, 0, 0);
+            for (int j = 0; j < 8; ++j) {
+                const int k = 8 * half + j;
+                float v = 0.0f;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int n0 = (2 * np) * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
-                h[4 * np + (r >> 3)][r & 7] = f2bf(softplus_f(a0[r] + bias[n0]));
-                h[4 * np + 2 + (r >> 3)][r & 7] = f2bf(softplus_f(a1[r] + bias[n0 + 32]));
+                for (int i = 0; i < MLP_NIN; ++i) {
+                    const float hi = (float)f2bf(f[i]);
+                    if (k == i) v = hi;
+                    if (k == i + MLP_NIN) v = (float)f2bf(f[i] - hi);
+                }
+                b1[j] = f2bf(v);
+            }
+#pragma unroll
+            for (int np = 0; np < 2; ++np) {
+                f32x16 a0 = {}, a1 = {};
+                a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(F1[(2 * np) * 64 + l], b1, a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(F1[(2 * np + 1) * 64 + l], b1, a1, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int n0 = (2 * np) * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+                    h[s][4 * np + (r >> 3)][r & 7] = f2bf(softplus_f(a0[r] + bias[n0]));
+                    h[s][4 * np + 2 + (r >> 3)][r & 7] = f2bf(softplus_f(a1[r] + bias[n0 + 32]));
+                }
             }
         }
         // layers 2, 3
@@ -307,62 +322,82 @@ __global__ __launch_bounds__(512) void mlp_infer_kernel(MlpInferArgs A)
         for (int L = 0; L < 2; ++L) {
             const bf16x8 *Fw = L == 0 ? F2 : F3;
             const float *bl = bias + 128 * (L + 1);
-            bf16x8 hn[8];
+            bf16x8 hn[2][8];
 #pragma unroll
             for (int np = 0; np < 2; ++np) {
-                f32x16 a0 = {}, a1 = {};
+                f32x16 a[2][2] = {{{}, {}}, {{}, {}}};
 #pragma unroll
                 for (int kc = 0; kc < 8; ++kc) {
-                    a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Fw[((2 * np) * 8 + kc) * 64 + l], h[kc], a0, 0, 0, 0);
-                    a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Fw[((2 * np + 1) * 8 + kc) * 64 + l], h[kc], a1, 0, 0, 0);
+                    const bf16x8 w0 = Fw[((2 * np) * 8 + kc) * 64 + l], w1 = Fw[((2 * np + 1) * 8 + kc) * 64 + l];
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        a[s][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, h[s][kc], a[s][0], 0, 0, 0);
+                        a[s][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, h[s][kc], a[s][1], 0, 0, 0);
+                    }
                 }
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int n0 = (2 * np) * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
-                    hn[4 * np + (r >> 3)][r & 7] = f2bf(softplus_f(a0[r] + bl[n0]));
-                    hn[4 * np + 2 + (r >> 3)][r & 7] = f2bf(softplus_f(a1[r] + bl[n0 + 32]));
-                }
+                for (int s = 0; s < 2; ++s)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int n0 = (2 * np) * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+                        hn[s][4 * np + (r >> 3)][r & 7] = f2bf(softplus_f(a[s][0][r] + bl[n0]));
+                        hn[s][4 * np + 2 + (r >> 3)][r & 7] = f2bf(softplus_f(a[s][1][r] + bl[n0 + 32]));
+                    }
             }
 #pragma unroll
-            for (int kc = 0; kc < 8; ++kc) h[kc] = hn[kc];
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int kc = 0; kc < 8; ++kc) h[s][kc] = hn[s][kc];
         }
-        // layer 4 (linear), two tiles at a time, fused output transform
-        const float *b4 = bias + 384, *ysd = bias + 640, *ymn = bias + 896;
-        float *tau_row = A.tau + orow * MLP_G;
-        float *pf_row = A.pfrac + orow * MLP_G;
-#pragma unroll 1
-        for (int np = 0; np < 4; ++np) {
-            f32x16 acc[2] = {{}, {}};
+        // layer 4 (linear), two output tiles at a time, fused output transform:
+        // y = acc ystd + (b4 ystd + ymean); tau = e^y col_dry, pfrac = max(y, 0)
+        const float *ysd = bias + 640, *yc = bias + 896;
+        // output tiles 2 np, 2 np + 1: tau for np < 2 (TAU), Planck fraction after
+        auto layer4 = [&](auto TAU, int np) {
+            f32x16 acc[2][2] = {{{}, {}}, {{}, {}}};
 #pragma unroll
             for (int kc = 0; kc < 8; ++kc) {
-                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(F4[((2 * np) * 8 + kc) * 64 + l], h[kc], acc[0], 0, 0, 0);
-                acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(F4[((2 * np + 1) * 8 + kc) * 64 + l], h[kc], acc[1], 0, 0, 0);
+                const bf16x8 w0 = F4[((2 * np) * 8 + kc) * 64 + l], w1 = F4[((2 * np + 1) * 8 + kc) * 64 + l];
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    acc[s][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, h[s][kc], acc[s][0], 0, 0, 0);
+                    acc[s][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, h[s][kc], acc[s][1], 0, 0, 0);
+                }
             }
-            if (valid) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (!valid[s]) continue;
+                float *orow_p = (TAU.value ? A.tau : A.pfrac) + orow[s] * MLP_G;
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
                     const int nt = 2 * np + u;
 #pragma unroll
                     for (int r4 = 0; r4 < 4; ++r4) {
                         const int n0 = nt * 32 + 8 * r4 + 4 * half;
+                        const float4 sc = *(const float4 *)(ysd + n0), cc = *(const float4 *)(yc + n0);
+                        const float *scp = (const float *)&sc, *ccp = (const float *)&cc;
                         float4 o;
                         float *op = (float *)&o;
 #pragma unroll
                         for (int i = 0; i < 4; ++i) {
-                            const int n = n0 + i;
-                            const float y = (acc[u][4 * r4 + i] + b4[n]) * ysd[n] + ymn[n];
-                            op[i] = nt < 4 ? __expf(y) * col_dry : fmaxf(y, 0.0f);
+                            const float y = acc[s][u][4 * r4 + i] * scp[i] + ccp[i];
+                            op[i] = TAU.value ? __expf(y) * col_dry[s] : fmaxf(y, 0.0f);
                         }
-                        if (nt < 4) *(float4 *)(tau_row + n0) = o;
-                        else *(float4 *)(pf_row + (n0 - MLP_G)) = o;
+                        *(float4 *)(orow_p + (n0 - (TAU.value ? 0 : MLP_G))) = o;
                     }
                 }
             }
-        }
-        if (valid && half == 0) {
-            A.oob[orow] = out ? 1 : 0;
-            if (out) A.colflag[d] = 1;
-        }
+        };
+#pragma unroll 1
+        for (int np = 0; np < 2; ++np) layer4(std::true_type{}, np);
+#pragma unroll 1
+        for (int np = 2; np < 4; ++np) layer4(std::false_type{}, np);
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+            if (valid[s] && half == 0) {
+                A.oob[orow[s]] = out[s] ? 1 : 0;
+                if (out[s]) A.colflag[dcol[s]] = 1;
+            }
     }
 }
 
@@ -479,7 +514,12 @@ static void build_image(const std::vector<float> &P, const float *ystd, const fl
     for (int i = 0; i < 3; ++i)
         for (int n = 0; n < 128; ++n) b[128 * i + n] = P[boff(i) + n];
     for (int n = 0; n < 256; ++n) b[384 + n] = P[boff(3) + n];
-    for (int n = 0; n < 256; ++n) { b[640 + n] = ystd[n]; b[896 + n] = ymean[n]; }
+    // output de-standardisation folded with the layer-4 bias:
+    // y = (acc + b4) ystd + ymean = acc ystd + (b4 ystd + ymean)
+    for (int n = 0; n < 256; ++n) {
+        b[640 + n] = ystd[n];
+        b[896 + n] = (float)((double)P[boff(3) + n] * ystd[n] + ymean[n]);
+    }
 }
 
 static int refresh(ccpprad_mlp *m)
