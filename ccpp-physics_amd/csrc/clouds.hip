@@ -257,8 +257,7 @@ __global__ __launch_bounds__(64) void mcica_kernel(McicaArgs A)
 {
     extern __shared__ unsigned char smem_raw[];
     uint32_t *mt = (uint32_t *)smem_raw;              // [624]
-    uint32_t *tmp = mt + 624;                          // [2][624]
-    double *cf = (double *)(tmp + 2 * 624);            // [nlay]
+    double *cf = (double *)(mt + 624);                 // [nlay]
     const int d = blockIdx.x, lane = threadIdx.x;
     const int od = A.col0 + d;
     const int c = A.col_idx ? A.col_idx[od] - 1 : od;
@@ -282,28 +281,41 @@ __global__ __launch_bounds__(64) void mcica_kernel(McicaArgs A)
     __syncthreads();
     int fst = -1;
     for (int l = 0; l < nlay; ++l) if (cf[l] > 0.0) { fst = l; break; }
-    int cur = -1;  // index of the newest tempered block
+    // twist the state in place, 64 words at a time (reads of a chunk before
+    // its writes)
+    auto twist = [&]() {
+        for (int k0 = 0; k0 < 624; k0 += 64) {
+            const int kk = k0 + lane;
+            uint32_t nv = 0;
+            if (kk < 624) {
+                const uint32_t y = (mt[kk] & 0x80000000u) | (mt[kk == 623 ? 0 : kk + 1] & 0x7fffffffu);
+                nv = mt[kk < 227 ? kk + 397 : kk - 227] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+            }
+            __syncthreads();
+            if (kk < 624) mt[kk] = nv;
+            __syncthreads();
+        }
+    };
+    int cur = -1;  // number of the twisted block the state holds (draws 624 cur ... 624 cur + 623)
     double lr[2] = {0.0, 0.0}, pr[2] = {0.0, 0.0};
     for (int l = 0; l < nlay; ++l) {
         const int base = vmax ? 0 : l * ngpt;
         const int need = (base + ngpt - 1) / 624;
-        while (cur < need) {
-            // twist (64 words at a time, reads before writes)
-            for (int k0 = 0; k0 < 624; k0 += 64) {
-                const int kk = k0 + lane;
-                uint32_t nv = 0;
-                if (kk < 624) {
-                    const uint32_t y = (mt[kk] & 0x80000000u) | (mt[kk == 623 ? 0 : kk + 1] & 0x7fffffffu);
-                    nv = mt[kk < 227 ? kk + 397 : kk - 227] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
-                }
-                __syncthreads();
-                if (kk < 624) mt[kk] = nv;
-                __syncthreads();
+        // the lane's draws (tempered on read): those of the block held now,
+        // then, if the layer reaches into the next block, twist and the rest
+        uint32_t yv[2] = {0u, 0u};
+        if (cur < need) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int k = base + lane + 64 * h;
+                if (lane + 64 * h < ngpt && k / 624 == cur) yv[h] = mt_temper(mt[k - 624 * cur]);
             }
-            ++cur;
-            uint32_t *slot = tmp + 624 * (cur & 1);
-            for (int i = lane; i < 624; i += 64) slot[i] = mt_temper(mt[i]);
-            __syncthreads();
+            while (cur < need) { twist(); ++cur; }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = base + lane + 64 * h;
+            if (lane + 64 * h < ngpt && k / 624 == cur) yv[h] = mt_temper(mt[k - 624 * cur]);
         }
         const double cfl = cf[l];
         const bool cl = cfl > 0.0;
@@ -315,8 +327,7 @@ __global__ __launch_bounds__(64) void mcica_kernel(McicaArgs A)
             const int g = lane + 64 * h;
             bool bit = false;
             if (g < ngpt && (cl || dcorr)) {
-                const int k = base + g;
-                const double r = mt_uniform(tmp[624 * ((k / 624) & 1) + (k % 624)]);
+                const double r = mt_uniform(yv[h]);
                 if (plain) {
                     if (cl) {
                         if (!clp) lr[h] = r;
@@ -396,7 +407,7 @@ void launch_cloud_optics(const LutView &L, const CloudArgs &A, int ncols, hipStr
 void launch_mcica(const McicaArgs &A, int ncols, hipStream_t st)
 {
     if (ncols <= 0) return;
-    const size_t smem = sizeof(uint32_t) * 3 * 624 + sizeof(double) * A.nlay;
+    const size_t smem = sizeof(uint32_t) * 624 + sizeof(double) * A.nlay;
     hipLaunchKernelGGL(mcica_kernel, dim3(ncols), dim3(64), smem, st, A);
 }
 
