@@ -58,8 +58,9 @@ int gas_record_words(const TablesView &T) { return rec_geom(T.nflav, T.nminor[0]
 // ---------------------------------------------------------------------------
 // interpolation: thread per (column, layer), FP64
 // ---------------------------------------------------------------------------
-// v: the lane's gas columns in LDS (index 0 = dry air); i outside 1..n-1
-// reads the dry-air column
+// v: the lane's gas columns in LDS (index 0 = dry air; a register array
+// indexed at run time would go to scratch); i outside 1..n-1 reads the
+// dry-air column
 __device__ __forceinline__ double pick(const double *v, int n, int i)
 {
     return v[(i >= 1 && i < n) ? i : 0];
@@ -70,7 +71,7 @@ __device__ __forceinline__ double pick(const double *v, int n, int i)
 // contiguous block with 16-byte stores.
 constexpr int INTERP_ROWS = 64;
 
-__device__ __forceinline__ void interp_row(const TablesView &T, const RecGeom &G, size_t i, int ncols, int nlay,
+__device__ __forceinline__ void interp_row(const TablesView &T, const RecGeom &G, unsigned i, int ncols, int nlay,
                                            int ncol_ld, const int *col_idx, int col0, const double *p_lay,
                                            const double *p_lev, const double *t_lay, const VmrPtrs &vmr,
                                            unsigned *r, double *col, int *err);
@@ -83,31 +84,40 @@ __global__ __launch_bounds__(INTERP_ROWS) void gas_interp_kernel(
     extern __shared__ unsigned srec[];   // [INTERP_ROWS][words]
     __shared__ double scol[INTERP_ROWS][MAX_GAS + 1];
     const RecGeom G = rec_geom(T.nflav, T.nminor[0], T.nminor[1]);
-    const size_t nrow = (size_t)ncols * nlay;
-    const size_t i0 = (size_t)blockIdx.x * INTERP_ROWS, i = i0 + threadIdx.x;
+    // a launch covers fewer than 2^32 rows: 32-bit row arithmetic
+    const unsigned nrow = (unsigned)ncols * (unsigned)nlay;
+    const unsigned i0 = blockIdx.x * INTERP_ROWS, i = i0 + threadIdx.x;
     // fallback launch: blocks without a flagged column write nothing
-    if (col_mask && !__any(i < nrow && col_mask[i % ncols])) return;   // one wavefront: uniform
+    if (col_mask && !__any(i < nrow && col_mask[i % (unsigned)ncols])) return;   // one wavefront: uniform
     // every row of a flagged column gets a record: the gather kernel stages
     // whole chunks, so no record it reads may be left unwritten (rows of
     // unflagged columns are written with whatever the slot holds; nothing
     // reads them)
-    if (i < nrow && (!col_mask || col_mask[i % ncols]))
+    if (i < nrow && (!col_mask || col_mask[i % (unsigned)ncols]))
         interp_row(T, G, i, ncols, nlay, ncol_ld, col_idx, col0, p_lay, p_lev, t_lay, vmr,
                    srec + threadIdx.x * G.words, scol[threadIdx.x], err);
     __syncthreads();
-    const size_t nr = nrow - i0 < (size_t)INTERP_ROWS ? nrow - i0 : (size_t)INTERP_ROWS;
+    const unsigned nr = nrow - i0 < (unsigned)INTERP_ROWS ? nrow - i0 : (unsigned)INTERP_ROWS;
     const int n4 = (int)nr * G.words / 4;
-    uint4 *dst = (uint4 *)(rec + i0 * G.words);
+    uint4 *dst = (uint4 *)(rec + (size_t)i0 * G.words);
     const uint4 *src = (const uint4 *)srec;
-    for (int k = threadIdx.x; k < n4; k += INTERP_ROWS) dst[k] = src[k];
+    // 16-byte copies, 6 LDS reads in flight per lane
+    for (int k0 = threadIdx.x; k0 < n4; k0 += 6 * INTERP_ROWS) {
+        uint4 v[6];
+#pragma unroll
+        for (int u = 0; u < 6; ++u) v[u] = src[min(k0 + u * INTERP_ROWS, n4 - 1)];
+#pragma unroll
+        for (int u = 0; u < 6; ++u)
+            if (k0 + u * INTERP_ROWS < n4) dst[k0 + u * INTERP_ROWS] = v[u];
+    }
 }
 
-__device__ __forceinline__ void interp_row(const TablesView &T, const RecGeom &G, size_t i, int ncols, int nlay,
+__device__ __forceinline__ void interp_row(const TablesView &T, const RecGeom &G, unsigned i, int ncols, int nlay,
                                            int ncol_ld, const int *col_idx, int col0, const double *p_lay,
                                            const double *p_lev, const double *t_lay, const VmrPtrs &vmr,
                                            unsigned *r, double *col, int *err)
 {
-    const int d = (int)(i % ncols), l = (int)(i / ncols);   // column fastest: coalesced CCPP reads
+    const int l = (int)(i / (unsigned)ncols), d = (int)(i - (unsigned)l * (unsigned)ncols);   // column fastest: coalesced CCPP reads
     const int gc = col_idx ? col_idx[col0 + d] - 1 : col0 + d;
     const size_t q = (size_t)gc + (size_t)ncol_ld * l;
     const double tmin = T.temp_ref_min, dT = T.temp_ref_delta;
