@@ -241,6 +241,10 @@ size_t gas_gather_smem(int is_lw, int nflav, int nminor0, int nminor1, int ngpt,
     return gather_smem(rec_geom(nflav, nminor0, nminor1));
 }
 
+// layers whose gathers are in flight together (3 or 4 measured slower for
+// both LW and SW: fewer wavefronts fit)
+template <bool LW> constexpr int gather_layers() { return 2; }
+
 template <bool LW, int NM>
 __global__ __launch_bounds__(128) void gas_optics_kernel(
     TablesView T, int nlay, int ncols, const unsigned *__restrict__ rec, float *__restrict__ tau,
@@ -319,14 +323,15 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
     __syncthreads();
     if (!act) return;
 
-    // two layers at a time: every table load of both layers (major, Planck
-    // fraction or Rayleigh, minor) is issued before any arithmetic; an odd
-    // last layer is computed twice and stored once
-    for (int j0 = 0; j0 < nl; j0 += 2) {
-        float k[2][2][4], q[2][2][4], kr[2][2][2], km[2][NM > 0 ? NM : 1][2][2];
-        int regv[2];
+    // GU layers at a time: every table load of them (major, Planck fraction
+    // or Rayleigh, minor) is issued before any arithmetic; layers past the
+    // chunk end repeat its last layer and are not stored
+    constexpr int GU = gather_layers<LW>();
+    for (int j0 = 0; j0 < nl; j0 += GU) {
+        float k[GU][2][4], q[GU][2][4], kr[GU][2][2], km[GU][NM > 0 ? NM : 1][2][2];
+        int regv[GU];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < GU; ++u) {
             const int j = min(j0 + u, nl - 1);
             const unsigned *r = srec + j * G.words;
             const int reg = (int)(__builtin_amdgcn_readfirstlane(r[0]) >> 16) - 1;
@@ -372,7 +377,7 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
             }
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < GU; ++u) {
             const int j = min(j0 + u, nl - 1), l = l0 + j, reg = regv[u];
             const unsigned *r = srec + j * G.words;
             const uint4 *Ij = sitem + (size_t)j * G.nf2 * 2;
