@@ -39,8 +39,8 @@ METRIC = "LW+SW columns/s at 127 layers (table vs NN) @ HR RMSE<=0.1 K/day vs FP
 # --no-nn --copy-probe`), each scaled by the reading of a 1 GiB device copy
 # (FETCH x 2.0, WRITE x 1.0).  Counters cannot be read from inside this
 # process, so the last measurement of the current kernels is recorded here.
-TRAFFIC_GB = {"sw_sw_solve": 21.732, "lw_lw_solve": 21.707, "lw_gas_optics": 11.730, "sw_gas_optics": 10.665,
-              "lw_gas_interp": 3.005, "sw_gas_interp": 3.005, "lw_cloud_optics": 1.239, "sw_cloud_optics": 4.304,
+TRAFFIC_GB = {"sw_sw_solve": 22.186, "lw_lw_solve": 23.174, "lw_gas_optics": 11.728, "sw_gas_optics": 10.665,
+              "lw_gas_interp": 3.007, "sw_gas_interp": 3.007, "lw_cloud_optics": 2.605, "sw_cloud_optics": 2.605,
               "lw_mcica": 0.669, "sw_mcica": 0.669}
 
 
