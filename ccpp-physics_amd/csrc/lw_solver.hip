@@ -16,9 +16,10 @@
 // intensity, Jacobian); tau and the Planck fraction are re-read in the second
 // sweep rather than stored, so the HBM traffic is two reads of tau/pfrac (8 B
 // per cell each) plus the FP64 level fluxes.
-// At every level each wavefront sums its lanes' FP32 fluxes in FP64 and
-// parks the partial in LDS; the epilogue adds the two wave partials, writes
-// the (nCol, nLay+1) fluxes and differences them into the heating rate.
+// At every level each wavefront sums its lanes' FP32 fluxes in FP64 and adds
+// the partial into one zeroed LDS slot per level (two adds onto 0, so the
+// result does not depend on their order); the epilogue writes the
+// (nCol, nLay+1) fluxes and differences them into the heating rate.
 #include "kernels.h"
 
 namespace ccpprad {
@@ -42,12 +43,13 @@ __device__ __forceinline__ void lw_trans(float x, float &e, float &E, float &f)
 
 constexpr int LW_BATCH = 8;   // also the number of level sums reduced together
 
-// LDS per block: [2 waves][3][nlev] FP64 level sums, then the column's band
-// Planck function at layers [nlay][nband], levels [nlev][nband] and the
-// surface [2][nband] (at T_sfc and T_sfc + 1 K).
+// LDS per block: [3][nlev] FP64 level sums (up, down, Jacobian) of both
+// wavefronts, then the column's band Planck function at layers
+// [nlay][nband], levels [nlev][nband] and the surface [2][nband] (at T_sfc
+// and T_sfc + 1 K).
 size_t lw_solve_smem(int nlay, int nband)
 {
-    return sizeof(double) * 2 * 3 * (size_t)(nlay + 1) + sizeof(float) * (size_t)nband * (2 * (size_t)nlay + 3);
+    return sizeof(double) * 3 * (size_t)(nlay + 1) + sizeof(float) * (size_t)nband * (2 * (size_t)nlay + 3);
 }
 
 template <int NANG, bool CLOUDS, bool JAC, bool OPTANG>
@@ -62,8 +64,11 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
     const bool act = g < ngpt;
     const int gg = act ? g : 0;
     const int b = T.g2b[gg];
-    double *rup = red + (size_t)wave * 3 * nlev, *rdn = rup + nlev, *rjc = rdn + nlev;
-    float *Blay = (float *)(red + 6 * (size_t)nlev), *Blev = Blay + (size_t)nlay * nb, *Bsfc = Blev + (size_t)nlev * nb;
+    // level sums of both wavefronts: each slot gets exactly two FP64 adds onto
+    // 0, so the total is the same in either order
+    double *rup = red, *rdn = red + nlev, *rjc = red + 2 * (size_t)nlev;
+    float *Blay = (float *)(red + 3 * (size_t)nlev), *Blev = Blay + (size_t)nlay * nb, *Bsfc = Blev + (size_t)nlev * nb;
+    for (int v = g; v < 3 * nlev; v += blockDim.x) red[v] = 0.0;
 
     // ---- the column's band Planck function (interpolate1D of totplnk),
     // shared by the g-points of a band
@@ -133,7 +138,6 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
     float I[NANG];
 #pragma unroll
     for (int a = 0; a < NANG; ++a) I[a] = 0.0f;
-    if (lane == 0) rdn[lev_toa] = 0.0;
     for (int k0 = 0; k0 < nlay; k0 += LW_BATCH) {
         LayerIn x[LW_BATCH];
 #pragma unroll
@@ -164,7 +168,7 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
         }
         const double sum = wave_sum_n<LW_BATCH>(fl);
         const int k = k0 + wave_sum_idx<LW_BATCH>(lane);
-        if (wave_sum_writer<LW_BATCH>(lane) && k < nlay) rdn[top ? k + 1 : nlay - 1 - k] = sum;
+        if (wave_sum_writer<LW_BATCH>(lane) && k < nlay) atomicAdd(&rdn[top ? k + 1 : nlay - 1 - k], sum);
     }
     // ---- surface
     const int sfc_lay = top ? nlay - 1 : 0;
@@ -184,10 +188,10 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
             }
         }
         const double ws = wave_sum(act ? (double)fsum : 0.0);
-        if (lane == 0) rup[lev_sfc] = ws;
+        if (lane == 0) atomicAdd(&rup[lev_sfc], ws);
         if (JAC) {
             const double wj = wave_sum(act ? (double)jsum : 0.0);
-            if (lane == 0) rjc[lev_sfc] = wj;
+            if (lane == 0) atomicAdd(&rjc[lev_sfc], wj);
         }
     }
     // ---- bottom-up sweep: upward intensity (and Jacobian)
@@ -223,30 +227,30 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
         if (JAC) {
             const double sum = wave_sum_n<2 * LW_BATCH>(fl);
             const int idx = wave_sum_idx<2 * LW_BATCH>(lane), k = k0 + (idx & (LW_BATCH - 1));
-            if (wave_sum_writer<2 * LW_BATCH>(lane) && k < nlay) (idx < LW_BATCH ? rup : rjc)[top ? nlay - 1 - k : k + 1] = sum;
+            if (wave_sum_writer<2 * LW_BATCH>(lane) && k < nlay)
+                atomicAdd(&(idx < LW_BATCH ? rup : rjc)[top ? nlay - 1 - k : k + 1], sum);
         } else {
             const double sum = wave_sum_n<LW_BATCH>(*(const float(*)[LW_BATCH])fl);
             const int k = k0 + wave_sum_idx<LW_BATCH>(lane);
-            if (wave_sum_writer<LW_BATCH>(lane) && k < nlay) rup[top ? nlay - 1 - k : k + 1] = sum;
+            if (wave_sum_writer<LW_BATCH>(lane) && k < nlay) atomicAdd(&rup[top ? nlay - 1 - k : k + 1], sum);
         }
     }
     __syncthreads();
-    // ---- epilogue: combine wave partials, write fluxes, heating rate (FP64)
-    double *r0 = red, *r1 = red + 3 * (size_t)nlev;
+    // ---- epilogue: write fluxes, heating rate (FP64)
     for (int v = g; v < nlev; v += blockDim.x) {
-        const double up = r0[v] + r1[v];
-        const double dn = r0[nlev + v] + r1[nlev + v];
+        const double up = rup[v];
+        const double dn = rdn[v];
         const size_t q = (size_t)c + (size_t)ld * v;
         A.flux_up[q] = up;
         A.flux_dn[q] = dn;
         if (A.flux_up2) A.flux_up2[q] = up;
         if (A.flux_dn2) A.flux_dn2[q] = dn;
-        if (JAC && A.flux_jac) A.flux_jac[q] = r0[2 * nlev + v] + r1[2 * nlev + v];
+        if (JAC && A.flux_jac) A.flux_jac[q] = rjc[v];
     }
     if (A.htr) {
         for (int l = g; l < nlay; l += blockDim.x) {
-            const double u0 = r0[l] + r1[l], u1 = r0[l + 1] + r1[l + 1];
-            const double d0 = r0[nlev + l] + r1[nlev + l], d1 = r0[nlev + l + 1] + r1[nlev + l + 1];
+            const double u0 = rup[l], u1 = rup[l + 1];
+            const double d0 = rdn[l], d1 = rdn[l + 1];
             const double p0 = A.p_lev[(size_t)c + (size_t)ld * l], p1 = A.p_lev[(size_t)c + (size_t)ld * (l + 1)];
             A.htr[(size_t)c + (size_t)ld * l] = (u1 - u0 - d1 + d0) * GRAV / (CP_DRY * (p1 - p0));
         }

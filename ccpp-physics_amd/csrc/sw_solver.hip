@@ -112,11 +112,15 @@ __device__ __forceinline__ TwoStream two_stream32(float mu0, float mu0_inv, cons
 // terms are recomputed in registers where they are needed.
 constexpr int SW_SEG = 8;   // 2 SW_SEG = 16 level sums per segment, reduced together
 
+// checkpoints kept in LDS: segment boundaries 1 .. nseg - 1 (the TOA and
+// surface values stay in registers)
+static __host__ __device__ inline int sw_nckpt(int nseg) { return nseg > 1 ? nseg - 1 : 0; }
+
 size_t sw_solve_smem(int nlay, int nband)
 {
     const int nseg = (nlay + SW_SEG - 1) / SW_SEG;
-    return sizeof(double) * (4 * (size_t)(nlay + 1) + 6 * (size_t)nband) + sizeof(float) * 2 * (size_t)(nseg + 1) * 128 +
-           2 * sizeof(unsigned long long);
+    return sizeof(double) * (2 * (size_t)(nlay + 1) + 6 * (size_t)nband) +
+           sizeof(float) * 2 * (size_t)sw_nckpt(nseg) * 128 + 2 * sizeof(unsigned long long);
 }
 
 template <bool CLOUDS, bool AER>
@@ -138,11 +142,15 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     const bool cloudy = CLOUDS && A.blkflag[od / A.blksz];
     const int top = A.top_at_1;
     const int lev_toa = top ? 0 : nlay;
-    double *rup = red + (size_t)wave * 2 * nlev, *rdn = rup + nlev;
-    double *rb = red + 4 * (size_t)nlev;  // [wave][3][nband]
-    float *cX = (float *)(red + 4 * (size_t)nlev + 6 * (size_t)nb);
-    float *cQ = cX + (size_t)(nseg + 1) * 128;
-    unsigned long long *sflag = (unsigned long long *)(cQ + (size_t)(nseg + 1) * 128);  // [2 waves]
+    // level sums of both wavefronts: each slot gets exactly two FP64 adds
+    // onto 0, so the total is the same in either order
+    double *rup = red, *rdn = red + nlev;
+    double *rb = red + 2 * (size_t)nlev;  // [wave][3][nband]
+    float *cX = (float *)(red + 2 * (size_t)nlev + 6 * (size_t)nb);   // checkpoint of boundary k at k - 1
+    float *cQ = cX + (size_t)sw_nckpt(nseg) * 128;
+    unsigned long long *sflag = (unsigned long long *)(cQ + (size_t)sw_nckpt(nseg) * 128);  // [2 waves]
+    for (int v = g; v < 2 * nlev; v += blockDim.x) red[v] = 0.0;
+    __syncthreads();
 
     const double mu0d = A.coszen[c];
     const float mu0 = (float)mu0d;
@@ -252,8 +260,6 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     // bounded by 1 (the upwelling a direct beam can feed), and this pass
     // needs no direct beam.  A and q are kept at segment boundaries.
     float Ab = adif, qb = adir;
-    cX[(size_t)nseg * 128 + g] = Ab;
-    cQ[(size_t)nseg * 128 + g] = qb;
     for (int s = nseg - 1; s >= 0; --s) {
         Opt2s o[SW_SEG];
         optics_seg(s, o);
@@ -267,8 +273,7 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
             const float qa = ts.Rdir + ts.Td * den * (qb * ts.T0 + Ab * ts.Tdir);
             Ab = Aa; qb = qa;
         }
-        cX[(size_t)s * 128 + g] = Ab;
-        cQ[(size_t)s * 128 + g] = qb;
+        if (s > 0) { cX[(size_t)(s - 1) * 128 + g] = Ab; cQ[(size_t)(s - 1) * 128 + g] = qb; }
     }
 
     // ---- pass 2, top-down: direct beam by products of T_noscat (as the
@@ -277,10 +282,10 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     float dnd = 0.0f;        // diffuse down at TOA
     float fdir = toa * mu0;  // direct beam at the current level
     {
-        const float up = dnd * cX[g] + cQ[g] * fdir;
+        const float up = dnd * Ab + qb * fdir;   // A, q at the TOA: the end of pass 1
         const double su = wave_sum(act ? (double)up : 0.0);
         const double sdn = wave_sum(act ? (double)(dnd + fdir) : 0.0);
-        if (lane == 0) { rup[lev_toa] = su; rdn[lev_toa] = sdn; }
+        if (lane == 0) { atomicAdd(&rup[lev_toa], su); atomicAdd(&rdn[lev_toa], sdn); }
     }
     // direct and total down flux at the surface level (TOA values if the
     // surface level is the TOA)
@@ -291,7 +296,8 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
         // direct transmittance T0, and A, q below the layer
         float al[SW_SEG], be[SW_SEG], T0[SW_SEG], Abl[SW_SEG], qbl[SW_SEG];
         optics_seg(s, o);
-        float a = cX[(size_t)(s + 1) * 128 + g], q = cQ[(size_t)(s + 1) * 128 + g];
+        float a = adif, q = adir;   // the surface below the last segment
+        if (s + 1 < nseg) { a = cX[(size_t)s * 128 + g]; q = cQ[(size_t)s * 128 + g]; }
 #pragma unroll
         for (int j = SW_SEG - 1; j >= 0; --j) {
             const int k = s * SW_SEG + j;
@@ -324,7 +330,7 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
         }
         const double sum = wave_sum_n<2 * SW_SEG>(fl);
         const int idx = wave_sum_idx<2 * SW_SEG>(lane), k = s * SW_SEG + (idx & (SW_SEG - 1));
-        if (wave_sum_writer<2 * SW_SEG>(lane) && k < nlay) (idx < SW_SEG ? rup : rdn)[lev(k + 1)] = sum;
+        if (wave_sum_writer<2 * SW_SEG>(lane) && k < nlay) atomicAdd(&(idx < SW_SEG ? rup : rdn)[lev(k + 1)], sum);
     }
     {
         // per-band surface sums for scmpsw
@@ -337,10 +343,9 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     }
     __syncthreads();
     // ---- epilogue
-    double *r0 = red, *r1 = red + 2 * (size_t)nlev;
     const bool clr_too = (A.role == 1) || !cloudy;  // fluxes equal the clear-sky ones
     for (int v = g; v < nlev; v += blockDim.x) {
-        const double up = r0[v] + r1[v], dn = r0[nlev + v] + r1[nlev + v];
+        const double up = rup[v], dn = rdn[v];
         const size_t q = (size_t)c + (size_t)ld * v;
         if (A.role == 0) { A.up_all[q] = up; A.dn_all[q] = dn; }
         if (clr_too) { A.up_clr[q] = up; A.dn_clr[q] = dn; }
@@ -355,8 +360,8 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     double *h_all = A.role == 0 ? A.htr : nullptr, *h_clr = (clr_too || A.role == 1) ? A.htr_clr : nullptr;
     if (h_all || h_clr) {
         for (int l = g; l < nlay; l += blockDim.x) {
-            const double u0 = r0[l] + r1[l], u1 = r0[l + 1] + r1[l + 1];
-            const double d0 = r0[nlev + l] + r1[nlev + l], d1 = r0[nlev + l + 1] + r1[nlev + l + 1];
+            const double u0 = rup[l], u1 = rup[l + 1];
+            const double d0 = rdn[l], d1 = rdn[l + 1];
             const double p0 = A.p_lev[(size_t)c + (size_t)ld * l], p1 = A.p_lev[(size_t)c + (size_t)ld * (l + 1)];
             const double hr = (u1 - u0 - d1 + d0) * GRAV / (CP_DRY * (p1 - p0));
             if (h_all) h_all[(size_t)c + (size_t)ld * l] = hr;
