@@ -178,8 +178,8 @@ def lut_struct(lut: syn.CloudLUT) -> Holder:
 
 
 def lw_default_ints(ncol, nlay, blksz=None, nangle=3, clrsky=False, jacobian=False,
-                    iovr="iovr_maxrand", isubc=1):
-    d = dict(doLWrad=1, doLWclrsky=int(clrsky), top_at_1=0, doGP_lwscat=0,
+                    iovr="iovr_maxrand", isubc=1, scat=False, top_at_1=False):
+    d = dict(doLWrad=1, doLWclrsky=int(clrsky), top_at_1=int(top_at_1), doGP_lwscat=int(scat),
              use_LW_jacobian=int(jacobian), doGP_sgs_cnv=0, doGP_sgs_pbl=0, nCol=ncol,
              nLay=nlay, nGases=6, rrtmgp_phys_blksz=blksz or ncol, nGauss_angles=nangle,
              iovr=IOVR[iovr], iovr_convcld=IOVR["iovr_maxrand"], isubc_lw=isubc)

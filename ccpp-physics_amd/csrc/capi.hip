@@ -254,7 +254,8 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
     if (kd->minor_lower.n > MAX_MINOR || kd->minor_upper.n > MAX_MINOR) { seterr(errmsg, errlen, "too many minor intervals"); return 1; }
     if (nLay < 2 || nLay > MAX_LAY) { seterr(errmsg, errlen, "nLay=%d outside 2..%d", nLay, MAX_LAY); return 1; }
     if (blksz < 1) { seterr(errmsg, errlen, "rrtmgp_phys_blksz must be positive"); return 1; }
-    if ((want_lw ? lw_solve_smem(nLay, kd->nband) : sw_solve_smem(nLay, kd->nband)) > 64 * 1024) {
+    if ((want_lw ? std::max(lw_solve_smem(nLay, kd->nband), lw_solve_2str_smem(nLay, kd->nband))
+                 : sw_solve_smem(nLay, kd->nband)) > 64 * 1024) {
         seterr(errmsg, errlen, "nLay=%d needs more than the solver's 64 KiB of LDS per column", nLay); return 1;
     }
     if (gas_gather_smem(want_lw, kd->nflav, kd->minor_lower.n, kd->minor_upper.n, kd->ngpt, kd->neta) > 64 * 1024) {
@@ -523,7 +524,11 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
     if (!a->doLWrad) return 0;
     hipStream_t st = (hipStream_t)stream;
     const int nC = a->nCol, nL = a->nLay;
-    if (a->doGP_lwscat) { seterr(errmsg, errlen, "doGP_lwscat (LW two-stream scattering) is not supported"); return 1; }
+    const bool scat = a->doGP_lwscat != 0;
+    if (scat && a->use_LW_jacobian) {
+        seterr(errmsg, errlen, "doGP_lwscat: rte_lw gives no surface-temperature Jacobian with two-stream optics");
+        return 1;
+    }
     if (nL != c->nlay) { seterr(errmsg, errlen, "nLay=%d differs from init (%d)", nL, c->nlay); return 1; }
     if (nC < 0) { seterr(errmsg, errlen, "nCol negative"); return 1; }
     if (nC == 0) return 0;
@@ -554,7 +559,7 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
     HIPCHK(w->w_aux.ensure(sizeof(float) * (size_t)CH * nL * G));
     HIPCHK(w->w_rec.ensure(sizeof(unsigned) * (size_t)CH * nL * gas_record_words(T)));
     HIPCHK(w->w_mask.ensure(sizeof(unsigned) * (size_t)CH * nL * MASK_WORDS));
-    HIPCHK(w->w_ctau.ensure(sizeof(float) * (size_t)CH * nL * B));
+    HIPCHK(w->w_ctau.ensure((scat ? sizeof(float3) : sizeof(float)) * (size_t)CH * nL * B));
     HIPCHK(w->w_blk.ensure((size_t)nblk + 1));
     HIPCHK(w->w_colf.ensure((size_t)nC + 1));
     if (c->mlp) {
@@ -574,7 +579,10 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
         const int nc = std::min(CH, nC - c0);
         float *tau = (float *)w->w_tau.p, *pf = (float *)w->w_aux.p;
         unsigned *mask = (unsigned *)w->w_mask.p;
-        float *ctau = (float *)w->w_ctau.p;
+        // cloud optics by band: absorption depth, or the full two-stream
+        // record when the all-sky solver scatters
+        float *ctau = scat ? nullptr : (float *)w->w_ctau.p;
+        float3 *copt = scat ? (float3 *)w->w_ctau.p : nullptr;
         if (have_clouds) {
             CloudArgs ca{};
             ca.nlay = nL; ca.ncol_ld = nC; ca.col0 = c0; ca.blksz = bs; ca.nband = B; ca.is_lw = 1;
@@ -584,7 +592,7 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
             if (a->doGP_sgs_pbl) { ca.pbl[0] = a->cld_pbl_lwp; ca.pbl[1] = a->cld_pbl_iwp; ca.pbl[2] = a->cld_pbl_reliq; ca.pbl[3] = a->cld_pbl_reice; }
             ca.rwp = a->cld_rwp; ca.swp = a->cld_swp; ca.resnow = a->cld_resnow;
             if (!ca.rwp || !ca.swp || !ca.resnow) { seterr(errmsg, errlen, "cld_rwp/cld_swp/cld_resnow required with clouds"); return 1; }
-            ca.ctau = ctau; ca.err = w->err;
+            ca.ctau = ctau; ca.copt = copt; ca.err = w->err;
             { KScope k_(c, K_CLOUD, st); launch_cloud_optics(c->L, ca, nc, st); }
             McicaArgs ma{};
             ma.nlay = nL; ma.ngpt = G; ma.ncol_ld = nC; ma.col0 = c0; ma.blksz = bs; ma.isubc = a->isubc_lw;
@@ -613,17 +621,23 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
         }
         LwSolveArgs s{};
         s.nlay = nL; s.ncol_ld = nC; s.col0 = c0; s.top_at_1 = a->top_at_1;
-        s.tau = tau; s.pfrac = pf; s.mask = mask; s.ctau = ctau; s.aer_tau = a->aerlw_tau;
+        s.tau = tau; s.pfrac = pf; s.mask = mask; s.ctau = ctau; s.copt = copt; s.aer_tau = a->aerlw_tau;
         s.t_lay = a->t_lay; s.t_lev = a->t_lev; s.tsfc = a->tsfg; s.semis = a->semis; s.p_lev = a->p_lev;
         for (int i = 0; i < MAX_ANG; ++i) { s.D[i] = D[i]; s.W[i] = W[i]; }
         s.flux_up = a->fluxlwUP_allsky; s.flux_dn = a->fluxlwDOWN_allsky;
         s.flux_up2 = a->fluxlwUP_radtime; s.flux_dn2 = a->fluxlwDOWN_radtime;
         s.flux_jac = a->use_LW_jacobian ? a->fluxlwUP_jac : nullptr;
         s.htr = a->htrlw;
-        { KScope k_(c, K_LWSOLVE, st); launch_lw_solve(T, s, nc, a->nGauss_angles, have_clouds, a->use_LW_jacobian != 0, false, st); }
+        if (scat) {
+            KScope k_(c, K_LWSOLVE, st);
+            launch_lw_solve_2str(T, s, nc, have_clouds, st);
+        } else {
+            KScope k_(c, K_LWSOLVE, st);
+            launch_lw_solve(T, s, nc, a->nGauss_angles, have_clouds, a->use_LW_jacobian != 0, false, st);
+        }
         if (a->doLWclrsky) {
             LwSolveArgs s2 = s;
-            s2.mask = nullptr; s2.ctau = nullptr;
+            s2.mask = nullptr; s2.ctau = nullptr; s2.copt = nullptr;
             s2.flux_up = a->fluxlwUP_clrsky; s2.flux_dn = a->fluxlwDOWN_clrsky;
             s2.flux_up2 = nullptr; s2.flux_dn2 = nullptr; s2.flux_jac = nullptr;
             s2.htr = a->htrlwc;

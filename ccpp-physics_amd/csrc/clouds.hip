@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs 
         const size_t o0 = (size_t)i * nb;
         if (!A.blkflag[blk]) {
             for (int b = 0; b < nb; ++b) {
-                if (A.is_lw) A.ctau[o0 + b] = 0.0f;
+                if (A.ctau) A.ctau[o0 + b] = 0.0f;
                 else A.copt[o0 + b] = make_float3(0.0f, 0.0f, 0.0f);
             }
             continue;
@@ -211,8 +211,10 @@ __global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs 
                 pr.g = asyw * frcp(1.0f + asyw);
             }
             inc2s(pr, o);
-            if (A.is_lw) A.ctau[o0 + b] = o.t * o.p;   // 1scl += 2str: absorption depth tau * (1 - ssa)
-            else A.copt[o0 + b] = make_float3(o.t, o.p, o.g);   // SW: tau, co-albedo, g in one 12-byte record
+            // LW without scattering, 1scl += 2str: absorption depth tau * (1 - ssa);
+            // SW and LW with scattering: tau, co-albedo, g in one 12-byte record
+            if (A.ctau) A.ctau[o0 + b] = o.t * o.p;
+            else A.copt[o0 + b] = make_float3(o.t, o.p, o.g);
         }
     }
     if (bad) atomicOr(A.err, 2);

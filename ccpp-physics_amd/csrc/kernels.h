@@ -35,6 +35,7 @@ struct LwSolveArgs {
     const float *pfrac;    // [d][l][g]
     const unsigned *mask;  // [d][l][MASK_WORDS] or null (clear sky)
     const float *ctau;     // [l][d][nband] cloud absorption depth (d = column of the launch) or null
+    const float3 *copt;    // [l][d][nband] cloud (tau, co-albedo, g) for the scattering solver, or null
     const double *aer_tau; // Fortran (ncol_ld, nlay, nband) or null
     const double *t_lay, *t_lev, *tsfc, *semis, *p_lev;
     float D[MAX_ANG], W[MAX_ANG];
@@ -43,6 +44,9 @@ struct LwSolveArgs {
 void launch_lw_solve(const TablesView &T, const LwSolveArgs &A, int ncols, int nang, bool clouds, bool jac,
                      bool optang, hipStream_t st);
 size_t lw_solve_smem(int nlay, int nband);
+// the two-stream solver with scattering (doGP_lwscat): all-sky fluxes, no Jacobian
+void launch_lw_solve_2str(const TablesView &T, const LwSolveArgs &A, int ncols, bool clouds, hipStream_t st);
+size_t lw_solve_2str_smem(int nlay, int nband);
 void launch_heating_rate(int ncol, int nlay, const double *up, const double *dn, const double *p_lev,
                          double *htr, hipStream_t st);
 
@@ -53,8 +57,8 @@ struct CloudArgs {
     const double *cld_frac, *lwp, *iwp, *reliq, *reice;
     const double *cnv[4], *pbl[4];  // lwp, iwp, reliq, reice (null = off)
     const double *rwp, *swp, *resnow;
-    float *ctau;              // LW: [l][d][b] absorption depth (d = column of the launch)
-    float3 *copt;             // SW: [l][d][b] (tau, co-albedo 1 - ssa, g)
+    float *ctau;              // LW without scattering: [l][d][b] absorption depth (d = column of the launch)
+    float3 *copt;             // SW, and LW with scattering: [l][d][b] (tau, co-albedo 1 - ssa, g)
     double *cldtau;           // SW: (ncol_ld, nlay) band-11 cloud tau, or null
     int cldtau_blk;           // the one scheme block whose columns write cldtau
     int *err;

@@ -9,7 +9,9 @@
 !    rte-rrtmgp errors and leaves errflg = 0, radiation_tools.F90:89-98);
 !  - the gas / cloud files are this package's table files (the netCDF files
 !    of rrtmgp_lw_gas_optics.F90:81-450 are replaced);
-!  - doGP_cldoptics_PADE and doGP_lwscat are rejected with an error.
+!  - doGP_cldoptics_PADE is rejected with an error, and so is doGP_lwscat
+!    together with use_LW_jacobian (rte_lw computes no Jacobian with
+!    two-stream optics).
 module rrtmgp_lw_main
   use mpi_f08,   only: MPI_Comm
   use iso_c_binding

@@ -39,6 +39,8 @@ def lib():
                                      P_D, P_D, P_D, P_D, P_D]
         L.orc_lw_solve.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, P_D, P_D, P_D,
                                    P_D, P_D, P_D, P_D, P_D, P_D, P_D, P_D]
+        L.orc_lw2s_gpt.argtypes = [C.c_int, C.c_int, P_D, P_D, P_D, P_D, C.c_double, C.c_double, P_D, P_D]
+        L.orc_lw_solve_2str.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [P_D] * 9
         L.orc_optimal_angles.argtypes = [C.c_void_p, C.c_int, C.c_int, P_D, P_D]
         L.orc_gauss.argtypes = [C.c_int, P_D, P_D]
         L.orc_heating_rate.argtypes = [C.c_int, C.c_int, P_D, P_D, P_D, P_D]
@@ -133,6 +135,29 @@ def lw_solve(kd, ncol, nlay, tau, src, emis_b, nang=3, Ds=None, top_at_1=False, 
                        _p(c(src["lev_inc"])), _p(c(src["lev_dec"])), _p(c(emis_b)),
                        _p(c(src["sfc_src"])), _p(c(src["sfc_jac"])), _p(up), _p(dn), _p(fj))
     return up, dn, fj
+
+
+def lw2s_gpt(tau, ssa, g, levs, emis, sfc, top_at_1=True):
+    """LW two-stream fluxes of one g-point column with given level sources
+    (array order); returns up, dn at the nlay+1 levels."""
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    nl = len(tau)
+    up, dn = np.zeros(nl + 1), np.zeros(nl + 1)
+    lib().orc_lw2s_gpt(nl, int(top_at_1), _p(c(tau)), _p(c(ssa)), _p(c(g)), _p(c(levs)), float(emis),
+                       float(sfc), _p(up), _p(dn))
+    return up, dn
+
+
+def lw_solve_2str(kd, ncol, nlay, tau, ssa, g, src, emis_b, top_at_1=False):
+    """lw_solver_2stream + band/broadband sums; tau/ssa/g (ncol, nlay, ngpt)."""
+    h = capi.kdist_struct(kd)
+    up = np.zeros((nlay + 1, ncol))
+    dn = np.zeros((nlay + 1, ncol))
+    c = np.ascontiguousarray
+    lib().orc_lw_solve_2str(C.byref(h.struct), ncol, nlay, int(top_at_1), _p(c(tau)), _p(c(ssa)), _p(c(g)),
+                            _p(c(src["lev_inc"])), _p(c(src["lev_dec"])), _p(c(emis_b)),
+                            _p(c(src["sfc_src"])), _p(up), _p(dn))
+    return up, dn
 
 
 def optimal_angles(kd, ncol, nlay, tau):

@@ -709,6 +709,162 @@ static inline void inc2s(double t1, double s1, double g1, double *t2, double *s2
 }
 
 /* ===================================================================== */
+/* adding (Shonk & Hogan 2008, rte's adding()), shared by the SW and the  */
+/* LW two-stream solvers.  Per array layer l: diffuse reflectance Rd,     */
+/* transmittance Td and the layer sources sup (upward, leaving its top)   */
+/* and sdn (downward, leaving its bottom); surface albedo alb_sfc and     */
+/* source src_sfc.  No diffuse flux enters at the top.  up/dn at the      */
+/* nlay+1 levels; alb/src (nlev) and den (nlay) are scratch.              */
+/* ===================================================================== */
+static void adding(int nlay, int top_at_1, double alb_sfc, double src_sfc, const double *Rd,
+                   const double *Td, const double *sup, const double *sdn, double *up, double *dn,
+                   double *alb, double *src, double *den)
+{
+    const int nlev = nlay + 1, vs = top_at_1 ? nlay : 0;
+    alb[vs] = alb_sfc;
+    src[vs] = src_sfc;
+    if (top_at_1) {
+        for (int l = nlay - 1; l >= 0; --l) {
+            den[l] = 1.0 / (1.0 - Rd[l] * alb[l + 1]);
+            alb[l] = Rd[l] + Td[l] * Td[l] * alb[l + 1] * den[l];
+            src[l] = sup[l] + Td[l] * den[l] * (src[l + 1] + alb[l + 1] * sdn[l]);
+        }
+        dn[0] = 0.0;
+        up[0] = dn[0] * alb[0] + src[0];
+        for (int v = 1; v < nlev; ++v) {
+            dn[v] = (Td[v - 1] * dn[v - 1] + Rd[v - 1] * src[v] + sdn[v - 1]) * den[v - 1];
+            up[v] = dn[v] * alb[v] + src[v];
+        }
+    } else {
+        for (int l = 0; l < nlay; ++l) {
+            den[l] = 1.0 / (1.0 - Rd[l] * alb[l]);
+            alb[l + 1] = Rd[l] + Td[l] * Td[l] * alb[l] * den[l];
+            src[l + 1] = sup[l] + Td[l] * den[l] * (src[l] + alb[l] * sdn[l]);
+        }
+        dn[nlay] = 0.0;
+        up[nlay] = dn[nlay] * alb[nlay] + src[nlay];
+        for (int v = nlay - 1; v >= 0; --v) {
+            dn[v] = (Td[v] * dn[v + 1] + Rd[v] * src[v] + sdn[v]) * den[v];
+            up[v] = dn[v] * alb[v] + src[v];
+        }
+    }
+}
+
+/* ===================================================================== */
+/* LW two-stream solver with scattering: rte_lw with two-stream optical   */
+/* properties (the doGP_lwscat branch, rrtmgp_lw_main.F90:552-575) runs   */
+/* lw_solver_2stream = lw_combine_sources + lw_two_stream (Fu et al. 1997 */
+/* coefficients, diffusivity secant 1.66) + lw_source_2str (Toon et al.  */
+/* 1989 linear-in-tau source) + adding, per g-point.                      */
+/* ===================================================================== */
+/* One g-point, level sources given: tau/ssa/g (nlay) and levs (nlev) by
+ * array index, stride 1; sfc the surface Planck source (pfrac * B(T_sfc));
+ * up/dn (nlev) out.  ws: scratch of 7 nlev doubles.  Pinned by
+ * tests/test_oracle_lw.py against the two-stream equations integrated by
+ * matrix exponentials, layer by layer, with the boundary conditions solved
+ * as one linear system (no adding), and against lw_solver_noscat at one
+ * angle (1.66, 0.5) when ssa = 0. */
+static void lw2s_core(int nlay, int top_at_1, const double *tau, const double *ssa, const double *gg,
+                      const double *levs, double emis, double sfc, double *up, double *dn, double *ws)
+{
+    const double D = 1.66;
+    const int nlev = nlay + 1;
+    double *Rd = ws, *Td = ws + nlev, *sup = ws + 2 * nlev, *sdn = ws + 3 * nlev;
+    double *alb = ws + 4 * nlev, *src = ws + 5 * nlev, *den = ws + 6 * nlev;
+    for (int l = 0; l < nlay; ++l) {
+        const double t = tau[l], w = ssa[l], g = gg[l];
+        /* lw_two_stream */
+        const double g1 = D * (1.0 - 0.5 * w * (1.0 + g));
+        const double g2 = D * 0.5 * w * (1.0 - g);
+        const double kk = sqrt(fmax((g1 - g2) * (g1 + g2), 1.0e-12));
+        const double e = exp(-t * kk), e2 = e * e;
+        const double RT = 1.0 / (kk * (1.0 + e2) + g1 * (1.0 - e2));
+        Rd[l] = RT * g2 * (1.0 - e2);
+        Td[l] = RT * 2.0 * kk * e;
+        /* lw_source_2str */
+        if (t > 1.0e-8) {
+            const double st = top_at_1 ? levs[l] : levs[l + 1];
+            const double sb = top_at_1 ? levs[l + 1] : levs[l];
+            const double Z = (sb - st) / (t * (g1 + g2));
+            const double zut = Z + st, zub = Z + sb, zdt = -Z + st, zdb = -Z + sb;
+            sup[l] = M_PI * (zut - Rd[l] * zdt - Td[l] * zub);
+            sdn[l] = M_PI * (zdb - Rd[l] * zub - Td[l] * zdt);
+        } else {
+            sup[l] = 0.0;
+            sdn[l] = 0.0;
+        }
+    }
+    adding(nlay, top_at_1, 1.0 - emis, M_PI * emis * sfc, Rd, Td, sup, sdn, up, dn, alb, src, den);
+}
+
+void orc_lw2s_gpt(int nlay, int top_at_1, const double *tau, const double *ssa, const double *g,
+                  const double *levs, double emis, double sfc, double *up, double *dn)
+{
+    double *ws = malloc(sizeof(double) * 7 * (size_t)(nlay + 1));
+    lw2s_core(nlay, top_at_1, tau, ssa, g, levs, emis, sfc, up, dn, ws);
+    free(ws);
+}
+
+/* lw_combine_sources: the level source at array level v is
+ * sqrt(lev_dec(v) * lev_inc(v-1)) between two layers, lev_dec(0) at the
+ * first level and lev_inc(nlay-1) at the last (inc/dec strided by ngpt). */
+static void lw_combine_sources(int nlay, int ngpt, const double *inc, const double *dec, double *levs)
+{
+    levs[0] = dec[0];
+    for (int v = 1; v < nlay; ++v) levs[v] = sqrt(dec[(size_t)v * ngpt] * inc[(size_t)(v - 1) * ngpt]);
+    levs[nlay] = inc[(size_t)(nlay - 1) * ngpt];
+}
+
+/* lw_solver_2stream over ncol columns with the band / broadband reduction of
+ * orc_lw_solve.  tau/ssa/g/lev_inc/lev_dec (ngpt, nlay, ncol), emis_b
+ * (nband, ncol), sfc_src (ngpt, ncol); flux_up/flux_dn (ncol, nlev). */
+void orc_lw_solve_2str(const ccpprad_kdist *k, int ncol, int nlay, int top_at_1, const double *tau,
+                       const double *ssa, const double *g, const double *lev_inc, const double *lev_dec,
+                       const double *emis_b, const double *sfc_src, double *flux_up, double *flux_dn)
+{
+    const int ng = k->ngpt, nb = k->nband, nlev = nlay + 1;
+#pragma omp parallel
+    {
+        double *gup = malloc(sizeof(double) * nlev * ng), *gdn = malloc(sizeof(double) * nlev * ng);
+        double *ws = malloc(sizeof(double) * 7 * nlev), *levs = malloc(sizeof(double) * nlev);
+        double *t1 = malloc(sizeof(double) * nlay), *s1 = malloc(sizeof(double) * nlay);
+        double *g1 = malloc(sizeof(double) * nlay);
+#pragma omp for schedule(static)
+        for (int c = 0; c < ncol; ++c) {
+            for (int b = 0; b < nb; ++b) {
+                double emis = F2(emis_b, b, c, nb);
+                for (int gp = F2(k->band_lims_gpt, 0, b, 2) - 1; gp < F2(k->band_lims_gpt, 1, b, 2); ++gp) {
+                    size_t o = gp + (size_t)ng * (size_t)nlay * c;
+                    for (int l = 0; l < nlay; ++l) {
+                        t1[l] = tau[o + (size_t)l * ng];
+                        s1[l] = ssa[o + (size_t)l * ng];
+                        g1[l] = g[o + (size_t)l * ng];
+                    }
+                    lw_combine_sources(nlay, ng, lev_inc + o, lev_dec + o, levs);
+                    lw2s_core(nlay, top_at_1, t1, s1, g1, levs, emis, sfc_src[gp + (size_t)ng * c],
+                              gup + (size_t)gp * nlev, gdn + (size_t)gp * nlev, ws);
+                }
+            }
+            for (int v = 0; v < nlev; ++v) {
+                double su_ = 0.0, sd_ = 0.0;
+                for (int b = 0; b < nb; ++b) {
+                    int g0 = F2(k->band_lims_gpt, 0, b, 2) - 1, gl = F2(k->band_lims_gpt, 1, b, 2);
+                    double bu = gup[(size_t)g0 * nlev + v], bd = gdn[(size_t)g0 * nlev + v];
+                    for (int gp = g0 + 1; gp < gl; ++gp) {
+                        bu += gup[(size_t)gp * nlev + v];
+                        bd += gdn[(size_t)gp * nlev + v];
+                    }
+                    su_ += bu; sd_ += bd;
+                }
+                F2(flux_up, c, v, ncol) = su_;
+                F2(flux_dn, c, v, ncol) = sd_;
+            }
+        }
+        free(gup); free(gdn); free(ws); free(levs); free(t1); free(s1); free(g1);
+    }
+}
+
+/* ===================================================================== */
 /* SW solver (sw_two_stream + sw_source_2str + adding), one column, one g */
 /* ===================================================================== */
 static void sw_one(int nlay, int ngpt, int top_at_1, double mu0, double toa, double adir,
@@ -755,37 +911,8 @@ static void sw_one(int nlay, int ngpt, int top_at_1, double mu0, double toa, dou
         sdn[l] = Tdir_l * fd[vin];
         fd[vout] = T0 * fd[vin];
     }
-    int vs = top_at_1 ? nlay : 0, vt = top_at_1 ? 0 : nlay;
-    double src_sfc = fd[vs] * adir;
-    /* adding */
-    alb[vs] = adif;
-    src[vs] = src_sfc;
-    if (top_at_1) {
-        for (int l = nlay - 1; l >= 0; --l) {
-            den[l] = 1.0 / (1.0 - Rd[l] * alb[l + 1]);
-            alb[l] = Rd[l] + Td[l] * Td[l] * alb[l + 1] * den[l];
-            src[l] = sup[l] + Td[l] * den[l] * (src[l + 1] + alb[l + 1] * sdn[l]);
-        }
-        dn[0] = 0.0;
-        up[0] = dn[0] * alb[0] + src[0];
-        for (int v = 1; v < nlev; ++v) {
-            dn[v] = (Td[v - 1] * dn[v - 1] + Rd[v - 1] * src[v] + sdn[v - 1]) * den[v - 1];
-            up[v] = dn[v] * alb[v] + src[v];
-        }
-    } else {
-        for (int l = 0; l < nlay; ++l) {
-            den[l] = 1.0 / (1.0 - Rd[l] * alb[l]);
-            alb[l + 1] = Rd[l] + Td[l] * Td[l] * alb[l] * den[l];
-            src[l + 1] = sup[l] + Td[l] * den[l] * (src[l] + alb[l] * sdn[l]);
-        }
-        dn[nlay] = 0.0;
-        up[nlay] = dn[nlay] * alb[nlay] + src[nlay];
-        for (int v = nlay - 1; v >= 0; --v) {
-            dn[v] = (Td[v] * dn[v + 1] + Rd[v] * src[v] + sdn[v]) * den[v];
-            up[v] = dn[v] * alb[v] + src[v];
-        }
-    }
-    (void)vt;
+    int vs = top_at_1 ? nlay : 0;
+    adding(nlay, top_at_1, adif, fd[vs] * adir, Rd, Td, sup, sdn, up, dn, alb, src, den);
     for (int v = 0; v < nlev; ++v) dn[v] += fd[v];
 }
 
@@ -858,7 +985,10 @@ int orc_lw_main_run(const ccpprad_kdist *k, const ccpprad_cloud_lut *lut, const 
     if (!a->doLWrad) return 0;
     const int nC = a->nCol, nL = a->nLay, nV = nL + 1, bs = a->rrtmgp_phys_blksz;
     const int ng = k->ngpt, nb = k->nband;
-    if (a->doGP_lwscat) { snprintf(errmsg, errlen, "doGP_lwscat (LW 2-stream) not supported"); return 1; }
+    if (a->doGP_lwscat && a->use_LW_jacobian) {
+        snprintf(errmsg, errlen, "doGP_lwscat: rte_lw gives no surface-temperature Jacobian with two-stream optics");
+        return 1;
+    }
     if (bs <= 0 || nC % bs || bs > 4096) { snprintf(errmsg, errlen, "nCol must be a multiple of rrtmgp_phys_blksz (<= 4096)"); return 1; }
     const double *vm[6] = {a->vmr_h2o, a->vmr_co2, a->vmr_o3, a->vmr_n2o, a->vmr_ch4, a->vmr_o2};
     size_t cl = (size_t)bs * nL, cg = cl * ng;
@@ -1002,6 +1132,38 @@ int orc_lw_main_run(const ccpprad_kdist *k, const ccpprad_cloud_lut *lut, const 
                     F2(a->fluxlwUP_clrsky, c0 + i, v, nC) = 0.0;
                     F2(a->fluxlwDOWN_clrsky, c0 + i, v, nC) = 0.0;
                 }
+        }
+        if (a->doGP_lwscat) {
+            /* all sky with scattering: the clear sky (1scl) incremented into
+             * the McICA-sampled clouds (2str), increment_2stream_by_1scalar:
+             * tau = tau_c + tau, ssa = tau_c ssa_c / tau (0 unless tau > eps),
+             * g = g_c; then lw_solver_2stream */
+            const double eps = 3.0 * DBL_MIN;
+            double *ssa_all = lay, *g_all = pf;   /* pfrac and the layer source are no longer needed */
+            for (int i = 0; i < bs; ++i)
+                for (int l = 0; l < nL; ++l)
+                    for (int b = 0; b < nb; ++b) {
+                        size_t q = i + (size_t)bs * (l + (size_t)nL * b);
+                        for (int gp = F2(k->band_lims_gpt, 0, b, 2) - 1; gp < F2(k->band_lims_gpt, 1, b, 2); ++gp) {
+                            size_t o = gp + (size_t)ng * (l + (size_t)nL * i);
+                            int m = mask[l + (size_t)nL * gp + (size_t)nL * ng * i];
+                            double t1 = m ? ctau[q] : 0.0, s1 = m ? cssa[q] : 0.0, g1 = m ? cg_[q] : 0.0;
+                            double t12 = t1 + tau[o];
+                            ssa_all[o] = t12 > eps ? t1 * s1 / t12 : 0.0;
+                            g_all[o] = g1;
+                            tau_all[o] = t12;
+                        }
+                    }
+            orc_lw_solve_2str(k, bs, nL, a->top_at_1, tau_all, ssa_all, g_all, inc, dec, emis, ss, fu, fd);
+            for (int v = 0; v < nV; ++v)
+                for (int i = 0; i < bs; ++i) {
+                    int c = c0 + i;
+                    F2(a->fluxlwUP_allsky, c, v, nC) = F2(fu, i, v, bs);
+                    F2(a->fluxlwDOWN_allsky, c, v, nC) = F2(fd, i, v, bs);
+                    if (a->fluxlwUP_radtime) F2(a->fluxlwUP_radtime, c, v, nC) = F2(fu, i, v, bs);
+                    if (a->fluxlwDOWN_radtime) F2(a->fluxlwDOWN_radtime, c, v, nC) = F2(fd, i, v, bs);
+                }
+            continue;
         }
         /* all sky: clouds (by g-point via McICA) incremented into clear sky
          * as absorption only: tau += tau_c * (1 - ssa_c) */

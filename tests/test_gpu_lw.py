@@ -40,11 +40,41 @@ def test_lw_gas_optics_parity(kd_lw, lut_lw):
     assert np.abs(pf_g - pf_o).max() < 2e-6
 
 
-def _run_both(kd, lut, ncol, nlay, clouds, nangle, clrsky, jac, blksz=None, iovr="iovr_maxrand"):
+def _aerosols(ncol, nlay, nband, seed, sw=False):
+    """Aerosol optics by band, Fortran (ncol, nlay, nband): tau up to ~0.1 per
+    layer, ssa 0.8-1, g 0.5-0.8 (LW uses tau only)."""
+    rng = np.random.default_rng(seed)
+    pre = "aersw_" if sw else "aerlw_"
+    f = lambda a: np.ascontiguousarray(a)
+    return {pre + "tau": f(10 ** rng.uniform(-5, -1, (nband, nlay, ncol))),
+            pre + "ssa": f(rng.uniform(0.8, 1.0, (nband, nlay, ncol))),
+            pre + "g": f(rng.uniform(0.5, 0.8, (nband, nlay, ncol)))}
+
+
+def _flip(arr, nlay):
+    """Reverse the vertical axis of every (nlay or nlay+1, ...) field: the
+    same atmosphere stored top first (top_at_1 true)."""
+    out = {}
+    for k, v in arr.items():
+        if v.ndim >= 2 and v.shape[0] in (nlay, nlay + 1):
+            v = np.ascontiguousarray(v[::-1])
+        elif v.ndim == 3 and v.shape[1] == nlay:
+            v = np.ascontiguousarray(v[:, ::-1])
+        out[k] = v
+    return out
+
+
+def _run_both(kd, lut, ncol, nlay, clouds, nangle, clrsky, jac, blksz=None, iovr="iovr_maxrand", scat=False,
+              aer=False, top_at_1=False):
     from ccpprad import lib, device, capi
     from oracle import oracle as O
     pr, arr = _setup(kd, lut, ncol, nlay, clouds)
-    ints = capi.lw_default_ints(ncol, nlay, blksz=blksz, nangle=nangle, clrsky=clrsky, jacobian=jac, iovr=iovr)
+    if aer:
+        arr.update(_aerosols(ncol, nlay, kd.nband, seed=11))
+    if top_at_1:
+        arr = _flip(arr, nlay)
+    ints = capi.lw_default_ints(ncol, nlay, blksz=blksz, nangle=nangle, clrsky=clrsky, jacobian=jac, iovr=iovr,
+                                scat=scat, top_at_1=top_at_1)
     ref = dict(arr)
     ref.update(O.lw_outputs(ncol, nlay))
     O.lw_main_run(kd, lut, ints, ref)
@@ -106,3 +136,51 @@ def test_lw_host_entry_matches_device_entry(kd_lw, lut_lw):
     out = device.to_host(dev)
     for n in ("fluxlwUP_allsky", "fluxlwDOWN_allsky", "htrlw"):
         assert np.array_equal(out[n], host[n]), n
+
+
+def test_lw_aerosols(kd_lw, lut_lw):
+    """Aerosol depths by band added to the clear and all sky (1scl += 1scl)."""
+    ref, out = _run_both(kd_lw, lut_lw, 128, 127, clouds=True, nangle=3, clrsky=True, jac=False, blksz=64,
+                         aer=True)
+    _check(ref, out, ["fluxlwUP_allsky", "fluxlwDOWN_allsky", "fluxlwUP_clrsky", "fluxlwDOWN_clrsky"],
+           ("htrlw", "htrlwc"))
+
+
+def test_lw_top_at_1(kd_lw, lut_lw):
+    """The same profiles stored top first (top_at_1 true), all sky."""
+    ref, out = _run_both(kd_lw, lut_lw, 128, 127, clouds=True, nangle=3, clrsky=True, jac=True, blksz=64,
+                         top_at_1=True)
+    _check(ref, out, ["fluxlwUP_allsky", "fluxlwDOWN_allsky", "fluxlwUP_clrsky", "fluxlwDOWN_clrsky",
+                      "fluxlwUP_jac"], ("htrlw", "htrlwc"))
+
+
+def test_lw_scattering_all_sky(kd_lw, lut_lw):
+    """doGP_lwscat: all-sky fluxes from the two-stream solver with cloud
+    scattering (the clear sky still from the no-scattering solver), 127
+    layers, McICA max-random, with aerosols."""
+    ref, out = _run_both(kd_lw, lut_lw, 256, 127, clouds=True, nangle=3, clrsky=True, jac=False, blksz=64,
+                         scat=True, aer=True)
+    _check(ref, out, ["fluxlwUP_allsky", "fluxlwDOWN_allsky", "fluxlwUP_clrsky", "fluxlwDOWN_clrsky",
+                      "fluxlwUP_radtime", "fluxlwDOWN_radtime"], ("htrlw", "htrlwc"))
+
+
+def test_lw_scattering_top_at_1_and_clear(kd_lw, lut_lw):
+    """doGP_lwscat with the profiles stored top first, and a clear column set
+    (no clouds: the two-stream solver with ssa = 0)."""
+    ref, out = _run_both(kd_lw, lut_lw, 128, 127, clouds=True, nangle=1, clrsky=False, jac=False, blksz=32,
+                         scat=True, top_at_1=True)
+    _check(ref, out, ["fluxlwUP_allsky", "fluxlwDOWN_allsky"])
+    ref, out = _run_both(kd_lw, lut_lw, 64, 64, clouds=False, nangle=1, clrsky=False, jac=False, scat=True)
+    _check(ref, out, ["fluxlwUP_allsky", "fluxlwDOWN_allsky"])
+
+
+def test_lw_scattering_rejects_jacobian(kd_lw, lut_lw):
+    from ccpprad import lib, device, capi
+    ncol, nlay = 32, 32
+    pr, arr = _setup(kd_lw, lut_lw, ncol, nlay, False)
+    ints = capi.lw_default_ints(ncol, nlay, jacobian=True, scat=True)
+    s = lib.LWScheme(kd_lw, lut_lw, nlay, blksz=ncol)
+    dev = device.to_device(arr)
+    dev.update(device.lw_outputs(ncol, nlay))
+    with pytest.raises(capi.CcppradError, match="Jacobian"):
+        s.run_dev(ints, dev)

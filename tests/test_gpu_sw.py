@@ -11,11 +11,19 @@ FLUX_TOL = 0.05
 HR_TOL = 0.01
 
 
-def _run_both(kd, lut, ncol, nlay, clouds, clrsky, blksz=None, nday=None, iovr="iovr_maxrand", seed=20261203):
+def _run_both(kd, lut, ncol, nlay, clouds, clrsky, blksz=None, nday=None, iovr="iovr_maxrand", seed=20261203,
+              aer=False):
     from ccpprad import synthetic as syn, capi, lib, device
     from oracle import oracle as O
     pr = syn.make_profiles(ncol, nlay, seed=seed, clouds=clouds, kd=kd)
     arr = capi.profile_fields(pr)
+    if aer:
+        # aerosol optics by band, Fortran (ncol, nlay, nband)
+        rng = np.random.default_rng(seed + 9)
+        shp = (kd.nband, nlay, ncol)
+        arr["aersw_tau"] = 10 ** rng.uniform(-5, -1, shp)
+        arr["aersw_ssa"] = rng.uniform(0.8, 1.0, shp)
+        arr["aersw_g"] = rng.uniform(0.5, 0.8, shp)
     nday = ncol if nday is None else nday
     rng = np.random.default_rng(seed + 5)
     idx = np.sort(rng.choice(ncol, nday, replace=False)) + 1 if nday < ncol else np.arange(1, ncol + 1)
@@ -79,4 +87,11 @@ def test_sw_all_sky_no_clrsky_partial_day(kd_sw, lut_sw):
 
 def test_sw_clear_sky(kd_sw, lut_sw):
     ref, out, cols = _run_both(kd_sw, lut_sw, 128, 64, clouds=False, clrsky=True)
+    _check(ref, out, cols)
+
+
+def test_sw_aerosols(kd_sw, lut_sw):
+    """Aerosol tau, ssa, g by band incremented into the clear and all sky
+    (increment_2stream_by_2stream)."""
+    ref, out, cols = _run_both(kd_sw, lut_sw, 128, 127, clouds=True, clrsky=True, blksz=64, aer=True)
     _check(ref, out, cols)
