@@ -55,6 +55,7 @@ def parse():
     p.add_argument("--mlp-weights", default=None,
                    help="trained emulator weights (.npz); default: the packaged ones")
     p.add_argument("--no-nn", action="store_true", help="skip the learned LW gas-optics leg")
+    p.add_argument("--no-lwscat", action="store_true", help="skip the LW cloud-scattering (doGP_lwscat) leg")
     p.add_argument("--copy-probe", action="store_true",
                    help="after the bench, run one 1 GiB device copy: the reference dispatch that scales "
                         "FETCH_SIZE / WRITE_SIZE counter readings to bytes")
@@ -241,6 +242,43 @@ def main():
                   "weights": os.path.relpath(wpath, REPO)}
             emu.close()
 
+    # ---- LW with cloud scattering (doGP_lwscat): the LW scheme alone with the
+    # two-stream adding solver in place of the no-scattering one
+    lws = None
+    lws_out = None
+    if not a.no_lwscat:
+        lw_s = lib.LWScheme(kd_lw, lut_lw, NLAY, blksz=blksz)
+        lws_out = device.lw_outputs(ncol, NLAY)
+        s_args = dict(dev)
+        s_args.update(lws_out)
+        s_ints = capi.lw_default_ints(ncol, NLAY, blksz=blksz, nangle=3, scat=True)
+        lw_s.run_dev(s_ints, s_args, stream=sp, validate=True)
+
+        def lw_alone(s, ints, args):
+            for _ in range(a.warmup):
+                s.run_dev(ints, args, stream=sp, validate=False)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(a.steps):
+                s.run_dev(ints, args, stream=sp, validate=False)
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) * 1e3 / a.steps
+
+        ms_scat, ms_noscat = lw_alone(lw_s, s_ints, s_args), lw_alone(lw, lw_ints, lw_args)
+        L.ccpprad_timing_enable(lw_s.ctx, 1)
+        lw_s.run_dev(s_ints, s_args, stream=sp, validate=False)
+        skt = {}
+        for kid, nm in {6: "gas_interp", 0: "gas_optics", 1: "lw_solve_2str", 3: "cloud_optics", 4: "mcica"}.items():
+            tot, cnt = C.c_double(), C.c_int()
+            L.ccpprad_timing_query(lw_s.ctx, kid, C.byref(tot), C.byref(cnt))
+            if cnt.value:
+                skt[nm] = tot.value
+        L.ccpprad_timing_enable(lw_s.ctx, 0)
+        lws = {"lw_only_columns_per_s": ncol / (ms_scat / 1e3), "ms_per_step": ms_scat,
+               "lw_only_noscat_columns_per_s": ncol / (ms_noscat / 1e3),
+               "kernel_ms_per_step": skt,
+               "solver": "two-stream adding with cloud scattering (all sky), no-scattering clear sky off"}
+
     # ---- CPU baseline: FP64 oracle on a bounded sample, and GPU error on it
     cpu = None
     acc = None
@@ -280,6 +318,20 @@ def main():
                 "lw_flux_maxabs_Wm2": max(e(gl["u"], lw_ref["fluxlwUP_allsky"])[0], e(gl["d"], lw_ref["fluxlwDOWN_allsky"])[0]),
                 "lw_flux_rmse_Wm2": max(e(gl["u"], lw_ref["fluxlwUP_allsky"])[1], e(gl["d"], lw_ref["fluxlwDOWN_allsky"])[1]),
                 "lw_hr_maxabs_Kday": hu[0], "lw_hr_rmse_Kday": hu[1], "columns": ns}
+        if lws_out is not None:
+            # the scattering path on the first 4,096 sample columns
+            n4 = min(4096, ns)
+            s4 = pr.subset(np.arange(n4))
+            sa4 = capi.profile_fields(s4)
+            sref = dict(sa4)
+            sref.update(O.lw_outputs(n4, NLAY))
+            O.lw_main_run(kd_lw, lut_lw, capi.lw_default_ints(n4, NLAY, blksz=blksz, nangle=3, scat=True), sref)
+            gs = {k: v[..., :n4].cpu().numpy() for k, v in
+                  {"u": lws_out["fluxlwUP_allsky"], "d": lws_out["fluxlwDOWN_allsky"], "h": lws_out["htrlw"]}.items()}
+            hs = e(gs["h"] * 86400, sref["htrlw"] * 86400)
+            lws["accuracy_vs_fp64_oracle"] = {
+                "lw_flux_maxabs_Wm2": max(e(gs["u"], sref["fluxlwUP_allsky"])[0], e(gs["d"], sref["fluxlwDOWN_allsky"])[0]),
+                "lw_hr_maxabs_Kday": hs[0], "lw_hr_rmse_Kday": hs[1], "columns": n4}
         acc = {
             "lw_flux_maxabs_Wm2": max(e(g["lwu"], lw_ref["fluxlwUP_allsky"])[0], e(g["lwd"], lw_ref["fluxlwDOWN_allsky"])[0]),
             "sw_flux_maxabs_Wm2": max(e(g["swu"], sw_ref["fluxswUP_allsky"])[0], e(g["swd"], sw_ref["fluxswDOWN_allsky"])[0]),
@@ -298,6 +350,7 @@ def main():
                                f"FP32 optics/transport, FP64 reductions and heating rate)",
                    "ncol": ncol, "nlay": NLAY, "path": "table", "blksz": blksz},
         "roofline": roof, "cpu_baseline": cpu, "accuracy_vs_fp64_oracle": acc, "nn": nn,
+        "lw_scattering": lws,
     }
     print(json.dumps(line), flush=True)
     if a.copy_probe:
