@@ -65,13 +65,16 @@ __device__ __forceinline__ float exp_diff_ratio(float a, float b, float d, float
 //    Rdir = w RT / (1+k mu) [a2 (2G - E2 - 2 T_R) + k g3 (E2 + 2 T_R)]
 //    Tdir = -w RT / (1+k mu) [(a1 - k g4) E2 (1 - F) - 2 k (g4 + mu a1) T_T].
 // gamma1 - gamma2 = 2 (1 - w) comes from the co-albedo p.
-__device__ __forceinline__ TwoStream two_stream32(float mu0, float mu0_inv, const Opt2s &o)
+__device__ __forceinline__ TwoStream two_stream32(float mu0, float mu0_inv, float c3, const Opt2s &o)
 {
     const float tau = o.t, w = o.w, p = o.p, g = o.g;
-    const float g1 = (8.0f - w * (5.0f + 3.0f * g)) * 0.25f;
-    const float g2 = 3.0f * (w * (1.0f - g)) * 0.25f;
-    const float g3 = (2.0f - 3.0f * mu0 * g) * 0.25f;
-    const float g4 = 1.0f - g3;
+    // (8 - w (5 + 3 g)) / 4, 3 w (1 - g) / 4, (2 - 3 mu0 g) / 4 and 1 - g3,
+    // each in one or two FMAs (c3 = 3 mu0 / 4 is the column's)
+    const float g1 = fmaf(-w, fmaf(0.75f, g, 1.25f), 2.0f);
+    const float w34 = 0.75f * w;
+    const float g2 = fmaf(-w34, g, w34);
+    const float g3 = fmaf(-c3, g, 0.5f);
+    const float g4 = fmaf(c3, g, 0.5f);
     const float a1 = g1 * g4 + g2 * g3;
     const float a2 = g1 * g3 + g2 * g4;
     const float k = __builtin_amdgcn_sqrtf(fmaxf(2.0f * p * (g1 + g2), 1.0e-12f));
@@ -98,8 +101,9 @@ __device__ __forceinline__ TwoStream two_stream32(float mu0, float mu0_inv, cons
         Tdir = -pre * ((a1 - k * g4) * E2 * (1.0f - F) - 2.0f * k * (g4 + mu0 * a1) * TT);
     }
     TwoStream r;
-    r.Rdir = fmaxf(0.0f, fminf(Rdir, F));
-    r.Tdir = fmaxf(0.0f, fminf(Tdir, F - r.Rdir));
+    // max(0, min(x, hi)) for hi >= 0, one v_med3 each
+    r.Rdir = __builtin_amdgcn_fmed3f(Rdir, 0.0f, F);
+    r.Tdir = __builtin_amdgcn_fmed3f(Tdir, 0.0f, F - r.Rdir);
     r.Rd = RT * g2 * E2;
     r.Td = RT * 2.0f * k * e;
     r.T0 = T0;
@@ -155,6 +159,7 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     const double mu0d = A.coszen[c];
     const float mu0 = (float)mu0d;
     const float mu0_inv = (float)(1.0 / mu0d);
+    const float c3 = 0.75f * mu0;
     const int cat = A.alb_cat[b];
     float adir = 0.0f, adif = 0.0f;
     if (cat == 0) { adir = (float)A.alb_nir_dir[c]; adif = (float)A.alb_nir_dif[c]; }
@@ -242,8 +247,9 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
                 te = tc + te; ts += sc; tab += tc * pc; tsg += sc * gc;
             }
             o[j].t = te;
-            o[j].w = ts * frcp(fmaxf(EPS3, te));
-            o[j].p = te > EPS3 ? tab * frcp(te) : 1.0f;
+            const float rte = frcp(fmaxf(EPS3, te));
+            o[j].w = ts * rte;
+            o[j].p = te > EPS3 ? tab * rte : 1.0f;
             o[j].g = tsg * frcp(fmaxf(EPS3, ts));
         }
     };
@@ -267,7 +273,7 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
         for (int j = SW_SEG - 1; j >= 0; --j) {
             const int k = s * SW_SEG + j;
             if (k >= nlay) continue;
-            const TwoStream ts = two_stream32(mu0, mu0_inv, o[j]);
+            const TwoStream ts = two_stream32(mu0, mu0_inv, c3, o[j]);
             const float den = frcp(1.0f - ts.Rd * Ab);
             const float Aa = ts.Rd + ts.Td * ts.Td * Ab * den;
             const float qa = ts.Rdir + ts.Td * den * (qb * ts.T0 + Ab * ts.Tdir);
@@ -302,7 +308,7 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
         for (int j = SW_SEG - 1; j >= 0; --j) {
             const int k = s * SW_SEG + j;
             if (k >= nlay) continue;
-            const TwoStream ts = two_stream32(mu0, mu0_inv, o[j]);
+            const TwoStream ts = two_stream32(mu0, mu0_inv, c3, o[j]);
             const float den = frcp(1.0f - ts.Rd * a);
             // dn_dif below = (Td dn_dif + Rd Q + Tdir F_dir) den with Q = q T0 F_dir
             al[j] = ts.Td * den;
