@@ -66,6 +66,12 @@ __device__ __forceinline__ double pick(const double *v, int n, int i)
     return v[(i >= 1 && i < n) ? i : 0];
 }
 
+// Reads of the read-only int tables at wavefront-uniform addresses, through
+// the scalar cache (constant address space): a vector load of one address
+// still returns 4 bytes for each of the 64 lanes through the texture path.
+typedef const __attribute__((address_space(4))) int cint;
+__device__ __forceinline__ int sld(const int *p, int i) { return ((cint *)p)[i]; }
+
 // One wavefront of 64 (column, layer) rows: each lane builds its record in
 // LDS, then the wavefront writes the 64 consecutive records as one
 // contiguous block with 16-byte stores.
@@ -147,7 +153,7 @@ __device__ __forceinline__ void interp_row(const TablesView &T, const RecGeom &G
     r[2] = __float_as_uint((float)(col[1] + col_dry));
     unsigned jeword = 0;
     for (int f = 0; f < T.nflav; ++f) {
-        const int g1 = T.flavor[2 * f], g2 = T.flavor[2 * f + 1];
+        const int g1 = sld(T.flavor, 2 * f), g2 = sld(T.flavor, 2 * f + 1);
         const double c1 = pick(col, ng1, g1), c2 = pick(col, ng1, g2);
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
@@ -174,15 +180,23 @@ __device__ __forceinline__ void interp_row(const TablesView &T, const RecGeom &G
     const double vf = 1.0 / col_dry;
     const double df = 1.0 / (1.0 + col[1] * vf);
     for (int m = 0; m < G.nmn; ++m) {
+        // the interval's parameters in both regimes (uniform, scalar loads),
+        // then the lane's regime
+        int pr[2][4];
+#pragma unroll
+        for (int rg = 0; rg < 2; ++rg)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) pr[rg][k] = m < T.nminor[rg] ? sld(T.minor_par[rg], 5 * m + k) : 0;
+        const int p0 = reg ? pr[1][0] : pr[0][0], p1 = reg ? pr[1][1] : pr[0][1];
+        const int p2 = reg ? pr[1][2] : pr[0][2], p3 = reg ? pr[1][3] : pr[0][3];
         double sc = 0.0;
         if (m < T.nminor[reg]) {
-            const int *mp = T.minor_par[reg] + 5 * m;
-            sc = pick(col, ng1, mp[0]);
-            if (mp[1]) {
+            sc = pick(col, ng1, p0);
+            if (p1) {
                 sc *= dens;
-                if (mp[2] > 0) {
-                    const double cs = pick(col, ng1, mp[2]);
-                    if (mp[3]) sc *= (1.0 - cs * vf * df);
+                if (p2 > 0) {
+                    const double cs = pick(col, ng1, p2);
+                    if (p3) sc *= (1.0 - cs * vf * df);
                     else sc *= cs * vf * df;
                 }
             }
