@@ -22,7 +22,8 @@ _lib = None
 EXPORTS = ["ccpprad_lw_init", "ccpprad_sw_init", "ccpprad_lw_init_files", "ccpprad_sw_init_files",
            "ccpprad_lw_run", "ccpprad_sw_run", "ccpprad_lw_run_dev", "ccpprad_sw_run_dev",
            "ccpprad_finalize", "ccpprad_heating_rate_dev", "ccpprad_lw_gas_optics_dev",
-           "ccpprad_sw_gas_optics_dev", "ccpprad_ngpt", "ccpprad_nband", "ccpprad_version"]
+           "ccpprad_sw_gas_optics_dev", "ccpprad_ngpt", "ccpprad_nband", "ccpprad_version",
+           "ccpprad_prepare_inputs_dev"]
 
 
 def load():
@@ -49,6 +50,7 @@ def load():
     L.ccpprad_heating_rate_dev.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_char_p, C.c_int]
     L.ccpprad_lw_gas_optics_dev.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp, C.c_char_p, C.c_int]
     L.ccpprad_sw_gas_optics_dev.argtypes = L.ccpprad_lw_gas_optics_dev.argtypes
+    L.ccpprad_prepare_inputs_dev.argtypes = [vp, C.c_int, C.c_int] + [vp] * 14 + [vp, C.c_char_p, C.c_int]
     L.ccpprad_ngpt.argtypes = [vp]
     L.ccpprad_nband.argtypes = [vp]
     L.ccpprad_version.restype = C.c_char_p
@@ -125,6 +127,19 @@ class LWScheme(_Scheme):
         h = capi.build_args(capi.LWArgs, ints, arrays)
         buf, n = capi.errbuf()
         rc = self.L.ccpprad_lw_run(self.ctx, C.addressof(h.struct), buf, n)
+        capi.check(rc, buf)
+
+    def prepare_inputs_dev(self, ncol, nlev, prsi, prsl, tgrs, q_h2o, q_o3, tsfc, out, top_at_1=None, stream=None):
+        """GFS_rrtmgp_pre's input preparation on the device; out holds the
+        device arrays p_lay, p_lev, t_lay, t_lev, vmr_h2o and optionally q_lay,
+        vmr_o3 (nlev = layers)."""
+        L = load()
+        buf, n = capi.errbuf()
+        rc = L.ccpprad_prepare_inputs_dev(self.ctx, ncol, nlev, _dp(prsi), _dp(prsl), _dp(tgrs), _dp(q_h2o),
+                                          _dp(q_o3), _dp(tsfc), _dp(out["p_lay"]), _dp(out["p_lev"]),
+                                          _dp(out["t_lay"]), _dp(out["t_lev"]), _dp(out.get("q_lay")),
+                                          _dp(out["vmr_h2o"]), _dp(out.get("vmr_o3")), _dp(top_at_1), stream,
+                                          buf, n)
         capi.check(rc, buf)
 
     def gas_optics_dev(self, ncol, nlay, p_lay, p_lev, t_lay, vmr, tau, pfrac, stream=None):

@@ -814,6 +814,32 @@ extern "C" int ccpprad_heating_rate_dev(int nCol, int nLay, const double *flux_u
     return 0;
 }
 
+extern "C" int ccpprad_prepare_inputs_dev(const ccpprad_ctx *c, int nCol, int nLev, const double *prsi,
+                                          const double *prsl, const double *tgrs, const double *qgrs_h2o,
+                                          const double *qgrs_o3, const double *tsfc, double *p_lay,
+                                          double *p_lev, double *t_lay, double *t_lev, double *q_lay,
+                                          double *vmr_h2o, double *vmr_o3, int *top_at_1, void *stream,
+                                          char *errmsg, int errlen)
+{
+    if (errmsg && errlen > 0) errmsg[0] = 0;
+    if (!c || !c->is_lw) { seterr(errmsg, errlen, "the table range comes from a LW context"); return 1; }
+    if (nCol < 0 || nLev < 2) { seterr(errmsg, errlen, "bad dimensions"); return 1; }
+    if (!prsi || !prsl || !tgrs || !qgrs_h2o || !tsfc || !p_lay || !p_lev || !t_lay || !t_lev || !vmr_h2o ||
+        (qgrs_o3 && !vmr_o3)) {
+        seterr(errmsg, errlen, "NULL array"); return 1;
+    }
+    PrepArgs A{};
+    A.ncol = nCol; A.nlev = nLev;
+    A.prsi = prsi; A.prsl = prsl; A.tgrs = tgrs; A.qh2o = qgrs_h2o; A.qo3 = qgrs_o3; A.tsfc = tsfc;
+    A.p_lay = p_lay; A.p_lev = p_lev; A.t_lay = t_lay; A.t_lev = t_lev; A.q_lay = q_lay;
+    A.vmr_h2o = vmr_h2o; A.vmr_o3 = vmr_o3; A.top_at_1 = top_at_1;
+    A.press_min = c->T.press_min; A.press_max = c->T.press_max;
+    A.temp_min = c->T.temp_min; A.temp_max = c->T.temp_max;
+    launch_prepare(A, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 static int gas_optics_entry(ccpprad_ctx *c, int want_lw, int nCol, int nLay, const double *p_lay,
                             const double *p_lev, const double *t_lay, const double *vmr, float *tau, float *aux,
                             void *stream, char *errmsg, int errlen)

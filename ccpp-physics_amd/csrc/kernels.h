@@ -50,6 +50,16 @@ size_t lw_solve_2str_smem(int nlay, int nband);
 void launch_heating_rate(int ncol, int nlay, const double *up, const double *dn, const double *p_lev,
                          double *htr, hipStream_t st);
 
+// GFS_rrtmgp_pre's preparation of the radiation inputs (prep.hip)
+struct PrepArgs {
+    int ncol, nlev;                                   // nlev = layers
+    const double *prsi, *prsl, *tgrs, *qh2o, *qo3, *tsfc;
+    double *p_lay, *p_lev, *t_lay, *t_lev, *q_lay, *vmr_h2o, *vmr_o3;   // q_lay, qo3/vmr_o3 optional
+    int *top_at_1;                                    // device int, optional
+    double press_min, press_max, temp_min, temp_max;  // the k-distribution's range
+};
+void launch_prepare(const PrepArgs &A, hipStream_t st);
+
 struct CloudArgs {
     int nlay, ncol_ld, col0, blksz, nband, is_lw;
     const int *col_idx;            // 1-based gather map (SW daylit) or null

@@ -8,6 +8,8 @@
  *        (physics/Radiation/RRTMGP/rrtmgp_sw_main.F90:34-79, :87-683)
  *   ccpprad_heating_rate               <- compute_heating_rate as called by
  *        GFS_rrtmgp_post_run (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_post.F90:159-171,251-266)
+ *   ccpprad_prepare_inputs_dev         <- the radiation-input preparation of
+ *        GFS_rrtmgp_pre_run (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_pre.F90:217-274,349-371)
  *   ccpprad_*_gas_optics_dev           <- ty_gas_optics_rrtmgp%gas_optics
  *        (call sites rrtmgp_lw_main.F90:337-345, rrtmgp_sw_main.F90:346-353)
  *
@@ -188,6 +190,23 @@ void ccpprad_finalize(ccpprad_ctx *ctx);
 int ccpprad_heating_rate_dev(int nCol, int nLay, const double *flux_up,
                              const double *flux_dn, const double *p_lev, double *htr,
                              void *stream, char *errmsg, int errlen);
+
+/* The radiation-input preparation of GFS_rrtmgp_pre_run
+ * (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_pre.F90:217-274,349-371),
+ * FP64 on the device: q_lay = max(qgrs_h2o, 1e-6); p_lev = prsi; p_lay, t_lay
+ * = prsl, tgrs bounded into the k-distribution's range of the LW context
+ * (min + epsilon, max - epsilon); t_lev by cmp_tlev (radiation_tools.F90:16-84)
+ * bounded the same way; vmr_h2o = q/(1-q) amd/amw; vmr_o3 = max(1e-10,
+ * qgrs_o3) amd/amo3 when qgrs_o3 is given (else vmr_o3 is not written: the
+ * host's ozone climatology).  nLev = layers; fields (nCol, nLev[+1]).
+ * top_at_1 (a device int, may be NULL) receives prsi(1,1) < prsi(1,nLev).
+ * q_lay may be NULL.  Device pointers; ordered on stream. */
+int ccpprad_prepare_inputs_dev(const ccpprad_ctx *ctx, int nCol, int nLev, const double *prsi,
+                               const double *prsl, const double *tgrs, const double *qgrs_h2o,
+                               const double *qgrs_o3, const double *tsfc, double *p_lay,
+                               double *p_lev, double *t_lay, double *t_lev, double *q_lay,
+                               double *vmr_h2o, double *vmr_o3, int *top_at_1, void *stream,
+                               char *errmsg, int errlen);
 
 /* ty_gas_optics_rrtmgp%gas_optics, LW (rrtmgp_lw_main.F90:337-345).  Device
  * pointers.  vmr is (nCol, nLay, ngas) in the k-distribution's gas order.

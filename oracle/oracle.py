@@ -42,6 +42,8 @@ def lib():
         L.orc_lw2s_gpt.argtypes = [C.c_int, C.c_int, P_D, P_D, P_D, P_D, C.c_double, C.c_double, P_D, P_D]
         L.orc_lw_solve_2str.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [P_D] * 9
         L.orc_optimal_angles.argtypes = [C.c_void_p, C.c_int, C.c_int, P_D, P_D]
+        L.orc_prepare.argtypes = [C.c_int, C.c_int] + [C.c_double] * 4 + [P_D] * 13
+        L.orc_prepare.restype = C.c_int
         L.orc_gauss.argtypes = [C.c_int, P_D, P_D]
         L.orc_heating_rate.argtypes = [C.c_int, C.c_int, P_D, P_D, P_D, P_D]
         L.orc_mt_draws.argtypes = [C.c_int, C.c_int, P_D]
@@ -158,6 +160,22 @@ def lw_solve_2str(kd, ncol, nlay, tau, ssa, g, src, emis_b, top_at_1=False):
                             _p(c(src["lev_inc"])), _p(c(src["lev_dec"])), _p(c(emis_b)),
                             _p(c(src["sfc_src"])), _p(up), _p(dn))
     return up, dn
+
+
+def prepare(kd, prsi, prsl, tgrs, q_h2o, q_o3, tsfc):
+    """GFS_rrtmgp_pre's input preparation; arrays (nlev[+1], ncol) numpy =
+    Fortran (ncol, nlev[+1]).  Returns (dict of outputs, top_at_1)."""
+    c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+    nl, nc = prsl.shape
+    out = {k: np.zeros((nl, nc)) for k in ("p_lay", "t_lay", "q_lay", "vmr_h2o", "vmr_o3")}
+    out["p_lev"] = np.zeros((nl + 1, nc))
+    out["t_lev"] = np.zeros((nl + 1, nc))
+    pr = np.asarray(kd.press_ref)
+    top = lib().orc_prepare(nc, nl, float(pr.min()), float(pr.max()), float(kd.temp_ref[0]),
+                            float(kd.temp_ref[-1]), _p(c(prsi)), _p(c(prsl)), _p(c(tgrs)), _p(c(q_h2o)),
+                            _p(c(q_o3)), _p(c(tsfc)), _p(out["p_lay"]), _p(out["p_lev"]), _p(out["t_lay"]),
+                            _p(out["t_lev"]), _p(out["q_lay"]), _p(out["vmr_h2o"]), _p(out["vmr_o3"]))
+    return out, bool(top)
 
 
 def optimal_angles(kd, ncol, nlay, tau):

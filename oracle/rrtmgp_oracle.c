@@ -473,6 +473,74 @@ void orc_lw_solve(const ccpprad_kdist *k, int ncol, int nlay, int top_at_1, int 
     }
 }
 
+/* ===================================================================== */
+/* GFS_rrtmgp_pre_run's preparation of the radiation inputs               */
+/* (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_pre.F90:217-274,     */
+/* 349-371) with cmp_tlev (physics/Radiation/radiation_tools.F90:16-84).   */
+/* Fortran (ncol, nlev[+1]) arrays; nlev = layers.  qo3 may be NULL (then  */
+/* vmr_o3 is not written).  Returns top_at_1.  cmp_tlev is pinned against  */
+/* the reference's own Fortran (oracle/_ref); the rest by invariants.      */
+/* ===================================================================== */
+int orc_prepare(int ncol, int nlev, double pmin, double pmax, double tmin, double tmax,
+                const double *prsi, const double *prsl, const double *tgrs, const double *qh2o,
+                const double *qo3, const double *tsfc, double *p_lay, double *p_lev, double *t_lay,
+                double *t_lev, double *q_lay, double *vmr_h2o, double *vmr_o3)
+{
+    const double amd = 28.9644, amw = 18.0154, amo3 = 47.9982, con_epsqs = 1.0e-10;
+    const double amdw = amd / amw, amdo3 = amd / amo3, eps = DBL_EPSILON;
+    const int top_at_1 = prsi[0] < prsi[(size_t)ncol * (nlev - 1)];
+    const size_t nl = (size_t)ncol * nlev, nv = (size_t)ncol * (nlev + 1);
+    for (size_t i = 0; i < nl; ++i) {
+        q_lay[i] = qh2o[i] < 1.0e-6 ? 1.0e-6 : qh2o[i];
+        p_lay[i] = prsl[i];
+        t_lay[i] = tgrs[i];
+        if (t_lay[i] <= tmin) t_lay[i] = tmin + eps;
+        if (p_lay[i] <= pmin) p_lay[i] = pmin + eps;
+        if (t_lay[i] >= tmax) t_lay[i] = tmax - eps;
+        if (p_lay[i] >= pmax) p_lay[i] = pmax - eps;
+    }
+    for (size_t i = 0; i < nv; ++i) p_lev[i] = prsi[i];
+    /* cmp_tlev(nCol, nLev, minP, p_lay, t_lay, p_lev, tsfc, t_lev) */
+    double *a = malloc(sizeof(double) * nl), *b = malloc(sizeof(double) * nl);
+    for (int c = 0; c < ncol; ++c) {
+        for (int k = 0; k < nlev; ++k) {
+            F2(a, c, k, ncol) = log(F2(p_lay, c, k, ncol));
+            F2(b, c, k, ncol) = log(F2(p_lev, c, k, ncol));
+        }
+        if (top_at_1) {
+            F2(b, c, 0, ncol) = log(fmax(pmin, F2(p_lev, c, 0, ncol)));
+            F2(t_lev, c, 0, ncol) = F2(t_lay, c, 0, ncol);
+            for (int k = 1; k < nlev; ++k)
+                F2(t_lev, c, k, ncol) = F2(t_lay, c, k, ncol) + (F2(t_lay, c, k - 1, ncol) - F2(t_lay, c, k, ncol)) *
+                                        (F2(b, c, k, ncol) - F2(a, c, k, ncol)) /
+                                        (F2(a, c, k - 1, ncol) - F2(a, c, k, ncol));
+            F2(t_lev, c, nlev, ncol) = tsfc[c];
+        } else {
+            F2(b, c, nlev - 1, ncol) = log(fmax(pmin, F2(p_lev, c, nlev - 1, ncol)));
+            F2(t_lev, c, 0, ncol) = tsfc[c];
+            for (int k = 0; k < nlev - 1; ++k)
+                F2(t_lev, c, k + 1, ncol) = F2(t_lay, c, k, ncol) + (F2(t_lay, c, k + 1, ncol) - F2(t_lay, c, k, ncol)) *
+                                            (F2(b, c, k + 1, ncol) - F2(a, c, k, ncol)) /
+                                            (F2(a, c, k + 1, ncol) - F2(a, c, k, ncol));
+            F2(t_lev, c, nlev, ncol) = F2(t_lay, c, nlev - 1, ncol);
+        }
+    }
+    free(a);
+    free(b);
+    for (size_t i = 0; i < nv; ++i) {
+        if (t_lev[i] <= tmin) t_lev[i] = tmin + eps;
+        if (t_lev[i] >= tmax) t_lev[i] = tmax - eps;
+    }
+    for (size_t i = 0; i < nl; ++i) {
+        vmr_h2o[i] = q_lay[i] != 1.0 ? (q_lay[i] / (1.0 - q_lay[i])) * amdw : 0.0;
+        if (qo3) {
+            const double o = fmax(con_epsqs, qo3[i]);
+            vmr_o3[i] = o > 0.0 ? o * amdo3 : 0.0;
+        }
+    }
+    return top_at_1;
+}
+
 /* compute_optimal_angles: Ds(g,c) = fit1(b) * exp(-sum_l tau) + fit2(b). */
 void orc_optimal_angles(const ccpprad_kdist *k, int ncol, int nlay, const double *tau, double *Ds)
 {
