@@ -76,6 +76,45 @@ _SW_FIELDS = (
                           "fluxswDOWN_clrsky", "cldtausw", "htrsw", "htrswc")])
 
 
+_POST_FIELDS = (
+    [(n, C.c_int) for n in ("nCol", "nLev", "nDay", "iSFC", "iTOA", "doLWrad", "doSWrad", "save_diag",
+                            "naerodp", "nfxr")]
+    + [(n, C.c_double) for n in ("fhlwr", "fhswr", "raddt")]
+    + [(n, P_D) for n in (
+        "fluxlwUP_allsky", "fluxlwDOWN_allsky", "fluxlwUP_clrsky", "fluxlwDOWN_clrsky",
+        "fluxswUP_allsky", "fluxswDOWN_allsky", "fluxswUP_clrsky", "fluxswDOWN_clrsky", "scmpsw",
+        "sfc_alb_nir_dir", "sfc_alb_nir_dif", "sfc_alb_uvvis_dir", "sfc_alb_uvvis_dif",
+        "tsfa", "coszen", "coszdg", "htrlw", "p_lev", "cldsa", "cldtaulw", "cldtausw", "aerodp")]
+    + [("mtopa", P_I), ("mbota", P_I)]
+    + [(n, P_D) for n in (
+        "topflw", "sfcflw", "sfcdlw", "sfculw", "tsflw", "htrlwu", "topfsw", "sfcfsw",
+        "nirbmdi", "nirdfdi", "visbmdi", "visdfdi", "nirbmui", "nirdfui", "visbmui", "visdfui",
+        "sfcnsw", "sfcdsw", "fluxr")])
+
+
+class PostArgs(C.Structure):
+    """ccpprad_post_args (GFS_rrtmgp_post_run's bookkeeping)."""
+    _fields_ = _POST_FIELDS
+
+
+def post_struct(ints: dict, reals: dict, arrays: dict) -> Holder:
+    """Fill a PostArgs from ints, reals (fhlwr, fhswr, raddt) and arrays
+    (numpy or torch; missing names become NULL)."""
+    s = PostArgs()
+    keep = []
+    for name, typ in PostArgs._fields_:
+        if typ is C.c_int:
+            setattr(s, name, int(ints.get(name, 0)))
+        elif typ is C.c_double:
+            setattr(s, name, float(reals.get(name, 0.0)))
+        else:
+            a = arrays.get(name)
+            if a is not None:
+                keep.append(a)
+            setattr(s, name, _ptr(a, typ))
+    return Holder(s, keep)
+
+
 class LWArgs(C.Structure):
     _fields_ = _LW_FIELDS
 

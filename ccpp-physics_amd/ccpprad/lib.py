@@ -23,7 +23,7 @@ EXPORTS = ["ccpprad_lw_init", "ccpprad_sw_init", "ccpprad_lw_init_files", "ccppr
            "ccpprad_lw_run", "ccpprad_sw_run", "ccpprad_lw_run_dev", "ccpprad_sw_run_dev",
            "ccpprad_finalize", "ccpprad_heating_rate_dev", "ccpprad_lw_gas_optics_dev",
            "ccpprad_sw_gas_optics_dev", "ccpprad_ngpt", "ccpprad_nband", "ccpprad_version",
-           "ccpprad_prepare_inputs_dev"]
+           "ccpprad_prepare_inputs_dev", "ccpprad_post_dev"]
 
 
 def load():
@@ -51,6 +51,7 @@ def load():
     L.ccpprad_lw_gas_optics_dev.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp, C.c_char_p, C.c_int]
     L.ccpprad_sw_gas_optics_dev.argtypes = L.ccpprad_lw_gas_optics_dev.argtypes
     L.ccpprad_prepare_inputs_dev.argtypes = [vp, C.c_int, C.c_int] + [vp] * 14 + [vp, C.c_char_p, C.c_int]
+    L.ccpprad_post_dev.argtypes = [vp, vp, C.c_char_p, C.c_int]
     L.ccpprad_ngpt.argtypes = [vp]
     L.ccpprad_nband.argtypes = [vp]
     L.ccpprad_version.restype = C.c_char_p
@@ -175,4 +176,14 @@ def heating_rate_dev(ncol, nlay, up, dn, p_lev, htr, stream=None):
     L = load()
     buf, n = capi.errbuf()
     rc = L.ccpprad_heating_rate_dev(ncol, nlay, _dp(up), _dp(dn), _dp(p_lev), _dp(htr), stream, buf, n)
+    capi.check(rc, buf)
+
+
+def post_dev(ints, reals, arrays, stream=None):
+    """GFS_rrtmgp_post's flux bookkeeping on the device (arrays: torch device
+    tensors by ccpprad_post_args field name)."""
+    L = load()
+    h = capi.post_struct(ints, reals, arrays)
+    buf, n = capi.errbuf()
+    rc = L.ccpprad_post_dev(C.addressof(h.struct), stream, buf, n)
     capi.check(rc, buf)

@@ -11,6 +11,7 @@
 #include <vector>
 #include <algorithm>
 #include <mutex>
+#include <initializer_list>
 #include <thread>
 
 #include "../../include/ccpprad.h"
@@ -810,6 +811,47 @@ extern "C" int ccpprad_heating_rate_dev(int nCol, int nLay, const double *flux_u
     if (nCol < 0 || nLay < 1) { seterr(errmsg, errlen, "bad dimensions"); return 1; }
     if (!flux_up || !flux_dn || !p_lev || !htr) { seterr(errmsg, errlen, "NULL array"); return 1; }
     launch_heating_rate(nCol, nLay, flux_up, flux_dn, p_lev, htr, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+namespace ccpprad {
+void launch_post(const ccpprad_post_args &A, hipStream_t st);   // prep.hip
+}
+
+extern "C" int ccpprad_post_dev(const ccpprad_post_args *a, void *stream, char *errmsg, int errlen)
+{
+    if (errmsg && errlen > 0) errmsg[0] = 0;
+    if (!a) { seterr(errmsg, errlen, "null arguments"); return 1; }
+    if (a->nCol < 0 || a->nLev < 1) { seterr(errmsg, errlen, "bad dimensions"); return 1; }
+    if (a->nCol == 0 || !(a->doLWrad || a->doSWrad)) return 0;
+    if (a->iSFC < 1 || a->iSFC > a->nLev + 1 || a->iTOA < 1 || a->iTOA > a->nLev + 1) {
+        seterr(errmsg, errlen, "iSFC / iTOA outside 1..nLev+1"); return 1;
+    }
+    auto all = [](std::initializer_list<const void *> l) {
+        for (const void *p : l) if (!p) return false;
+        return true;
+    };
+    if (a->doLWrad && !all({a->fluxlwUP_allsky, a->fluxlwDOWN_allsky, a->fluxlwUP_clrsky, a->fluxlwDOWN_clrsky,
+                            a->tsfa, a->htrlw, a->topflw, a->sfcflw, a->sfcdlw, a->sfculw, a->tsflw, a->htrlwu})) {
+        seterr(errmsg, errlen, "LW: NULL array"); return 1;
+    }
+    if (a->doSWrad && !all({a->fluxswUP_allsky, a->fluxswDOWN_allsky, a->fluxswUP_clrsky, a->fluxswDOWN_clrsky,
+                            a->scmpsw, a->sfc_alb_nir_dir, a->sfc_alb_nir_dif, a->sfc_alb_uvvis_dir,
+                            a->sfc_alb_uvvis_dif, a->topfsw, a->sfcfsw, a->nirbmdi, a->nirdfdi, a->visbmdi,
+                            a->visdfdi, a->nirbmui, a->nirdfui, a->visbmui, a->visdfui, a->sfcnsw, a->sfcdsw})) {
+        seterr(errmsg, errlen, "SW: NULL array"); return 1;
+    }
+    if (a->save_diag) {
+        if (!a->fluxr || a->nfxr < 45 || !a->cldsa || !a->mtopa || !a->mbota) {
+            seterr(errmsg, errlen, "save_diag needs fluxr (nfxr >= 45), cldsa, mtopa and mbota"); return 1;
+        }
+        if (a->doLWrad && !a->cldtaulw) { seterr(errmsg, errlen, "save_diag needs cldtaulw"); return 1; }
+        if (a->doSWrad && (!a->cldtausw || !a->aerodp || a->naerodp < 6 || !a->coszen || !a->coszdg || !a->p_lev)) {
+            seterr(errmsg, errlen, "save_diag needs cldtausw, aerodp (>= 6), coszen, coszdg and p_lev"); return 1;
+        }
+    }
+    launch_post(*a, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return 0;
 }

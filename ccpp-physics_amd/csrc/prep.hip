@@ -1,6 +1,6 @@
-// Radiation input preparation of the caller, GFS_rrtmgp_pre_run, on gfx950
-// (FP64), so a host that keeps its state on the GPU need not round-trip the
-// fields through the CPU before the radiation schemes:
+// The callers either side of the radiation schemes, on gfx950 (FP64), so a
+// host that keeps its state on the GPU need not round-trip the fields
+// through the CPU.  First the input preparation of GFS_rrtmgp_pre_run:
 //   q_lay = max(qgrs(:,:,1), 1e-6)                   GFS_rrtmgp_pre.F90:236-237
 //   p_lev = prsi, p_lay = prsl, t_lay = tgrs         :239-246
 //   t_lay, p_lay bounded into the k-distribution's   :248-263
@@ -14,6 +14,7 @@
 // layers); threads of a wavefront are consecutive columns, so every read and
 // write of the column-fastest fields is coalesced.
 #include "kernels.h"
+#include "../../include/ccpprad.h"
 
 #include <cfloat>
 
@@ -87,6 +88,119 @@ void launch_prepare(const PrepArgs &A, hipStream_t st)
 {
     if (A.ncol <= 0) return;
     hipLaunchKernelGGL(prepare_kernel, dim3((A.ncol + 255) / 256), dim3(256), 0, st, A);
+}
+
+// GFS_rrtmgp_post_run's flux bookkeeping (GFS_rrtmgp_post.F90:173-389, less
+// the heating rates): thread per column.  The statements keep the
+// reference's order, so fluxr accumulates bit for bit as the host loops do.
+__global__ __launch_bounds__(256) void post_kernel(ccpprad_post_args A)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.nCol) return;
+    const int nc = A.nCol, nl = A.nLev;
+    const size_t it = (size_t)i + (size_t)nc * (A.iTOA - 1), is = (size_t)i + (size_t)nc * (A.iSFC - 1);
+    auto fx = [&](int j) -> double & { return A.fluxr[(size_t)i + (size_t)nc * (j - 1)]; };   // fluxr(i, j)
+    auto ilev = [&](const int *m, int j) { return m[(size_t)i + (size_t)nc * (j - 1)]; };     // (nCol, 3)
+    if (A.doLWrad) {
+        double *tf = A.topflw + 2 * (size_t)i, *sf = A.sfcflw + 4 * (size_t)i;
+        tf[0] = A.fluxlwUP_allsky[it];
+        tf[1] = A.fluxlwUP_clrsky[it];
+        sf[0] = A.fluxlwUP_allsky[is];
+        sf[1] = A.fluxlwUP_clrsky[is];
+        sf[2] = A.fluxlwDOWN_allsky[is];
+        sf[3] = A.fluxlwDOWN_clrsky[is];
+        A.tsflw[i] = A.tsfa[i];
+        A.sfcdlw[i] = sf[2];
+        A.sfculw[i] = sf[0];
+        for (int k = 0; k < nl; ++k) A.htrlwu[(size_t)i + (size_t)nc * k] = A.htrlw[(size_t)i + (size_t)nc * k];
+        if (A.save_diag) {
+            fx(1) = fx(1) + A.fhlwr * A.fluxlwUP_allsky[it];
+            fx(19) = fx(19) + A.fhlwr * A.fluxlwDOWN_allsky[is];
+            fx(20) = fx(20) + A.fhlwr * A.fluxlwUP_allsky[is];
+            fx(28) = fx(28) + A.fhlwr * A.fluxlwUP_clrsky[it];
+            fx(30) = fx(30) + A.fhlwr * A.fluxlwDOWN_clrsky[is];
+            fx(33) = fx(33) + A.fhlwr * A.fluxlwUP_clrsky[is];
+            for (int j = 1; j <= 3; ++j) {
+                const double tem0d = A.raddt * A.cldsa[(size_t)i + (size_t)nc * (j - 1)];
+                // cloud top / base levels outside 1..nLev (the reference assumes
+                // valid ones) sum nothing instead of reading out of bounds
+                const int itop = min(ilev(A.mtopa, j), nl), ibtc = max(ilev(A.mbota, j), 1);
+                double tem2 = 0.0;
+                for (int k = ibtc; k <= itop; ++k) tem2 = tem2 + A.cldtaulw[(size_t)i + (size_t)nc * (k - 1)];
+                fx(46 - j) = fx(46 - j) + tem0d * (1.0 - exp(-tem2));
+            }
+        }
+    }
+    if (A.doSWrad) {
+        double *tf = A.topfsw + 3 * (size_t)i, *sf = A.sfcfsw + 4 * (size_t)i;
+        const double *cm = A.scmpsw + 6 * (size_t)i;
+        if (A.nDay > 0) {
+            tf[0] = A.fluxswUP_allsky[it];
+            tf[2] = A.fluxswUP_clrsky[it];
+            tf[1] = A.fluxswDOWN_allsky[it];
+            sf[0] = A.fluxswUP_allsky[is];
+            sf[2] = A.fluxswUP_clrsky[is];
+            sf[1] = A.fluxswDOWN_allsky[is];
+            sf[3] = A.fluxswDOWN_clrsky[is];
+            A.nirbmdi[i] = cm[2];
+            A.nirdfdi[i] = cm[3];
+            A.visbmdi[i] = cm[4];
+            A.visdfdi[i] = cm[5];
+            A.nirbmui[i] = cm[2] * A.sfc_alb_nir_dir[i];
+            A.nirdfui[i] = cm[3] * A.sfc_alb_nir_dif[i];
+            A.visbmui[i] = cm[4] * A.sfc_alb_uvvis_dir[i];
+            A.visdfui[i] = cm[5] * A.sfc_alb_uvvis_dif[i];
+        } else {
+            for (int k = 0; k < 4; ++k) sf[k] = 0.0;
+            for (int k = 0; k < 3; ++k) tf[k] = 0.0;
+            A.nirbmdi[i] = 0.0; A.nirdfdi[i] = 0.0; A.visbmdi[i] = 0.0; A.visdfdi[i] = 0.0;
+            A.nirbmui[i] = 0.0; A.nirdfui[i] = 0.0; A.visbmui[i] = 0.0; A.visdfui[i] = 0.0;
+        }
+        A.sfcnsw[i] = sf[1] - sf[0];
+        A.sfcdsw[i] = sf[1];
+        if (A.save_diag) {
+            for (int j = 1; j <= 6; ++j) fx(33 + j) = A.aerodp[(size_t)i + (size_t)nc * (j - 1)];
+            if (A.coszen[i] > 0.0) {
+                const double tem0d = A.fhswr * A.coszdg[i] / A.coszen[i];
+                fx(2) = fx(2) + tf[0] * tem0d;
+                fx(3) = fx(3) + sf[0] * tem0d;
+                fx(4) = fx(4) + sf[1] * tem0d;
+                fx(21) = fx(21) + cm[0] * tem0d;
+                fx(22) = fx(22) + cm[1] * tem0d;
+                fx(23) = fx(23) + tf[1] * tem0d;
+                fx(24) = fx(24) + A.visbmdi[i] * tem0d;
+                fx(25) = fx(25) + A.visdfdi[i] * tem0d;
+                fx(26) = fx(26) + A.nirbmdi[i] * tem0d;
+                fx(27) = fx(27) + A.nirdfdi[i] * tem0d;
+                fx(29) = fx(29) + tf[2] * tem0d;
+                fx(31) = fx(31) + sf[2] * tem0d;
+                fx(32) = fx(32) + sf[3] * tem0d;
+            }
+            fx(17) = fx(17) + A.raddt * A.cldsa[(size_t)i + (size_t)nc * 3];
+            fx(18) = fx(18) + A.raddt * A.cldsa[(size_t)i + (size_t)nc * 4];
+            for (int j = 1; j <= 3; ++j) {
+                const double tem0d = A.raddt * A.cldsa[(size_t)i + (size_t)nc * (j - 1)];
+                // levels clamped into 1..nLev+1 for the pressures, 1..nLev for the sums
+                const int mt = min(max(ilev(A.mtopa, j), 1), nl + 1), mb = min(max(ilev(A.mbota, j), 1), nl + 1);
+                const double ptop = A.p_lev[(size_t)i + (size_t)nc * (mt - 1)];
+                const double pbtc = A.p_lev[(size_t)i + (size_t)nc * (mb - 1)];
+                const int itop = min(ilev(A.mtopa, j), nl), ibtc = max(ilev(A.mbota, j), 1);
+                fx(8 - j) = fx(8 - j) + tem0d;
+                fx(11 - j) = fx(11 - j) + tem0d * ptop;
+                fx(14 - j) = fx(14 - j) + tem0d * pbtc;
+                fx(17 - j) = fx(17 - j) + tem0d * ptop;
+                double tem1 = 0.0;
+                for (int k = ibtc; k <= itop; ++k) tem1 = tem1 + A.cldtausw[(size_t)i + (size_t)nc * (k - 1)];
+                fx(43 - j) = fx(43 - j) + tem0d * tem1;
+            }
+        }
+    }
+}
+
+void launch_post(const ccpprad_post_args &A, hipStream_t st)
+{
+    if (A.nCol <= 0) return;
+    hipLaunchKernelGGL(post_kernel, dim3((A.nCol + 255) / 256), dim3(256), 0, st, A);
 }
 
 }  // namespace ccpprad

@@ -8,6 +8,8 @@
  *        (physics/Radiation/RRTMGP/rrtmgp_sw_main.F90:34-79, :87-683)
  *   ccpprad_heating_rate               <- compute_heating_rate as called by
  *        GFS_rrtmgp_post_run (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_post.F90:159-171,251-266)
+ *   ccpprad_post_dev                   <- GFS_rrtmgp_post_run's flux bookkeeping
+ *        (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_post.F90:173-389)
  *   ccpprad_prepare_inputs_dev         <- the radiation-input preparation of
  *        GFS_rrtmgp_pre_run (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_pre.F90:217-274,349-371)
  *   ccpprad_*_gas_optics_dev           <- ty_gas_optics_rrtmgp%gas_optics
@@ -190,6 +192,40 @@ void ccpprad_finalize(ccpprad_ctx *ctx);
 int ccpprad_heating_rate_dev(int nCol, int nLay, const double *flux_up,
                              const double *flux_dn, const double *p_lev, double *htr,
                              void *stream, char *errmsg, int errlen);
+
+/* Argument list of GFS_rrtmgp_post_run
+ * (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_post.F90:35-389) less
+ * its heating rates (those come fused out of the _run entries, or from
+ * ccpprad_heating_rate_dev).  Device pointers, FP64, Fortran layouts:
+ * level fluxes (nCol, nLev+1); the flux types are arrays of derived types,
+ * i.e. per column their fields in declaration order
+ * (Radiation/RRTMG/radlw_param.f:78-92, radsw_param.f:79-118):
+ *   topflw (2, nCol) upfxc, upfx0         sfcflw (4, nCol) upfxc, upfx0, dnfxc, dnfx0
+ *   topfsw (3, nCol) upfxc, dnfxc, upfx0  sfcfsw (4, nCol) upfxc, dnfxc, upfx0, dnfx0
+ *   scmpsw (6, nCol) uvbfc, uvbf0, nirbm, nirdf, visbm, visdf
+ * iSFC / iTOA, mtopa / mbota are 1-based level indices; cldsa (nCol, 5);
+ * aerodp (nCol, naerodp >= 6); fluxr (nCol, nfxr >= 45) is accumulated when
+ * save_diag.  The cloud and aerosol diagnostics may be NULL when save_diag
+ * is 0. */
+typedef struct ccpprad_post_args {
+    int nCol, nLev, nDay, iSFC, iTOA;
+    int doLWrad, doSWrad, save_diag, naerodp, nfxr;
+    double fhlwr, fhswr, raddt;
+    const double *fluxlwUP_allsky, *fluxlwDOWN_allsky, *fluxlwUP_clrsky, *fluxlwDOWN_clrsky;
+    const double *fluxswUP_allsky, *fluxswDOWN_allsky, *fluxswUP_clrsky, *fluxswDOWN_clrsky;
+    const double *scmpsw;
+    const double *sfc_alb_nir_dir, *sfc_alb_nir_dif, *sfc_alb_uvvis_dir, *sfc_alb_uvvis_dif;
+    const double *tsfa, *coszen, *coszdg, *htrlw, *p_lev;
+    const double *cldsa, *cldtaulw, *cldtausw, *aerodp;
+    const int *mtopa, *mbota;
+    double *topflw, *sfcflw, *sfcdlw, *sfculw, *tsflw, *htrlwu;
+    double *topfsw, *sfcfsw;
+    double *nirbmdi, *nirdfdi, *visbmdi, *visdfdi, *nirbmui, *nirdfui, *visbmui, *visdfui;
+    double *sfcnsw, *sfcdsw, *fluxr;
+} ccpprad_post_args;
+
+/* GFS_rrtmgp_post_run's flux bookkeeping on the device (see the struct). */
+int ccpprad_post_dev(const ccpprad_post_args *args, void *stream, char *errmsg, int errlen);
 
 /* The radiation-input preparation of GFS_rrtmgp_pre_run
  * (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_pre.F90:217-274,349-371),

@@ -44,6 +44,7 @@ def lib():
         L.orc_optimal_angles.argtypes = [C.c_void_p, C.c_int, C.c_int, P_D, P_D]
         L.orc_prepare.argtypes = [C.c_int, C.c_int] + [C.c_double] * 4 + [P_D] * 13
         L.orc_prepare.restype = C.c_int
+        L.orc_post.argtypes = [C.c_void_p]
         L.orc_gauss.argtypes = [C.c_int, P_D, P_D]
         L.orc_heating_rate.argtypes = [C.c_int, C.c_int, P_D, P_D, P_D, P_D]
         L.orc_mt_draws.argtypes = [C.c_int, C.c_int, P_D]
@@ -176,6 +177,12 @@ def prepare(kd, prsi, prsl, tgrs, q_h2o, q_o3, tsfc):
                             _p(c(q_o3)), _p(c(tsfc)), _p(out["p_lay"]), _p(out["p_lev"]), _p(out["t_lay"]),
                             _p(out["t_lev"]), _p(out["q_lay"]), _p(out["vmr_h2o"]), _p(out["vmr_o3"]))
     return out, bool(top)
+
+
+def post(ints, reals, arrays):
+    """GFS_rrtmgp_post's bookkeeping on host arrays (updated in place)."""
+    h = capi.post_struct(ints, reals, arrays)
+    lib().orc_post(h.ref())
 
 
 def optimal_angles(kd, ncol, nlay, tau):

@@ -541,6 +541,124 @@ int orc_prepare(int ncol, int nlev, double pmin, double pmax, double tmin, doubl
     return top_at_1;
 }
 
+/* GFS_rrtmgp_post_run's flux bookkeeping (GFS_rrtmgp_post.F90:173-389, the
+ * heating rates aside) over host arrays, statement by statement as the
+ * reference's loops.  Pinned by hand cases (tests/test_oracle_prepare.py):
+ * it is data movement and time-weighted sums, with no arithmetic of its own
+ * beyond them. */
+void orc_post(const ccpprad_post_args *a)
+{
+    const int nc = a->nCol, nl = a->nLev;
+#define FX(i, j) a->fluxr[(size_t)(i) + (size_t)nc * ((j) - 1)]
+#define C2(p, i, k) (p)[(size_t)(i) + (size_t)nc * ((k) - 1)]
+    if (!(a->doLWrad || a->doSWrad)) return;
+    if (a->doLWrad) {
+        for (int i = 0; i < nc; ++i) {
+            a->topflw[2 * i + 0] = C2(a->fluxlwUP_allsky, i, a->iTOA);
+            a->topflw[2 * i + 1] = C2(a->fluxlwUP_clrsky, i, a->iTOA);
+            a->sfcflw[4 * i + 0] = C2(a->fluxlwUP_allsky, i, a->iSFC);
+            a->sfcflw[4 * i + 1] = C2(a->fluxlwUP_clrsky, i, a->iSFC);
+            a->sfcflw[4 * i + 2] = C2(a->fluxlwDOWN_allsky, i, a->iSFC);
+            a->sfcflw[4 * i + 3] = C2(a->fluxlwDOWN_clrsky, i, a->iSFC);
+            a->tsflw[i] = a->tsfa[i];
+            a->sfcdlw[i] = a->sfcflw[4 * i + 2];
+            a->sfculw[i] = a->sfcflw[4 * i + 0];
+        }
+        for (size_t q = 0; q < (size_t)nc * nl; ++q) a->htrlwu[q] = a->htrlw[q];
+        if (a->save_diag) {
+            for (int i = 0; i < nc; ++i) {
+                FX(i, 1) = FX(i, 1) + a->fhlwr * C2(a->fluxlwUP_allsky, i, a->iTOA);
+                FX(i, 19) = FX(i, 19) + a->fhlwr * C2(a->fluxlwDOWN_allsky, i, a->iSFC);
+                FX(i, 20) = FX(i, 20) + a->fhlwr * C2(a->fluxlwUP_allsky, i, a->iSFC);
+                FX(i, 28) = FX(i, 28) + a->fhlwr * C2(a->fluxlwUP_clrsky, i, a->iTOA);
+                FX(i, 30) = FX(i, 30) + a->fhlwr * C2(a->fluxlwDOWN_clrsky, i, a->iSFC);
+                FX(i, 33) = FX(i, 33) + a->fhlwr * C2(a->fluxlwUP_clrsky, i, a->iSFC);
+            }
+            for (int j = 1; j <= 3; ++j)
+                for (int i = 0; i < nc; ++i) {
+                    double tem0d = a->raddt * C2(a->cldsa, i, j);
+                    int itop = C2(a->mtopa, i, j), ibtc = C2(a->mbota, i, j);
+                    double tem2 = 0.0;
+                    for (int k = ibtc; k <= itop; ++k) tem2 = tem2 + C2(a->cldtaulw, i, k);
+                    FX(i, 46 - j) = FX(i, 46 - j) + tem0d * (1.0 - exp(-tem2));
+                }
+        }
+    }
+    if (a->doSWrad) {
+        for (int i = 0; i < nc; ++i) {
+            double *tf = a->topfsw + 3 * (size_t)i, *sf = a->sfcfsw + 4 * (size_t)i;
+            const double *cm = a->scmpsw + 6 * (size_t)i;
+            if (a->nDay > 0) {
+                tf[0] = C2(a->fluxswUP_allsky, i, a->iTOA);
+                tf[2] = C2(a->fluxswUP_clrsky, i, a->iTOA);
+                tf[1] = C2(a->fluxswDOWN_allsky, i, a->iTOA);
+                sf[0] = C2(a->fluxswUP_allsky, i, a->iSFC);
+                sf[2] = C2(a->fluxswUP_clrsky, i, a->iSFC);
+                sf[1] = C2(a->fluxswDOWN_allsky, i, a->iSFC);
+                sf[3] = C2(a->fluxswDOWN_clrsky, i, a->iSFC);
+                a->nirbmdi[i] = cm[2];
+                a->nirdfdi[i] = cm[3];
+                a->visbmdi[i] = cm[4];
+                a->visdfdi[i] = cm[5];
+                a->nirbmui[i] = cm[2] * a->sfc_alb_nir_dir[i];
+                a->nirdfui[i] = cm[3] * a->sfc_alb_nir_dif[i];
+                a->visbmui[i] = cm[4] * a->sfc_alb_uvvis_dir[i];
+                a->visdfui[i] = cm[5] * a->sfc_alb_uvvis_dif[i];
+            } else {
+                for (int k = 0; k < 4; ++k) sf[k] = 0.0;
+                for (int k = 0; k < 3; ++k) tf[k] = 0.0;
+                a->nirbmdi[i] = a->nirdfdi[i] = a->visbmdi[i] = a->visdfdi[i] = 0.0;
+                a->nirbmui[i] = a->nirdfui[i] = a->visbmui[i] = a->visdfui[i] = 0.0;
+            }
+        }
+        for (int i = 0; i < nc; ++i) {
+            a->sfcnsw[i] = a->sfcfsw[4 * i + 1] - a->sfcfsw[4 * i + 0];
+            a->sfcdsw[i] = a->sfcfsw[4 * i + 1];
+        }
+        if (a->save_diag) {
+            for (int i = 0; i < nc; ++i) {
+                for (int j = 1; j <= 6; ++j) FX(i, 33 + j) = C2(a->aerodp, i, j);
+                if (a->coszen[i] > 0.0) {
+                    const double *tf = a->topfsw + 3 * (size_t)i, *sf = a->sfcfsw + 4 * (size_t)i;
+                    const double *cm = a->scmpsw + 6 * (size_t)i;
+                    double tem0d = a->fhswr * a->coszdg[i] / a->coszen[i];
+                    FX(i, 2) = FX(i, 2) + tf[0] * tem0d;
+                    FX(i, 3) = FX(i, 3) + sf[0] * tem0d;
+                    FX(i, 4) = FX(i, 4) + sf[1] * tem0d;
+                    FX(i, 21) = FX(i, 21) + cm[0] * tem0d;
+                    FX(i, 22) = FX(i, 22) + cm[1] * tem0d;
+                    FX(i, 23) = FX(i, 23) + tf[1] * tem0d;
+                    FX(i, 24) = FX(i, 24) + a->visbmdi[i] * tem0d;
+                    FX(i, 25) = FX(i, 25) + a->visdfdi[i] * tem0d;
+                    FX(i, 26) = FX(i, 26) + a->nirbmdi[i] * tem0d;
+                    FX(i, 27) = FX(i, 27) + a->nirdfdi[i] * tem0d;
+                    FX(i, 29) = FX(i, 29) + tf[2] * tem0d;
+                    FX(i, 31) = FX(i, 31) + sf[2] * tem0d;
+                    FX(i, 32) = FX(i, 32) + sf[3] * tem0d;
+                }
+            }
+            for (int i = 0; i < nc; ++i) {
+                FX(i, 17) = FX(i, 17) + a->raddt * C2(a->cldsa, i, 4);
+                FX(i, 18) = FX(i, 18) + a->raddt * C2(a->cldsa, i, 5);
+            }
+            for (int j = 1; j <= 3; ++j)
+                for (int i = 0; i < nc; ++i) {
+                    double tem0d = a->raddt * C2(a->cldsa, i, j);
+                    int itop = C2(a->mtopa, i, j), ibtc = C2(a->mbota, i, j);
+                    FX(i, 8 - j) = FX(i, 8 - j) + tem0d;
+                    FX(i, 11 - j) = FX(i, 11 - j) + tem0d * C2(a->p_lev, i, itop);
+                    FX(i, 14 - j) = FX(i, 14 - j) + tem0d * C2(a->p_lev, i, ibtc);
+                    FX(i, 17 - j) = FX(i, 17 - j) + tem0d * C2(a->p_lev, i, itop);
+                    double tem1 = 0.0;
+                    for (int k = ibtc; k <= itop; ++k) tem1 = tem1 + C2(a->cldtausw, i, k);
+                    FX(i, 43 - j) = FX(i, 43 - j) + tem0d * tem1;
+                }
+        }
+    }
+#undef FX
+#undef C2
+}
+
 /* compute_optimal_angles: Ds(g,c) = fit1(b) * exp(-sum_l tau) + fit2(b). */
 void orc_optimal_angles(const ccpprad_kdist *k, int ncol, int nlay, const double *tau, double *Ds)
 {

@@ -31,3 +31,30 @@ def test_prepare_inputs_matches_oracle(kd_lw, lut_lw, top):
             assert np.allclose(g, r, rtol=1e-13, atol=0), k
         else:
             assert np.array_equal(g, r), k
+
+
+@pytest.mark.parametrize("nday", [1, 0])
+def test_post_matches_oracle(nday):
+    """GFS_rrtmgp_post's bookkeeping: the copies bit-identical; the fluxr
+    sums to 1e-15 relative (the host and device compilers may contract
+    different products into FMAs, and exp(-tau) may differ in the last
+    bit)."""
+    from ccpprad import lib
+    from oracle import oracle as O
+    from test_oracle_prepare import post_inputs
+    ints, reals, arr, out = post_inputs(nc=1000, nl=64, nday=nday, seed=6)
+    ref = {k: v.copy() for k, v in out.items()}
+    a = dict(arr)
+    a.update(ref)
+    O.post(ints, reals, a)
+    d = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in arr.items()}
+    dout = {k: torch.from_numpy(v.copy()).cuda() for k, v in out.items()}
+    d.update(dout)
+    lib.post_dev(ints, reals, d)
+    torch.cuda.synchronize()
+    for k, r in ref.items():
+        g = dout[k].cpu().numpy()
+        if k in ("fluxr", "nirbmui", "nirdfui", "visbmui", "visdfui", "sfcnsw"):
+            assert np.allclose(g, r, rtol=1e-15, atol=0), k
+        else:
+            assert np.array_equal(g, r), k

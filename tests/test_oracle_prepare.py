@@ -71,3 +71,73 @@ def test_bounds_floor_and_mixing_ratios(kd_lw, top):
     assert np.allclose(qb, out["q_lay"][ok], rtol=1e-13, atol=0)
     assert (out["vmr_h2o"][~ok] == 0).all() and (~ok).sum() == 1
     assert np.allclose(out["vmr_o3"] / AMDO3, np.maximum(x["o3"], 1e-10), rtol=1e-15)
+
+
+def post_inputs(nc=37, nl=20, nday=1, seed=4):
+    """Random level fluxes, components and diagnostics for GFS_rrtmgp_post;
+    numpy arrays shaped as the Fortran arrays reversed (column fastest)."""
+    rng = np.random.default_rng(seed)
+    f = lambda *s: rng.uniform(0.0, 400.0, s[::-1]) if s else None
+    arr = {k: f(nc, nl + 1) for k in ("fluxlwUP_allsky", "fluxlwDOWN_allsky", "fluxlwUP_clrsky",
+                                      "fluxlwDOWN_clrsky", "fluxswUP_allsky", "fluxswDOWN_allsky",
+                                      "fluxswUP_clrsky", "fluxswDOWN_clrsky")}
+    arr["scmpsw"] = rng.uniform(0.0, 300.0, (nc, 6))
+    for k in ("sfc_alb_nir_dir", "sfc_alb_nir_dif", "sfc_alb_uvvis_dir", "sfc_alb_uvvis_dif"):
+        arr[k] = rng.uniform(0.05, 0.8, nc)
+    arr["tsfa"] = rng.uniform(250.0, 300.0, nc)
+    arr["coszen"] = rng.uniform(-0.5, 1.0, nc)
+    arr["coszdg"] = rng.uniform(0.0, 1.0, nc)
+    arr["htrlw"] = rng.normal(0.0, 1e-5, (nl, nc))
+    arr["p_lev"] = np.linspace(1e5, 100.0, nl + 1)[:, None] * np.ones((1, nc))
+    arr["cldsa"] = rng.uniform(0.0, 1.0, (5, nc))
+    mb = rng.integers(1, nl // 2, (3, nc))
+    arr["mbota"] = mb.astype(np.int32)
+    arr["mtopa"] = (mb + rng.integers(0, nl // 2, (3, nc))).astype(np.int32)
+    arr["cldtaulw"] = rng.uniform(0.0, 2.0, (nl, nc))
+    arr["cldtausw"] = rng.uniform(0.0, 5.0, (nl, nc))
+    arr["aerodp"] = rng.uniform(0.0, 0.5, (6, nc))
+    z = lambda *s: np.zeros(s[::-1])
+    out = dict(topflw=np.zeros((nc, 2)), sfcflw=np.zeros((nc, 4)), topfsw=np.zeros((nc, 3)),
+               sfcfsw=np.zeros((nc, 4)), htrlwu=z(nc, nl), fluxr=rng.uniform(0.0, 10.0, (45, nc)))
+    for k in ("sfcdlw", "sfculw", "tsflw", "nirbmdi", "nirdfdi", "visbmdi", "visdfdi", "nirbmui", "nirdfui",
+              "visbmui", "visdfui", "sfcnsw", "sfcdsw"):
+        out[k] = np.zeros(nc)
+    ints = dict(nCol=nc, nLev=nl, nDay=nday, iSFC=1, iTOA=nl + 1, doLWrad=1, doSWrad=1, save_diag=1,
+                naerodp=6, nfxr=45)
+    reals = dict(fhlwr=3600.0, fhswr=3600.0, raddt=1800.0)
+    return ints, reals, arr, out
+
+
+def test_post_bookkeeping_hand_cases():
+    """TOA / surface picks, surface components and the time-weighted fluxr
+    sums against their definitions; a sunless call zeroes the SW fields."""
+    from oracle import oracle as O
+    ints, reals, arr, out = post_inputs()
+    f0 = out["fluxr"].copy()
+    a = dict(arr)
+    a.update(out)
+    O.post(ints, reals, a)
+    nl, it, isf = ints["nLev"], ints["iTOA"] - 1, ints["iSFC"] - 1
+    assert np.array_equal(out["topflw"][:, 0], arr["fluxlwUP_allsky"][it])
+    assert np.array_equal(out["sfcflw"][:, 2], arr["fluxlwDOWN_allsky"][isf])
+    assert np.array_equal(out["topfsw"][:, 1], arr["fluxswDOWN_allsky"][it])
+    assert np.array_equal(out["sfcfsw"][:, 3], arr["fluxswDOWN_clrsky"][isf])
+    assert np.array_equal(out["sfcnsw"], out["sfcfsw"][:, 1] - out["sfcfsw"][:, 0])
+    assert np.array_equal(out["nirbmui"], arr["scmpsw"][:, 2] * arr["sfc_alb_nir_dir"])
+    assert np.array_equal(out["htrlwu"], arr["htrlw"])
+    fr = out["fluxr"]
+    # the sums may be contracted into FMAs (one rounding instead of two)
+    assert np.allclose(fr[0], f0[0] + 3600.0 * arr["fluxlwUP_allsky"][it], rtol=1e-15, atol=0)
+    day = arr["coszen"] > 0
+    w = 3600.0 * arr["coszdg"] / arr["coszen"]
+    assert np.allclose(fr[1][day], (f0[1] + out["topfsw"][:, 0] * w)[day], rtol=1e-15, atol=0)
+    assert np.array_equal(fr[1][~day], f0[1][~day])
+    assert np.array_equal(fr[33:39], arr["aerodp"])
+    # cloud emissivity of the high cloud (j = 1 -> fluxr(45)): sum of tau over base..top
+    c = 5
+    b_, t_ = arr["mbota"][0, c], arr["mtopa"][0, c]
+    tau = arr["cldtaulw"][b_ - 1:t_, c].sum()
+    assert fr[44, c] == pytest.approx(f0[44, c] + 1800.0 * arr["cldsa"][0, c] * (1 - np.exp(-tau)), rel=1e-14)
+    ints["nDay"] = 0
+    O.post(ints, reals, a)
+    assert not out["sfcfsw"].any() and not out["topfsw"].any() and not out["nirbmui"].any()
