@@ -350,48 +350,56 @@ __global__ __launch_bounds__(512) void mlp_infer_kernel(MlpInferArgs A)
                 for (int kc = 0; kc < 8; ++kc) h[s][kc] = hn[s][kc];
         }
         // layer 4 (linear), two output tiles at a time, fused output transform:
-        // y = acc ystd + (b4 ystd + ymean); tau = e^y col_dry, pfrac = max(y, 0)
+        // y = acc ystd + (b4 ystd + ymean); tau = e^y col_dry, pfrac = max(y, 0).
+        // The operands are swapped (rows x K times K x features): the layer-3
+        // output already sits in the A-operand layout, and the weight fragments
+        // in the B-operand layout, so the result has one feature per lane and
+        // 16 rows per lane, and each store writes one 128-byte line per half
+        // wavefront (32 consecutive features of one row) instead of 32 partial
+        // lines.  A lane's rows come from the lanes that prepared them.
         const float *ysd = bias + 640, *yc = bias + 896;
-        // output tiles 2 np, 2 np + 1: tau for np < 2 (TAU), Planck fraction after
-        auto layer4 = [&](auto TAU, int np) {
-            f32x16 acc[2][2] = {{{}, {}}, {{}, {}}};
+        // one tile at a time (unrolled: h[s] must stay in registers): its 16
+        // rows per lane (output row, dry-air column) from the lanes that
+        // prepared them, then its 8 output tiles
 #pragma unroll
-            for (int kc = 0; kc < 8; ++kc) {
-                const bf16x8 w0 = F4[((2 * np) * 8 + kc) * 64 + l], w1 = F4[((2 * np + 1) * 8 + kc) * 64 + l];
+        for (int s = 0; s < 2; ++s) {
+            int orr[16];
+            float cdr[16];
+            const int mine = valid[s] ? (int)orow[s] : -1;
 #pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                    acc[s][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, h[s][kc], acc[s][0], 0, 0, 0);
-                    acc[s][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, h[s][kc], acc[s][1], 0, 0, 0);
-                }
+            for (int r = 0; r < 16; ++r) {
+                const int row = 8 * (r >> 2) + 4 * half + (r & 3);
+                orr[r] = __shfl(mine, row);
+                cdr[r] = __shfl(col_dry[s], row);
             }
+            // output tiles 2 np, 2 np + 1: tau for np < 2 (TAU), Planck fraction after
+            auto layer4 = [&](auto TAU, int np) {
+                f32x16 acc[2] = {{}, {}};
 #pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                if (!valid[s]) continue;
-                float *orow_p = (TAU.value ? A.tau : A.pfrac) + orow[s] * MLP_G;
+                for (int kc = 0; kc < 8; ++kc) {
+                    const bf16x8 w0 = F4[((2 * np) * 8 + kc) * 64 + l], w1 = F4[((2 * np + 1) * 8 + kc) * 64 + l];
+                    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(h[s][kc], w0, acc[0], 0, 0, 0);
+                    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(h[s][kc], w1, acc[1], 0, 0, 0);
+                }
+                float *out_p = TAU.value ? A.tau : A.pfrac;
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
-                    const int nt = 2 * np + u;
+                    const int feat = (2 * np + u) * 32 + rl;
+                    const float sc = ysd[feat], cc = yc[feat];
+                    const int col = feat - (TAU.value ? 0 : MLP_G);
 #pragma unroll
-                    for (int r4 = 0; r4 < 4; ++r4) {
-                        const int n0 = nt * 32 + 8 * r4 + 4 * half;
-                        const float4 sc = *(const float4 *)(ysd + n0), cc = *(const float4 *)(yc + n0);
-                        const float *scp = (const float *)&sc, *ccp = (const float *)&cc;
-                        float4 o;
-                        float *op = (float *)&o;
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            const float y = acc[s][u][4 * r4 + i] * scp[i] + ccp[i];
-                            op[i] = TAU.value ? __expf(y) * col_dry[s] : fmaxf(y, 0.0f);
-                        }
-                        *(float4 *)(orow_p + (n0 - (TAU.value ? 0 : MLP_G))) = o;
+                    for (int r = 0; r < 16; ++r) {
+                        const float y = acc[u][r] * sc + cc;
+                        const float v = TAU.value ? __expf(y) * cdr[r] : fmaxf(y, 0.0f);
+                        if (orr[r] >= 0) out_p[(long)orr[r] * MLP_G + col] = v;
                     }
                 }
-            }
-        };
+            };
 #pragma unroll 1
-        for (int np = 0; np < 2; ++np) layer4(std::true_type{}, np);
+            for (int np = 0; np < 2; ++np) layer4(std::true_type{}, np);
 #pragma unroll 1
-        for (int np = 2; np < 4; ++np) layer4(std::false_type{}, np);
+            for (int np = 2; np < 4; ++np) layer4(std::false_type{}, np);
+        }
 #pragma unroll
         for (int s = 0; s < 2; ++s)
             if (valid[s] && half == 0) {
