@@ -278,6 +278,14 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
     (void)hipGetDevice(&c->device);
     const char *ch = getenv("CCPPRAD_CHUNK");
     if (ch && atoi(ch) > 0) c->chunk = atoi(ch);
+    // the solvers form byte offsets into a launch's cloud records
+    // ([layer][column][band], 12 bytes each) in 32 bits: bound the columns
+    // per launch accordingly
+    {
+        const size_t per_col = (size_t)std::max(nLay, 1) * std::max(kd->nband, 1) * 12;
+        const size_t cmax = (((size_t)1 << 32) - 1) / per_col;
+        if ((size_t)c->chunk > cmax) c->chunk = (int)std::max<size_t>(cmax, 1);
+    }
     hipError_t e = hipSuccess;
     TablesView &T = c->T;
     T.maxminor_g = 0;

@@ -94,6 +94,16 @@ __device__ __forceinline__ float planck_band(const TablesView &t, int b, float T
     return a0 + frac * (a1 - a0);
 }
 
+// base[idx] for a wavefront-uniform base and a 32-bit element index, with the
+// byte offset formed in 32 bits: the load can then take the base in SGPRs
+// and the offset in one VGPR (no 64-bit address add per load).  The caller
+// guarantees idx * sizeof(T) < 2^32.
+template <typename T>
+__device__ __forceinline__ T ldu(const T *base, unsigned idx)
+{
+    return *(const T *)((const char *)base + idx * (unsigned)sizeof(T));
+}
+
 // 1/x from v_rcp_f32 (about 1 ulp); x must be a normal number.
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 

@@ -151,15 +151,15 @@ __global__ __launch_bounds__(128) void lw_solve_2str_kernel(TablesView T, LwSolv
 #pragma unroll
         for (int j = -1; j <= L2_SEG; ++j) {
             const int k = min(max(s * L2_SEG + j, 0), nlay - 1);
-            pf[j + 1] = pf_c[(unsigned)lay(k) * ngpt + gg];
+            pf[j + 1] = ldu(pf_c, (unsigned)lay(k) * ngpt + gg);
         }
 #pragma unroll
         for (int j = 0; j < L2_SEG; ++j) {
             const int l = lay(min(s * L2_SEG + j, nlay - 1));
-            tau[j] = tau_c[(unsigned)l * ngpt + gg];
+            tau[j] = ldu(tau_c, (unsigned)l * ngpt + gg);
             if (CLOUDS) {
-                mw[j] = mk_c[(unsigned)l * MASK_WORDS + mword];
-                const float3 v = co_c[(unsigned)l * co_ls + b];
+                mw[j] = ldu(mk_c, (unsigned)l * MASK_WORDS + mword);
+                const float3 v = ldu(co_c, (unsigned)l * co_ls + b);
                 cx[j] = v.x; cy[j] = v.y; cz[j] = v.z;
             }
         }

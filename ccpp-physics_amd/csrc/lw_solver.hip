@@ -109,10 +109,10 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
     // selected by the mask bit), so a batch costs one memory latency.
     struct LayerIn { float tau, pf, ct; unsigned mw; double aer; };
     auto load_layer = [&](int l, LayerIn &x) {
-        x.tau = tau_c[(unsigned)l * ngpt + gg];
-        x.pf = pf_c[(unsigned)l * ngpt + gg];
+        x.tau = ldu(tau_c, (unsigned)l * ngpt + gg);
+        x.pf = ldu(pf_c, (unsigned)l * ngpt + gg);
         x.aer = A.aer_tau ? A.aer_tau[(size_t)c + (size_t)ld * (l + (size_t)nlay * b)] : 0.0;
-        if (CLOUDS) { x.mw = mk_c[(unsigned)l * MASK_WORDS + mword]; x.ct = ct_c[(unsigned)l * ct_ls + b]; }
+        if (CLOUDS) { x.mw = ldu(mk_c, (unsigned)l * MASK_WORDS + mword); x.ct = ldu(ct_c, (unsigned)l * ct_ls + b); }
     };
     auto layer_tau = [&](const LayerIn &x) -> float {
         float t = x.tau;

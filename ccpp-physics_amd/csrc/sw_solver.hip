@@ -222,15 +222,15 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
         for (int j = 0; j < SW_SEG; ++j) {
             const int l = lay(min(sg * SW_SEG + j, nlay - 1));
             const unsigned og = (unsigned)l * ngpt + gg;
-            ta[j] = ta_c[og]; tr[j] = tr_c[og];
+            ta[j] = ldu(ta_c, og); tr[j] = ldu(tr_c, og);
             if (AER) {
                 const size_t q = (size_t)c + (size_t)ld * (l + (size_t)nlay * b);
                 const double sd = A.aer_ssa[q];
                 at[j] = (float)A.aer_tau[q]; aw[j] = (float)sd; ap[j] = (float)(1.0 - sd); ag[j] = (float)A.aer_g[q];
             }
             if (cl.value) {
-                mw[j] = mk_c[(unsigned)l * MASK_WORDS + mword];
-                cv[j] = co_c[(unsigned)l * co_ls + b];
+                mw[j] = ldu(mk_c, (unsigned)l * MASK_WORDS + mword);
+                cv[j] = ldu(co_c, (unsigned)l * co_ls + b);
             }
         }
 #pragma unroll
