@@ -56,6 +56,8 @@ def parse():
                    help="trained emulator weights (.npz); default: the packaged ones")
     p.add_argument("--no-nn", action="store_true", help="skip the learned LW gas-optics leg")
     p.add_argument("--no-lwscat", action="store_true", help="skip the LW cloud-scattering (doGP_lwscat) leg")
+    p.add_argument("--no-callers", action="store_true",
+                   help="skip the leg timing the device callers (pre, geometry, overlap, post)")
     p.add_argument("--copy-probe", action="store_true",
                    help="after the bench, run one 1 GiB device copy: the reference dispatch that scales "
                         "FETCH_SIZE / WRITE_SIZE counter readings to bytes")
@@ -78,6 +80,86 @@ def build_inputs(kd_lw, kd_sw, ncol, torch):
         dev[k] = t
     dev["idx"] = torch.arange(1, ncol + 1, dtype=torch.int32, device="cuda")
     return pr, arr, dev
+
+
+def time_callers(a, torch, lw, dev, ncol):
+    """The callers either side of the schemes on the device, at the workload
+    size: GFS_rrtmgp_pre's input preparation and layer geometry,
+    GFS_rrtmgp_cloud_overlap_run (Hogan length, exponential-random overlap,
+    convective overlap on) and GFS_rrtmgp_post's bookkeeping with the
+    diagnostics on.  HIP events on the launch stream, mean over the timed
+    repeats.  Algorithmic bytes per column: prepare 8 (9 nl + 3 nv + 1),
+    geometry 8 (7 nl + nv), overlap 8 (6 nl + 2) (every field read and
+    written once); post is reported as time only."""
+    from ccpprad import lib
+    nl, nv = NLAY, NLAY + 1
+    sp = torch.cuda.current_stream().cuda_stream
+    f64 = dict(dtype=torch.float64, device="cuda")
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    rnd = lambda *s, lo=0.0, hi=1.0: torch.rand(*s, generator=gen, **f64) * (hi - lo) + lo
+    q = rnd(nl, ncol, hi=0.02)
+    o3 = rnd(nl, ncol, hi=1e-5)
+    prep = {k: torch.empty(nv if k in ("p_lev", "t_lev") else nl, ncol, **f64)
+            for k in ("p_lay", "p_lev", "t_lay", "t_lev", "q_lay", "vmr_h2o", "vmr_o3")}
+    geo = {k: torch.empty(nl, ncol, **f64) for k in ("tv", "dz", "dzc", "dp")}
+    ovl_in = dict(lat=rnd(ncol, lo=-1.5, hi=1.5), deltaZc=geo["dzc"], cld_frac=dev["cld_frac"],
+                  cld_cnv_frac=rnd(nl, ncol, hi=0.2))
+    ovl_out = dict(de_lgth=torch.empty(ncol, **f64), cloud_overlap_param=torch.empty(nl, ncol, **f64),
+                   cnv_cloud_overlap_param=torch.empty(nl, ncol, **f64),
+                   precip_overlap_param=torch.empty(nl, ncol, **f64))
+    ovl_ints = dict(nCol=ncol, nLev=nl, yearlen=365, doSWrad=1, doLWrad=1, literal_r8=1, idcor=1, idcor_con=0,
+                    idcor_hogan=1, idcor_oreopoulos=2, iovr=5, iovr_convcld=5, iovr_dcorr=3, iovr_exp=4,
+                    iovr_exprand=5, imfdeepcnv=2, imfdeepcnv_gf=3, imfdeepcnv_samf=2)
+    ovl_reals = dict(julian=172.0, con_pi=np.pi, dcorr_con=2.5)
+    post_in = {k: rnd(nv, ncol, hi=400.0) for k in ("fluxlwUP_allsky", "fluxlwDOWN_allsky", "fluxlwUP_clrsky",
+                                                    "fluxlwDOWN_clrsky", "fluxswUP_allsky", "fluxswDOWN_allsky",
+                                                    "fluxswUP_clrsky", "fluxswDOWN_clrsky")}
+    post_in.update(scmpsw=rnd(ncol, 6, hi=300.0), tsfa=rnd(ncol, lo=250.0, hi=300.0), coszen=rnd(ncol),
+                   coszdg=rnd(ncol), htrlw=rnd(nl, ncol, hi=1e-5), p_lev=dev["p_lev"], cldsa=rnd(5, ncol),
+                   cldtaulw=rnd(nl, ncol, hi=2.0), cldtausw=rnd(nl, ncol, hi=5.0), aerodp=rnd(6, ncol),
+                   mbota=torch.full((3, ncol), 10, dtype=torch.int32, device="cuda"),
+                   mtopa=torch.full((3, ncol), 40, dtype=torch.int32, device="cuda"))
+    for k in ("sfc_alb_nir_dir", "sfc_alb_nir_dif", "sfc_alb_uvvis_dir", "sfc_alb_uvvis_dif"):
+        post_in[k] = rnd(ncol, lo=0.05, hi=0.8)
+    post_out = dict(topflw=torch.zeros(ncol, 2, **f64), sfcflw=torch.zeros(ncol, 4, **f64),
+                    topfsw=torch.zeros(ncol, 3, **f64), sfcfsw=torch.zeros(ncol, 4, **f64),
+                    htrlwu=torch.zeros(nl, ncol, **f64), fluxr=torch.zeros(45, ncol, **f64))
+    for k in ("sfcdlw", "sfculw", "tsflw", "nirbmdi", "nirdfdi", "visbmdi", "visdfdi", "nirbmui", "nirdfui",
+              "visbmui", "visdfui", "sfcnsw", "sfcdsw"):
+        post_out[k] = torch.zeros(ncol, **f64)
+    post_ints = dict(nCol=ncol, nLev=nl, nDay=ncol, iSFC=1, iTOA=nv, doLWrad=1, doSWrad=1, save_diag=1,
+                     naerodp=6, nfxr=45)
+    post_reals = dict(fhlwr=3600.0, fhswr=3600.0, raddt=1800.0)
+    legs = {
+        "prepare_inputs": (lambda: lw.prepare_inputs_dev(ncol, nl, dev["p_lev"], dev["p_lay"], dev["t_lay"], q, o3,
+                                                         dev["tsfg"], prep, stream=sp), 8 * (9 * nl + 3 * nv + 1)),
+        "layer_geometry": (lambda: lib.layer_geometry_dev(ncol, nl, prep["p_lev"], prep["p_lay"], prep["t_lay"],
+                                                          prep["q_lay"], geo["tv"], geo["dz"], geo["dzc"], geo["dp"],
+                                                          stream=sp), 8 * (7 * nl + nv)),
+        "cloud_overlap": (lambda: lib.cloud_overlap_dev(ovl_ints, ovl_reals, {**ovl_in, **ovl_out}, stream=sp),
+                          8 * (6 * nl + 2)),
+        "post": (lambda: lib.post_dev(post_ints, post_reals, {**post_in, **post_out}, stream=sp), None),
+    }
+    res = {}
+    reps = max(a.steps, 3)
+    for name, (fn, bpc) in legs.items():
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        r = {"ms": ms}
+        if bpc is not None:
+            gbs = bpc * ncol / (ms * 1e-3) / 1e9
+            r.update(algorithmic_bytes_per_column=bpc, achieved_GBs=gbs, frac_of_hbm_peak=gbs / HBM_PEAK_GBS)
+        res[name] = r
+    print("# callers: " + ", ".join(f"{k} {v['ms']:.3f} ms" for k, v in res.items()), file=sys.stderr, flush=True)
+    return {"columns": ncol, "layers": nl, "kernels": res,
+            "note": "thread per column (prepare, geometry, post) or per (column, layer) (overlap), FP64"}
 
 
 def main():
@@ -279,6 +361,8 @@ def main():
                "kernel_ms_per_step": skt,
                "solver": "two-stream adding with cloud scattering (all sky), no-scattering clear sky off"}
 
+    callers = None if a.no_callers else time_callers(a, torch, lw, dev, ncol)
+
     # ---- CPU baseline: FP64 oracle on a bounded sample, and GPU error on it
     cpu = None
     acc = None
@@ -350,7 +434,7 @@ def main():
                                f"FP32 optics/transport, FP64 reductions and heating rate)",
                    "ncol": ncol, "nlay": NLAY, "path": "table", "blksz": blksz},
         "roofline": roof, "cpu_baseline": cpu, "accuracy_vs_fp64_oracle": acc, "nn": nn,
-        "lw_scattering": lws,
+        "lw_scattering": lws, "callers": callers,
     }
     print(json.dumps(line), flush=True)
     if a.copy_probe:

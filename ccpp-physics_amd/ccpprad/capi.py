@@ -97,12 +97,37 @@ class PostArgs(C.Structure):
     _fields_ = _POST_FIELDS
 
 
+_OVL_FIELDS = (
+    [(n, C.c_int) for n in ("nCol", "nLev", "yearlen", "doSWrad", "doLWrad", "literal_r8",
+                            "idcor", "idcor_con", "idcor_hogan", "idcor_oreopoulos",
+                            "iovr", "iovr_convcld", "iovr_dcorr", "iovr_exp", "iovr_exprand",
+                            "imfdeepcnv", "imfdeepcnv_gf", "imfdeepcnv_samf")]
+    + [(n, C.c_double) for n in ("julian", "con_pi", "dcorr_con")]
+    + [(n, P_D) for n in ("lat", "deltaZc", "cld_frac", "cld_cnv_frac", "de_lgth", "cloud_overlap_param",
+                          "cnv_cloud_overlap_param", "precip_overlap_param")])
+
+
+class OverlapArgs(C.Structure):
+    """ccpprad_cloud_overlap_args (GFS_rrtmgp_cloud_overlap_run)."""
+    _fields_ = _OVL_FIELDS
+
+
 def post_struct(ints: dict, reals: dict, arrays: dict) -> Holder:
     """Fill a PostArgs from ints, reals (fhlwr, fhswr, raddt) and arrays
     (numpy or torch; missing names become NULL)."""
-    s = PostArgs()
+    return _fill(PostArgs, ints, reals, arrays)
+
+
+def overlap_struct(ints: dict, reals: dict, arrays: dict) -> Holder:
+    """Fill an OverlapArgs from ints, reals (julian, con_pi, dcorr_con) and
+    arrays (numpy or torch; missing names become NULL)."""
+    return _fill(OverlapArgs, ints, reals, arrays)
+
+
+def _fill(cls, ints: dict, reals: dict, arrays: dict) -> Holder:
+    s = cls()
     keep = []
-    for name, typ in PostArgs._fields_:
+    for name, typ in cls._fields_:
         if typ is C.c_int:
             setattr(s, name, int(ints.get(name, 0)))
         elif typ is C.c_double:

@@ -23,7 +23,8 @@ EXPORTS = ["ccpprad_lw_init", "ccpprad_sw_init", "ccpprad_lw_init_files", "ccppr
            "ccpprad_lw_run", "ccpprad_sw_run", "ccpprad_lw_run_dev", "ccpprad_sw_run_dev",
            "ccpprad_finalize", "ccpprad_heating_rate_dev", "ccpprad_lw_gas_optics_dev",
            "ccpprad_sw_gas_optics_dev", "ccpprad_ngpt", "ccpprad_nband", "ccpprad_version",
-           "ccpprad_prepare_inputs_dev", "ccpprad_post_dev"]
+           "ccpprad_prepare_inputs_dev", "ccpprad_post_dev", "ccpprad_layer_geometry_dev",
+           "ccpprad_cloud_overlap_dev"]
 
 
 def load():
@@ -52,6 +53,9 @@ def load():
     L.ccpprad_sw_gas_optics_dev.argtypes = L.ccpprad_lw_gas_optics_dev.argtypes
     L.ccpprad_prepare_inputs_dev.argtypes = [vp, C.c_int, C.c_int] + [vp] * 14 + [vp, C.c_char_p, C.c_int]
     L.ccpprad_post_dev.argtypes = [vp, vp, C.c_char_p, C.c_int]
+    L.ccpprad_layer_geometry_dev.argtypes = ([C.c_int, C.c_int] + [C.c_double] * 3 + [vp] * 8 +
+                                             [vp, C.c_char_p, C.c_int])
+    L.ccpprad_cloud_overlap_dev.argtypes = [vp, vp, C.c_char_p, C.c_int]
     L.ccpprad_ngpt.argtypes = [vp]
     L.ccpprad_nband.argtypes = [vp]
     L.ccpprad_version.restype = C.c_char_p
@@ -176,6 +180,28 @@ def heating_rate_dev(ncol, nlay, up, dn, p_lev, htr, stream=None):
     L = load()
     buf, n = capi.errbuf()
     rc = L.ccpprad_heating_rate_dev(ncol, nlay, _dp(up), _dp(dn), _dp(p_lev), _dp(htr), stream, buf, n)
+    capi.check(rc, buf)
+
+
+def layer_geometry_dev(ncol, nlev, p_lev, p_lay, t_lay, q_lay, tv_lay, dz, dzc, dp, con_rd=287.05,
+                       con_g=9.80665, con_fvirt=0.60779, stream=None):
+    """GFS_rrtmgp_pre's virtual temperature and layer thicknesses on the
+    device (torch device tensors, Fortran (nCol, nLev[+1]) order; tv_lay, dz
+    and dp may be None)."""
+    L = load()
+    buf, n = capi.errbuf()
+    rc = L.ccpprad_layer_geometry_dev(ncol, nlev, con_rd, con_g, con_fvirt, _dp(p_lev), _dp(p_lay), _dp(t_lay),
+                                      _dp(q_lay), _dp(tv_lay), _dp(dz), _dp(dzc), _dp(dp), stream, buf, n)
+    capi.check(rc, buf)
+
+
+def cloud_overlap_dev(ints, reals, arrays, stream=None):
+    """GFS_rrtmgp_cloud_overlap_run on the device (arrays: torch device
+    tensors by ccpprad_cloud_overlap_args field name)."""
+    L = load()
+    h = capi.overlap_struct(ints, reals, arrays)
+    buf, n = capi.errbuf()
+    rc = L.ccpprad_cloud_overlap_dev(C.addressof(h.struct), stream, buf, n)
     capi.check(rc, buf)
 
 

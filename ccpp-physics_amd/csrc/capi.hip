@@ -824,7 +824,8 @@ extern "C" int ccpprad_heating_rate_dev(int nCol, int nLay, const double *flux_u
 }
 
 namespace ccpprad {
-void launch_post(const ccpprad_post_args &A, hipStream_t st);   // prep.hip
+void launch_post(const ccpprad_post_args &A, hipStream_t st);                // prep.hip
+void launch_cloud_overlap(const ccpprad_cloud_overlap_args &A, hipStream_t st);   // prep.hip
 }
 
 extern "C" int ccpprad_post_dev(const ccpprad_post_args *a, void *stream, char *errmsg, int errlen)
@@ -860,6 +861,59 @@ extern "C" int ccpprad_post_dev(const ccpprad_post_args *a, void *stream, char *
         }
     }
     launch_post(*a, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ccpprad_layer_geometry_dev(int nCol, int nLev, double con_rd, double con_g, double con_fvirt,
+                                          const double *p_lev, const double *p_lay, const double *t_lay,
+                                          const double *q_lay, double *tv_lay, double *deltaZ,
+                                          double *deltaZc, double *deltaP, void *stream, char *errmsg,
+                                          int errlen)
+{
+    if (errmsg && errlen > 0) errmsg[0] = 0;
+    if (nCol < 0 || nLev < 1) { seterr(errmsg, errlen, "bad dimensions"); return 1; }
+    if (nCol == 0) return 0;
+    if (!p_lev || !p_lay || !t_lay || !q_lay || !deltaZc) { seterr(errmsg, errlen, "NULL array"); return 1; }
+    if (!(con_g != 0.0)) { seterr(errmsg, errlen, "con_g must be nonzero"); return 1; }
+    GeomArgs A{};
+    A.ncol = nCol; A.nlev = nLev;
+    A.rdog = con_rd / con_g; A.fvirt = con_fvirt;
+    A.p_lev = p_lev; A.p_lay = p_lay; A.t_lay = t_lay; A.q_lay = q_lay;
+    A.tv_lay = tv_lay; A.dz = deltaZ; A.dzc = deltaZc; A.dp = deltaP;
+    launch_layer_geometry(A, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ccpprad_cloud_overlap_dev(const ccpprad_cloud_overlap_args *a, void *stream, char *errmsg, int errlen)
+{
+    if (errmsg && errlen > 0) errmsg[0] = 0;
+    if (!a) { seterr(errmsg, errlen, "null arguments"); return 1; }
+    if (a->nCol < 0 || a->nLev < 1) { seterr(errmsg, errlen, "bad dimensions"); return 1; }
+    if (a->nLev > 65535) { seterr(errmsg, errlen, "nLev above 65535"); return 1; }
+    if (a->nCol == 0 || !(a->doLWrad || a->doSWrad)) return 0;
+    if (!a->de_lgth || !a->cloud_overlap_param || !a->precip_overlap_param) {
+        seterr(errmsg, errlen, "NULL output array"); return 1;
+    }
+    auto expo = [&](int iovr) { return iovr == a->iovr_dcorr || iovr == a->iovr_exp || iovr == a->iovr_exprand; };
+    const bool cnv = a->imfdeepcnv == a->imfdeepcnv_samf || a->imfdeepcnv == a->imfdeepcnv_gf;
+    if ((a->idcor == a->idcor_hogan || a->idcor == a->idcor_oreopoulos) && !a->lat) {
+        seterr(errmsg, errlen, "the decorrelation length of idcor needs lat"); return 1;
+    }
+    if ((expo(a->iovr) || (cnv && expo(a->iovr_convcld))) && !a->deltaZc) {
+        seterr(errmsg, errlen, "an exponential overlap needs deltaZc"); return 1;
+    }
+    if (expo(a->iovr) && a->iovr == a->iovr_exprand && !a->cld_frac) {
+        seterr(errmsg, errlen, "exponential-random overlap needs cld_frac"); return 1;
+    }
+    if (cnv && !a->cnv_cloud_overlap_param) {
+        seterr(errmsg, errlen, "imfdeepcnv selects SAMF or GF: cnv_cloud_overlap_param is NULL"); return 1;
+    }
+    if (cnv && expo(a->iovr_convcld) && a->iovr_convcld == a->iovr_exprand && !a->cld_cnv_frac) {
+        seterr(errmsg, errlen, "exponential-random convective overlap needs cld_cnv_frac"); return 1;
+    }
+    launch_cloud_overlap(*a, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -60,6 +60,15 @@ struct PrepArgs {
 };
 void launch_prepare(const PrepArgs &A, hipStream_t st);
 
+// GFS_rrtmgp_pre's virtual temperature and layer thicknesses (prep.hip)
+struct GeomArgs {
+    int ncol, nlev;                                   // nlev = layers
+    double rdog, fvirt;                               // con_rd / con_g, con_fvirt
+    const double *p_lev, *p_lay, *t_lay, *q_lay;
+    double *tv_lay, *dz, *dzc, *dp;                   // tv_lay, dz, dp optional
+};
+void launch_layer_geometry(const GeomArgs &A, hipStream_t st);
+
 struct CloudArgs {
     int nlay, ncol_ld, col0, blksz, nband, is_lw;
     const int *col_idx;            // 1-based gather map (SW daylit) or null

@@ -203,4 +203,107 @@ void launch_post(const ccpprad_post_args &A, hipStream_t st)
     hipLaunchKernelGGL(post_kernel, dim3((A.nCol + 255) / 256), dim3(256), 0, st, A);
 }
 
+// GFS_rrtmgp_pre_run's virtual temperature and layer thicknesses
+// (GFS_rrtmgp_pre.F90:289,294-341): thread per column, one pass from the
+// surface up carrying the boundary height and the centre height of the layer
+// below, so each layer's thickness between centres is formed as it is
+// reached.  Heights accumulate from the surface in the reference's order.
+__global__ __launch_bounds__(256) void geometry_kernel(GeomArgs A)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= A.ncol) return;
+    const int nc = A.ncol, nl = A.nlev;
+    auto at = [&](int k) { return (size_t)c + (size_t)nc * k; };
+    const bool top = A.p_lev[0] < A.p_lev[(size_t)nc * (nl - 1)];   // from the first column
+    double hb_lo = 0.0, hc_lo = 0.0;   // boundary height below the layer, centre height of the layer below
+    for (int i = 0; i < nl; ++i) {
+        const int k = top ? nl - 1 - i : i;          // layer index, surface first
+        const int kb = top ? k + 1 : k, kt = top ? k : k + 1;   // its bottom and top level
+        const size_t q = at(k);
+        const double tv = A.t_lay[q] * (1.0 + A.fvirt * A.q_lay[q]);
+        const double lb = log(A.p_lev[at(kb)]), lt = log(A.p_lev[at(kt)]);
+        const double dz = A.rdog * fabs(lt - lb) * tv;
+        const double hb = hb_lo + dz;
+        const double pfac = fabs(lb - log(A.p_lay[q])) / fabs(lb - lt);
+        const double hc = hb_lo + pfac * (hb - hb_lo);
+        A.dzc[q] = hc - (i == 0 ? hb_lo : hc_lo);
+        if (A.tv_lay) A.tv_lay[q] = tv;
+        if (A.dz) A.dz[q] = dz;
+        if (A.dp) A.dp[q] = fabs(A.p_lev[at(k + 1)] - A.p_lev[q]);
+        hb_lo = hb;
+        hc_lo = hc;
+    }
+}
+
+void launch_layer_geometry(const GeomArgs &A, hipStream_t st)
+{
+    if (A.ncol <= 0) return;
+    hipLaunchKernelGGL(geometry_kernel, dim3((A.ncol + 255) / 256), dim3(256), 0, st, A);
+}
+
+// GFS_rrtmgp_cloud_overlap_run (GFS_rrtmgp_cloud_overlap.F90:97-137) with
+// cmp_dcorr_lgth and get_alpha_exper (radiation_cloud_overlap.F90:28-145):
+// thread per (column, layer), grid (columns / 256, layers).  Each thread
+// forms its column's decorrelation length (a handful of flops) instead of
+// reading it back, so the launch has no dependency between threads; the
+// layer-1 threads write it.
+__device__ __forceinline__ double dcorr_length(const ccpprad_cloud_overlap_args &A, int c)
+{
+    // each operation rounded as the reference writes it (no FMA), so the
+    // length is the reference's to the bit; unsuffixed literals of the
+    // reference take its default real's precision
+#pragma clang fp contract(off)
+    const bool r8 = A.literal_r8 != 0;
+    double L = 0.0;
+    if (A.idcor == A.idcor_hogan) {
+        const double lo = r8 ? 0.6 : (double)0.6f, a = r8 ? 2.78 : (double)2.78f, b = r8 ? 4.6 : (double)4.6f;
+        L = fmax(lo, a - b * fabs(A.lat[c] / A.con_pi));                                    // Hogan eq. 13
+    }
+    if (A.idcor == A.idcor_oreopoulos) {
+        constexpr double am1 = 1.4315, am2 = 2.1219, am4 = -25.584, amr = 7.0;
+        const double latd = A.lat[c] * (180.0 / A.con_pi);
+        const double am3 = A.julian > 181.0 ? (-4.0 * amr * (A.julian - 272.0)) / (double)A.yearlen   // eq. 11a
+                                            : (4.0 * amr * (A.julian - 91.0)) / (double)A.yearlen;    // eq. 11b
+        const double x = latd - am3;
+        L = am1 + am2 * exp(-(x * x) / (am4 * am4));                                        // eq. 10
+    }
+    if (A.idcor == A.idcor_con) L = A.dcorr_con;
+    return L;
+}
+
+__device__ __forceinline__ bool exponential_overlap(const ccpprad_cloud_overlap_args &A, int iovr)
+{
+    return iovr == A.iovr_dcorr || iovr == A.iovr_exp || iovr == A.iovr_exprand;
+}
+
+__global__ __launch_bounds__(256) void overlap_kernel(ccpprad_cloud_overlap_args A)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (c >= A.nCol) return;
+    const size_t q = (size_t)c + (size_t)A.nCol * k;
+    const double L = dcorr_length(A, c);
+    if (k == 0) A.de_lgth[c] = L;
+    const double km = A.literal_r8 ? 0.001 : (double)0.001f;   // deltaZc * 0.001: m -> km
+    auto alpha = [&](int iovr, const double *cf) {
+        if (k == 0) return 0.0;
+        double a = exp(-(A.deltaZc[q] * km) / L);
+        // exponential-random: a clear layer after a cloudy one decorrelates
+        if (iovr == A.iovr_exprand && cf[q] == 0.0 && cf[q - (size_t)A.nCol] > 0.0) a = 0.0;
+        return a;
+    };
+    const double a = exponential_overlap(A, A.iovr) ? alpha(A.iovr, A.cld_frac) : 0.0;
+    A.cloud_overlap_param[q] = a;
+    A.precip_overlap_param[q] = a;
+    if (A.imfdeepcnv == A.imfdeepcnv_samf || A.imfdeepcnv == A.imfdeepcnv_gf)
+        A.cnv_cloud_overlap_param[q] =
+            exponential_overlap(A, A.iovr_convcld) ? alpha(A.iovr_convcld, A.cld_cnv_frac) : 0.0;
+}
+
+void launch_cloud_overlap(const ccpprad_cloud_overlap_args &A, hipStream_t st)
+{
+    if (A.nCol <= 0 || A.nLev <= 0) return;
+    hipLaunchKernelGGL(overlap_kernel, dim3((A.nCol + 255) / 256, A.nLev), dim3(256), 0, st, A);
+}
+
 }  // namespace ccpprad

@@ -12,6 +12,10 @@
  *        (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_post.F90:173-389)
  *   ccpprad_prepare_inputs_dev         <- the radiation-input preparation of
  *        GFS_rrtmgp_pre_run (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_pre.F90:217-274,349-371)
+ *   ccpprad_layer_geometry_dev         <- the virtual temperature and layer thicknesses of
+ *        GFS_rrtmgp_pre_run (GFS_rrtmgp_pre.F90:289,294-341)
+ *   ccpprad_cloud_overlap_dev          <- GFS_rrtmgp_cloud_overlap_run
+ *        (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_cloud_overlap.F90:31-138)
  *   ccpprad_*_gas_optics_dev           <- ty_gas_optics_rrtmgp%gas_optics
  *        (call sites rrtmgp_lw_main.F90:337-345, rrtmgp_sw_main.F90:346-353)
  *
@@ -243,6 +247,57 @@ int ccpprad_prepare_inputs_dev(const ccpprad_ctx *ctx, int nCol, int nLev, const
                                double *p_lev, double *t_lay, double *t_lev, double *q_lay,
                                double *vmr_h2o, double *vmr_o3, int *top_at_1, void *stream,
                                char *errmsg, int errlen);
+
+/* The virtual temperature and layer thicknesses of GFS_rrtmgp_pre_run
+ * (GFS_rrtmgp_pre.F90:289,294-341), FP64 on the device, from the prepared
+ * fields (bounded t_lay, floored q_lay, p_lev = prsi):
+ *   tv_lay  = t_lay (1 + con_fvirt q_lay)
+ *   deltaP  = |p_lev(k+1) - p_lev(k)|
+ *   deltaZ  = con_rd/con_g |ln p_lev(k+1) - ln p_lev(k)| tv_lay       (m)
+ *   deltaZc = height of the layer centre above the centre of the layer
+ *             below it (above the surface for the lowest layer); the centre
+ *             lies at the fraction |ln p_lev(bottom) - ln p_lay| /
+ *             |ln p_lev(bottom) - ln p_lev(top)| of the layer's thickness.
+ * The vertical order comes from the first column (p_lev(1,1) < p_lev(1,nLev)
+ * means the top is at index 1), as in the reference.  nLev = layers; fields
+ * (nCol, nLev), p_lev (nCol, nLev+1).  tv_lay, deltaZ and deltaP may be NULL
+ * (deltaZc is always written).  Device pointers; ordered on stream. */
+int ccpprad_layer_geometry_dev(int nCol, int nLev, double con_rd, double con_g, double con_fvirt,
+                               const double *p_lev, const double *p_lay, const double *t_lay,
+                               const double *q_lay, double *tv_lay, double *deltaZ,
+                               double *deltaZc, double *deltaP, void *stream, char *errmsg,
+                               int errlen);
+
+/* GFS_rrtmgp_cloud_overlap_run (GFS_rrtmgp_cloud_overlap.F90:31-138) with
+ * cmp_dcorr_lgth_hogan / _oreopoulos and get_alpha_exper
+ * (physics/Radiation/radiation_cloud_overlap.F90:28-145), FP64 on the device.
+ * Nothing is written when neither doSWrad nor doLWrad.  de_lgth (nCol) is 0
+ * unless idcor selects Hogan et al. (2010), Oreopoulos et al. (2012) or the
+ * constant dcorr_con; cloud_overlap_param (nCol, nLev) is
+ * exp(-deltaZc 0.001 / de_lgth) from the second layer on (0 in layer 1) for
+ * iovr in {iovr_dcorr, iovr_exp, iovr_exprand}, with exponential-random
+ * setting 0 where a clear layer follows a cloudy one, else 0;
+ * cnv_cloud_overlap_param is computed the same way from cld_cnv_frac and
+ * iovr_convcld, and written only when imfdeepcnv is imfdeepcnv_samf or
+ * imfdeepcnv_gf; precip_overlap_param = cloud_overlap_param.
+ * The reference's unsuffixed constants (0.6, 2.78 and 4.6 of the Hogan fit,
+ * the 0.001 m -> km factor) take the precision of its build's default real:
+ * literal_r8 = 1 for physics compiled with default-real-8 (UFS and the SCM),
+ * 0 for the compiler's default single-precision literals.  The reference's
+ * p_lev, p_lay, tv_lay, con_g, con_rd, con_epsq and top_at_1 arguments are
+ * unused by it and omitted.  lat in radians; julian day; yearlen in days. */
+typedef struct ccpprad_cloud_overlap_args {
+    int nCol, nLev, yearlen, doSWrad, doLWrad, literal_r8;
+    int idcor, idcor_con, idcor_hogan, idcor_oreopoulos;
+    int iovr, iovr_convcld, iovr_dcorr, iovr_exp, iovr_exprand;
+    int imfdeepcnv, imfdeepcnv_gf, imfdeepcnv_samf;
+    double julian, con_pi, dcorr_con;
+    const double *lat, *deltaZc, *cld_frac, *cld_cnv_frac;
+    double *de_lgth, *cloud_overlap_param, *cnv_cloud_overlap_param, *precip_overlap_param;
+} ccpprad_cloud_overlap_args;
+
+int ccpprad_cloud_overlap_dev(const ccpprad_cloud_overlap_args *args, void *stream, char *errmsg,
+                              int errlen);
 
 /* ty_gas_optics_rrtmgp%gas_optics, LW (rrtmgp_lw_main.F90:337-345).  Device
  * pointers.  vmr is (nCol, nLay, ngas) in the k-distribution's gas order.

@@ -45,6 +45,8 @@ def lib():
         L.orc_prepare.argtypes = [C.c_int, C.c_int] + [C.c_double] * 4 + [P_D] * 13
         L.orc_prepare.restype = C.c_int
         L.orc_post.argtypes = [C.c_void_p]
+        L.orc_cloud_overlap.argtypes = [C.c_void_p]
+        L.orc_layer_geometry.argtypes = [C.c_int, C.c_int] + [C.c_double] * 3 + [P_D] * 8
         L.orc_gauss.argtypes = [C.c_int, P_D, P_D]
         L.orc_heating_rate.argtypes = [C.c_int, C.c_int, P_D, P_D, P_D, P_D]
         L.orc_mt_draws.argtypes = [C.c_int, C.c_int, P_D]
@@ -77,8 +79,31 @@ def ref_lib():
         R.ref_mt_draws.argtypes = [C.c_int, C.c_int, P_D]
         R.ref_mt_draws_chunked.argtypes = [C.c_int, C.c_int, C.c_int, P_D]
         R.ref_cmp_tlev.argtypes = [C.c_int, C.c_int, C.c_double, P_D, P_D, P_D, P_D, P_D]
+        R.ref_cloud_overlap.argtypes = _REF_OVL_ARGTYPES
         _ref = R
     return _ref
+
+
+# ref_cloud_overlap(ncol, nlev, yearlen, julian, con_pi, dcorr_con, idcor,
+# idcor_con, idcor_hogan, idcor_oreopoulos, iovr, iovr_convcld, iovr_dcorr,
+# iovr_exp, iovr_exprand, imfdeepcnv, imfdeepcnv_gf, imfdeepcnv_samf, lat,
+# deltaZc, cld_frac, cld_cnv_frac, de_lgth, cloud, cnv, precip)
+_REF_OVL_ARGTYPES = [C.c_int] * 3 + [C.c_double] * 3 + [C.c_int] * 12 + [P_D] * 8
+_ref_r8 = None
+
+
+def ref_lib_r8():
+    """The reference's cloud-overlap scheme built with default-real-8
+    (`make -C oracle ref`, _ref/r8/); None when not built."""
+    global _ref_r8
+    if _ref_r8 is None:
+        path = os.path.join(_HERE, "_ref", "r8", "libccpp_ref_r8.so")
+        if not os.path.exists(path):
+            return None
+        R = C.CDLL(path)
+        R.ref_cloud_overlap.argtypes = _REF_OVL_ARGTYPES
+        _ref_r8 = R
+    return _ref_r8
 
 
 def _p(a, t=P_D):
@@ -183,6 +208,46 @@ def post(ints, reals, arrays):
     """GFS_rrtmgp_post's bookkeeping on host arrays (updated in place)."""
     h = capi.post_struct(ints, reals, arrays)
     lib().orc_post(h.ref())
+
+
+def layer_geometry(p_lev, p_lay, t_lay, q_lay, con_rd=287.05, con_g=9.80665, con_fvirt=0.60779):
+    """GFS_rrtmgp_pre's tv_lay, deltaZ, deltaZc, deltaP from (nlev[+1], ncol)
+    C-order arrays (= Fortran (nCol, nLev[+1])); returns them the same way."""
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    nl, nc = np.shape(t_lay)
+    tv, dz, dzc, dp = (np.zeros((nl, nc)) for _ in range(4))
+    lib().orc_layer_geometry(nc, nl, con_rd, con_g, con_fvirt, _p(c(p_lev)), _p(c(p_lay)), _p(c(t_lay)),
+                             _p(c(q_lay)), _p(tv), _p(dz), _p(dzc), _p(dp))
+    return tv, dz, dzc, dp
+
+
+def cloud_overlap(ints, reals, arrays):
+    """GFS_rrtmgp_cloud_overlap_run on host arrays (outputs updated in place)."""
+    h = capi.overlap_struct(ints, reals, arrays)
+    lib().orc_cloud_overlap(h.ref())
+
+
+def ref_cloud_overlap(ints, reals, arrays, r8=False):
+    """The reference's own GFS_rrtmgp_cloud_overlap_run (built from its
+    sources into _ref/); the outputs in arrays are updated in place.  None
+    when that build is absent."""
+    R = ref_lib_r8() if r8 else ref_lib()
+    if R is None:
+        return None
+    g = lambda k: int(ints.get(k, 0))
+    nc, nl = g("nCol"), g("nLev")
+    zero = np.zeros((nl, nc))
+    a = {k: arrays.get(k, zero) for k in ("lat", "deltaZc", "cld_frac", "cld_cnv_frac")}
+    if "lat" not in arrays:
+        a["lat"] = np.zeros(nc)
+    R.ref_cloud_overlap(nc, nl, g("yearlen"), float(reals.get("julian", 0.0)), float(reals["con_pi"]),
+                        float(reals.get("dcorr_con", 0.0)), g("idcor"), g("idcor_con"), g("idcor_hogan"),
+                        g("idcor_oreopoulos"), g("iovr"), g("iovr_convcld"), g("iovr_dcorr"), g("iovr_exp"),
+                        g("iovr_exprand"), g("imfdeepcnv"), g("imfdeepcnv_gf"), g("imfdeepcnv_samf"),
+                        *(_p(np.ascontiguousarray(a[k])) for k in ("lat", "deltaZc", "cld_frac", "cld_cnv_frac")),
+                        *(_p(arrays[k]) for k in ("de_lgth", "cloud_overlap_param", "cnv_cloud_overlap_param",
+                                                  "precip_overlap_param")))
+    return True
 
 
 def optimal_angles(kd, ncol, nlay, tau):

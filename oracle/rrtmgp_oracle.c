@@ -546,6 +546,116 @@ int orc_prepare(int ncol, int nlev, double pmin, double pmax, double tmin, doubl
  * reference's loops.  Pinned by hand cases (tests/test_oracle_prepare.py):
  * it is data movement and time-weighted sums, with no arithmetic of its own
  * beyond them. */
+/* GFS_rrtmgp_pre_run's virtual temperature and layer thicknesses
+ * (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_pre.F90:289,294-341), in
+ * the reference's loop structure: per column, deltaZ, the boundary heights
+ * from the surface, the centre heights, then the thicknesses between
+ * centres.  No FMA contraction, so the statements round as written. */
+__attribute__((optimize("fp-contract=off")))
+void orc_layer_geometry(int ncol, int nlev, double con_rd, double con_g, double con_fvirt,
+                        const double *p_lev, const double *p_lay, const double *t_lay,
+                        const double *q_lay, double *tv_lay, double *dz, double *dzc, double *dp)
+{
+#define L2(p, i, k) (p)[(size_t)(i) + (size_t)ncol * ((k) - 1)]
+    const double rdog = con_rd / con_g;
+    const int top_at_1 = p_lev[0] < p_lev[(size_t)ncol * (nlev - 1)];
+    double *hgtb = (double *)malloc(sizeof(double) * (nlev + 2));
+    double *hgtc = (double *)malloc(sizeof(double) * (nlev + 1));
+    for (int k = 1; k <= nlev; ++k)
+        for (int i = 0; i < ncol; ++i) {
+            L2(tv_lay, i, k) = L2(t_lay, i, k) * (1.0 + con_fvirt * L2(q_lay, i, k));
+            L2(dp, i, k) = fabs(L2(p_lev, i, k + 1) - L2(p_lev, i, k));
+        }
+    for (int i = 0; i < ncol; ++i) {
+        if (top_at_1) {
+            for (int k = 1; k <= nlev; ++k)
+                L2(dz, i, k) = rdog * fabs(log(L2(p_lev, i, k + 1)) - log(L2(p_lev, i, k))) * L2(tv_lay, i, k);
+            hgtb[nlev + 1] = 0.0;
+            for (int k = nlev; k >= 1; --k) hgtb[k] = hgtb[k + 1] + L2(dz, i, k);
+            for (int k = nlev; k >= 1; --k) {
+                const double pfac = fabs(log(L2(p_lev, i, k + 1)) - log(L2(p_lay, i, k))) /
+                                    fabs(log(L2(p_lev, i, k + 1)) - log(L2(p_lev, i, k)));
+                hgtc[k] = hgtb[k + 1] + pfac * (hgtb[k] - hgtb[k + 1]);
+            }
+            for (int k = nlev - 1; k >= 1; --k) L2(dzc, i, k) = hgtc[k] - hgtc[k + 1];
+            L2(dzc, i, nlev) = hgtc[nlev] - hgtb[nlev + 1];
+        } else {
+            for (int k = nlev; k >= 1; --k)
+                L2(dz, i, k) = rdog * fabs(log(L2(p_lev, i, k)) - log(L2(p_lev, i, k + 1))) * L2(tv_lay, i, k);
+            hgtb[1] = 0.0;
+            for (int k = 1; k <= nlev; ++k) hgtb[k + 1] = hgtb[k] + L2(dz, i, k);
+            for (int k = 1; k <= nlev; ++k) {
+                const double pfac = fabs(log(L2(p_lev, i, k)) - log(L2(p_lay, i, k))) /
+                                    fabs(log(L2(p_lev, i, k)) - log(L2(p_lev, i, k + 1)));
+                hgtc[k] = hgtb[k] + pfac * (hgtb[k + 1] - hgtb[k]);
+            }
+            for (int k = 2; k <= nlev; ++k) L2(dzc, i, k) = hgtc[k] - hgtc[k - 1];
+            L2(dzc, i, 1) = hgtc[1] - hgtb[1];
+        }
+    }
+    free(hgtb);
+    free(hgtc);
+#undef L2
+}
+
+/* GFS_rrtmgp_cloud_overlap_run (GFS_rrtmgp_cloud_overlap.F90:97-137) with
+ * cmp_dcorr_lgth_hogan (radiation_cloud_overlap.F90:28-50),
+ * cmp_dcorr_lgth_oreopoulos (:58-95) and get_alpha_exper (:100-143).  The
+ * unsuffixed literals take the precision literal_r8 selects. */
+__attribute__((optimize("fp-contract=off")))
+void orc_cloud_overlap(const ccpprad_cloud_overlap_args *a)
+{
+    const int nc = a->nCol, nl = a->nLev;
+    const int r8 = a->literal_r8 != 0;
+    if (!(a->doSWrad || a->doLWrad)) return;
+    for (int i = 0; i < nc; ++i) a->de_lgth[i] = 0.0;
+    if (a->idcor == a->idcor_hogan) {
+        const double min_dcorr = r8 ? 0.6 : (double)0.6f;
+        const double c1 = r8 ? 2.78 : (double)2.78f, c2 = r8 ? 4.6 : (double)4.6f;
+        for (int i = 0; i < nc; ++i) {
+            const double v = c1 - c2 * fabs(a->lat[i] / a->con_pi);
+            a->de_lgth[i] = min_dcorr > v ? min_dcorr : v;   /* max(min_dcorr, v) */
+        }
+    }
+    if (a->idcor == a->idcor_oreopoulos) {
+        const double am1 = 1.4315, am2 = 2.1219, am4 = -25.584, amr = 7.0;
+        const double f = 180.0 / a->con_pi;
+        for (int i = 0; i < nc; ++i) {
+            const double latd = a->lat[i] * f;
+            double am3;
+            if (a->julian > 181.0) am3 = -4.0 * amr * (a->julian - 272.0) / a->yearlen;
+            else am3 = 4.0 * amr * (a->julian - 91.0) / a->yearlen;
+            const double x = latd - am3;
+            a->de_lgth[i] = am1 + am2 * exp(-(x * x) / (am4 * am4));
+        }
+    }
+    if (a->idcor == a->idcor_con)
+        for (int i = 0; i < nc; ++i) a->de_lgth[i] = a->dcorr_con;
+    const double km = r8 ? 0.001 : (double)0.001f;
+#define OV(p, i, k) (p)[(size_t)(i) + (size_t)nc * ((k) - 1)]
+    /* get_alpha_exper, for the clouds and then the convective clouds */
+    for (int pass = 0; pass < 2; ++pass) {
+        const int iovr = pass == 0 ? a->iovr : a->iovr_convcld;
+        const double *cf = pass == 0 ? a->cld_frac : a->cld_cnv_frac;
+        double *out = pass == 0 ? a->cloud_overlap_param : a->cnv_cloud_overlap_param;
+        if (pass == 1 && !(a->imfdeepcnv == a->imfdeepcnv_samf || a->imfdeepcnv == a->imfdeepcnv_gf)) break;
+        if (iovr == a->iovr_dcorr || iovr == a->iovr_exp || iovr == a->iovr_exprand) {
+            for (int i = 0; i < nc; ++i) {
+                OV(out, i, 1) = 0.0;
+                for (int k = 2; k <= nl; ++k) OV(out, i, k) = exp(-(OV(a->deltaZc, i, k) * km) / a->de_lgth[i]);
+            }
+            if (iovr == a->iovr_exprand)
+                for (int k = 2; k <= nl; ++k)
+                    for (int i = 0; i < nc; ++i)
+                        if (OV(cf, i, k) == 0.0 && OV(cf, i, k - 1) > 0.0) OV(out, i, k) = 0.0;
+        } else {
+            for (size_t j = 0; j < (size_t)nc * nl; ++j) out[j] = 0.0;
+        }
+    }
+    for (size_t j = 0; j < (size_t)nc * nl; ++j) a->precip_overlap_param[j] = a->cloud_overlap_param[j];
+#undef OV
+}
+
 void orc_post(const ccpprad_post_args *a)
 {
     const int nc = a->nCol, nl = a->nLev;
