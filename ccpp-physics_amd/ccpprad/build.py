@@ -17,7 +17,7 @@ F_LIB = os.path.join(LIBDIR, "libccpprad_f.so")
 F_DRIVER = os.path.join(LIBDIR, "ccpp_cap_driver")
 
 HIP_SOURCES = ["capi.hip", "host.hip", "gas_optics.hip", "lw_solver.hip", "lw_scat.hip", "sw_solver.hip",
-               "clouds.hip", "util.hip", "mlp.hip", "prep.hip"]
+               "clouds.hip", "util.hip", "mlp.hip", "prep.hip", "nc_tables.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 # Packed FP32 instructions are disabled for all device code.

@@ -95,6 +95,23 @@ class _Scheme:
         capi.check(rc, buf)
         self.ctx = ctx
 
+    @classmethod
+    def from_files(cls, gas_file, cloud_file, nlay: int, blksz: int = 1, gas_names=syn.GAS_NAMES):
+        """rrtmgp_*_main_init from table files: the reference's netCDF files
+        (classic formats) or this package's binary table files."""
+        self = cls.__new__(cls)
+        self.L = load()
+        self.kd, self.lut, self.nlay = None, None, nlay
+        names = _gas_names(list(gas_names))
+        ctx = C.c_void_p()
+        buf, n = capi.errbuf()
+        fn = self.L.ccpprad_lw_init_files if cls.is_lw else self.L.ccpprad_sw_init_files
+        rc = fn(str(gas_file).encode(), (str(cloud_file) if cloud_file else "").encode(), names, len(gas_names),
+                nlay, blksz, C.byref(ctx), buf, n)
+        capi.check(rc, buf)
+        self.ctx = ctx
+        return self
+
     @property
     def ngpt(self):
         return self.L.ccpprad_ngpt(self.ctx)

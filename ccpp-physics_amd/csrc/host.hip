@@ -16,11 +16,18 @@
 #include <map>
 
 #include "../../include/ccpprad.h"
+#include "nc_tables.h"
 
 extern "C" int ccpprad_ctx_stage(ccpprad_ctx *c, void *stream, int slot, size_t bytes, void **p);
 extern "C" void *ccpprad_ctx_host_stream(ccpprad_ctx *c);
 
 namespace {
+
+using ccpprad::NcKdistHold;
+using ccpprad::NcLutHold;
+using ccpprad::is_netcdf_file;
+using ccpprad::load_kdist_nc;
+using ccpprad::load_lut_nc;
 
 enum Shape { S_COL, S_LAY, S_LEV, S_BAND, S_SCMP, S_DAY };
 struct Field {
@@ -245,20 +252,9 @@ bool fill_minor(RecMap &m, const char *pre, int neta, int ntemp, ccpprad_minor &
     return ok;
 }
 
-int init_files(int lw, const char *gas_file, const char *cloud_file, const char *const *gas_names, int ngases,
-               int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg, int errlen)
+// The binary table file's k-distribution records into h.kd.
+bool kdist_from_records(TableHold &h)
 {
-    TableHold h;
-    std::string err;
-    if (!gas_file || !read_file(gas_file, h.k, err)) {
-        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "%s", gas_file ? err.c_str() : "null gas file");
-        return 1;
-    }
-    bool have_lut = cloud_file && cloud_file[0];
-    if (have_lut && !read_file(cloud_file, h.l, err)) {
-        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "%s", err.c_str());
-        return 1;
-    }
     bool ok = true;
     ccpprad_kdist &k = h.kd;
     k.is_lw = geti(h.k, "is_lw", &ok);
@@ -291,26 +287,65 @@ int init_files(int lw, const char *gas_file, const char *cloud_file, const char 
             k.solar_src = pd(h.k, "solar_src", &ok, k.ngpt);
         }
     }
-    if (ok && have_lut) {
-        ccpprad_cloud_lut &L = h.lut;
-        L.nband = geti(h.l, "nband", &ok); L.nsize_liq = geti(h.l, "nsize_liq", &ok);
-        L.nsize_ice = geti(h.l, "nsize_ice", &ok); L.nrghice = geti(h.l, "nrghice", &ok);
-        L.icergh = geti(h.l, "icergh", &ok);
-        L.radliq_lwr = getd(h.l, "radliq_lwr", &ok); L.radliq_upr = getd(h.l, "radliq_upr", &ok);
-        L.radice_lwr = getd(h.l, "radice_lwr", &ok); L.radice_upr = getd(h.l, "radice_upr", &ok);
-        if (ok && (L.nband <= 0 || L.nsize_liq < 2 || L.nsize_ice < 2 || L.nrghice < 1)) ok = false;
-        if (ok) {
-            const size_t nl = (size_t)L.nsize_liq * L.nband, ni = (size_t)L.nsize_ice * L.nband * L.nrghice;
-            L.extliq = pd(h.l, "extliq", &ok, nl); L.ssaliq = pd(h.l, "ssaliq", &ok, nl); L.asyliq = pd(h.l, "asyliq", &ok, nl);
-            L.extice = pd(h.l, "extice", &ok, ni); L.ssaice = pd(h.l, "ssaice", &ok, ni); L.asyice = pd(h.l, "asyice", &ok, ni);
+    return ok;
+}
+
+// The binary table file's cloud records into h.lut.
+bool lut_from_records(TableHold &h)
+{
+    bool ok = true;
+    ccpprad_cloud_lut &L = h.lut;
+    L.nband = geti(h.l, "nband", &ok); L.nsize_liq = geti(h.l, "nsize_liq", &ok);
+    L.nsize_ice = geti(h.l, "nsize_ice", &ok); L.nrghice = geti(h.l, "nrghice", &ok);
+    L.icergh = geti(h.l, "icergh", &ok);
+    L.radliq_lwr = getd(h.l, "radliq_lwr", &ok); L.radliq_upr = getd(h.l, "radliq_upr", &ok);
+    L.radice_lwr = getd(h.l, "radice_lwr", &ok); L.radice_upr = getd(h.l, "radice_upr", &ok);
+    if (ok && (L.nband <= 0 || L.nsize_liq < 2 || L.nsize_ice < 2 || L.nrghice < 1)) ok = false;
+    if (ok) {
+        const size_t nl = (size_t)L.nsize_liq * L.nband, ni = (size_t)L.nsize_ice * L.nband * L.nrghice;
+        L.extliq = pd(h.l, "extliq", &ok, nl); L.ssaliq = pd(h.l, "ssaliq", &ok, nl); L.asyliq = pd(h.l, "asyliq", &ok, nl);
+        L.extice = pd(h.l, "extice", &ok, ni); L.ssaice = pd(h.l, "ssaice", &ok, ni); L.asyice = pd(h.l, "asyice", &ok, ni);
+    }
+    return ok;
+}
+
+// Tables from the k-distribution and cloud files named at _init: the
+// reference's netCDF files (read natively, classic formats) or this
+// package's binary table files, each file's format taken from its signature.
+int init_files(int lw, const char *gas_file, const char *cloud_file, const char *const *gas_names, int ngases,
+               int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg, int errlen)
+{
+    TableHold h;
+    NcKdistHold kh;
+    NcLutHold lh;
+    std::string err;
+    auto fail = [&](const char *file, const std::string &e) {
+        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "%s: %s", file ? file : "(null)", e.c_str());
+        return 1;
+    };
+    if (!gas_file) return fail(gas_file, "null gas file");
+    const ccpprad_kdist *kd = nullptr;
+    const ccpprad_cloud_lut *lut = nullptr;
+    if (is_netcdf_file(gas_file)) {
+        if (!load_kdist_nc(gas_file, lw, gas_names, ngases, kh, err)) return fail(gas_file, err);
+        kd = &kh.kd;
+    } else {
+        if (!read_file(gas_file, h.k, err)) return fail(gas_file, err);
+        if (!kdist_from_records(h)) return fail(gas_file, "table file lacks a required record or has a bad size");
+        kd = &h.kd;
+    }
+    if (cloud_file && cloud_file[0]) {
+        if (is_netcdf_file(cloud_file)) {
+            if (!load_lut_nc(cloud_file, lh, err)) return fail(cloud_file, err);
+            lut = &lh.lut;
+        } else {
+            if (!read_file(cloud_file, h.l, err)) return fail(cloud_file, err);
+            if (!lut_from_records(h)) return fail(cloud_file, "table file lacks a required record or has a bad size");
+            lut = &h.lut;
         }
     }
-    if (!ok) {
-        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "table file lacks a required record or has a bad size");
-        return 1;
-    }
-    return lw ? ccpprad_lw_init(&h.kd, have_lut ? &h.lut : nullptr, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen)
-              : ccpprad_sw_init(&h.kd, have_lut ? &h.lut : nullptr, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen);
+    return lw ? ccpprad_lw_init(kd, lut, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen)
+              : ccpprad_sw_init(kd, lut, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen);
 }
 }  // namespace
 
