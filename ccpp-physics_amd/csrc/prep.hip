@@ -38,15 +38,22 @@ __global__ __launch_bounds__(256) void prepare_kernel(PrepArgs A)
         if (t >= A.temp_max) t = A.temp_max - DBL_EPSILON;
         return t;
     };
+    // one pass over the layers: the bounded layer fields, the mixing ratios,
+    // p_lev, and cmp_tlev's level between this layer and the previous one
+    // (linear in log p between the layer centres), carrying the previous
+    // layer's log pressure and temperature
+    double la_prev = 0.0, t_prev = 0.0;
     for (int k = 0; k < nl; ++k) {
         const size_t q = at(k);
         double t = A.tgrs[q], p = A.prsl[q];
+        const double pv = A.prsi[q];
         if (t <= A.temp_min) t = A.temp_min + DBL_EPSILON;
         if (p <= A.press_min) p = A.press_min + DBL_EPSILON;
         if (t >= A.temp_max) t = A.temp_max - DBL_EPSILON;
         if (p >= A.press_max) p = A.press_max - DBL_EPSILON;
         A.t_lay[q] = t;
         A.p_lay[q] = p;
+        A.p_lev[q] = pv;
         double qv = A.qh2o[q];
         if (qv < 1.0e-6) qv = 1.0e-6;
         if (A.q_lay) A.q_lay[q] = qv;
@@ -55,33 +62,24 @@ __global__ __launch_bounds__(256) void prepare_kernel(PrepArgs A)
             const double o = fmax(CON_EPSQS, A.qo3[q]);
             A.vmr_o3[q] = o > 0.0 ? o * AMDO3 : 0.0;
         }
-    }
-    for (int k = 0; k <= nl; ++k) A.p_lev[at(k)] = A.prsi[at(k)];
-    // cmp_tlev on the bounded layer fields: linear in log p between the
-    // layer centres; the level beyond the last layer takes the surface or
-    // top layer temperature
-    if (top) {
-        A.t_lev[at(0)] = bound_t(A.t_lay[at(0)]);
-        double la_up = log(A.p_lay[at(0)]), t_up = A.t_lay[at(0)];
-        for (int k = 1; k < nl; ++k) {
-            const double la = log(A.p_lay[at(k)]), t = A.t_lay[at(k)];
-            const double lb = log(A.prsi[at(k)]);
-            A.t_lev[at(k)] = bound_t(t + (t_up - t) * (lb - la) / (la_up - la));
-            la_up = la; t_up = t;
+        const double la = log(p);
+        if (k > 0) {
+            if (top) {
+                // the layer above is the previous one
+                A.t_lev[q] = bound_t(t + (t_prev - t) * (log(pv) - la) / (la_prev - la));
+            } else {
+                const double lb = log(k == nl - 1 ? fmax(A.press_min, pv) : pv);
+                A.t_lev[q] = bound_t(t_prev + (t - t_prev) * (lb - la_prev) / (la - la_prev));
+            }
         }
-        A.t_lev[at(nl)] = bound_t(A.tsfc[c]);
-    } else {
-        A.t_lev[at(0)] = bound_t(A.tsfc[c]);
-        double la_lo = log(A.p_lay[at(0)]), t_lo = A.t_lay[at(0)];
-        for (int k = 1; k < nl; ++k) {
-            const double la = log(A.p_lay[at(k)]), t = A.t_lay[at(k)];
-            const double pv = A.prsi[at(k)];
-            const double lb = log(k == nl - 1 ? fmax(A.press_min, pv) : pv);
-            A.t_lev[at(k)] = bound_t(t_lo + (t - t_lo) * (lb - la_lo) / (la - la_lo));
-            la_lo = la; t_lo = t;
-        }
-        A.t_lev[at(nl)] = bound_t(A.t_lay[at(nl - 1)]);
+        la_prev = la;
+        t_prev = t;
     }
+    A.p_lev[at(nl)] = A.prsi[at(nl)];
+    // the levels beyond the first and last layers: the surface temperature
+    // or the outermost layer's
+    A.t_lev[at(0)] = bound_t(top ? A.t_lay[at(0)] : A.tsfc[c]);
+    A.t_lev[at(nl)] = bound_t(top ? A.tsfc[c] : t_prev);
 }
 
 void launch_prepare(const PrepArgs &A, hipStream_t st)
@@ -216,12 +214,13 @@ __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs A)
     auto at = [&](int k) { return (size_t)c + (size_t)nc * k; };
     const bool top = A.p_lev[0] < A.p_lev[(size_t)nc * (nl - 1)];   // from the first column
     double hb_lo = 0.0, hc_lo = 0.0;   // boundary height below the layer, centre height of the layer below
+    double lb = log(A.p_lev[at(top ? nl : 0)]);   // log pressure at the layer's bottom level
     for (int i = 0; i < nl; ++i) {
         const int k = top ? nl - 1 - i : i;          // layer index, surface first
-        const int kb = top ? k + 1 : k, kt = top ? k : k + 1;   // its bottom and top level
+        const int kt = top ? k : k + 1;              // its top level
         const size_t q = at(k);
         const double tv = A.t_lay[q] * (1.0 + A.fvirt * A.q_lay[q]);
-        const double lb = log(A.p_lev[at(kb)]), lt = log(A.p_lev[at(kt)]);
+        const double lt = log(A.p_lev[at(kt)]);
         const double dz = A.rdog * fabs(lt - lb) * tv;
         const double hb = hb_lo + dz;
         const double pfac = fabs(lb - log(A.p_lay[q])) / fabs(lb - lt);
@@ -232,6 +231,7 @@ __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs A)
         if (A.dp) A.dp[q] = fabs(A.p_lev[at(k + 1)] - A.p_lev[q]);
         hb_lo = hb;
         hc_lo = hc;
+        lb = lt;                                      // the next layer's bottom level
     }
 }
 
