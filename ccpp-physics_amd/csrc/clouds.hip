@@ -281,8 +281,15 @@ __global__ __launch_bounds__(64) void mcica_kernel(McicaArgs A)
     if (seed == 0) seed = 4357;
     for (int i = lane; i < 624; i += 64) mt[i] = (uint32_t)seed * A.mtpow[i];
     __syncthreads();
-    int fst = -1;
-    for (int l = 0; l < nlay; ++l) if (cf[l] > 0.0) { fst = l; break; }
+    // First and last cloudy layer.  No overlap method sets a mask bit outside
+    // them (the decorrelation method compares draws in [0, 1] with 1 - cf = 1
+    // in clear layers), and every column seeds its own stream, so the stream
+    // is not advanced past the last cloudy layer: the words above it are 0,
+    // and a column without cloud draws nothing.
+    int fst = -1, lst = -1;
+    for (int l = 0; l < nlay; ++l)
+        if (cf[l] > 0.0) { if (fst < 0) fst = l; lst = l; }
+    for (int i = lane + (lst + 1) * MASK_WORDS; i < nlay * MASK_WORDS; i += 64) mout[i] = 0u;
     // twist the state in place, 64 words at a time (reads of a chunk before
     // its writes)
     auto twist = [&]() {
@@ -300,7 +307,7 @@ __global__ __launch_bounds__(64) void mcica_kernel(McicaArgs A)
     };
     int cur = -1;  // number of the twisted block the state holds (draws 624 cur ... 624 cur + 623)
     double lr[2] = {0.0, 0.0}, pr[2] = {0.0, 0.0};
-    for (int l = 0; l < nlay; ++l) {
+    for (int l = 0; l <= lst; ++l) {
         const int base = vmax ? 0 : l * ngpt;
         const int need = (base + ngpt - 1) / 624;
         // the lane's draws (tempered on read): those of the block held now,
