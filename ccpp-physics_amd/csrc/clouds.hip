@@ -62,6 +62,12 @@ __device__ __forceinline__ void inc2s(const Opt2 &a, Opt2 &b)
     b.t = t12;
 }
 
+// The effective-radius range check lut_cloud makes for a layer's condensate.
+__device__ __forceinline__ bool lut_out_of_range(const LutView &L, float lwp, float iwp, float rl, float ri)
+{
+    return (lwp > 0.0f && !(rl >= L.liq_lwr && rl <= L.liq_upr)) || (iwp > 0.0f && !(ri >= L.ice_lwr && ri <= L.ice_upr));
+}
+
 __device__ __forceinline__ bool lut_cloud(const LutView &L, int b, float lwp, float iwp, float rl, float ri, Opt2 &o)
 {
     float lt = 0, lts = 0, lta = 0, ltsg = 0, it = 0, its = 0, ita = 0, itsg = 0;
@@ -147,16 +153,23 @@ __global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs 
         const int blk = od / A.blksz;
         const size_t q = (size_t)c + (size_t)A.ncol_ld * l;
         const size_t o0 = (size_t)i * nb;
-        if (!A.blkflag[blk]) {
-            for (int b = 0; b < nb; ++b) {
-                if (A.ctau) A.ctau[o0 + b] = 0.0f;
-                else A.copt[o0 + b] = make_float3(0.0f, 0.0f, 0.0f);
+        // The solvers read a record only where the McICA mask has a bit, and
+        // the mask has none in a block without cloud or in a cloud-free layer
+        // (every overlap method); those records are not written.  The SW
+        // band-11 depth (cldtausw) is still formed for its block.
+        if (!A.blkflag[blk]) continue;
+        const double cf = A.cld_frac[q];
+        const float lwp = (float)A.lwp[q], iwp = (float)A.iwp[q];
+        const float rl = (float)A.reliq[q], ri = (float)A.reice[q];
+        if (!(cf > 0.0) && !(!A.is_lw && A.cldtau && blk == A.cldtau_blk)) {
+            // the range check of every layer's condensate still applies
+            bad |= lut_out_of_range(S, lwp, iwp, rl, ri);
+            for (int k = 0; k < 2; ++k) {
+                const double *const *src = k == 0 ? A.cnv : A.pbl;
+                if (src[0]) bad |= lut_out_of_range(S, (float)src[0][q], (float)src[1][q], (float)src[2][q], (float)src[3][q]);
             }
             continue;
         }
-        const float lwp = (float)A.lwp[q], iwp = (float)A.iwp[q];
-        const float rl = (float)A.reliq[q], ri = (float)A.reice[q];
-        const double cf = A.cld_frac[q];
         // precipitation, band independent parts
         float tr = 0.0f, tsn = 0.0f, res = 0.0f;
         bool prec = false;
