@@ -141,3 +141,49 @@ def write_lut_nc(path, lut, version=2):
         var("lut_" + k, ("nrghice", "nband", "nsize_ice"), getattr(lut, k))
     var("bnd_limits_wavenumber", ("nband", "pair"), np.zeros((lut.nband, 2)))
     f.close()
+
+
+def to_cdf5(src, dst):
+    """Re-encode a netCDF classic file (read with scipy) as CDF-5, the format
+    `nccopy -k cdf5` writes: 8-byte counts, lengths and offsets, no record
+    dimension, attributes dropped (the tables carry none the loader reads)."""
+    import struct
+    f = netcdf_file(src, "r", mmap=False)
+    types = {"b": (1, 1), "c": (2, 1), "h": (3, 2), "i": (4, 4), "f": (5, 4), "d": (6, 8)}
+    dims = list(f.dimensions.items())
+    dim_id = {n: i for i, (n, _) in enumerate(dims)}
+
+    def name(s):
+        b = s.encode()
+        return struct.pack(">q", len(b)) + b + b"\0" * (-len(b) % 4)
+
+    vars_ = []
+    for n, v in f.variables.items():
+        code, size = types[v.typecode()]
+        data = np.ascontiguousarray(v[...])
+        if v.typecode() == "c":
+            raw = data.astype("S1").tobytes()
+        else:
+            raw = data.astype(data.dtype.newbyteorder(">")).tobytes()
+        vars_.append((n, [dim_id[d] for d in v.dimensions], code, raw))
+    f.close()
+
+    def header(begins):
+        h = b"CDF\x05" + struct.pack(">q", 0)
+        h += struct.pack(">iq", 0x0A, len(dims)) + b"".join(name(n) + struct.pack(">q", L) for n, L in dims)
+        h += struct.pack(">iq", 0, 0)
+        h += struct.pack(">iq", 0x0B, len(vars_))
+        for (n, ids, code, raw), b in zip(vars_, begins):
+            h += name(n) + struct.pack(">q", len(ids)) + b"".join(struct.pack(">q", i) for i in ids)
+            h += struct.pack(">iq", 0, 0) + struct.pack(">iqq", code, len(raw) + (-len(raw) % 4), b)
+        return h
+
+    hlen = len(header([0] * len(vars_)))
+    begins, off = [], hlen
+    for _, _, _, raw in vars_:
+        begins.append(off)
+        off += len(raw) + (-len(raw) % 4)
+    with open(dst, "wb") as out:
+        out.write(header(begins))
+        for _, _, _, raw in vars_:
+            out.write(raw + b"\0" * (-len(raw) % 4))

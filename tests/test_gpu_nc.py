@@ -8,19 +8,24 @@ which leaves every result unchanged."""
 import numpy as np
 import pytest
 
-from nc_writer import write_kdist_nc, write_lut_nc
+from nc_writer import to_cdf5, write_kdist_nc, write_lut_nc
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("cdf", [2, 5])
 @pytest.mark.parametrize("is_lw", [True, False])
-def test_netcdf_tables_match_direct_tables(tmp_path, is_lw):
+def test_netcdf_tables_match_direct_tables(tmp_path, is_lw, cdf):
     from ccpprad import capi, device, lib, synthetic as syn
     kd, lut = syn.make_kdist(is_lw), syn.make_cloud_lut(is_lw)
     lut.icergh = lut.nrghice           # the netCDF path selects the last roughness category
     write_kdist_nc(tmp_path / "k.nc", kd)
     write_lut_nc(tmp_path / "c.nc", lut)
+    if cdf == 5:                       # the format `nccopy -k cdf5` writes
+        for n in ("k", "c"):
+            to_cdf5(tmp_path / f"{n}.nc", tmp_path / f"{n}5.nc")
+            (tmp_path / f"{n}5.nc").replace(tmp_path / f"{n}.nc")
     ncol, nlay = 128, 127
     print(f"netCDF tables: {'LW' if is_lw else 'SW'} {ncol} columns x {nlay} layers", flush=True)
     pr = syn.make_profiles(ncol, nlay, seed=17, clouds=True, kd=kd)

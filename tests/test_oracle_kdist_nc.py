@@ -8,7 +8,7 @@ g-point and layer, the same vmr_ref columns, the same minor intervals with
 their contributors repacked.  A key species that is not active is an error.
 
 Library (no GPU needed): the native classic-format reader parses the same
-files in both classic versions up to the device upload, and refuses netCDF-4
+files as CDF-1, CDF-2 and CDF-5 up to the device upload, and refuses netCDF-4
 (HDF5) containers, truncated headers, missing variables and inactive key
 species with a message."""
 import ctypes as C
@@ -16,7 +16,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from nc_writer import write_kdist_nc, write_lut_nc
+from nc_writer import to_cdf5, write_kdist_nc, write_lut_nc
 
 
 def _pairs(kd):
@@ -70,7 +70,7 @@ def _init(L, kfile, cfile, names=None, lw=True):
     return rc, buf.value.decode(errors="replace")
 
 
-@pytest.mark.parametrize("version", [1, 2])
+@pytest.mark.parametrize("version", [1, 2, 5])
 @pytest.mark.parametrize("is_lw", [True, False])
 def test_library_parses_netcdf_tables(tmp_path, version, is_lw):
     import torch
@@ -78,8 +78,13 @@ def test_library_parses_netcdf_tables(tmp_path, version, is_lw):
     if torch.cuda.is_available():
         pytest.skip("covered by the GPU tests")
     L = lib.load()
-    write_kdist_nc(tmp_path / "k.nc", syn.make_kdist(is_lw), version=version)
-    write_lut_nc(tmp_path / "c.nc", syn.make_cloud_lut(is_lw), version=version)
+    write_kdist_nc(tmp_path / "k.nc", syn.make_kdist(is_lw), version=min(version, 2))
+    write_lut_nc(tmp_path / "c.nc", syn.make_cloud_lut(is_lw), version=min(version, 2))
+    if version == 5:
+        for n in ("k", "c"):
+            to_cdf5(tmp_path / f"{n}.nc", tmp_path / f"{n}5.nc")
+            (tmp_path / f"{n}5.nc").replace(tmp_path / f"{n}.nc")
+        assert (tmp_path / "k.nc").read_bytes()[:4] == b"CDF\x05"
     rc, msg = _init(L, tmp_path / "k.nc", tmp_path / "c.nc", syn.GAS_NAMES, lw=is_lw)
     # no GPU here: the tables parse and reduce, then the device upload fails
     assert rc != 0
