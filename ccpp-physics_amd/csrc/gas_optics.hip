@@ -375,18 +375,27 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
                 const unsigned v = reg == 0 ? mit[0][ii] : mit[1][ii];
                 const int fv = mi_fv(v);
                 const unsigned ci = (unsigned)mi_ci(v);
+                // only lanes whose slot holds an item fetch (an empty slot's
+                // scale is 0, so its value is never seen): the gathers are
+                // bound by the data the texture path returns per lane
+                const bool item = mi_iv(v) != G.nmn;
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
                     const unsigned b0 = __umul24(Ij[(2 * fv + t) * 2 + 1].x, nc) + ci;
-                    if (LW) {
-                        const float2 m2 = ld2(reg == 0 ? T.km2[0] : T.km2[1], b0);
-                        km[u][ii][t][0] = m2.x;
-                        km[u][ii][t][1] = m2.y;
-                    } else {
-                        const float *kmt = reg == 0 ? T.kmin[0] : T.kmin[1];
-                        km[u][ii][t][0] = ld1(kmt, b0);
-                        km[u][ii][t][1] = ld1(kmt, b0 + nc);
+                    float k0 = 0.0f, k1 = 0.0f;
+                    if (item) {
+                        if (LW) {
+                            const float2 m2 = ld2(reg == 0 ? T.km2[0] : T.km2[1], b0);
+                            k0 = m2.x;
+                            k1 = m2.y;
+                        } else {
+                            const float *kmt = reg == 0 ? T.kmin[0] : T.kmin[1];
+                            k0 = ld1(kmt, b0);
+                            k1 = ld1(kmt, b0 + nc);
+                        }
                     }
+                    km[u][ii][t][0] = k0;
+                    km[u][ii][t][1] = k1;
                 }
             }
         }
