@@ -161,9 +161,15 @@ int ccpprad_lw_init(const ccpprad_kdist *kd, const ccpprad_cloud_lut *lut,
 int ccpprad_sw_init(const ccpprad_kdist *kd, const ccpprad_cloud_lut *lut,
                     const char *const *gas_names, int ngases, int nLay,
                     int rrtmgp_phys_blksz, ccpprad_ctx **ctx, char *errmsg, int errlen);
-/* Init from table files written by the package's table writer
- * (the netCDF files named at rrtmgp_lw_gas_optics.F90:81-450 are replaced by
- * this binary format). */
+/* Init from table files: the reference's netCDF k-distribution and cloud
+ * files (rrtmgp_lw_gas_optics.F90:130-327, rrtmgp_sw_gas_optics.F90:288-376,
+ * rrtmgp_lw_cloud_optics.F90:130-290), read natively in the netCDF classic
+ * formats (CDF-1, CDF-2, CDF-5; netCDF-4/HDF5 files are refused with a
+ * message) and reduced to the active gases as ty_gas_optics_rrtmgp%load
+ * does, or files in the package's binary table format; each file's format is
+ * taken from its signature.  Lookup-table cloud optics only (Pade
+ * coefficients are refused); the netCDF cloud path uses the file's last
+ * ice-roughness category. */
 int ccpprad_lw_init_files(const char *gas_file, const char *cloud_file,
                           const char *const *gas_names, int ngases, int nLay,
                           int rrtmgp_phys_blksz, ccpprad_ctx **ctx, char *errmsg, int errlen);
