@@ -1,4 +1,5 @@
-"""Writer of the binary table files read by ccpprad_{lw,sw}_init_files.
+"""Writer of the package's binary table files, which ccpprad_{lw,sw}_init_files
+read besides the reference's netCDF files.
 
 Format: b"CCPPRADT", int32 version 1, int32 record count, then records of
 char name[32], int32 type (1 int32, 2 float64), int32 ndim, int64 dims[4] and

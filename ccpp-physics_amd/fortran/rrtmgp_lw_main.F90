@@ -7,8 +7,10 @@
 ! Deviations from the reference, by design:
 !  - errors set errflg = 1 with the library's message (the reference prints
 !    rte-rrtmgp errors and leaves errflg = 0, radiation_tools.F90:89-98);
-!  - the gas / cloud files are this package's table files (the netCDF files
-!    of rrtmgp_lw_gas_optics.F90:81-450 are replaced);
+!  - the gas / cloud files are read by the library: the reference's netCDF
+!    files in the netCDF classic formats (netCDF-4 files are refused with a
+!    message; `nccopy -k cdf5` converts them), or the package's binary table
+!    files;
 !  - doGP_cldoptics_PADE is rejected with an error, and so is doGP_lwscat
 !    together with use_LW_jacobian (rte_lw computes no Jacobian with
 !    two-stream optics).
