@@ -24,9 +24,11 @@ def _chars(names, width=32):
     return a
 
 
-def _minor_file_form(m, is_lower, rng, neta, ntemp):
+def _minor_file_form(m, is_lower, rng, neta, ntemp, scaled_by_inactive=False):
     """The synthetic intervals with an extra-gas interval after the first one
-    and another at the end, each with its own contributors."""
+    and another at the end, each with its own contributors; with
+    scaled_by_inactive, also an o3 interval scaled by n2 (a gas the run does
+    not use), which must stay, without its scaling."""
     n = m["n"]
     lims, kmin, start = m["limits_gpt"], m["kminor"], m["kminor_start"]
     rows = []
@@ -42,6 +44,9 @@ def _minor_file_form(m, is_lower, rng, neta, ntemp):
             lo = int(lims[i, 0])
             rows.append(dict(gas=gas, lims=np.array([lo, lo + 2], dtype=np.int32), dens=1, comp=0, scal="h2o",
                              k=rng.uniform(1e-22, 1e-20, (ntemp, neta, 3))))
+    if scaled_by_inactive:
+        rows.append(dict(gas="o3", lims=np.array([1, 4], dtype=np.int32), dens=1, comp=0, scal="n2",
+                         k=rng.uniform(1e-21, 1e-19, (ntemp, neta, 4))))
     ks, starts = 1, []
     for r in rows:
         starts.append(ks)
@@ -49,13 +54,13 @@ def _minor_file_form(m, is_lower, rng, neta, ntemp):
     return rows, np.array(starts, dtype=np.int32), np.concatenate([r["k"] for r in rows], axis=2)
 
 
-def write_kdist_nc(path, kd, version=2, seed=3):
+def write_kdist_nc(path, kd, version=2, seed=3, scaled_by_inactive=False):
     """kd: a synthetic KDist.  version: 1 (CDF-1) or 2 (64-bit offset)."""
     rng = np.random.default_rng(seed)
     f = netcdf_file(path, "w", version=version)
     ntemp, npres, neta, ngpt, nbnd = kd.ntemp, kd.npres, kd.neta, kd.ngpt, kd.nband
     nabs = len(FILE_GASES)
-    lo_rows, lo_start, lo_k = _minor_file_form(kd.minor_lower, True, rng, neta, ntemp)
+    lo_rows, lo_start, lo_k = _minor_file_form(kd.minor_lower, True, rng, neta, ntemp, scaled_by_inactive)
     up_rows, up_start, up_k = _minor_file_form(kd.minor_upper, False, rng, neta, ntemp)
     ident = sorted({r["gas"] for r in lo_rows + up_rows})
     dims = dict(temperature=ntemp, pressure=npres, pressure_interp=npres + 1, absorber=nabs,

@@ -51,3 +51,36 @@ def test_netcdf_tables_match_direct_tables(tmp_path, is_lw, cdf):
     for n in names:
         assert np.array_equal(res[0][n], res[1][n]), n
     assert np.any(res[0][names[0]] != 0.0)
+
+
+def test_netcdf_minor_scaled_by_inactive_gas_matches_oracle(tmp_path):
+    """A file whose extra o3 interval is scaled by n2, which the run does not
+    use: the library keeps the interval without its scaling, as the oracle's
+    reduction does; all-sky LW against the oracle run on that reduction."""
+    from ccpprad import capi, device, lib, synthetic as syn
+    from oracle import kdist_nc, oracle as O
+    from util import report
+    kd, lut = syn.make_kdist(True), syn.make_cloud_lut(True)
+    lut.icergh = lut.nrghice
+    write_kdist_nc(tmp_path / "k.nc", kd, scaled_by_inactive=True)
+    write_lut_nc(tmp_path / "c.nc", lut)
+    kr = kdist_nc.load_kdist(tmp_path / "k.nc", True)
+    ncol, nlay = 64, 127
+    print(f"netCDF tables, o3 interval scaled by inactive n2: LW {ncol} columns", flush=True)
+    pr = syn.make_profiles(ncol, nlay, seed=19, clouds=True, kd=kd)
+    arr = capi.profile_fields(pr)
+    ints = capi.lw_default_ints(ncol, nlay, blksz=ncol, nangle=3)
+    ref = dict(arr)
+    ref.update(O.lw_outputs(ncol, nlay))
+    O.lw_main_run(kr, lut, ints, ref)
+    s = lib.LWScheme.from_files(tmp_path / "k.nc", tmp_path / "c.nc", nlay, blksz=ncol)
+    dev = device.to_device(arr)
+    dev.update(device.lw_outputs(ncol, nlay))
+    s.run_dev(ints, dev)
+    torch.cuda.synchronize()
+    out = device.to_host(dev)
+    for n in ("fluxlwUP_allsky", "fluxlwDOWN_allsky"):
+        mx, _ = report(n, out[n], ref[n], "W/m2")
+        assert mx < 0.05, n
+    mx, _ = report("htrlw", out["htrlw"] * 86400, ref["htrlw"] * 86400, "K/day")
+    assert mx < 0.01

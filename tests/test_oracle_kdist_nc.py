@@ -49,6 +49,21 @@ def test_oracle_reduction_round_trip(tmp_path, is_lw):
             np.testing.assert_array_equal(getattr(r, k), getattr(kd, k), err_msg=k)
 
 
+def test_oracle_minor_scaled_by_an_inactive_gas(tmp_path):
+    """An interval whose gas is active but whose scaling gas is not stays,
+    with no scaling gas (index 0), and its contributors are appended."""
+    from ccpprad import synthetic as syn
+    from oracle import kdist_nc
+    kd = syn.make_kdist(True)
+    write_kdist_nc(tmp_path / "k.nc", kd, scaled_by_inactive=True)
+    r = kdist_nc.load_kdist(tmp_path / "k.nc", True)
+    a, b = r.minor_lower, kd.minor_lower
+    assert a["n"] == b["n"] + 1 and a["ncontrib"] == b["ncontrib"] + 4
+    assert a["idx_minor"][-1] == 3 and a["idx_scaling"][-1] == 0 and a["scales_with_density"][-1] == 1
+    assert list(a["limits_gpt"][-1]) == [1, 4] and a["kminor_start"][-1] == b["ncontrib"] + 1
+    np.testing.assert_array_equal(a["kminor"][:, :, :b["ncontrib"]], b["kminor"])
+
+
 def test_oracle_inactive_key_species_is_an_error(tmp_path):
     from ccpprad import synthetic as syn
     from oracle import kdist_nc
