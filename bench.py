@@ -165,8 +165,8 @@ def time_callers(a, torch, lw, dev, ncol):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU column sharding is described "
-                                   "in DESIGN.md and not launched", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; the path shards by columns across GPUs "
+                                   "with no exchange, and a multi-GPU run is not launched", "n_gpus": a.gpus}))
         sys.exit(2)
     import torch
     from ccpprad import synthetic as syn, capi, lib, device
