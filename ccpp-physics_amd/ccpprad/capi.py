@@ -33,12 +33,18 @@ class KDistC(C.Structure):
                 ("krayl_upper", P_D), ("solar_src", P_D)]
 
 
+PADE_FIELDS = ("pade_extliq", "pade_ssaliq", "pade_asyliq", "pade_extice", "pade_ssaice", "pade_asyice",
+               "pade_sizreg_extliq", "pade_sizreg_ssaliq", "pade_sizreg_asyliq", "pade_sizreg_extice",
+               "pade_sizreg_ssaice", "pade_sizreg_asyice")
+
+
 class CloudLUTC(C.Structure):
     _fields_ = [("nband", C.c_int), ("nsize_liq", C.c_int), ("nsize_ice", C.c_int),
                 ("nrghice", C.c_int), ("icergh", C.c_int), ("radliq_lwr", C.c_double),
                 ("radliq_upr", C.c_double), ("radice_lwr", C.c_double),
                 ("radice_upr", C.c_double), ("extliq", P_D), ("ssaliq", P_D), ("asyliq", P_D),
-                ("extice", P_D), ("ssaice", P_D), ("asyice", P_D)]
+                ("extice", P_D), ("ssaice", P_D), ("asyice", P_D), ("pade", C.c_int)]
+    _fields_ += [(n, P_D) for n in PADE_FIELDS]
 
 
 _LW_FIELDS = (
@@ -234,10 +240,12 @@ def lut_struct(lut: syn.CloudLUT) -> Holder:
         keep.append(a)
         return _ptr(a, P_D)
 
+    pade = getattr(lut, "pade", None)
     s = CloudLUTC(lut.nband, lut.nsize_liq, lut.nsize_ice, lut.nrghice, lut.icergh,
                   lut.radliq_lwr, lut.radliq_upr, lut.radice_lwr, lut.radice_upr,
-                  d(lut.extliq), d(lut.ssaliq), d(lut.asyliq), d(lut.extice), d(lut.ssaice),
-                  d(lut.asyice))
+                  *(d(getattr(lut, k)) if getattr(lut, k) is not None else None
+                    for k in ("extliq", "ssaliq", "asyliq", "extice", "ssaice", "asyice")),
+                  int(pade is not None), *(d(pade[k]) if pade is not None else None for k in PADE_FIELDS))
     return Holder(s, keep)
 
 

@@ -270,6 +270,47 @@ class CloudLUT:
     ssaice: np.ndarray
     asyice: np.ndarray
     icergh: int = 2      # ice roughness category used (1-based)
+    # Pade form: None, or a dict of the ccpprad_cloud_lut pade_* arrays
+    # (numpy shapes reversed from Fortran: (ncoeff, 3, nband), ice
+    # (nrghice, ncoeff, 3, nband); bounds (4,))
+    pade: dict | None = None
+
+
+def make_cloud_pade(is_lw: bool, seed: int = 20261203) -> CloudLUT:
+    """Pade-form cloud optics of the same band count: per band and size
+    regime, extinction ~ a re / (1 + b re + c re^2 + d re^3) (falling like
+    1/re), co-albedo and asymmetry nearly constant, positive denominators;
+    liquid regimes split at 6.5 and 11 um (bounds 2.5..21.5), ice at 30 and
+    70 um (10..180)."""
+    rng = np.random.default_rng(seed + (11 if is_lw else 1011))
+    nb, nr = (16 if is_lw else 14), 3
+    lut = make_cloud_lut(is_lw, seed)
+
+    def coeffs(nband, ext_scale, coa, asy):
+        e = np.zeros((6, nr, nband))
+        e[1] = ext_scale * rng.uniform(0.9, 1.1, (nr, nband))        # numerator: a re
+        e[3] = rng.uniform(0.0, 0.2, (nr, nband))                     # denominator: 1 + b re + c re^2 + d re^3
+        e[4] = rng.uniform(0.8, 1.2, (nr, nband))
+        e[5] = rng.uniform(0.0, 1e-3, (nr, nband))
+        s = np.zeros((5, nr, nband))
+        s[0] = coa * rng.uniform(0.5, 1.5, (nr, nband))
+        s[1] = coa * rng.uniform(-0.01, 0.01, (nr, nband))
+        s[3] = rng.uniform(0.0, 0.05, (nr, nband))
+        a = np.zeros((5, nr, nband))
+        a[0] = asy * rng.uniform(0.9, 1.0, (nr, nband))
+        a[3] = rng.uniform(0.0, 0.01, (nr, nband))
+        return e, s, a
+
+    el, sl, al = coeffs(nb, 1.5, 0.3 if is_lw else 1e-4, 0.85)
+    ice = [coeffs(nb, 2.5, 0.4 if is_lw else 1e-3, 0.8) for _ in range(lut.nrghice)]
+    pade = dict(pade_extliq=el, pade_ssaliq=sl, pade_asyliq=al,
+                pade_extice=np.stack([c[0] for c in ice]), pade_ssaice=np.stack([c[1] for c in ice]),
+                pade_asyice=np.stack([c[2] for c in ice]))
+    bl, bi = np.array([2.5, 6.5, 11.0, 21.5]), np.array([10.0, 30.0, 70.0, 180.0])
+    for k in ("ext", "ssa", "asy"):
+        pade[f"pade_sizreg_{k}liq"] = bl.copy()
+        pade[f"pade_sizreg_{k}ice"] = bi.copy()
+    return dataclasses.replace(lut, radliq_lwr=2.5, radliq_upr=21.5, radice_lwr=10.0, radice_upr=180.0, pade=pade)
 
 
 def make_cloud_lut(is_lw: bool, seed: int = 20261203) -> CloudLUT:

@@ -433,6 +433,36 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
         LutView &L = c->L;
         int icergh = lut->icergh > 0 ? lut->icergh : 1;
         if (icergh > lut->nrghice) { delete c; seterr(errmsg, errlen, "ice roughness category out of range"); return 1; }
+        L.pade = lut->pade != 0;
+        if (L.pade) {
+            // Pade form: coefficients [phase][band][regime][16] and the bounds
+            const double *P[2][3] = {{lut->pade_extliq, lut->pade_ssaliq, lut->pade_asyliq},
+                                     {lut->pade_extice, lut->pade_ssaice, lut->pade_asyice}};
+            const double *B[2][3] = {{lut->pade_sizreg_extliq, lut->pade_sizreg_ssaliq, lut->pade_sizreg_asyliq},
+                                     {lut->pade_sizreg_extice, lut->pade_sizreg_ssaice, lut->pade_sizreg_asyice}};
+            for (int ph = 0; ph < 2; ++ph)
+                for (int f = 0; f < 3; ++f)
+                    if (!P[ph][f] || !B[ph][f]) { delete c; seterr(errmsg, errlen, "Pade cloud optics: NULL array"); return 1; }
+            const int nb = lut->nband;
+            const int nco[3] = {6, 5, 5}, off[3] = {0, 6, 11};
+            std::vector<float> pc((size_t)2 * nb * 48, 0.0f);
+            for (int ph = 0; ph < 2; ++ph)
+                for (int f = 0; f < 3; ++f) {
+                    const double *src = P[ph][f] + (ph ? (size_t)nb * 3 * nco[f] * (icergh - 1) : 0);
+                    for (int b = 0; b < nb; ++b)
+                        for (int r = 0; r < 3; ++r)
+                            for (int k = 0; k < nco[f]; ++k)
+                                pc[((size_t)(ph * nb + b) * 3 + r) * 16 + off[f] + k] =
+                                    (float)src[(size_t)b + (size_t)nb * (r + 3 * (size_t)k)];
+                    for (int i = 0; i < 4; ++i) L.pbnd[ph][f][i] = (float)B[ph][f][i];
+                    if (!(B[ph][f][2] > 0.0)) { delete c; seterr(errmsg, errlen, "Pade size-regime bounds must be increasing"); return 1; }
+                }
+            L.pcoef = upload(c, pc.data(), pc.size(), &e);
+            L.nband = nb; L.nsize_liq = 0; L.nsize_ice = 0;
+            L.liq_lwr = (float)B[0][0][0]; L.liq_upr = (float)B[0][0][3];
+            L.ice_lwr = (float)B[1][0][0]; L.ice_upr = (float)B[1][0][3];
+            c->have_lut = 1;
+        } else {
         L.nband = lut->nband; L.nsize_liq = lut->nsize_liq; L.nsize_ice = lut->nsize_ice;
         L.liq_lwr = (float)lut->radliq_lwr; L.liq_upr = (float)lut->radliq_upr;
         L.ice_lwr = (float)lut->radice_lwr; L.ice_upr = (float)lut->radice_upr;
@@ -461,6 +491,7 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
             return 1;
         }
         c->have_lut = 1;
+        }
     }
     if (e != hipSuccess) {
         seterr(errmsg, errlen, "device table upload failed: %s", hipGetErrorString(e));

@@ -77,6 +77,10 @@ struct LutView {
     const float *extliq, *ssaliq, *asyliq;   // [nband][nsize_liq]
     const float *extice, *ssaice, *asyice;   // [nband][nsize_ice] (roughness selected)
     const float *coaliq, *coaice;            // co-albedo 1 - ssa, formed in FP64 at init
+    int pade;                                // Pade form: coefficients in place of the tables
+    const float *pcoef;                      // [phase][nband][3 regimes][16]: extinction 0-5,
+                                             // co-albedo 6-10, asymmetry 11-15
+    float pbnd[2][3][4];                     // [phase][extinction, co-albedo, asymmetry] regime bounds
 };
 
 __device__ __forceinline__ int d_min(int a, int b) { return a < b ? a : b; }

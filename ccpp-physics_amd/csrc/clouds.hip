@@ -42,6 +42,39 @@ __device__ __forceinline__ void lut_eval(const float *ext, const float *ssa, con
     tsg = ts * (a0 + fint * (a1 - a0));
 }
 
+// rte-rrtmgp's pade_eval: numerator degree m, denominator degree n with
+// constant term 1, coefficients c[0 .. m + n] (Horner, FP32).
+__device__ __forceinline__ float pade_f(const float *c, int m, int n, float re)
+{
+    float den = c[n + m];
+    for (int i = n + m - 1; i >= m + 1; --i) den = fmaf(re, den, c[i]);
+    den = fmaf(re, den, 1.0f);
+    float num = c[m];
+    for (int i = m - 1; i >= 0; --i) num = fmaf(re, num, c[i]);
+    return num / den;
+}
+
+// the size regime of a radius, rte's min(floor((re - b2) / b3) + 2, 3), 0-based
+__device__ __forceinline__ int pade_reg(const float *bd, float re)
+{
+    return min(max((int)floorf((re - bd[1]) / bd[2]) + 2, 1), 3) - 1;
+}
+
+// compute_all_from_pade for one phase (0 liquid, 1 ice) and band:
+// t = path ext, absorption t max(0, co-albedo), ts = t - that, tsg = ts g.
+__device__ __forceinline__ void pade_phase(const LutView &L, int ph, int b, float path, float re, float &t,
+                                           float &ts, float &ta, float &tsg)
+{
+    const float *pc = L.pcoef + (size_t)(ph * L.nband + b) * 48;
+    const float e = pade_f(pc + 16 * pade_reg(L.pbnd[ph][0], re), 2, 3, re);
+    const float co = fmaxf(0.0f, pade_f(pc + 16 * pade_reg(L.pbnd[ph][1], re) + 6, 2, 2, re));
+    const float g = pade_f(pc + 16 * pade_reg(L.pbnd[ph][2], re) + 11, 2, 2, re);
+    t = path * e;
+    ta = t * co;
+    ts = t * (1.0f - co);
+    tsg = ts * g;
+}
+
 // Two-stream optics carried with the co-albedo: (tau, ssa, 1 - ssa, g).
 struct Opt2 {
     float t, w, p, g;
@@ -62,7 +95,8 @@ __device__ __forceinline__ void inc2s(const Opt2 &a, Opt2 &b)
     b.t = t12;
 }
 
-// The effective-radius range check lut_cloud makes for a layer's condensate.
+// The effective-radius range check lut_cloud makes for a layer's condensate
+// (LUT and Pade forms alike: the Pade bounds are the extinction size bounds).
 __device__ __forceinline__ bool lut_out_of_range(const LutView &L, float lwp, float iwp, float rl, float ri)
 {
     return (lwp > 0.0f && !(rl >= L.liq_lwr && rl <= L.liq_upr)) || (iwp > 0.0f && !(ri >= L.ice_lwr && ri <= L.ice_upr));
@@ -72,6 +106,16 @@ __device__ __forceinline__ bool lut_cloud(const LutView &L, int b, float lwp, fl
 {
     float lt = 0, lts = 0, lta = 0, ltsg = 0, it = 0, its = 0, ita = 0, itsg = 0;
     bool bad = false;
+    if (L.pade) {
+        if (lwp > 0.0f) {
+            bad |= !(rl >= L.liq_lwr && rl <= L.liq_upr);
+            pade_phase(L, 0, b, lwp, fminf(fmaxf(rl, L.liq_lwr), L.liq_upr), lt, lts, lta, ltsg);
+        }
+        if (iwp > 0.0f) {
+            bad |= !(ri >= L.ice_lwr && ri <= L.ice_upr);
+            pade_phase(L, 1, b, iwp, fminf(fmaxf(ri, L.ice_lwr), L.ice_upr), it, its, ita, itsg);
+        }
+    } else {
     if (lwp > 0.0f) {
         bad |= !(rl >= L.liq_lwr && rl <= L.liq_upr);
         const size_t o = (size_t)b * L.nsize_liq;
@@ -83,6 +127,7 @@ __device__ __forceinline__ bool lut_cloud(const LutView &L, int b, float lwp, fl
         const size_t o = (size_t)b * L.nsize_ice;
         lut_eval(L.extice + o, L.ssaice + o, L.coaice + o, L.asyice + o, L.nsize_ice, L.ice_lwr, L.ice_istep, iwp,
                  fminf(fmaxf(ri, L.ice_lwr), L.ice_upr), it, its, ita, itsg);
+    }
     }
     o.t = lt + it;
     const float ts = lts + its, ta = lta + ita, tsg = ltsg + itsg;
@@ -105,6 +150,7 @@ __constant__ float PREC_C0S[14] = {0.970f, 0.970f, 0.970f, 0.970f, 0.970f, 0.970
 // liquid then ice, [band][size] each.
 size_t cloud_lut_floats(const LutView &L)
 {
+    if (L.pade) return 2 * (size_t)L.nband * 48;
     return 4 * (size_t)L.nband * ((size_t)L.nsize_liq + L.nsize_ice);
 }
 
@@ -123,7 +169,12 @@ __global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs 
 {
     extern __shared__ float slut[];
     const int nl = L.nband * L.nsize_liq, ni = L.nband * L.nsize_ice, ntot = 4 * (nl + ni);
-    {
+    LutView S = L;
+    if (L.pade) {
+        const int np = 2 * L.nband * 48;
+        for (int k = threadIdx.x; k < np; k += blockDim.x) slut[k] = L.pcoef[k];
+        S.pcoef = slut;
+    } else {
         constexpr int NLD = 4;   // staging loads in flight per thread
         for (int k0 = threadIdx.x; k0 < ntot; k0 += NLD * blockDim.x) {
             float v[NLD];
@@ -138,10 +189,9 @@ __global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs 
             for (int u = 0; u < NLD; ++u)
                 if (k0 + u * (int)blockDim.x < ntot) slut[k0 + u * blockDim.x] = v[u];
         }
+        S.extliq = slut; S.ssaliq = slut + nl; S.coaliq = slut + 2 * nl; S.asyliq = slut + 3 * nl;
+        S.extice = slut + 4 * nl; S.ssaice = S.extice + ni; S.coaice = S.extice + 2 * ni; S.asyice = S.extice + 3 * ni;
     }
-    LutView S = L;
-    S.extliq = slut; S.ssaliq = slut + nl; S.coaliq = slut + 2 * nl; S.asyliq = slut + 3 * nl;
-    S.extice = slut + 4 * nl; S.ssaice = S.extice + ni; S.coaice = S.extice + 2 * ni; S.asyice = S.extice + 3 * ni;
     __syncthreads();
 
     const int nb = A.nband;

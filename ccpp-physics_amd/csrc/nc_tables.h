@@ -26,6 +26,7 @@ struct NcKdistHold {
 struct NcLutHold {
     ccpprad_cloud_lut lut{};
     std::vector<double> extliq, ssaliq, asyliq, extice, ssaice, asyice;
+    std::vector<double> pade[6], sizreg[6];   // Pade form: ext, ssa, asy (liquid, then ice)
 };
 
 // true when the file starts with a netCDF classic or netCDF-4 (HDF5) signature
@@ -34,7 +35,8 @@ bool is_netcdf_file(const char *path);
 // load does; false with a message in err on any missing or malformed field
 bool load_kdist_nc(const char *path, int want_lw, const char *const *gas_names, int ngases, NcKdistHold &h,
                    std::string &err);
-// the lookup-table cloud optics (the last ice-roughness category selected)
+// the cloud optics: the lookup tables when the file has them, else the Pade
+// coefficients (the last ice-roughness category selected)
 bool load_lut_nc(const char *path, NcLutHold &h, std::string &err);
 
 }  // namespace ccpprad

@@ -510,11 +510,39 @@ bool load_lut_nc(const char *path, NcLutHold &h, std::string &err)
 {
     Reader R;
     if (!nc_open(path, R.f, err)) return false;
-    const size_t nb = R.dim("nband"), nl = R.dim("nsize_liq"), ni = R.dim("nsize_ice"), nr = R.dim("nrghice");
+    const size_t nb = R.dim("nband"), nr = R.dim("nrghice");
+    const size_t nl = R.has("lut_extliq") ? R.dim("nsize_liq") : 0, ni = R.has("lut_extliq") ? R.dim("nsize_ice") : 0;
     if (!R.missing.empty()) { err = "netCDF cloud-optics file lacks " + R.missing; return false; }
     if (!R.has("lut_extliq")) {
-        err = "the netCDF cloud-optics file has no lookup tables (Pade coefficients are not supported)";
-        return false;
+        // Pade form (rrtmgp_lw_cloud_optics.F90:192-205,261-288): coefficients
+        // (nband, nsizereg, ncoeff[, nrghice]) and the size-regime bounds
+        const size_t ns = R.dim("nsizereg"), ne = R.dim("ncoeff_ext"), nsg = R.dim("ncoeff_ssa_g");
+        const size_t nbd = R.dim("nbound");
+        if (!R.missing.empty()) { err = "netCDF cloud-optics file has neither lookup tables nor Pade " + R.missing; return false; }
+        if (nb < 1 || nr < 1 || ns != 3 || ne != 6 || nsg != 5 || nbd != 4) {
+            err = "Pade cloud optics need 3 size regimes, 6 extinction and 5 ssa/g coefficients, 4 bounds";
+            return false;
+        }
+        const char *cn[6] = {"pade_extliq", "pade_ssaliq", "pade_asyliq", "pade_extice", "pade_ssaice", "pade_asyice"};
+        const char *bn[6] = {"pade_sizreg_extliq", "pade_sizreg_ssaliq", "pade_sizreg_asyliq",
+                             "pade_sizreg_extice", "pade_sizreg_ssaice", "pade_sizreg_asyice"};
+        for (int k = 0; k < 6; ++k) {
+            h.pade[k] = R.dbl(cn[k], nb * ns * (k % 3 == 0 ? ne : nsg) * (k >= 3 ? nr : 1));
+            h.sizreg[k] = R.dbl(bn[k], nbd);
+        }
+        if (!R.missing.empty()) { err = "netCDF cloud-optics file lacks " + R.missing; return false; }
+        ccpprad_cloud_lut &L = h.lut;
+        L = ccpprad_cloud_lut{};
+        L.nband = (int)nb; L.nrghice = (int)nr; L.icergh = (int)nr;
+        L.radliq_lwr = h.sizreg[0][0]; L.radliq_upr = h.sizreg[0][3];
+        L.radice_lwr = h.sizreg[3][0]; L.radice_upr = h.sizreg[3][3];
+        L.pade = 1;
+        L.pade_extliq = h.pade[0].data(); L.pade_ssaliq = h.pade[1].data(); L.pade_asyliq = h.pade[2].data();
+        L.pade_extice = h.pade[3].data(); L.pade_ssaice = h.pade[4].data(); L.pade_asyice = h.pade[5].data();
+        L.pade_sizreg_extliq = h.sizreg[0].data(); L.pade_sizreg_ssaliq = h.sizreg[1].data();
+        L.pade_sizreg_asyliq = h.sizreg[2].data(); L.pade_sizreg_extice = h.sizreg[3].data();
+        L.pade_sizreg_ssaice = h.sizreg[4].data(); L.pade_sizreg_asyice = h.sizreg[5].data();
+        return true;
     }
     if (nb < 1 || nl < 2 || ni < 2 || nr < 1) { err = "netCDF cloud-optics file has bad dimensions"; return false; }
     h.extliq = R.dbl("lut_extliq", nl * nb); h.ssaliq = R.dbl("lut_ssaliq", nl * nb); h.asyliq = R.dbl("lut_asyliq", nl * nb);

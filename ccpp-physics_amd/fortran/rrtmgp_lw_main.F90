@@ -11,7 +11,10 @@
 !    files in the netCDF classic formats (netCDF-4 files are refused with a
 !    message; `nccopy -k cdf5` converts them), or the package's binary table
 !    files;
-!  - doGP_cldoptics_PADE is rejected with an error, and so is doGP_lwscat
+!  - doGP_cldoptics_PADE is rejected with an error (the library evaluates
+!    Pade cloud optics given through ccpprad_lw_init or a cloud file that
+!    holds only Pade coefficients, but the file entry has no switch to pick
+!    them over a file's lookup tables), and so is doGP_lwscat
 !    together with use_LW_jacobian (rte_lw computes no Jacobian with
 !    two-stream optics).
 module rrtmgp_lw_main

@@ -89,6 +89,17 @@ typedef struct ccpprad_cloud_lut {
     double radliq_lwr, radliq_upr, radice_lwr, radice_upr;
     const double *extliq, *ssaliq, *asyliq;   /* (nsize_liq, nband)          */
     const double *extice, *ssaice, *asyice;   /* (nsize_ice, nband, nrghice) */
+    /* Pade form (the Pade arguments of ty_cloud_optics%load,
+     * rrtmgp_lw_cloud_optics.F90:192-205,345-350), used when pade != 0 (the
+     * LUT arrays and sizes above are then unused; radliq/radice bounds are
+     * the extinction size bounds).  Coefficients (nband, 3 size regimes,
+     * ncoeff): extinction 6 (numerator degree 2, denominator 3), co-albedo
+     * and asymmetry 5 (2, 2); ice (.., nrghice).  Size-regime bounds (4). */
+    int pade;
+    const double *pade_extliq, *pade_ssaliq, *pade_asyliq;
+    const double *pade_extice, *pade_ssaice, *pade_asyice;
+    const double *pade_sizreg_extliq, *pade_sizreg_ssaliq, *pade_sizreg_asyliq;
+    const double *pade_sizreg_extice, *pade_sizreg_ssaice, *pade_sizreg_asyice;
 } ccpprad_cloud_lut;
 
 /* Argument list of rrtmgp_lw_main_run (rrtmgp_lw_main.F90:104-194), in order.
@@ -167,9 +178,9 @@ int ccpprad_sw_init(const ccpprad_kdist *kd, const ccpprad_cloud_lut *lut,
  * formats (CDF-1, CDF-2, CDF-5; netCDF-4/HDF5 files are refused with a
  * message) and reduced to the active gases as ty_gas_optics_rrtmgp%load
  * does, or files in the package's binary table format; each file's format is
- * taken from its signature.  Lookup-table cloud optics only (Pade
- * coefficients are refused); the netCDF cloud path uses the file's last
- * ice-roughness category. */
+ * taken from its signature.  A netCDF cloud file gives the lookup tables
+ * when it has them, else its Pade coefficients; the netCDF cloud path uses
+ * the file's last ice-roughness category. */
 int ccpprad_lw_init_files(const char *gas_file, const char *cloud_file,
                           const char *const *gas_names, int ngases, int nLay,
                           int rrtmgp_phys_blksz, ccpprad_ctx **ctx, char *errmsg, int errlen);

@@ -957,6 +957,46 @@ static void lut_interp(const double *ext, const double *ssa, const double *asy, 
     *t = tt;
 }
 
+/* rte-rrtmgp's pade_eval: numerator degree m, denominator degree n with
+ * constant term 1, coefficient k of band b and size regime r at
+ * c[b + nb (r + 3 k)] (the Fortran (nband, nsizereg, ncoeff) array).
+ * PARITY UNPINNED: the rte-rrtmgp source is absent from the reference tree;
+ * this follows the published algorithm (mo_cloud_optics compute_all_from_pade). */
+static double pade_eval(const double *c, int nb, int b, int r, int m, int n, double re)
+{
+#define PC(k) c[(size_t)b + (size_t)nb * ((size_t)r + 3 * (size_t)(k))]
+    double denom = PC(n + m);
+    for (int i = n - 1 + m; i >= 1 + m; --i) denom = PC(i) + re * denom;
+    denom = 1.0 + re * denom;
+    double numer = PC(m);
+    for (int i = m - 1; i >= 0; --i) numer = PC(i) + re * numer;
+#undef PC
+    return numer / denom;
+}
+
+/* The size regime (0-based) of a radius: min(floor((re - b2) / b3) + 2, 3),
+ * rte's expression for three regimes (four bounds). */
+static int pade_regime(const double *bnd, double re)
+{
+    int ir = (int)floor((re - bnd[1]) / bnd[2]) + 2;
+    if (ir > 3) ir = 3;
+    if (ir < 1) ir = 1;
+    return ir - 1;
+}
+
+/* compute_all_from_pade for one phase and band: t = path ext,
+ * ts = t (1 - max(0, co-albedo)), tsg = ts g. */
+static void pade_phase(const double *ce, const double *cs, const double *ca, const double *be,
+                       const double *bs, const double *ba, int nb, int b, double path, double re,
+                       double *t, double *ts, double *tsg)
+{
+    const double tt = path * pade_eval(ce, nb, b, pade_regime(be, re), 2, 3, re);
+    const double s = tt * (1.0 - fmax(0.0, pade_eval(cs, nb, b, pade_regime(bs, re), 2, 2, re)));
+    *tsg = s * pade_eval(ca, nb, b, pade_regime(ba, re), 2, 2, re);
+    *ts = s;
+    *t = tt;
+}
+
 /* Returns nonzero if an effective radius of a masked element is out of range. */
 int orc_cloud_optics(const ccpprad_cloud_lut *lut, int ncol, int nlay, const double *lwp,
                      const double *iwp, const double *reliq, const double *reice,
@@ -976,6 +1016,17 @@ int orc_cloud_optics(const ccpprad_cloud_lut *lut, int ncol, int nlay, const dou
             for (int c = 0; c < ncol; ++c) {
                 double lt = 0, lts = 0, ltsg = 0, it = 0, its = 0, itsg = 0;
                 double cl = F2(lwp, c, l, ncol), ci = F2(iwp, c, l, ncol);
+                if (lut->pade) {
+                    const size_t io = (size_t)nb * 3 * 6 * (icergh - 1), is = (size_t)nb * 3 * 5 * (icergh - 1);
+                    if (cl > 0.0)
+                        pade_phase(lut->pade_extliq, lut->pade_ssaliq, lut->pade_asyliq, lut->pade_sizreg_extliq,
+                                   lut->pade_sizreg_ssaliq, lut->pade_sizreg_asyliq, nb, b, cl,
+                                   F2(reliq, c, l, ncol), &lt, &lts, &ltsg);
+                    if (ci > 0.0)
+                        pade_phase(lut->pade_extice + io, lut->pade_ssaice + is, lut->pade_asyice + is,
+                                   lut->pade_sizreg_extice, lut->pade_sizreg_ssaice, lut->pade_sizreg_asyice, nb,
+                                   b, ci, F2(reice, c, l, ncol), &it, &its, &itsg);
+                } else {
                 if (cl > 0.0)
                     lut_interp(lut->extliq + (size_t)lut->nsize_liq * b, lut->ssaliq + (size_t)lut->nsize_liq * b,
                                lut->asyliq + (size_t)lut->nsize_liq * b, lut->nsize_liq, lut->radliq_lwr,
@@ -984,6 +1035,7 @@ int orc_cloud_optics(const ccpprad_cloud_lut *lut, int ncol, int nlay, const dou
                     size_t o = (size_t)lut->nsize_ice * (b + (size_t)nb * (icergh - 1));
                     lut_interp(lut->extice + o, lut->ssaice + o, lut->asyice + o, lut->nsize_ice,
                                lut->radice_lwr, lut->radice_upr, ci, F2(reice, c, l, ncol), &it, &its, &itsg);
+                }
                 }
                 double t = lt + it, ts = lts + its, tsg = ltsg + itsg;
                 F3(tau, c, l, b, ncol, nlay) = t;

@@ -192,3 +192,29 @@ def to_cdf5(src, dst):
         out.write(header(begins))
         for _, _, _, raw in vars_:
             out.write(raw + b"\0" * (-len(raw) % 4))
+
+
+def write_pade_nc(path, lut, version=2):
+    """Pade-form cloud optics (lut.pade) in the reference's variable names and
+    dimension orders (rrtmgp_lw_cloud_optics.F90:130-150,192-205,261-288)."""
+    f = netcdf_file(path, "w", version=version)
+    for k, v in dict(nband=lut.nband, nrghice=lut.nrghice, nsizereg=3, ncoeff_ext=6, ncoeff_ssa_g=5,
+                     nbound=4, pair=2).items():
+        f.createDimension(k, v)
+
+    def var(name, dn, data):
+        v = f.createVariable(name, "d", dn)
+        v[...] = data
+
+    for k in ("radliq_lwr", "radliq_upr", "radice_lwr", "radice_upr"):
+        var(k, (), getattr(lut, k))
+    P = lut.pade
+    for ph in ("liq", "ice"):
+        lead = ("nrghice",) if ph == "ice" else ()
+        var(f"pade_ext{ph}", lead + ("ncoeff_ext", "nsizereg", "nband"), P[f"pade_ext{ph}"])
+        var(f"pade_ssa{ph}", lead + ("ncoeff_ssa_g", "nsizereg", "nband"), P[f"pade_ssa{ph}"])
+        var(f"pade_asy{ph}", lead + ("ncoeff_ssa_g", "nsizereg", "nband"), P[f"pade_asy{ph}"])
+        for k in ("ext", "ssa", "asy"):
+            var(f"pade_sizreg_{k}{ph}", ("nbound",), P[f"pade_sizreg_{k}{ph}"])
+    var("bnd_limits_wavenumber", ("nband", "pair"), np.zeros((lut.nband, 2)))
+    f.close()
