@@ -24,7 +24,7 @@ EXPORTS = ["ccpprad_lw_init", "ccpprad_sw_init", "ccpprad_lw_init_files", "ccppr
            "ccpprad_finalize", "ccpprad_heating_rate_dev", "ccpprad_lw_gas_optics_dev",
            "ccpprad_sw_gas_optics_dev", "ccpprad_ngpt", "ccpprad_nband", "ccpprad_version",
            "ccpprad_prepare_inputs_dev", "ccpprad_post_dev", "ccpprad_layer_geometry_dev",
-           "ccpprad_cloud_overlap_dev"]
+           "ccpprad_cloud_overlap_dev", "ccpprad_lw_init_files_pade", "ccpprad_sw_init_files_pade"]
 
 
 def load():
@@ -42,6 +42,8 @@ def load():
     L.ccpprad_lw_init_files.argtypes = [C.c_char_p, C.c_char_p, vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp),
                                         C.c_char_p, C.c_int]
     L.ccpprad_sw_init_files.argtypes = L.ccpprad_lw_init_files.argtypes
+    L.ccpprad_lw_init_files_pade.argtypes = L.ccpprad_lw_init_files.argtypes
+    L.ccpprad_sw_init_files_pade.argtypes = L.ccpprad_lw_init_files.argtypes
     L.ccpprad_lw_run.argtypes = [vp, vp, C.c_char_p, C.c_int]
     L.ccpprad_sw_run.argtypes = [vp, vp, C.c_char_p, C.c_int]
     L.ccpprad_lw_run_dev.argtypes = [vp, vp, vp, C.c_int, C.c_char_p, C.c_int]
@@ -96,16 +98,20 @@ class _Scheme:
         self.ctx = ctx
 
     @classmethod
-    def from_files(cls, gas_file, cloud_file, nlay: int, blksz: int = 1, gas_names=syn.GAS_NAMES):
+    def from_files(cls, gas_file, cloud_file, nlay: int, blksz: int = 1, gas_names=syn.GAS_NAMES, pade=False):
         """rrtmgp_*_main_init from table files: the reference's netCDF files
-        (classic formats) or this package's binary table files."""
+        (classic formats) or this package's binary table files; pade: the
+        cloud file's Pade coefficients (doGP_cldoptics_PADE)."""
         self = cls.__new__(cls)
         self.L = load()
         self.kd, self.lut, self.nlay = None, None, nlay
         names = _gas_names(list(gas_names))
         ctx = C.c_void_p()
         buf, n = capi.errbuf()
-        fn = self.L.ccpprad_lw_init_files if cls.is_lw else self.L.ccpprad_sw_init_files
+        if pade:
+            fn = self.L.ccpprad_lw_init_files_pade if cls.is_lw else self.L.ccpprad_sw_init_files_pade
+        else:
+            fn = self.L.ccpprad_lw_init_files if cls.is_lw else self.L.ccpprad_sw_init_files
         rc = fn(str(gas_file).encode(), (str(cloud_file) if cloud_file else "").encode(), names, len(gas_names),
                 nlay, blksz, C.byref(ctx), buf, n)
         capi.check(rc, buf)

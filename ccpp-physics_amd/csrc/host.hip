@@ -313,7 +313,7 @@ bool lut_from_records(TableHold &h)
 // reference's netCDF files (read natively, classic formats) or this
 // package's binary table files, each file's format taken from its signature.
 int init_files(int lw, const char *gas_file, const char *cloud_file, const char *const *gas_names, int ngases,
-               int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg, int errlen)
+               int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg, int errlen, bool pade = false)
 {
     TableHold h;
     NcKdistHold kh;
@@ -336,9 +336,10 @@ int init_files(int lw, const char *gas_file, const char *cloud_file, const char 
     }
     if (cloud_file && cloud_file[0]) {
         if (is_netcdf_file(cloud_file)) {
-            if (!load_lut_nc(cloud_file, lh, err)) return fail(cloud_file, err);
+            if (!load_lut_nc(cloud_file, lh, err, pade)) return fail(cloud_file, err);
             lut = &lh.lut;
         } else {
+            if (pade) return fail(cloud_file, "Pade cloud optics come from a netCDF cloud file");
             if (!read_file(cloud_file, h.l, err)) return fail(cloud_file, err);
             if (!lut_from_records(h)) return fail(cloud_file, "table file lacks a required record or has a bad size");
             lut = &h.lut;
@@ -359,4 +360,18 @@ extern "C" int ccpprad_sw_init_files(const char *gas_file, const char *cloud_fil
                                      int ngases, int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg, int errlen)
 {
     return init_files(0, gas_file, cloud_file, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen);
+}
+
+extern "C" int ccpprad_lw_init_files_pade(const char *gas_file, const char *cloud_file, const char *const *gas_names,
+                                          int ngases, int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg,
+                                          int errlen)
+{
+    return init_files(1, gas_file, cloud_file, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen, true);
+}
+
+extern "C" int ccpprad_sw_init_files_pade(const char *gas_file, const char *cloud_file, const char *const *gas_names,
+                                          int ngases, int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg,
+                                          int errlen)
+{
+    return init_files(0, gas_file, cloud_file, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen, true);
 }

@@ -37,6 +37,7 @@ bool load_kdist_nc(const char *path, int want_lw, const char *const *gas_names, 
                    std::string &err);
 // the cloud optics: the lookup tables when the file has them, else the Pade
 // coefficients (the last ice-roughness category selected)
-bool load_lut_nc(const char *path, NcLutHold &h, std::string &err);
+// (force_pade: the Pade coefficients even when the file also has tables)
+bool load_lut_nc(const char *path, NcLutHold &h, std::string &err, bool force_pade = false);
 
 }  // namespace ccpprad

@@ -506,14 +506,15 @@ bool load_kdist_nc(const char *path, int want_lw, const char *const *gas_names, 
     return true;
 }
 
-bool load_lut_nc(const char *path, NcLutHold &h, std::string &err)
+bool load_lut_nc(const char *path, NcLutHold &h, std::string &err, bool force_pade)
 {
     Reader R;
     if (!nc_open(path, R.f, err)) return false;
     const size_t nb = R.dim("nband"), nr = R.dim("nrghice");
-    const size_t nl = R.has("lut_extliq") ? R.dim("nsize_liq") : 0, ni = R.has("lut_extliq") ? R.dim("nsize_ice") : 0;
+    const bool lut = R.has("lut_extliq") && !force_pade;
+    const size_t nl = lut ? R.dim("nsize_liq") : 0, ni = lut ? R.dim("nsize_ice") : 0;
     if (!R.missing.empty()) { err = "netCDF cloud-optics file lacks " + R.missing; return false; }
-    if (!R.has("lut_extliq")) {
+    if (!lut) {
         // Pade form (rrtmgp_lw_cloud_optics.F90:192-205,261-288): coefficients
         // (nband, nsizereg, ncoeff[, nrghice]) and the size-regime bounds
         const size_t ns = R.dim("nsizereg"), ne = R.dim("ncoeff_ext"), nsg = R.dim("ncoeff_ssa_g");

@@ -62,6 +62,26 @@ module ccpprad_c
        character(kind=c_char), dimension(*) :: errmsg
        integer(c_int), value :: errlen
      end function ccpprad_sw_init_files
+     integer(c_int) function ccpprad_lw_init_files_pade(gas_file, cloud_file, gas_names, ngases, nLay, blksz, &
+          ctx, errmsg, errlen) bind(C, name="ccpprad_lw_init_files_pade")
+       import :: c_int, c_ptr, c_char
+       character(kind=c_char), dimension(*) :: gas_file, cloud_file
+       type(c_ptr), value :: gas_names
+       integer(c_int), value :: ngases, nLay, blksz
+       type(c_ptr) :: ctx
+       character(kind=c_char), dimension(*) :: errmsg
+       integer(c_int), value :: errlen
+     end function ccpprad_lw_init_files_pade
+     integer(c_int) function ccpprad_sw_init_files_pade(gas_file, cloud_file, gas_names, ngases, nLay, blksz, &
+          ctx, errmsg, errlen) bind(C, name="ccpprad_sw_init_files_pade")
+       import :: c_int, c_ptr, c_char
+       character(kind=c_char), dimension(*) :: gas_file, cloud_file
+       type(c_ptr), value :: gas_names
+       integer(c_int), value :: ngases, nLay, blksz
+       type(c_ptr) :: ctx
+       character(kind=c_char), dimension(*) :: errmsg
+       integer(c_int), value :: errlen
+     end function ccpprad_sw_init_files_pade
      integer(c_int) function ccpprad_lw_run(ctx, args, errmsg, errlen) bind(C, name="ccpprad_lw_run")
        import :: c_int, c_ptr, c_char, ccpprad_lw_args_t
        type(c_ptr), value :: ctx

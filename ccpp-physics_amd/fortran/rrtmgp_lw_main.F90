@@ -11,12 +11,10 @@
 !    files in the netCDF classic formats (netCDF-4 files are refused with a
 !    message; `nccopy -k cdf5` converts them), or the package's binary table
 !    files;
-!  - doGP_cldoptics_PADE is rejected with an error (the library evaluates
-!    Pade cloud optics given through ccpprad_lw_init or a cloud file that
-!    holds only Pade coefficients, but the file entry has no switch to pick
-!    them over a file's lookup tables), and so is doGP_lwscat
-!    together with use_LW_jacobian (rte_lw computes no Jacobian with
-!    two-stream optics).
+!  - doGP_lwscat together with use_LW_jacobian is rejected with an error
+!    (rte_lw computes no Jacobian with two-stream optics);
+!  - doGP_cldoptics_PADE without doGP_cldoptics_LUT reads the cloud file's
+!    Pade coefficients (ccpprad_lw_init_files_pade).
 module rrtmgp_lw_main
   use mpi_f08,   only: MPI_Comm
   use iso_c_binding
@@ -49,22 +47,23 @@ contains
 
     errmsg = ''
     errflg = 0
-    if (doGP_cldoptics_PADE .and. .not. doGP_cldoptics_LUT) then
-       errmsg = 'rrtmgp_lw_main_init: Pade cloud optics are not supported'
-       errflg = 1
-       return
-    endif
     do i = 1, size(active_gases_array)
        names(i) = trim(active_gases_array(i))//c_null_char
        pnames(i) = c_loc(names(i))
     enddo
     cloud = ''
-    if (doGP_cldoptics_LUT) cloud = trim(rrtmgp_root_dir)//trim(rrtmgp_lw_file_clouds)
+    if (doGP_cldoptics_LUT .or. doGP_cldoptics_PADE) cloud = trim(rrtmgp_root_dir)//trim(rrtmgp_lw_file_clouds)
     if (c_associated(lw_ctx)) call ccpprad_finalize(lw_ctx)
     buf = c_null_char
-    rc = ccpprad_lw_init_files(c_str(trim(rrtmgp_root_dir)//trim(rrtmgp_lw_file_gas)), c_str(cloud), &
-         c_loc(pnames), int(size(active_gases_array), c_int), int(nLay, c_int),                       &
-         int(rrtmgp_phys_blksz, c_int), lw_ctx, buf, int(size(buf), c_int))
+    if (doGP_cldoptics_PADE .and. .not. doGP_cldoptics_LUT) then
+        rc = ccpprad_lw_init_files_pade(c_str(trim(rrtmgp_root_dir)//trim(rrtmgp_lw_file_gas)), c_str(cloud), &
+            c_loc(pnames), int(size(active_gases_array), c_int), int(nLay, c_int),                       &
+            int(rrtmgp_phys_blksz, c_int), lw_ctx, buf, int(size(buf), c_int))
+    else
+        rc = ccpprad_lw_init_files(c_str(trim(rrtmgp_root_dir)//trim(rrtmgp_lw_file_gas)), c_str(cloud), &
+            c_loc(pnames), int(size(active_gases_array), c_int), int(nLay, c_int),                       &
+            int(rrtmgp_phys_blksz, c_int), lw_ctx, buf, int(size(buf), c_int))
+    endif
     if (rc /= 0) then
        call f_msg(buf, errmsg)
        errflg = 1

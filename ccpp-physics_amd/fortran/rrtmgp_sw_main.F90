@@ -44,22 +44,23 @@ contains
 
     errmsg = ''
     errflg = 0
-    if (doGP_cldoptics_PADE .and. .not. doGP_cldoptics_LUT) then
-       errmsg = 'rrtmgp_sw_main_init: Pade cloud optics are not supported'
-       errflg = 1
-       return
-    endif
     do i = 1, size(active_gases_array)
        names(i) = trim(active_gases_array(i))//c_null_char
        pnames(i) = c_loc(names(i))
     enddo
     cloud = ''
-    if (doGP_cldoptics_LUT) cloud = trim(rrtmgp_root_dir)//trim(rrtmgp_sw_file_clouds)
+    if (doGP_cldoptics_LUT .or. doGP_cldoptics_PADE) cloud = trim(rrtmgp_root_dir)//trim(rrtmgp_sw_file_clouds)
     if (c_associated(sw_ctx)) call ccpprad_finalize(sw_ctx)
     buf = c_null_char
-    rc = ccpprad_sw_init_files(c_str(trim(rrtmgp_root_dir)//trim(rrtmgp_sw_file_gas)), c_str(cloud), &
-         c_loc(pnames), int(size(active_gases_array), c_int), int(nLay, c_int),                       &
-         int(rrtmgp_phys_blksz, c_int), sw_ctx, buf, int(size(buf), c_int))
+    if (doGP_cldoptics_PADE .and. .not. doGP_cldoptics_LUT) then
+        rc = ccpprad_sw_init_files_pade(c_str(trim(rrtmgp_root_dir)//trim(rrtmgp_sw_file_gas)), c_str(cloud), &
+            c_loc(pnames), int(size(active_gases_array), c_int), int(nLay, c_int),                       &
+            int(rrtmgp_phys_blksz, c_int), sw_ctx, buf, int(size(buf), c_int))
+    else
+        rc = ccpprad_sw_init_files(c_str(trim(rrtmgp_root_dir)//trim(rrtmgp_sw_file_gas)), c_str(cloud), &
+            c_loc(pnames), int(size(active_gases_array), c_int), int(nLay, c_int),                       &
+            int(rrtmgp_phys_blksz, c_int), sw_ctx, buf, int(size(buf), c_int))
+    endif
     if (rc /= 0) then
        call f_msg(buf, errmsg)
        errflg = 1

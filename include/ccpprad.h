@@ -187,6 +187,15 @@ int ccpprad_lw_init_files(const char *gas_file, const char *cloud_file,
 int ccpprad_sw_init_files(const char *gas_file, const char *cloud_file,
                           const char *const *gas_names, int ngases, int nLay,
                           int rrtmgp_phys_blksz, ccpprad_ctx **ctx, char *errmsg, int errlen);
+/* The same with doGP_cldoptics_PADE (rrtmgp_lw_cloud_optics.F90:192-205,
+ * 261-288,345-350): the netCDF cloud file's Pade coefficients, also where the
+ * file holds lookup tables too. */
+int ccpprad_lw_init_files_pade(const char *gas_file, const char *cloud_file,
+                               const char *const *gas_names, int ngases, int nLay,
+                               int rrtmgp_phys_blksz, ccpprad_ctx **ctx, char *errmsg, int errlen);
+int ccpprad_sw_init_files_pade(const char *gas_file, const char *cloud_file,
+                               const char *const *gas_names, int ngases, int nLay,
+                               int rrtmgp_phys_blksz, ccpprad_ctx **ctx, char *errmsg, int errlen);
 
 /* rrtmgp_lw_main_run / rrtmgp_sw_main_run with host arrays. */
 int ccpprad_lw_run(ccpprad_ctx *ctx, const ccpprad_lw_args *a, char *errmsg, int errlen);

@@ -194,12 +194,15 @@ def to_cdf5(src, dst):
             out.write(raw + b"\0" * (-len(raw) % 4))
 
 
-def write_pade_nc(path, lut, version=2):
+def write_pade_nc(path, lut, version=2, with_lut=False):
     """Pade-form cloud optics (lut.pade) in the reference's variable names and
-    dimension orders (rrtmgp_lw_cloud_optics.F90:130-150,192-205,261-288)."""
+    dimension orders (rrtmgp_lw_cloud_optics.F90:130-150,192-205,261-288);
+    with_lut: the lookup tables too, as the reference's files hold both."""
     f = netcdf_file(path, "w", version=version)
-    for k, v in dict(nband=lut.nband, nrghice=lut.nrghice, nsizereg=3, ncoeff_ext=6, ncoeff_ssa_g=5,
-                     nbound=4, pair=2).items():
+    dims = dict(nband=lut.nband, nrghice=lut.nrghice, nsizereg=3, ncoeff_ext=6, ncoeff_ssa_g=5, nbound=4, pair=2)
+    if with_lut:
+        dims.update(nsize_liq=lut.nsize_liq, nsize_ice=lut.nsize_ice)
+    for k, v in dims.items():
         f.createDimension(k, v)
 
     def var(name, dn, data):
@@ -216,5 +219,10 @@ def write_pade_nc(path, lut, version=2):
         var(f"pade_asy{ph}", lead + ("ncoeff_ssa_g", "nsizereg", "nband"), P[f"pade_asy{ph}"])
         for k in ("ext", "ssa", "asy"):
             var(f"pade_sizreg_{k}{ph}", ("nbound",), P[f"pade_sizreg_{k}{ph}"])
+    if with_lut:
+        for k in ("extliq", "ssaliq", "asyliq"):
+            var("lut_" + k, ("nband", "nsize_liq"), getattr(lut, k))
+        for k in ("extice", "ssaice", "asyice"):
+            var("lut_" + k, ("nrghice", "nband", "nsize_ice"), getattr(lut, k))
     var("bnd_limits_wavenumber", ("nband", "pair"), np.zeros((lut.nband, 2)))
     f.close()

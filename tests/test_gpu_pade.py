@@ -75,3 +75,46 @@ def test_pade_from_netcdf_matches_struct(tmp_path, kd_lw):
         res.append(device.to_host(dev))
     for n in ("fluxlwUP_allsky", "fluxlwDOWN_allsky", "htrlw"):
         assert np.array_equal(res[0][n], res[1][n]), n
+
+
+@pytest.mark.parametrize("is_lw", [True, False])
+def test_cloud_file_with_both_forms(tmp_path, kd_lw, kd_sw, is_lw):
+    """A cloud file with lookup tables and Pade coefficients (as the
+    reference's files): the Pade entry (doGP_cldoptics_PADE) gives the Pade
+    struct's results, the plain entry the lookup tables', bit for bit."""
+    import dataclasses
+    from ccpprad import capi, device, lib, synthetic as syn
+    from nc_writer import write_kdist_nc, write_pade_nc
+    kd = kd_lw if is_lw else kd_sw
+    both = syn.make_cloud_pade(is_lw)
+    both = dataclasses.replace(both, icergh=both.nrghice)
+    write_kdist_nc(tmp_path / "k.nc", kd)
+    write_pade_nc(tmp_path / "c.nc", both, with_lut=True)
+    lut_only = dataclasses.replace(syn.make_cloud_lut(is_lw), icergh=both.nrghice)
+    ncol, nlay = 64, 127
+    print(f"cloud file with both forms: {'LW' if is_lw else 'SW'} {ncol} columns", flush=True)
+    arr = capi.profile_fields(syn.make_profiles(ncol, nlay, seed=37, clouds=True, kd=kd))
+    cls = lib.LWScheme if is_lw else lib.SWScheme
+    if is_lw:
+        ints, outs, names = capi.lw_default_ints(ncol, nlay, blksz=ncol, nangle=3), device.lw_outputs, \
+            ("fluxlwUP_allsky", "fluxlwDOWN_allsky", "htrlw")
+    else:
+        arr["idx"] = np.arange(1, ncol + 1, dtype=np.int32)
+        ints, outs, names = capi.sw_default_ints(ncol, ncol, nlay, blksz=ncol), device.sw_outputs, \
+            ("fluxswUP_allsky", "fluxswDOWN_allsky", "htrsw")
+
+    def run(s):
+        dev = device.to_device(arr)
+        dev.update(outs(ncol, nlay))
+        s.run_dev(ints, dev)
+        torch.cuda.synchronize()
+        return device.to_host(dev)
+
+    f_pade = run(cls.from_files(tmp_path / "k.nc", tmp_path / "c.nc", nlay, blksz=ncol, pade=True))
+    f_lut = run(cls.from_files(tmp_path / "k.nc", tmp_path / "c.nc", nlay, blksz=ncol))
+    s_pade = run(cls(kd, both, nlay, blksz=ncol))
+    s_lut = run(cls(kd, lut_only, nlay, blksz=ncol))
+    for n in names:
+        assert np.array_equal(f_pade[n], s_pade[n]), n
+        assert np.array_equal(f_lut[n], s_lut[n]), n
+    assert not np.array_equal(f_pade[names[0]], f_lut[names[0]])
