@@ -102,11 +102,12 @@ __device__ __forceinline__ bool lut_out_of_range(const LutView &L, float lwp, fl
     return (lwp > 0.0f && !(rl >= L.liq_lwr && rl <= L.liq_upr)) || (iwp > 0.0f && !(ri >= L.ice_lwr && ri <= L.ice_upr));
 }
 
+template <bool PADE>
 __device__ __forceinline__ bool lut_cloud(const LutView &L, int b, float lwp, float iwp, float rl, float ri, Opt2 &o)
 {
     float lt = 0, lts = 0, lta = 0, ltsg = 0, it = 0, its = 0, ita = 0, itsg = 0;
     bool bad = false;
-    if (L.pade) {
+    if (PADE) {
         if (lwp > 0.0f) {
             bad |= !(rl >= L.liq_lwr && rl <= L.liq_upr);
             pade_phase(L, 0, b, lwp, fminf(fmaxf(rl, L.liq_lwr), L.liq_upr), lt, lts, lta, ltsg);
@@ -165,12 +166,13 @@ __device__ __forceinline__ const float *lut_table(const LutView &L, int t)
 // for all bands, and its [l][d][b] outputs form one contiguous block.  The
 // LUT is staged in LDS once per block, so an item waits on one round of
 // global loads.  Items are strided over a grid of a few blocks per CU.
+template <bool PADE>
 __global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs A, int ncols, unsigned n)
 {
     extern __shared__ float slut[];
     const int nl = L.nband * L.nsize_liq, ni = L.nband * L.nsize_ice, ntot = 4 * (nl + ni);
     LutView S = L;
-    if (L.pade) {
+    if (PADE) {
         const int np = 2 * L.nband * 48;
         for (int k = threadIdx.x; k < np; k += blockDim.x) slut[k] = L.pcoef[k];
         S.pcoef = slut;
@@ -241,13 +243,13 @@ __global__ __launch_bounds__(256) void cloud_optics_kernel(LutView L, CloudArgs 
         }
         for (int b = 0; b < nb; ++b) {
             Opt2 o;
-            bad |= lut_cloud(S, b, lwp, iwp, rl, ri, o);
+            bad |= lut_cloud<PADE>(S, b, lwp, iwp, rl, ri, o);
             if (!A.is_lw && A.cldtau && b == 10 && blk == A.cldtau_blk) A.cldtau[q] = (double)o.t;
             for (int k = 0; k < 2; ++k) {
                 const double *const *src = k == 0 ? A.cnv : A.pbl;
                 if (!src[0]) continue;
                 Opt2 o2;
-                bad |= lut_cloud(S, b, (float)src[0][q], (float)src[1][q], (float)src[2][q], (float)src[3][q], o2);
+                bad |= lut_cloud<PADE>(S, b, (float)src[0][q], (float)src[1][q], (float)src[2][q], (float)src[3][q], o2);
                 inc2s(o2, o);
             }
             Opt2 pr = {0.0f, 0.0f, 1.0f, 0.0f};
@@ -472,8 +474,12 @@ void launch_cloud_optics(const LutView &L, const CloudArgs &A, int ncols, hipStr
     const size_t n = (size_t)ncols * A.nlay;
     if (!n) return;
     const size_t nblk = std::min<size_t>((n + 255) / 256, 2048);
-    hipLaunchKernelGGL(cloud_optics_kernel, dim3((unsigned)nblk), dim3(256), sizeof(float) * cloud_lut_floats(L), st,
-                       L, A, ncols, (unsigned)n);
+    if (L.pade)
+        hipLaunchKernelGGL(cloud_optics_kernel<true>, dim3((unsigned)nblk), dim3(256),
+                           sizeof(float) * cloud_lut_floats(L), st, L, A, ncols, (unsigned)n);
+    else
+        hipLaunchKernelGGL(cloud_optics_kernel<false>, dim3((unsigned)nblk), dim3(256),
+                           sizeof(float) * cloud_lut_floats(L), st, L, A, ncols, (unsigned)n);
 }
 
 void launch_mcica(const McicaArgs &A, int ncols, hipStream_t st)
