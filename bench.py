@@ -39,9 +39,9 @@ METRIC = "LW+SW columns/s at 127 layers (table vs NN) @ HR RMSE<=0.1 K/day vs FP
 # --no-nn --copy-probe`), each scaled by the reading of a 1 GiB device copy
 # (FETCH x 2.0, WRITE x 1.0).  Counters cannot be read from inside this
 # process, so the last measurement of the current kernels is recorded here.
-TRAFFIC_GB = {"sw_sw_solve": 19.585, "lw_lw_solve": 22.660, "lw_gas_optics": 11.728, "sw_gas_optics": 10.669,
+TRAFFIC_GB = {"sw_sw_solve": 19.579, "lw_lw_solve": 22.667, "lw_gas_optics": 11.728, "sw_gas_optics": 10.670,
               "lw_gas_interp": 3.026, "sw_gas_interp": 3.026, "lw_cloud_optics": 0.776, "sw_cloud_optics": 0.776,
-              "lw_mcica": 0.670, "sw_mcica": 0.670, "lw_solve_2str": 24.527}
+              "lw_mcica": 0.670, "sw_mcica": 0.670, "lw_solve_2str": 24.485}
 
 
 def parse():
