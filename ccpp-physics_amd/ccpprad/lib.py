@@ -24,7 +24,9 @@ EXPORTS = ["ccpprad_lw_init", "ccpprad_sw_init", "ccpprad_lw_init_files", "ccppr
            "ccpprad_finalize", "ccpprad_heating_rate_dev", "ccpprad_lw_gas_optics_dev",
            "ccpprad_sw_gas_optics_dev", "ccpprad_ngpt", "ccpprad_nband", "ccpprad_version",
            "ccpprad_prepare_inputs_dev", "ccpprad_post_dev", "ccpprad_layer_geometry_dev",
-           "ccpprad_cloud_overlap_dev", "ccpprad_lw_init_files_pade", "ccpprad_sw_init_files_pade"]
+           "ccpprad_cloud_overlap_dev", "ccpprad_lw_init_files_pade", "ccpprad_sw_init_files_pade",
+           "ccpprad_heating_rate", "ccpprad_set_launch_columns", "ccpprad_launch_columns",
+           "ccpprad_lw_mcica_mask_dev", "ccpprad_sw_mcica_mask_dev"]
 
 
 def load():
@@ -39,8 +41,8 @@ def load():
     vp = C.c_void_p
     L.ccpprad_lw_init.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp), C.c_char_p, C.c_int]
     L.ccpprad_sw_init.argtypes = L.ccpprad_lw_init.argtypes
-    L.ccpprad_lw_init_files.argtypes = [C.c_char_p, C.c_char_p, vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp),
-                                        C.c_char_p, C.c_int]
+    L.ccpprad_lw_init_files.argtypes = [C.c_char_p, C.c_char_p, C.c_int, vp, C.c_int, C.c_int, C.c_int,
+                                        C.POINTER(vp), C.c_char_p, C.c_int]
     L.ccpprad_sw_init_files.argtypes = L.ccpprad_lw_init_files.argtypes
     L.ccpprad_lw_init_files_pade.argtypes = L.ccpprad_lw_init_files.argtypes
     L.ccpprad_sw_init_files_pade.argtypes = L.ccpprad_lw_init_files.argtypes
@@ -51,6 +53,11 @@ def load():
     L.ccpprad_finalize.argtypes = [vp]
     L.ccpprad_finalize.restype = None
     L.ccpprad_heating_rate_dev.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_char_p, C.c_int]
+    L.ccpprad_heating_rate.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, C.c_char_p, C.c_int]
+    L.ccpprad_set_launch_columns.argtypes = [vp, C.c_int]
+    L.ccpprad_launch_columns.argtypes = [vp]
+    L.ccpprad_lw_mcica_mask_dev.argtypes = [vp, vp, vp, vp, C.c_char_p, C.c_int]
+    L.ccpprad_sw_mcica_mask_dev.argtypes = [vp, vp, vp, vp, C.c_char_p, C.c_int]
     L.ccpprad_lw_gas_optics_dev.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp, C.c_char_p, C.c_int]
     L.ccpprad_sw_gas_optics_dev.argtypes = L.ccpprad_lw_gas_optics_dev.argtypes
     L.ccpprad_prepare_inputs_dev.argtypes = [vp, C.c_int, C.c_int] + [vp] * 14 + [vp, C.c_char_p, C.c_int]
@@ -98,10 +105,12 @@ class _Scheme:
         self.ctx = ctx
 
     @classmethod
-    def from_files(cls, gas_file, cloud_file, nlay: int, blksz: int = 1, gas_names=syn.GAS_NAMES, pade=False):
+    def from_files(cls, gas_file, cloud_file, nlay: int, blksz: int = 1, gas_names=syn.GAS_NAMES, pade=False,
+                   nrghice=0):
         """rrtmgp_*_main_init from table files: the reference's netCDF files
         (classic formats) or this package's binary table files; pade: the
-        cloud file's Pade coefficients (doGP_cldoptics_PADE)."""
+        cloud file's Pade coefficients (doGP_cldoptics_PADE); nrghice: the
+        namelist's ice-roughness category (0 = the file's)."""
         self = cls.__new__(cls)
         self.L = load()
         self.kd, self.lut, self.nlay = None, None, nlay
@@ -112,11 +121,17 @@ class _Scheme:
             fn = self.L.ccpprad_lw_init_files_pade if cls.is_lw else self.L.ccpprad_sw_init_files_pade
         else:
             fn = self.L.ccpprad_lw_init_files if cls.is_lw else self.L.ccpprad_sw_init_files
-        rc = fn(str(gas_file).encode(), (str(cloud_file) if cloud_file else "").encode(), names, len(gas_names),
-                nlay, blksz, C.byref(ctx), buf, n)
+        rc = fn(str(gas_file).encode(), (str(cloud_file) if cloud_file else "").encode(), int(nrghice), names,
+                len(gas_names), nlay, blksz, C.byref(ctx), buf, n)
         capi.check(rc, buf)
         self.ctx = ctx
         return self
+
+    def set_launch_columns(self, ncol):
+        """Columns per launch (bounds the per-stream work buffers)."""
+        if self.L.ccpprad_set_launch_columns(self.ctx, int(ncol)) != 0:
+            raise capi.CcppradError("ccpprad_set_launch_columns failed")
+        return self.L.ccpprad_launch_columns(self.ctx)
 
     @property
     def ngpt(self):
@@ -170,6 +185,14 @@ class LWScheme(_Scheme):
                                           buf, n)
         capi.check(rc, buf)
 
+    def mcica_mask_dev(self, ints: dict, arrays: dict, mask, stream=None):
+        """The McICA mask of a _run call (mask: device uint32/int32 tensor of
+        nCol * nLay * 4 words, [col][lay][word])."""
+        h = capi.build_args(capi.LWArgs, ints, arrays)
+        buf, n = capi.errbuf()
+        rc = self.L.ccpprad_lw_mcica_mask_dev(self.ctx, C.addressof(h.struct), _dp(mask), stream, buf, n)
+        capi.check(rc, buf)
+
     def gas_optics_dev(self, ncol, nlay, p_lay, p_lev, t_lay, vmr, tau, pfrac, stream=None):
         buf, n = capi.errbuf()
         rc = self.L.ccpprad_lw_gas_optics_dev(self.ctx, ncol, nlay, _dp(p_lay), _dp(p_lev), _dp(t_lay),
@@ -192,6 +215,14 @@ class SWScheme(_Scheme):
         rc = self.L.ccpprad_sw_run(self.ctx, C.addressof(h.struct), buf, n)
         capi.check(rc, buf)
 
+    def mcica_mask_dev(self, ints: dict, arrays: dict, mask, stream=None):
+        """The McICA mask of a _run call over the daylit columns (mask:
+        device tensor of nDay * nLay * 4 32-bit words)."""
+        h = capi.build_args(capi.SWArgs, ints, arrays)
+        buf, n = capi.errbuf()
+        rc = self.L.ccpprad_sw_mcica_mask_dev(self.ctx, C.addressof(h.struct), _dp(mask), stream, buf, n)
+        capi.check(rc, buf)
+
     def gas_optics_dev(self, ncol, nlay, p_lay, p_lev, t_lay, vmr, tau_abs, tau_ray, stream=None):
         buf, n = capi.errbuf()
         rc = self.L.ccpprad_sw_gas_optics_dev(self.ctx, ncol, nlay, _dp(p_lay), _dp(p_lev), _dp(t_lay),
@@ -204,6 +235,19 @@ def heating_rate_dev(ncol, nlay, up, dn, p_lev, htr, stream=None):
     buf, n = capi.errbuf()
     rc = L.ccpprad_heating_rate_dev(ncol, nlay, _dp(up), _dp(dn), _dp(p_lev), _dp(htr), stream, buf, n)
     capi.check(rc, buf)
+
+
+def heating_rate(ncol, nlay, up, dn, p_lev):
+    """compute_heating_rate with host arrays (numpy FP64, Fortran (ncol,
+    nlay+1) order); returns the (nlay, ncol) numpy heating rate in K/s."""
+    L = load()
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    up, dn, p_lev = c(up), c(dn), c(p_lev)
+    htr = np.zeros((nlay, ncol))
+    buf, n = capi.errbuf()
+    rc = L.ccpprad_heating_rate(ncol, nlay, _dp(up), _dp(dn), _dp(p_lev), _dp(htr), buf, n)
+    capi.check(rc, buf)
+    return htr
 
 
 def layer_geometry_dev(ncol, nlev, p_lev, p_lay, t_lay, q_lay, tv_lay, dz, dzc, dp, con_rd=287.05,

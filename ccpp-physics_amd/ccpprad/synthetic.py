@@ -279,9 +279,11 @@ class CloudLUT:
 def make_cloud_pade(is_lw: bool, seed: int = 20261203) -> CloudLUT:
     """Pade-form cloud optics of the same band count: per band and size
     regime, extinction ~ a re / (1 + b re + c re^2 + d re^3) (falling like
-    1/re), co-albedo and asymmetry nearly constant, positive denominators;
-    liquid regimes split at 6.5 and 11 um (bounds 2.5..21.5), ice at 30 and
-    70 um (10..180)."""
+    1/re), co-albedo and asymmetry nearly constant, positive denominators.
+    Size-regime bounds (b1, b2, b3, b4): liquid (2.5, 6.5, 11, 21.5), ice
+    (10, 30, 70, 180); the regime rule min(floor((re - b2) / b3) + 2, 3) puts
+    the regime edges at b2 and b2 + b3: liquid 6.5 and 17.5 um, ice 30 and
+    100 um."""
     rng = np.random.default_rng(seed + (11 if is_lw else 1011))
     nb, nr = (16 if is_lw else 14), 3
     lut = make_cloud_lut(is_lw, seed)

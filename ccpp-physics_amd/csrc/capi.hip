@@ -100,7 +100,8 @@ struct ccpprad_ctx {
     int *alb_cat = nullptr;
     int ibd = 0, ibd_uv = 0;
     const double *solar_dev = nullptr;
-    int chunk = 65536;
+    int chunk = 65536;                  // columns per launch
+    int chunk_max = 65536;              // its bound (32-bit cloud-record offsets)
     // learned gas optics (LW): MLP with table fallback, or null = table
     ccpprad_mlp *mlp = nullptr;
     // workspaces and the lock for them and for the timing records; held while
@@ -276,15 +277,14 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
     c->nlay = nLay;
     c->blksz = blksz;
     (void)hipGetDevice(&c->device);
-    const char *ch = getenv("CCPPRAD_CHUNK");
-    if (ch && atoi(ch) > 0) c->chunk = atoi(ch);
     // the solvers form byte offsets into a launch's cloud records
     // ([layer][column][band], 12 bytes each) in 32 bits: bound the columns
     // per launch accordingly
     {
         const size_t per_col = (size_t)std::max(nLay, 1) * std::max(kd->nband, 1) * 12;
         const size_t cmax = (((size_t)1 << 32) - 1) / per_col;
-        if ((size_t)c->chunk > cmax) c->chunk = (int)std::max<size_t>(cmax, 1);
+        c->chunk_max = (int)std::min<size_t>(std::max<size_t>(cmax, 1), (size_t)1 << 30);
+        c->chunk = std::min(c->chunk, c->chunk_max);
     }
     hipError_t e = hipSuccess;
     TablesView &T = c->T;
@@ -455,7 +455,16 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
                                 pc[((size_t)(ph * nb + b) * 3 + r) * 16 + off[f] + k] =
                                     (float)src[(size_t)b + (size_t)nb * (r + 3 * (size_t)k)];
                     for (int i = 0; i < 4; ++i) L.pbnd[ph][f][i] = (float)B[ph][f][i];
-                    if (!(B[ph][f][2] > 0.0)) { delete c; seterr(errmsg, errlen, "Pade size-regime bounds must be increasing"); return 1; }
+                    // regime min(floor((re - b2) / b3) + 2, 3) of sizes in [b1, b4]:
+                    // b1 <= b2 <= b4 and b2 - b3 <= b1 keep it at 1 or above
+                    // without a clamp; the kernel's clamp then never acts
+                    const double *b = B[ph][f];
+                    if (!(b[2] > 0.0) || !(b[0] <= b[1]) || !(b[1] <= b[3]) || !(b[1] - b[2] <= b[0])) {
+                        delete c;
+                        seterr(errmsg, errlen, "Pade size-regime bounds (%g, %g, %g, %g) need b1 <= b2 <= b4, b3 > 0 "
+                               "and b2 - b3 <= b1", b[0], b[1], b[2], b[3]);
+                        return 1;
+                    }
                 }
             L.pcoef = upload(c, pc.data(), pc.size(), &e);
             L.nband = nb; L.nsize_liq = 0; L.nsize_ice = 0;
@@ -532,6 +541,16 @@ extern "C" int ccpprad_lw_set_gas_optics_mlp(ccpprad_ctx *c, ccpprad_mlp *m)
     return 0;
 }
 
+extern "C" int ccpprad_set_launch_columns(ccpprad_ctx *c, int ncol)
+{
+    if (!c || ncol < 1) return 1;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->chunk = std::min(ncol, c->chunk_max);
+    return 0;
+}
+
+extern "C" int ccpprad_launch_columns(const ccpprad_ctx *c) { return c ? c->chunk : 0; }
+
 extern "C" int ccpprad_ngpt(const ccpprad_ctx *c) { return c ? c->T.ngpt : 0; }
 extern "C" int ccpprad_nband(const ccpprad_ctx *c) { return c ? c->T.nband : 0; }
 extern "C" const char *ccpprad_version(void) { return "ccpprad-mi355x 0.1 (gfx950)"; }
@@ -551,6 +570,87 @@ static void gauss_table(int n, float *D, float *W)
 }
 
 static bool need(const void *p) { return p != nullptr; }
+
+// McICA arguments of the LW / SW scheme for the launch starting at column
+// (LW) or daylit position (SW) c0: seeds ngpt + column (isubc 1) or
+// icseed (isubc 2), the overlap flags, and the mask words of that launch
+// (rrtmgp_lw_main.F90:441-493, rrtmgp_sw_main.F90:479-523).
+static McicaArgs lw_mcica_args(const ccpprad_ctx *c, const ccpprad_lw_args *a, const unsigned char *blk, int c0,
+                               unsigned *mask)
+{
+    McicaArgs ma{};
+    ma.nlay = a->nLay; ma.ngpt = c->T.ngpt; ma.ncol_ld = a->nCol; ma.col0 = c0; ma.blksz = a->rrtmgp_phys_blksz;
+    ma.isubc = a->isubc_lw;
+    ma.iovr = a->iovr; ma.iovr_max = a->iovr_max; ma.iovr_maxrand = a->iovr_maxrand; ma.iovr_rand = a->iovr_rand;
+    ma.iovr_dcorr = a->iovr_dcorr; ma.iovr_exp = a->iovr_exp; ma.iovr_exprand = a->iovr_exprand;
+    ma.col_idx = nullptr; ma.blkflag = blk; ma.cld_frac = a->cld_frac; ma.overlap = a->cloud_overlap_param;
+    ma.icseed = a->icseed_lw; ma.mtpow = c->mtpow; ma.mask = mask;
+    return ma;
+}
+
+static McicaArgs sw_mcica_args(const ccpprad_ctx *c, const ccpprad_sw_args *a, const unsigned char *blk, int c0,
+                               unsigned *mask)
+{
+    McicaArgs ma{};
+    ma.nlay = a->nLay; ma.ngpt = c->T.ngpt; ma.ncol_ld = a->nCol; ma.col0 = c0; ma.blksz = a->rrtmgp_phys_blksz;
+    ma.isubc = a->isubc_sw;
+    ma.iovr = a->iovr; ma.iovr_max = a->iovr_max; ma.iovr_maxrand = a->iovr_maxrand; ma.iovr_rand = a->iovr_rand;
+    ma.iovr_dcorr = a->iovr_dcorr; ma.iovr_exp = a->iovr_exp; ma.iovr_exprand = a->iovr_exprand;
+    ma.col_idx = a->idx; ma.blkflag = blk; ma.cld_frac = a->cld_frac; ma.overlap = a->cloud_overlap_param;
+    ma.icseed = a->icseed_sw; ma.mtpow = c->mtpow; ma.mask = mask;
+    return ma;
+}
+
+// The sampled cloud mask of a scheme call on its own: the block cloud flags,
+// then the McICA kernel over every column (LW) or daylit column (SW), in
+// launches of the context's column count, into mask[col][lay][MASK_WORDS].
+static int mcica_mask_entry(ccpprad_ctx *c, int lw, const ccpprad_lw_args *al, const ccpprad_sw_args *as,
+                            unsigned *mask, void *stream, char *errmsg, int errlen)
+{
+    if (errmsg && errlen > 0) errmsg[0] = 0;
+    if (!c || (!al && !as) || !mask) { seterr(errmsg, errlen, "null context, arguments or mask"); return 1; }
+    if (!!c->is_lw != !!lw) { seterr(errmsg, errlen, "context is for the other spectral range"); return 1; }
+    const int nL = lw ? al->nLay : as->nLay, nC = lw ? al->nCol : as->nCol;
+    const int n = lw ? nC : as->nDay, bs = lw ? al->rrtmgp_phys_blksz : as->rrtmgp_phys_blksz;
+    const int isubc = lw ? al->isubc_lw : as->isubc_sw, iovr = lw ? al->iovr : as->iovr;
+    const bool expo = lw ? (iovr == al->iovr_exp || iovr == al->iovr_exprand || iovr == al->iovr_dcorr)
+                         : (iovr == as->iovr_exp || iovr == as->iovr_exprand || iovr == as->iovr_dcorr);
+    if (nL != c->nlay) { seterr(errmsg, errlen, "nLay=%d differs from init (%d)", nL, c->nlay); return 1; }
+    if (n <= 0) return 0;
+    if (bs < 1 || n % bs) { seterr(errmsg, errlen, "column count %d must be a multiple of rrtmgp_phys_blksz=%d", n, bs); return 1; }
+    if (!(lw ? al->cld_frac : as->cld_frac) || (!lw && !as->idx)) { seterr(errmsg, errlen, "required array argument is NULL"); return 1; }
+    if (isubc == 2 && !(lw ? al->icseed_lw : as->icseed_sw)) { seterr(errmsg, errlen, "isubc=2 needs icseed"); return 1; }
+    if (expo && !(lw ? al->cloud_overlap_param : as->cloud_overlap_param)) {
+        seterr(errmsg, errlen, "exponential/decorrelation overlap needs cloud_overlap_param"); return 1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(c->mu);
+    Workspace *w = ws_for_stream(c, st);
+    if (!w) { seterr(errmsg, errlen, "workspace allocation failed"); return 2; }
+    HIPCHK(w->w_blk.ensure((size_t)(n / bs) + 1));
+    unsigned char *blk = (unsigned char *)w->w_blk.p;
+    launch_block_flags(n, nL, nC, lw ? nullptr : as->idx, bs, lw ? al->cld_frac : as->cld_frac, blk, nullptr, st);
+    const int CH = std::min(n, c->chunk);
+    for (int c0 = 0; c0 < n; c0 += CH) {
+        unsigned *m = mask + (size_t)c0 * nL * MASK_WORDS;
+        const McicaArgs ma = lw ? lw_mcica_args(c, al, blk, c0, m) : sw_mcica_args(c, as, blk, c0, m);
+        launch_mcica(ma, std::min(CH, n - c0), st);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ccpprad_lw_mcica_mask_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, unsigned *mask, void *stream,
+                                         char *errmsg, int errlen)
+{
+    return mcica_mask_entry(c, 1, a, nullptr, mask, stream, errmsg, errlen);
+}
+
+extern "C" int ccpprad_sw_mcica_mask_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, unsigned *mask, void *stream,
+                                         char *errmsg, int errlen)
+{
+    return mcica_mask_entry(c, 0, nullptr, a, mask, stream, errmsg, errlen);
+}
 
 // ---------------------------------------------------------------------------
 // LW run, device pointers
@@ -634,12 +734,7 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
             if (!ca.rwp || !ca.swp || !ca.resnow) { seterr(errmsg, errlen, "cld_rwp/cld_swp/cld_resnow required with clouds"); return 1; }
             ca.ctau = ctau; ca.copt = copt; ca.err = w->err;
             { KScope k_(c, K_CLOUD, st); launch_cloud_optics(c->L, ca, nc, st); }
-            McicaArgs ma{};
-            ma.nlay = nL; ma.ngpt = G; ma.ncol_ld = nC; ma.col0 = c0; ma.blksz = bs; ma.isubc = a->isubc_lw;
-            ma.iovr = a->iovr; ma.iovr_max = a->iovr_max; ma.iovr_maxrand = a->iovr_maxrand; ma.iovr_rand = a->iovr_rand;
-            ma.iovr_dcorr = a->iovr_dcorr; ma.iovr_exp = a->iovr_exp; ma.iovr_exprand = a->iovr_exprand;
-            ma.col_idx = nullptr; ma.blkflag = blk; ma.cld_frac = a->cld_frac; ma.overlap = a->cloud_overlap_param;
-            ma.icseed = a->icseed_lw; ma.mtpow = c->mtpow; ma.mask = mask;
+            const McicaArgs ma = lw_mcica_args(c, a, blk, c0, mask);
             { KScope k_(c, K_MCICA, st); launch_mcica(ma, nc, st); }
         }
         if (c->mlp) {
@@ -793,12 +888,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
             ca.cldtau = (c0 + nc > last_blk_start) ? a->cldtausw : nullptr;
             ca.cldtau_blk = nblk - 1;
             { KScope k_(c, K_CLOUD, st); launch_cloud_optics(c->L, ca, nc, st); }
-            McicaArgs ma{};
-            ma.nlay = nL; ma.ngpt = G; ma.ncol_ld = nC; ma.col0 = c0; ma.blksz = bs; ma.isubc = a->isubc_sw;
-            ma.iovr = a->iovr; ma.iovr_max = a->iovr_max; ma.iovr_maxrand = a->iovr_maxrand; ma.iovr_rand = a->iovr_rand;
-            ma.iovr_dcorr = a->iovr_dcorr; ma.iovr_exp = a->iovr_exp; ma.iovr_exprand = a->iovr_exprand;
-            ma.col_idx = a->idx; ma.blkflag = blk; ma.cld_frac = a->cld_frac; ma.overlap = a->cloud_overlap_param;
-            ma.icseed = a->icseed_sw; ma.mtpow = c->mtpow; ma.mask = mask;
+            const McicaArgs ma = sw_mcica_args(c, a, blk, c0, mask);
             { KScope k_(c, K_MCICA, st); launch_mcica(ma, nc, st); }
         }
         { KScope k_(c, K_INTERP, st); launch_gas_interp(T, nc, nL, nC, a->idx, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)w->w_rec.p, w->err, st); }
@@ -991,7 +1081,27 @@ static int gas_optics_entry(ccpprad_ctx *c, int want_lw, int nCol, int nLay, con
     if (!w) { seterr(errmsg, errlen, "workspace allocation failed"); return 2; }
     launch_fill_i32(w->err, 1, 0, st);
     HIPCHK(w->w_rec.ensure(sizeof(unsigned) * (size_t)nCol * nLay * gas_record_words(c->T)));
-    launch_gas_optics(c->T, nCol, nLay, nCol, nullptr, 0, p_lay, p_lev, t_lay, vm, (unsigned *)w->w_rec.p, tau, aux, w->err, st);
+    if (want_lw && c->mlp) {
+        // the gas-optics stage of the LW run with the emulator attached:
+        // MLP inference, then the table path on the layers outside its hull
+        if (nLay != c->nlay) { seterr(errmsg, errlen, "nLay=%d differs from init (%d)", nLay, c->nlay); return 1; }
+        HIPCHK(w->w_oob.ensure((size_t)nCol * nLay));
+        HIPCHK(w->w_colm.ensure((size_t)nCol + 1));
+        MlpInferArgs mi{};
+        mi.nlay = nLay; mi.ncol_ld = nCol; mi.col0 = 0; mi.ncols = nCol;
+        mi.p_lay = p_lay; mi.p_lev = p_lev; mi.t_lay = t_lay;
+        mi.vmr[0] = vm.v[0]; mi.vmr[1] = vm.v[2]; mi.vmr[2] = vm.v[1]; mi.vmr[3] = vm.v[4]; mi.vmr[4] = vm.v[3];
+        mi.tau = tau; mi.pfrac = aux;
+        mi.oob = (unsigned char *)w->w_oob.p; mi.colflag = (unsigned char *)w->w_colm.p;
+        launch_fill_u8(mi.colflag, (size_t)nCol, st);
+        launch_mlp_infer(c->mlp, mi, st);
+        launch_gas_interp(c->T, nCol, nLay, nCol, nullptr, 0, p_lay, p_lev, t_lay, vm, (unsigned *)w->w_rec.p, w->err, st,
+                          mi.colflag);
+        launch_gas_gather(c->T, nCol, nLay, (const unsigned *)w->w_rec.p, tau, aux, st, mi.oob, mi.colflag);
+    } else {
+        launch_gas_optics(c->T, nCol, nLay, nCol, nullptr, 0, p_lay, p_lev, t_lay, vm, (unsigned *)w->w_rec.p, tau, aux,
+                          w->err, st);
+    }
     HIPCHK(hipGetLastError());
     return 0;
 }

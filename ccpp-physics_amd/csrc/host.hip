@@ -312,8 +312,8 @@ bool lut_from_records(TableHold &h)
 // Tables from the k-distribution and cloud files named at _init: the
 // reference's netCDF files (read natively, classic formats) or this
 // package's binary table files, each file's format taken from its signature.
-int init_files(int lw, const char *gas_file, const char *cloud_file, const char *const *gas_names, int ngases,
-               int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg, int errlen, bool pade = false)
+int init_files(int lw, const char *gas_file, const char *cloud_file, int nrghice, const char *const *gas_names,
+               int ngases, int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg, int errlen, bool pade = false)
 {
     TableHold h;
     NcKdistHold kh;
@@ -344,34 +344,94 @@ int init_files(int lw, const char *gas_file, const char *cloud_file, const char 
             if (!lut_from_records(h)) return fail(cloud_file, "table file lacks a required record or has a bad size");
             lut = &h.lut;
         }
+        // the namelist's ice-roughness category overrides the file's when
+        // nonzero (rrtmgp_lw_cloud_optics.F90:171-173,377;
+        // rrtmgp_sw_cloud_optics.F90:160-162,366)
+        if (nrghice < 0 || nrghice > lut->nrghice) {
+            char m[96];
+            snprintf(m, sizeof m, "nrghice=%d outside 0..%d (the file's ice-roughness categories)", nrghice,
+                     lut->nrghice);
+            return fail(cloud_file, m);
+        }
+        if (nrghice > 0) (lut == &lh.lut ? lh.lut : h.lut).icergh = nrghice;
     }
     return lw ? ccpprad_lw_init(kd, lut, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen)
               : ccpprad_sw_init(kd, lut, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen);
 }
 }  // namespace
 
-extern "C" int ccpprad_lw_init_files(const char *gas_file, const char *cloud_file, const char *const *gas_names,
+extern "C" int ccpprad_lw_init_files(const char *gas_file, const char *cloud_file, int nrghice,
+                                     const char *const *gas_names,
                                      int ngases, int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg, int errlen)
 {
-    return init_files(1, gas_file, cloud_file, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen);
+    return init_files(1, gas_file, cloud_file, nrghice, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen);
 }
 
-extern "C" int ccpprad_sw_init_files(const char *gas_file, const char *cloud_file, const char *const *gas_names,
+extern "C" int ccpprad_sw_init_files(const char *gas_file, const char *cloud_file, int nrghice,
+                                     const char *const *gas_names,
                                      int ngases, int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg, int errlen)
 {
-    return init_files(0, gas_file, cloud_file, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen);
+    return init_files(0, gas_file, cloud_file, nrghice, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen);
 }
 
-extern "C" int ccpprad_lw_init_files_pade(const char *gas_file, const char *cloud_file, const char *const *gas_names,
+extern "C" int ccpprad_lw_init_files_pade(const char *gas_file, const char *cloud_file, int nrghice,
+                                     const char *const *gas_names,
                                           int ngases, int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg,
                                           int errlen)
 {
-    return init_files(1, gas_file, cloud_file, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen, true);
+    return init_files(1, gas_file, cloud_file, nrghice, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen, true);
 }
 
-extern "C" int ccpprad_sw_init_files_pade(const char *gas_file, const char *cloud_file, const char *const *gas_names,
+extern "C" int ccpprad_sw_init_files_pade(const char *gas_file, const char *cloud_file, int nrghice,
+                                     const char *const *gas_names,
                                           int ngases, int nLay, int blksz, ccpprad_ctx **ctx, char *errmsg,
                                           int errlen)
 {
-    return init_files(0, gas_file, cloud_file, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen, true);
+    return init_files(0, gas_file, cloud_file, nrghice, gas_names, ngases, nLay, blksz, ctx, errmsg, errlen, true);
+}
+
+// compute_heating_rate with host arrays (GFS_rrtmgp_post.F90:159-171,249-266):
+// the fluxes and pressures are copied to the device, the FP64 kernel runs
+// there and the heating rate is copied back.  Synchronous; the device
+// staging is allocated per call.
+extern "C" int ccpprad_heating_rate(int nCol, int nLay, const double *flux_up, const double *flux_dn,
+                                    const double *p_lev, double *htr, char *errmsg, int errlen)
+{
+    if (errmsg && errlen > 0) errmsg[0] = 0;
+    if (nCol < 0 || nLay < 1) {
+        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "heating_rate: bad dimensions nCol=%d nLay=%d", nCol, nLay);
+        return 1;
+    }
+    if (!flux_up || !flux_dn || !p_lev || !htr) {
+        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "heating_rate: NULL array");
+        return 1;
+    }
+    if (nCol == 0) return 0;
+    const size_t nv = (size_t)nCol * (nLay + 1), nl = (size_t)nCol * nLay;
+    double *d = nullptr;
+    hipStream_t st = nullptr;
+    int rc = 0;
+    auto fail = [&](const char *what, hipError_t e) {
+        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "heating_rate: %s: %s", what, hipGetErrorString(e));
+        rc = 2;
+    };
+    hipError_t e = hipStreamCreate(&st);
+    if (e != hipSuccess) { fail("stream creation", e); return rc; }
+    e = hipMalloc(&d, sizeof(double) * (3 * nv + nl));
+    if (e != hipSuccess) { fail("device allocation", e); (void)hipStreamDestroy(st); return rc; }
+    double *du = d, *dd = d + nv, *dp = d + 2 * nv, *dh = d + 3 * nv;
+    if ((e = hipMemcpyAsync(du, flux_up, sizeof(double) * nv, hipMemcpyHostToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(dd, flux_dn, sizeof(double) * nv, hipMemcpyHostToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(dp, p_lev, sizeof(double) * nv, hipMemcpyHostToDevice, st)) != hipSuccess) {
+        fail("host to device copy", e);
+    } else {
+        rc = ccpprad_heating_rate_dev(nCol, nLay, du, dd, dp, dh, (void *)st, errmsg, errlen);
+        if (!rc && (e = hipMemcpyAsync(htr, dh, sizeof(double) * nl, hipMemcpyDeviceToHost, st)) != hipSuccess)
+            fail("device to host copy", e);
+        if (!rc && (e = hipStreamSynchronize(st)) != hipSuccess) fail("stream synchronisation", e);
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(d);
+    (void)hipStreamDestroy(st);
+    return rc;
 }

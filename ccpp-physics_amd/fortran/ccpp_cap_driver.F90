@@ -77,8 +77,8 @@ program ccpp_cap_driver
        vn2o, vco2, cf, lwp, rel, iwp, rei, swp, resnow, rwp, rerain, zl, zl, reliq0, zl, reice0, zl, reliq0,  &
        zl, reice0, ovl, gases, aer, aer, aer, 1361.0_kp, scmpsw, sup, sdn, supc, sdnc, cldtau, errmsg, errsw)
   if (errsw /= 0) print *, 'sw run: ', trim(errmsg)
-  call rrtmgp_lw_main_finalize()
-  call rrtmgp_sw_main_finalize()
+  call rrtmgp_lw_main_release()
+  call rrtmgp_sw_main_release()
 
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(u) errlw, errsw

@@ -42,40 +42,44 @@ module ccpprad_c
   end type ccpprad_sw_args_t
 
   interface
-     integer(c_int) function ccpprad_lw_init_files(gas_file, cloud_file, gas_names, ngases, nLay, blksz, &
-          ctx, errmsg, errlen) bind(C, name="ccpprad_lw_init_files")
+     integer(c_int) function ccpprad_lw_init_files(gas_file, cloud_file, nrghice, gas_names, ngases, nLay, &
+          blksz, ctx, errmsg, errlen) bind(C, name="ccpprad_lw_init_files")
        import :: c_int, c_ptr, c_char
        character(kind=c_char), dimension(*) :: gas_file, cloud_file
+       integer(c_int), value :: nrghice
        type(c_ptr), value :: gas_names
        integer(c_int), value :: ngases, nLay, blksz
        type(c_ptr) :: ctx
        character(kind=c_char), dimension(*) :: errmsg
        integer(c_int), value :: errlen
      end function ccpprad_lw_init_files
-     integer(c_int) function ccpprad_sw_init_files(gas_file, cloud_file, gas_names, ngases, nLay, blksz, &
-          ctx, errmsg, errlen) bind(C, name="ccpprad_sw_init_files")
+     integer(c_int) function ccpprad_sw_init_files(gas_file, cloud_file, nrghice, gas_names, ngases, nLay, &
+          blksz, ctx, errmsg, errlen) bind(C, name="ccpprad_sw_init_files")
        import :: c_int, c_ptr, c_char
        character(kind=c_char), dimension(*) :: gas_file, cloud_file
+       integer(c_int), value :: nrghice
        type(c_ptr), value :: gas_names
        integer(c_int), value :: ngases, nLay, blksz
        type(c_ptr) :: ctx
        character(kind=c_char), dimension(*) :: errmsg
        integer(c_int), value :: errlen
      end function ccpprad_sw_init_files
-     integer(c_int) function ccpprad_lw_init_files_pade(gas_file, cloud_file, gas_names, ngases, nLay, blksz, &
-          ctx, errmsg, errlen) bind(C, name="ccpprad_lw_init_files_pade")
+     integer(c_int) function ccpprad_lw_init_files_pade(gas_file, cloud_file, nrghice, gas_names, ngases, nLay, &
+          blksz, ctx, errmsg, errlen) bind(C, name="ccpprad_lw_init_files_pade")
        import :: c_int, c_ptr, c_char
        character(kind=c_char), dimension(*) :: gas_file, cloud_file
+       integer(c_int), value :: nrghice
        type(c_ptr), value :: gas_names
        integer(c_int), value :: ngases, nLay, blksz
        type(c_ptr) :: ctx
        character(kind=c_char), dimension(*) :: errmsg
        integer(c_int), value :: errlen
      end function ccpprad_lw_init_files_pade
-     integer(c_int) function ccpprad_sw_init_files_pade(gas_file, cloud_file, gas_names, ngases, nLay, blksz, &
-          ctx, errmsg, errlen) bind(C, name="ccpprad_sw_init_files_pade")
+     integer(c_int) function ccpprad_sw_init_files_pade(gas_file, cloud_file, nrghice, gas_names, ngases, nLay, &
+          blksz, ctx, errmsg, errlen) bind(C, name="ccpprad_sw_init_files_pade")
        import :: c_int, c_ptr, c_char
        character(kind=c_char), dimension(*) :: gas_file, cloud_file
+       integer(c_int), value :: nrghice
        type(c_ptr), value :: gas_names
        integer(c_int), value :: ngases, nLay, blksz
        type(c_ptr) :: ctx
@@ -96,6 +100,14 @@ module ccpprad_c
        character(kind=c_char), dimension(*) :: errmsg
        integer(c_int), value :: errlen
      end function ccpprad_sw_run
+     integer(c_int) function ccpprad_heating_rate(nCol, nLay, flux_up, flux_dn, p_lev, htr, errmsg, &
+          errlen) bind(C, name="ccpprad_heating_rate")
+       import :: c_int, c_ptr, c_char
+       integer(c_int), value :: nCol, nLay
+       type(c_ptr), value :: flux_up, flux_dn, p_lev, htr
+       character(kind=c_char), dimension(*) :: errmsg
+       integer(c_int), value :: errlen
+     end function ccpprad_heating_rate
      subroutine ccpprad_finalize(ctx) bind(C, name="ccpprad_finalize")
        import :: c_ptr
        type(c_ptr), value :: ctx

@@ -23,7 +23,7 @@ module rrtmgp_lw_main
   integer, parameter, private :: kind_phys = c_double
   type(c_ptr), save, private :: lw_ctx = c_null_ptr
 
-  public rrtmgp_lw_main_init, rrtmgp_lw_main_run, rrtmgp_lw_main_finalize
+  public rrtmgp_lw_main_init, rrtmgp_lw_main_run, rrtmgp_lw_main_release
 contains
 
   subroutine rrtmgp_lw_main_init(rrtmgp_root_dir, rrtmgp_lw_file_gas, rrtmgp_lw_file_clouds,&
@@ -57,11 +57,11 @@ contains
     buf = c_null_char
     if (doGP_cldoptics_PADE .and. .not. doGP_cldoptics_LUT) then
         rc = ccpprad_lw_init_files_pade(c_str(trim(rrtmgp_root_dir)//trim(rrtmgp_lw_file_gas)), c_str(cloud), &
-            c_loc(pnames), int(size(active_gases_array), c_int), int(nLay, c_int),                       &
+            int(nrghice, c_int), c_loc(pnames), int(size(active_gases_array), c_int), int(nLay, c_int), &
             int(rrtmgp_phys_blksz, c_int), lw_ctx, buf, int(size(buf), c_int))
     else
         rc = ccpprad_lw_init_files(c_str(trim(rrtmgp_root_dir)//trim(rrtmgp_lw_file_gas)), c_str(cloud), &
-            c_loc(pnames), int(size(active_gases_array), c_int), int(nLay, c_int),                       &
+            int(nrghice, c_int), c_loc(pnames), int(size(active_gases_array), c_int), int(nLay, c_int), &
             int(rrtmgp_phys_blksz, c_int), lw_ctx, buf, int(size(buf), c_int))
     endif
     if (rc /= 0) then
@@ -154,9 +154,13 @@ contains
     endif
   end subroutine rrtmgp_lw_main_run
 
-  ! Releases the device tables (no counterpart in the reference scheme).
-  subroutine rrtmgp_lw_main_finalize()
+  ! Releases the device tables and work buffers.  No counterpart in the
+  ! reference scheme, and not a CCPP phase (the name is none of _init, _run,
+  ! _finalize, _timestep_init, _timestep_finalize, so the unchanged .meta
+  ! needs no argument table for it): the host calls it where it tears
+  ! radiation down, e.g. beside GFS_rrtmgp_setup_finalize.
+  subroutine rrtmgp_lw_main_release()
     if (c_associated(lw_ctx)) call ccpprad_finalize(lw_ctx)
     lw_ctx = c_null_ptr
-  end subroutine rrtmgp_lw_main_finalize
+  end subroutine rrtmgp_lw_main_release
 end module rrtmgp_lw_main

@@ -20,7 +20,7 @@ module rrtmgp_sw_main
   integer, parameter, private :: kind_phys = c_double
   type(c_ptr), save, private :: sw_ctx = c_null_ptr
 
-  public rrtmgp_sw_main_init, rrtmgp_sw_main_run, rrtmgp_sw_main_finalize
+  public rrtmgp_sw_main_init, rrtmgp_sw_main_run, rrtmgp_sw_main_release
 contains
 
   subroutine rrtmgp_sw_main_init(rrtmgp_root_dir, rrtmgp_sw_file_gas, rrtmgp_sw_file_clouds,&
@@ -54,11 +54,11 @@ contains
     buf = c_null_char
     if (doGP_cldoptics_PADE .and. .not. doGP_cldoptics_LUT) then
         rc = ccpprad_sw_init_files_pade(c_str(trim(rrtmgp_root_dir)//trim(rrtmgp_sw_file_gas)), c_str(cloud), &
-            c_loc(pnames), int(size(active_gases_array), c_int), int(nLay, c_int),                       &
+            int(nrghice, c_int), c_loc(pnames), int(size(active_gases_array), c_int), int(nLay, c_int), &
             int(rrtmgp_phys_blksz, c_int), sw_ctx, buf, int(size(buf), c_int))
     else
         rc = ccpprad_sw_init_files(c_str(trim(rrtmgp_root_dir)//trim(rrtmgp_sw_file_gas)), c_str(cloud), &
-            c_loc(pnames), int(size(active_gases_array), c_int), int(nLay, c_int),                       &
+            int(nrghice, c_int), c_loc(pnames), int(size(active_gases_array), c_int), int(nLay, c_int), &
             int(rrtmgp_phys_blksz, c_int), sw_ctx, buf, int(size(buf), c_int))
     endif
     if (rc /= 0) then
@@ -155,8 +155,13 @@ contains
     endif
   end subroutine rrtmgp_sw_main_run
 
-  subroutine rrtmgp_sw_main_finalize()
+  ! Releases the device tables and work buffers.  No counterpart in the
+  ! reference scheme, and not a CCPP phase (the name is none of _init, _run,
+  ! _finalize, _timestep_init, _timestep_finalize, so the unchanged .meta
+  ! needs no argument table for it): the host calls it where it tears
+  ! radiation down, e.g. beside GFS_rrtmgp_setup_finalize.
+  subroutine rrtmgp_sw_main_release()
     if (c_associated(sw_ctx)) call ccpprad_finalize(sw_ctx)
     sw_ctx = c_null_ptr
-  end subroutine rrtmgp_sw_main_finalize
+  end subroutine rrtmgp_sw_main_release
 end module rrtmgp_sw_main

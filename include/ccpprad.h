@@ -179,21 +179,25 @@ int ccpprad_sw_init(const ccpprad_kdist *kd, const ccpprad_cloud_lut *lut,
  * message) and reduced to the active gases as ty_gas_optics_rrtmgp%load
  * does, or files in the package's binary table format; each file's format is
  * taken from its signature.  A netCDF cloud file gives the lookup tables
- * when it has them, else its Pade coefficients; the netCDF cloud path uses
- * the file's last ice-roughness category. */
-int ccpprad_lw_init_files(const char *gas_file, const char *cloud_file,
+ * when it has them, else its Pade coefficients.  nrghice is the namelist's
+ * ice-roughness category: nonzero selects that category of the cloud file
+ * (an error beyond the file's count), as the reference's nonzero nrghice
+ * overrides the file's (rrtmgp_lw_cloud_optics.F90:171-173,377,
+ * rrtmgp_sw_cloud_optics.F90:160-162,366); 0 keeps the file's choice (the
+ * last category of a netCDF file). */
+int ccpprad_lw_init_files(const char *gas_file, const char *cloud_file, int nrghice,
                           const char *const *gas_names, int ngases, int nLay,
                           int rrtmgp_phys_blksz, ccpprad_ctx **ctx, char *errmsg, int errlen);
-int ccpprad_sw_init_files(const char *gas_file, const char *cloud_file,
+int ccpprad_sw_init_files(const char *gas_file, const char *cloud_file, int nrghice,
                           const char *const *gas_names, int ngases, int nLay,
                           int rrtmgp_phys_blksz, ccpprad_ctx **ctx, char *errmsg, int errlen);
 /* The same with doGP_cldoptics_PADE (rrtmgp_lw_cloud_optics.F90:192-205,
  * 261-288,345-350): the netCDF cloud file's Pade coefficients, also where the
  * file holds lookup tables too. */
-int ccpprad_lw_init_files_pade(const char *gas_file, const char *cloud_file,
+int ccpprad_lw_init_files_pade(const char *gas_file, const char *cloud_file, int nrghice,
                                const char *const *gas_names, int ngases, int nLay,
                                int rrtmgp_phys_blksz, ccpprad_ctx **ctx, char *errmsg, int errlen);
-int ccpprad_sw_init_files_pade(const char *gas_file, const char *cloud_file,
+int ccpprad_sw_init_files_pade(const char *gas_file, const char *cloud_file, int nrghice,
                                const char *const *gas_names, int ngases, int nLay,
                                int rrtmgp_phys_blksz, ccpprad_ctx **ctx, char *errmsg, int errlen);
 
@@ -215,13 +219,40 @@ int ccpprad_sw_run_dev(ccpprad_ctx *ctx, const ccpprad_sw_args *a, void *stream,
  * (GFS_rrtmgp_setup.F90:260-273) for the device state. */
 void ccpprad_finalize(ccpprad_ctx *ctx);
 
+/* Columns per launch (default 65,536, at most what keeps the solvers'
+ * 32-bit cloud-record offsets in range): the work buffers of a stream hold
+ * one launch, so this bounds their size.  No counterpart in the reference
+ * (its block size rrtmgp_phys_blksz is the host's, and stays that). */
+int ccpprad_set_launch_columns(ccpprad_ctx *ctx, int ncol);
+int ccpprad_launch_columns(const ccpprad_ctx *ctx);
+
 /* ---- component entry points ----------------------------------------- */
 
-/* compute_heating_rate (GFS_rrtmgp_post.F90:167-171): FP64 on the device.
- * htr(nCol,nLay) [K/s] = (dF_up - dF_dn) * g / (cp * dp).  Device pointers. */
+/* compute_heating_rate (mo_heating_rates, called at
+ * GFS_rrtmgp_post.F90:11,159-171,249-266): FP64 on the device.
+ * htr(nCol,nLay) [K/s] = (F_up(k+1) - F_up(k) - F_dn(k+1) + F_dn(k)) g / (cp (p(k+1) - p(k))),
+ * fluxes and p_lev (nCol, nLay+1).  Host pointers: copies in, runs, copies
+ * out, synchronous (the Fortran mo_heating_rates module calls this). */
+int ccpprad_heating_rate(int nCol, int nLay, const double *flux_up, const double *flux_dn,
+                         const double *p_lev, double *htr, char *errmsg, int errlen);
+/* The same with device pointers, ordered on stream. */
 int ccpprad_heating_rate_dev(int nCol, int nLay, const double *flux_up,
                              const double *flux_dn, const double *p_lev, double *htr,
                              void *stream, char *errmsg, int errlen);
+
+/* The McICA cloud mask of a scheme call on its own: random_setseed /
+ * random_number per column and sampled_mask (rrtmgp_lw_main.F90:353-362,
+ * 441-493; rrtmgp_sw_main.F90:263-273,479-523; rrtmgp_sampling.F90:84-182)
+ * for the flags, seeds and overlap of the args struct (only the fields the
+ * sampling reads are used: dimensions, blksz, isubc, icseed, the iovr
+ * values, cld_frac, cloud_overlap_param and, for SW, nDay and idx).  Blocks
+ * with no cloudy column get no bits, as the reference skips them.  Device
+ * pointers; mask (MASK words = 4, nLay, n) uint32 with n = nCol (LW) or
+ * nDay (SW, daylit order): bit (g-1) mod 32 of word (g-1)/32 is g-point g. */
+int ccpprad_lw_mcica_mask_dev(ccpprad_ctx *ctx, const ccpprad_lw_args *a, unsigned *mask,
+                              void *stream, char *errmsg, int errlen);
+int ccpprad_sw_mcica_mask_dev(ccpprad_ctx *ctx, const ccpprad_sw_args *a, unsigned *mask,
+                              void *stream, char *errmsg, int errlen);
 
 /* Argument list of GFS_rrtmgp_post_run
  * (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_post.F90:35-389) less
@@ -328,7 +359,10 @@ int ccpprad_cloud_overlap_dev(const ccpprad_cloud_overlap_args *args, void *stre
 /* ty_gas_optics_rrtmgp%gas_optics, LW (rrtmgp_lw_main.F90:337-345).  Device
  * pointers.  vmr is (nCol, nLay, ngas) in the k-distribution's gas order.
  * Outputs g-point innermost, FP32: tau(ngpt,nLay,nCol),
- * pfrac(ngpt,nLay,nCol) (Planck fraction; lay_src = pfrac * B_band(T)). */
+ * pfrac(ngpt,nLay,nCol) (Planck fraction; lay_src = pfrac * B_band(T)).
+ * With an emulator attached (ccpprad_lw_set_gas_optics_mlp) this is the
+ * gas-optics stage of ccpprad_lw_run_dev: the MLP, and the table path on
+ * the layers outside its training hull (nLay must then equal init's). */
 int ccpprad_lw_gas_optics_dev(ccpprad_ctx *ctx, int nCol, int nLay,
                               const double *p_lay, const double *p_lev,
                               const double *t_lay, const double *vmr,

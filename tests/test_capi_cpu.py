@@ -36,8 +36,11 @@ def test_fortran_modules_built_with_reference_names():
     if not os.path.exists(build.F_LIB):
         pytest.skip("flang not available")
     out = subprocess.run(["nm", "-D", "--defined-only", build.F_LIB], capture_output=True, text=True).stdout
-    for sym in ("rrtmgp_lw_main_init", "rrtmgp_lw_main_run", "rrtmgp_sw_main_init", "rrtmgp_sw_main_run"):
+    for sym in ("rrtmgp_lw_main_init", "rrtmgp_lw_main_run", "rrtmgp_sw_main_init", "rrtmgp_sw_main_run",
+                "compute_heating_rate"):
         assert sym in out, sym
+    # the device release is no CCPP phase name (the unchanged .meta has no table for it)
+    assert "rrtmgp_lw_main_finalize" not in out and "rrtmgp_lw_main_release" in out
 
 
 def test_table_file_loader_rejects_malformed_files(tmp_path):
@@ -49,9 +52,9 @@ def test_table_file_loader_rejects_malformed_files(tmp_path):
     bad.write_bytes(b"NOTATABLE" + b"\0" * 64)
     ctx = C.c_void_p()
     buf, n = capi.errbuf()
-    rc = L.ccpprad_lw_init_files(str(bad).encode(), b"", None, 0, 127, 1, C.byref(ctx), buf, n)
+    rc = L.ccpprad_lw_init_files(str(bad).encode(), b"", 0, None, 0, 127, 1, C.byref(ctx), buf, n)
     assert rc != 0 and b"malformed" in buf.value
-    rc = L.ccpprad_lw_init_files(str(tmp_path / "missing.tab").encode(), b"", None, 0, 127, 1,
+    rc = L.ccpprad_lw_init_files(str(tmp_path / "missing.tab").encode(), b"", 0, None, 0, 127, 1,
                                  C.byref(ctx), buf, n)
     assert rc != 0 and b"cannot open" in buf.value
 
@@ -69,6 +72,55 @@ def test_table_file_round_trip_parses(tmp_path):
     tables.write_lut(tmp_path / "c.tab", syn.make_cloud_lut(True))
     ctx = C.c_void_p()
     buf, n = capi.errbuf()
-    rc = L.ccpprad_lw_init_files(str(tmp_path / "k.tab").encode(), str(tmp_path / "c.tab").encode(), None, 0,
+    rc = L.ccpprad_lw_init_files(str(tmp_path / "k.tab").encode(), str(tmp_path / "c.tab").encode(), 0, None, 0,
                                  127, 1, C.byref(ctx), buf, n)
     assert rc != 0 and b"malformed" not in buf.value and b"lacks" not in buf.value
+
+
+def test_unknown_active_gas_is_an_init_error():
+    """An active gas the k-distribution lacks is rejected at init, on the
+    host, before any device work."""
+    import ctypes as C
+    from ccpprad import lib, capi, synthetic as syn
+    L = lib.load()
+    hk = capi.kdist_struct(syn.make_kdist(True))
+    names = (C.c_char_p * 3)(b"h2o", b"co2", b"cfc11")
+    ctx = C.c_void_p()
+    buf, n = capi.errbuf()
+    rc = L.ccpprad_lw_init(C.addressof(hk.struct), None, names, 3, 127, 1, C.byref(ctx), buf, n)
+    assert rc != 0 and b"cfc11" in buf.value
+
+
+def test_nrghice_beyond_the_file_is_an_init_error(tmp_path):
+    """nrghice selects a category of the cloud file; beyond the file's count
+    it is an error before any device work."""
+    import ctypes as C
+    from ccpprad import lib, capi, tables, synthetic as syn
+    L = lib.load()
+    tables.write_kdist(tmp_path / "k.tab", syn.make_kdist(True))
+    tables.write_lut(tmp_path / "c.tab", syn.make_cloud_lut(True))
+    ctx = C.c_void_p()
+    buf, n = capi.errbuf()
+    rc = L.ccpprad_lw_init_files(str(tmp_path / "k.tab").encode(), str(tmp_path / "c.tab").encode(), 4, None, 0,
+                                 127, 1, C.byref(ctx), buf, n)
+    assert rc != 0 and b"nrghice=4" in buf.value
+
+
+def test_pade_bounds_that_need_the_regime_clamp_are_rejected():
+    """Pade size-regime bounds with b2 - b3 > b1 would let rte's regime rule
+    fall below 1 (where this build clamps): rejected at init, on the host."""
+    import ctypes as C
+    import dataclasses
+    import numpy as np
+    from ccpprad import lib, capi, synthetic as syn
+    L = lib.load()
+    hk = capi.kdist_struct(syn.make_kdist(True))
+    lut = syn.make_cloud_pade(True)
+    pade = dict(lut.pade)
+    pade["pade_sizreg_extliq"] = np.array([2.5, 15.0, 5.0, 21.5])   # 15 - 5 > 2.5
+    bad = dataclasses.replace(lut, pade=pade)
+    hl = capi.lut_struct(bad)
+    ctx = C.c_void_p()
+    buf, n = capi.errbuf()
+    rc = L.ccpprad_lw_init(C.addressof(hk.struct), C.addressof(hl.struct), None, 0, 127, 1, C.byref(ctx), buf, n)
+    assert rc != 0 and b"b2 - b3 <= b1" in buf.value

@@ -1,6 +1,7 @@
 """End to end through the Fortran scheme modules: a cap-like Fortran driver
 calls rrtmgp_lw_main_init/_run and rrtmgp_sw_main_init/_run with the
 reference's argument lists; results are compared with the FP64 oracle."""
+import dataclasses
 import os
 import subprocess
 
@@ -51,12 +52,15 @@ def test_fortran_cap_driver_matches_oracle(tmp_path, kd_lw, kd_sw, lut_lw, lut_s
                  ("scmpsw", 6 * ncol), ("cldtau", ncol * nlay)):
         out[k] = vals[o:o + n]
         o += n
+    # the driver passes the namelist nrghice = 3, which selects the cloud
+    # file's third ice-roughness category
+    lut_lw3, lut_sw3 = dataclasses.replace(lut_lw, icergh=3), dataclasses.replace(lut_sw, icergh=3)
     lw = dict(arr)
     lw.update(O.lw_outputs(ncol, nlay))
-    O.lw_main_run(kd_lw, lut_lw, capi.lw_default_ints(ncol, nlay, blksz=blksz, nangle=3), lw)
+    O.lw_main_run(kd_lw, lut_lw3, capi.lw_default_ints(ncol, nlay, blksz=blksz, nangle=3), lw)
     sw = dict(arr)
     sw.update(O.sw_outputs(ncol, nlay))
-    O.sw_main_run(kd_sw, lut_sw, capi.sw_default_ints(ncol, ncol, nlay, blksz=blksz), sw)
+    O.sw_main_run(kd_sw, lut_sw3, capi.sw_default_ints(ncol, ncol, nlay, blksz=blksz), sw)
     d = lambda a, b: float(np.abs(a.reshape(-1) - b.reshape(-1)).max())
     print("lw up", d(out["lup"], lw["fluxlwUP_allsky"]), "sw dn", d(out["sdn"], sw["fluxswDOWN_allsky"]))
     assert d(out["lup"], lw["fluxlwUP_allsky"]) < 0.05

@@ -157,10 +157,35 @@ def test_training_reduces_loss_and_out_of_hull_falls_back(kd_lw, lut_lw):
     torch.cuda.synchronize()
     o = oob.cpu().numpy().reshape(ncol, nlay)
     assert o[3, 60] == 1 and o.sum() == 1 and cf.cpu().numpy()[3] == 1
-    # through the scheme: the flagged layer carries exactly the table's optics
+    # through the scheme's gas-optics stage (the emulator attached): the
+    # flagged layer carries exactly the table's optics, every other layer
+    # exactly the emulator's
     e.attach(s)
+    tau_s = torch.zeros_like(tau_t)
+    pf_s = torch.zeros_like(tau_t)
+    s.gas_optics_dev(ncol, nlay, d2["p_lay"], d2["p_lev"], d2["t_lay"], v, tau_s, pf_s)
+    torch.cuda.synchronize()
+    ts, ps = tau_s.cpu().numpy(), pf_s.cpu().numpy()
+    tt, pt = tau_t.cpu().numpy(), pf_t.cpu().numpy()
+    tn, pn = tau.cpu().numpy(), pf.cpu().numpy()
+    assert np.array_equal(ts[3, 60], tt[3, 60]) and np.array_equal(ps[3, 60], pt[3, 60])
+    keep = np.ones((ncol, nlay), dtype=bool)
+    keep[3, 60] = False
+    assert np.array_equal(ts[keep], tn[keep]) and np.array_equal(ps[keep], pn[keep])
+    assert not np.array_equal(tn[3, 60], tt[3, 60])   # the emulator's own values there differ
+    # and the scheme runs on that mix (a 3-epoch network: finite, positive
+    # fluxes; the trained network's accuracy is test_gpu_mlp_gain's)
     ints = capi.lw_default_ints(ncol, nlay, nangle=3)
     full = device.to_device(arr2)
     full.update(device.lw_outputs(ncol, nlay))
     s.run_dev(ints, full)
     torch.cuda.synchronize()
+    up_nn = full["fluxlwUP_allsky"].cpu().numpy()
+    assert np.isfinite(up_nn).all() and (up_nn > 0).all()
+    mlp.detach(s)
+    tab = device.to_device(arr2)
+    tab.update(device.lw_outputs(ncol, nlay))
+    s.run_dev(ints, tab)
+    torch.cuda.synchronize()
+    d_up = np.abs(up_nn - tab["fluxlwUP_allsky"].cpu().numpy()).max()
+    print("NN (3 epochs) vs table LW up flux max-abs", d_up)

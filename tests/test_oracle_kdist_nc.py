@@ -80,7 +80,7 @@ def _init(L, kfile, cfile, names=None, lw=True):
     if names is not None:
         arr = (C.c_char_p * len(names))(*[s.encode() for s in names])
     fn = L.ccpprad_lw_init_files if lw else L.ccpprad_sw_init_files
-    rc = fn(str(kfile).encode(), str(cfile).encode() if cfile else b"", arr, len(names or []), 127, 1,
+    rc = fn(str(kfile).encode(), str(cfile).encode() if cfile else b"", 0, arr, len(names or []), 127, 1,
             C.byref(ctx), buf, n)
     return rc, buf.value.decode(errors="replace")
 

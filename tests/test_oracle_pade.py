@@ -93,7 +93,7 @@ def test_library_parses_pade_cloud_file(tmp_path):
     names = (C.c_char_p * 6)(*[s.encode() for s in syn.GAS_NAMES])
     ctx = C.c_void_p()
     buf, n = capi.errbuf()
-    rc = L.ccpprad_lw_init_files(str(tmp_path / "k.nc").encode(), str(tmp_path / "c.nc").encode(), names, 6,
+    rc = L.ccpprad_lw_init_files(str(tmp_path / "k.nc").encode(), str(tmp_path / "c.nc").encode(), 0, names, 6,
                                  127, 1, C.byref(ctx), buf, n)
     msg = buf.value.decode()
     assert rc != 0 and "upload" in msg, msg
