@@ -42,6 +42,43 @@ METRIC = "LW+SW columns/s at 127 layers (table vs NN) @ HR RMSE<=0.1 K/day vs FP
 TRAFFIC_GB = {"sw_sw_solve": 19.579, "lw_lw_solve": 22.667, "lw_gas_optics": 11.728, "sw_gas_optics": 10.670,
               "lw_gas_interp": 3.026, "sw_gas_interp": 3.026, "lw_cloud_optics": 0.776, "sw_cloud_optics": 0.776,
               "lw_mcica": 0.670, "sw_mcica": 0.670, "lw_solve_2str": 24.485}
+# sha256 (first 16 hex digits) of the kernel sources the figures above were
+# measured with (kernel_source_sha()); on any other sources the bench line
+# reports traffic null rather than a stale figure.
+TRAFFIC_SRC_SHA = "unmeasured"
+
+
+def kernel_source_sha():
+    """sha256 over the HIP sources and headers of the library, in name order."""
+    import hashlib
+    d = os.path.join(REPO, "ccpp-physics_amd", "csrc")
+    h = hashlib.sha256()
+    for f in sorted(os.listdir(d)):
+        if f.endswith((".hip", ".h")):
+            h.update(f.encode())
+            with open(os.path.join(d, f), "rb") as fh:
+                h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def file_sha(path):
+    import hashlib
+    with open(path, "rb") as fh:
+        return hashlib.sha256(fh.read()).hexdigest()[:16]
+
+
+def host_info():
+    """The host cores the CPU baseline runs on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "cpu_model": model}
 
 
 def parse():
@@ -58,6 +95,11 @@ def parse():
     p.add_argument("--no-lwscat", action="store_true", help="skip the LW cloud-scattering (doGP_lwscat) leg")
     p.add_argument("--no-callers", action="store_true",
                    help="skip the leg timing the device callers (pre, geometry, overlap, post)")
+    p.add_argument("--no-e2e", action="store_true",
+                   help="skip the end-to-end leg through the host-pointer entries (host<->device copies included)")
+    p.add_argument("--e2e-steps", type=int, default=2)
+    p.add_argument("--cpu-sample-1core", type=int, default=1024,
+                   help="columns of the single-core CPU baseline sample")
     p.add_argument("--copy-probe", action="store_true",
                    help="after the bench, run one 1 GiB device copy: the reference dispatch that scales "
                         "FETCH_SIZE / WRITE_SIZE counter readings to bytes")
@@ -171,6 +213,8 @@ def main():
     import torch
     from ccpprad import synthetic as syn, capi, lib, device
 
+    lib.load()
+    lib_path = lib.loaded_path()
     kd_lw, kd_sw = syn.make_kdist(True), syn.make_kdist(False)
     lut_lw, lut_sw = syn.make_cloud_lut(True), syn.make_cloud_lut(False)
     ncol = a.ncol
@@ -241,6 +285,7 @@ def main():
         "lw_cloud_optics": 2_540 + 8_128,
         "sw_cloud_optics": 2_540 + 21_336,
     }
+    src_sha = kernel_source_sha()
     dom = max(kt.items(), key=lambda kv: kv[1][0])
     dname, (dtot, dcnt) = dom
     avg_ms = dtot / dcnt
@@ -248,7 +293,9 @@ def main():
     achieved = bytes_launch / (avg_ms / 1e3) / 1e9 if bytes_launch else None
     roof = {"kernel": dname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": TRAFFIC_GB[dname] * 1e9 if dname in TRAFFIC_GB and chunk == 65536 else None,
+            "traffic": (TRAFFIC_GB[dname] * 1e9 if dname in TRAFFIC_GB and chunk == 65536 and src_sha == TRAFFIC_SRC_SHA
+                        else None),
+            "traffic_measured_on_sources": TRAFFIC_SRC_SHA, "kernel_sources": src_sha,
             "traffic_unit": "bytes per launch",
             "avg_launch_ms": avg_ms, "bytes_per_column": per_col.get(dname), "bytes_per_launch": bytes_launch,
             "columns_per_launch": chunk,
@@ -363,6 +410,41 @@ def main():
 
     callers = None if a.no_callers else time_callers(a, torch, lw, dev, ncol)
 
+    # ---- end to end through the host-pointer entries (the ones the Fortran
+    # scheme modules call): host arrays in pageable memory, as a Fortran
+    # host holds them; every input copied in and every output copied out
+    # each call (intent(inout) outputs are copied both ways), synchronous.
+    # LW then SW, one after the other.
+    e2e = None
+    if not a.no_e2e:
+        reps = (ncol + UNIQUE - 1) // UNIQUE
+        host = {}
+        for k, v in arr.items():
+            host[k] = np.ascontiguousarray(np.tile(v, reps)[:ncol] if v.ndim == 1 else np.tile(v, (1, reps))[:, :ncol])
+        host["idx"] = np.arange(1, ncol + 1, dtype=np.int32)
+        hl = dict(host)
+        hl.update({k: np.zeros(tuple(v.shape), dtype=np.float64) for k, v in lw_out.items()})
+        hs = dict(host)
+        hs.update({k: np.zeros(tuple(v.shape), dtype=np.float64) for k, v in sw_out.items()})
+        moved = 0
+        for d_, outs in ((hl, set(lw_out)), (hs, set(sw_out))):
+            for k, v in d_.items():
+                if isinstance(v, np.ndarray):
+                    moved += v.nbytes * (2 if k in outs else 1)
+        lw.run(lw_ints, hl)
+        sw.run(sw_ints, hs)
+        t0 = time.perf_counter()
+        for _ in range(a.e2e_steps):
+            lw.run(lw_ints, hl)
+            sw.run(sw_ints, hs)
+        ms_e2e = (time.perf_counter() - t0) * 1e3 / a.e2e_steps
+        e2e = {"value": ncol / (ms_e2e / 1e3), "unit": "columns/s", "ms_per_step": ms_e2e, "steps": a.e2e_steps,
+               "host_device_bytes_per_step": moved,
+               "path": "ccpprad_lw_run + ccpprad_sw_run (host pointers, pageable memory, the entries the Fortran "
+                       "rrtmgp_*_main_run call), copies in and out included, LW then SW"}
+        print(f"# end to end: {ms_e2e:.1f} ms/step ({moved / 1e9:.1f} GB moved)", file=sys.stderr, flush=True)
+        del host, hl, hs
+
     # ---- CPU baseline: FP64 oracle on a bounded sample, and GPU error on it
     cpu = None
     acc = None
@@ -384,7 +466,24 @@ def main():
         tc = time.perf_counter() - t0
         cpu = {"value": ns / tc, "unit": "columns/s", "cores": O.nthreads(), "kind": "port",
                "sample": f"FP64 oracle, LW+SW all-sky, first {ns} of the {UNIQUE} distinct columns x {NLAY} "
-                         f"layers, {tc:.2f} s"}
+                         f"layers, {tc:.2f} s", "host": host_info()}
+        # the same on one core, over a smaller sample
+        n1 = min(a.cpu_sample_1core, ns)
+        s1 = pr.subset(np.arange(n1))
+        a1 = capi.profile_fields(s1)
+        a1["idx"] = np.arange(1, n1 + 1, dtype=np.int32)
+        O.set_threads(1)
+        l1 = dict(a1)
+        l1.update(O.lw_outputs(n1, NLAY))
+        w1 = dict(a1)
+        w1.update(O.sw_outputs(n1, NLAY))
+        t0 = time.perf_counter()
+        O.lw_main_run(kd_lw, lut_lw, capi.lw_default_ints(n1, NLAY, blksz=blksz, nangle=3), l1)
+        O.sw_main_run(kd_sw, lut_sw, capi.sw_default_ints(n1, n1, NLAY, blksz=blksz), w1)
+        t1c = time.perf_counter() - t0
+        O.set_threads(nth)
+        cpu["one_core"] = {"value": n1 / t1c, "unit": "columns/s", "cores": 1,
+                           "sample": f"first {n1} distinct columns, {t1c:.2f} s"}
         # GPU results on the same columns (the first ns columns of the device run)
         g = {k: v[..., :ns].cpu().numpy() if v.dim() == 2 else v for k, v in
              {"lwu": lw_out["fluxlwUP_allsky"], "lwd": lw_out["fluxlwDOWN_allsky"], "lwh": lw_out["htrlw"],
@@ -434,7 +533,9 @@ def main():
                                f"FP32 optics/transport, FP64 reductions and heating rate)",
                    "ncol": ncol, "nlay": NLAY, "path": "table", "blksz": blksz},
         "roofline": roof, "cpu_baseline": cpu, "accuracy_vs_fp64_oracle": acc, "nn": nn,
-        "lw_scattering": lws, "callers": callers,
+        "lw_scattering": lws, "callers": callers, "end_to_end": e2e,
+        "library": {"path": os.path.relpath(lib_path, REPO) if lib_path.startswith(REPO) else lib_path,
+                    "sha256_16": file_sha(lib_path)},
     }
     print(json.dumps(line), flush=True)
     if a.copy_probe:

@@ -17,6 +17,7 @@ P_D = C.POINTER(C.c_double)
 P_F = C.POINTER(C.c_float)
 
 _lib = None
+_lib_path = None
 
 # Every symbol the C-ABI header declares (checked by the CPU test suite).
 EXPORTS = ["ccpprad_lw_init", "ccpprad_sw_init", "ccpprad_lw_init_files", "ccpprad_sw_init_files",
@@ -31,7 +32,7 @@ EXPORTS = ["ccpprad_lw_init", "ccpprad_sw_init", "ccpprad_lw_init_files", "ccppr
 
 def load():
     """Load the HIP library (raises if it has not been built)."""
-    global _lib
+    global _lib, _lib_path
     if _lib is not None:
         return _lib
     path = os.environ.get("CCPPRAD_LIB", HIP_LIB)   # an alternative build of the same library
@@ -69,7 +70,14 @@ def load():
     L.ccpprad_nband.argtypes = [vp]
     L.ccpprad_version.restype = C.c_char_p
     _lib = L
+    _lib_path = os.path.abspath(path)
     return L
+
+
+def loaded_path():
+    """Path of the loaded HIP library."""
+    load()
+    return _lib_path
 
 
 def _gas_names(names):
