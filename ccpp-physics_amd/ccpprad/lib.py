@@ -27,7 +27,7 @@ EXPORTS = ["ccpprad_lw_init", "ccpprad_sw_init", "ccpprad_lw_init_files", "ccppr
            "ccpprad_prepare_inputs_dev", "ccpprad_post_dev", "ccpprad_layer_geometry_dev",
            "ccpprad_cloud_overlap_dev", "ccpprad_lw_init_files_pade", "ccpprad_sw_init_files_pade",
            "ccpprad_heating_rate", "ccpprad_set_launch_columns", "ccpprad_launch_columns",
-           "ccpprad_lw_mcica_mask_dev", "ccpprad_sw_mcica_mask_dev"]
+           "ccpprad_lw_mcica_mask_dev", "ccpprad_sw_mcica_mask_dev", "ccpprad_lw_mcica_mask_noalloc_dev"]
 
 
 def load():
@@ -59,6 +59,7 @@ def load():
     L.ccpprad_launch_columns.argtypes = [vp]
     L.ccpprad_lw_mcica_mask_dev.argtypes = [vp, vp, vp, vp, C.c_char_p, C.c_int]
     L.ccpprad_sw_mcica_mask_dev.argtypes = [vp, vp, vp, vp, C.c_char_p, C.c_int]
+    L.ccpprad_lw_mcica_mask_noalloc_dev.argtypes = [vp, vp, vp, vp, C.c_char_p, C.c_int]
     L.ccpprad_lw_gas_optics_dev.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp, C.c_char_p, C.c_int]
     L.ccpprad_sw_gas_optics_dev.argtypes = L.ccpprad_lw_gas_optics_dev.argtypes
     L.ccpprad_prepare_inputs_dev.argtypes = [vp, C.c_int, C.c_int] + [vp] * 14 + [vp, C.c_char_p, C.c_int]
@@ -199,6 +200,14 @@ class LWScheme(_Scheme):
         h = capi.build_args(capi.LWArgs, ints, arrays)
         buf, n = capi.errbuf()
         rc = self.L.ccpprad_lw_mcica_mask_dev(self.ctx, C.addressof(h.struct), _dp(mask), stream, buf, n)
+        capi.check(rc, buf)
+
+    def mcica_mask_noalloc_dev(self, ints: dict, arrays: dict, mask, stream=None):
+        """mcica_mask_dev through the allocation-free entry (its block flags
+        in the library's fixed device array)."""
+        h = capi.build_args(capi.LWArgs, ints, arrays)
+        buf, n = capi.errbuf()
+        rc = self.L.ccpprad_lw_mcica_mask_noalloc_dev(self.ctx, C.addressof(h.struct), _dp(mask), stream, buf, n)
         capi.check(rc, buf)
 
     def gas_optics_dev(self, ncol, nlay, p_lay, p_lev, t_lay, vmr, tau, pfrac, stream=None):

@@ -242,6 +242,7 @@ static std::vector<float> to_f32(const double *p, size_t n)
 }
 
 static const char *KGAS[MAX_GAS] = {"h2o", "co2", "o3", "n2o", "ch4", "o2"};
+static int scratch_setup(int dev, char *errmsg, int errlen);
 
 static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lut *lut,
                     const char *const *gas_names, int ngases, int nLay, int blksz, ccpprad_ctx **out,
@@ -507,6 +508,10 @@ static int ctx_init(int want_lw, const ccpprad_kdist *kd, const ccpprad_cloud_lu
         ccpprad_finalize(c);
         return 2;
     }
+    if (scratch_setup(c->device, errmsg, errlen)) {
+        ccpprad_finalize(c);
+        return 2;
+    }
     *out = c;
     return 0;
 }
@@ -644,6 +649,83 @@ extern "C" int ccpprad_lw_mcica_mask_dev(ccpprad_ctx *c, const ccpprad_lw_args *
                                          char *errmsg, int errlen)
 {
     return mcica_mask_entry(c, 1, a, nullptr, mask, stream, errmsg, errlen);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free LW McICA mask: the scheme-block cloud flags live in a
+// file-scope device array instead of the per-stream workspace.  Every device
+// the library runs on holds its own copy until the process ends; host code
+// uses only the address hipGetSymbolAddress returned for the current device
+// (taken once per device at context init), never the array's name.  Calls
+// that overlap on one device share the array and race.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_MAX_BLOCKS = (size_t)1 << 20;   // one flag byte per scheme block
+static __device__ __align__(256) unsigned char scratch[NOALLOC_MAX_BLOCKS];
+constexpr int NOALLOC_MAX_DEV = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[NOALLOC_MAX_DEV];
+
+// The scratch address on device dev (current): looked up once per device.
+static int scratch_setup(int dev, char *errmsg, int errlen)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEV) { seterr(errmsg, errlen, "device %d beyond %d", dev, NOALLOC_MAX_DEV); return 2; }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return 0;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        seterr(errmsg, errlen, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        return 2;
+    }
+    g_scratch_ptr[dev] = (unsigned char *)p;
+    return 0;
+}
+
+static unsigned char *scratch_ptr(int dev)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEV) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[dev];
+}
+
+extern "C" int ccpprad_lw_mcica_mask_noalloc_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, unsigned *mask,
+                                                 void *stream, char *errmsg, int errlen)
+{
+    if (errmsg && errlen > 0) errmsg[0] = 0;
+    if (!c || !a || !mask) { seterr(errmsg, errlen, "null context, arguments or mask"); return 1; }
+    if (!c->is_lw) { seterr(errmsg, errlen, "context is not a LW context"); return 1; }
+    const int nL = a->nLay, n = a->nCol, bs = a->rrtmgp_phys_blksz, iovr = a->iovr;
+    if (nL != c->nlay) { seterr(errmsg, errlen, "nLay=%d differs from init (%d)", nL, c->nlay); return 1; }
+    if (n <= 0) return 0;
+    if (bs < 1 || n % bs) { seterr(errmsg, errlen, "column count %d must be a multiple of rrtmgp_phys_blksz=%d", n, bs); return 1; }
+    if ((size_t)(n / bs) > NOALLOC_MAX_BLOCKS) {
+        seterr(errmsg, errlen, "nCol / rrtmgp_phys_blksz = %d blocks exceeds the allocation-free limit of %zu blocks",
+               n / bs, NOALLOC_MAX_BLOCKS);
+        return 1;
+    }
+    if (!a->cld_frac) { seterr(errmsg, errlen, "required array argument is NULL"); return 1; }
+    if (a->isubc_lw == 2 && !a->icseed_lw) { seterr(errmsg, errlen, "isubc=2 needs icseed"); return 1; }
+    if ((iovr == a->iovr_exp || iovr == a->iovr_exprand || iovr == a->iovr_dcorr) && !a->cloud_overlap_param) {
+        seterr(errmsg, errlen, "exponential/decorrelation overlap needs cloud_overlap_param"); return 1;
+    }
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    unsigned char *blk = scratch_ptr(dev);
+    if (!blk) { seterr(errmsg, errlen, "no context was initialised on device %d", dev); return 1; }
+    hipStream_t st = (hipStream_t)stream;
+    int CH;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        CH = std::min(n, c->chunk);
+    }
+    // every flag of this call is written (cleared, then set) before it is read
+    launch_block_flags(n, nL, n, nullptr, bs, a->cld_frac, blk, nullptr, st);
+    for (int c0 = 0; c0 < n; c0 += CH) {
+        const McicaArgs ma = lw_mcica_args(c, a, blk, c0, mask + (size_t)c0 * nL * MASK_WORDS);
+        launch_mcica(ma, std::min(CH, n - c0), st);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
 }
 
 extern "C" int ccpprad_sw_mcica_mask_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, unsigned *mask, void *stream,

@@ -254,6 +254,18 @@ int ccpprad_lw_mcica_mask_dev(ccpprad_ctx *ctx, const ccpprad_lw_args *a, unsign
 int ccpprad_sw_mcica_mask_dev(ccpprad_ctx *ctx, const ccpprad_sw_args *a, unsigned *mask,
                               void *stream, char *errmsg, int errlen);
 
+/* ccpprad_lw_mcica_mask_dev without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  The per-block cloud flags live in a
+ * library-owned device array of 2^20 bytes per GPU, held until the process
+ * ends, so nCol / rrtmgp_phys_blksz may not exceed 1,048,576 (larger calls
+ * are rejected with a message).  The caller still passes the inputs and
+ * the mask.  Calls that overlap on one GPU share that array and race; a
+ * context must have been initialised on the current device first.  Same
+ * result as ccpprad_lw_mcica_mask_dev. */
+int ccpprad_lw_mcica_mask_noalloc_dev(ccpprad_ctx *ctx, const ccpprad_lw_args *a, unsigned *mask,
+                                      void *stream, char *errmsg, int errlen);
+
 /* Argument list of GFS_rrtmgp_post_run
  * (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_post.F90:35-389) less
  * its heating rates (those come fused out of the _run entries, or from
