@@ -284,7 +284,9 @@ def profile_fields(pr: syn.Profiles) -> dict:
         semis=pr.semis, tsfg=pr.tsfc, coszen=pr.coszen, sfc_alb_nir_dir=pr.alb_nir_dir,
         sfc_alb_nir_dif=pr.alb_nir_dif, sfc_alb_uvvis_dir=pr.alb_uvvis_dir,
         sfc_alb_uvvis_dif=pr.alb_uvvis_dif)
-    return {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in f.items()}
+    # every field its own buffer (several start as the same zeros), so that
+    # setting one field in place leaves the others alone
+    return {k: np.array(v, dtype=np.float64, order="C", copy=True) for k, v in f.items()}
 
 
 def build_args(cls, ints: dict, arrays: dict, solcon=None) -> Holder:

@@ -161,23 +161,43 @@ def test_planck_fraction_sums_to_one_per_band(kd_lw):
     assert np.allclose(s, 1.0, atol=1e-13)
 
 
-def test_rayleigh_proportional_to_moist_column(kd_sw):
-    """Doubling every gas amount (same mixing ratios) doubles tau_ray through
-    (col_dry + col_h2o); Rayleigh of a dry column with constant krayl equals
-    krayl * col_dry."""
+def test_humid_col_dry_is_the_hydrostatic_mass(kd_lw):
+    """With water vapour: the layer's dry-air and water columns together
+    weigh the layer, (col_dry m_dry + col_h2o m_h2o) / N_A 1e4 = dp / g,
+    and col_h2o = vmr_h2o col_dry (get_col_dry's moist-air molar mass)."""
+    vmr = np.zeros(kd_lw.ngas)
+    for vh in (0.0, 0.005, 0.03):
+        vmr[0] = vh
+        r = _interp(kd_lw, 50000.0, 270.0, vmr, dp=2500.0)
+        cd, ch = r["col"][0], r["col"][1]
+        mass = (cd * 0.028964 + ch * 0.018016) / 6.02214076e23 * 1e4
+        assert mass == pytest.approx(2500.0 / 9.80665, rel=1e-12)
+        assert ch == pytest.approx(vh * cd, rel=1e-15, abs=0.0)
+
+
+def test_rayleigh_uses_the_moist_column(kd_sw):
+    """Rayleigh depth with a constant krayl K is K (col_dry + col_h2o): in
+    a humid layer the ratio tau_ray / (K col_dry) is 1 + vmr_h2o (so neither
+    col_dry alone nor the total including other gases), in a dry layer it
+    is 1.  col_dry is the oracle's interpolation column, itself pinned by
+    the hydrostatic mass above."""
     from oracle import oracle as O
     kd = copy.deepcopy(kd_sw)
-    kd.krayl_lower = np.full_like(kd.krayl_lower, 3e-27)
-    kd.krayl_upper = np.full_like(kd.krayl_upper, 3e-27)
-    pr = _prof(kd, ncol=2, nlay=20, seed=5)
+    K = 3e-27
+    kd.krayl_lower = np.full_like(kd.krayl_lower, K)
+    kd.krayl_upper = np.full_like(kd.krayl_upper, K)
+    pr = _prof(kd, ncol=3, nlay=20, seed=5)
+    pr.vmr["h2o"] = pr.vmr["h2o"].copy()
+    pr.vmr["h2o"][:, 0] = 0.0            # a dry column
+    pr.vmr["h2o"][:, 1] = 0.03           # a very humid column
     _, ray = O.gas_optics(kd, pr)
-    pr2 = _prof(kd, ncol=2, nlay=20, seed=5)
-    pr2.p_lev = pr2.p_lev.copy()
-    # doubling the pressure thickness doubles every column amount
-    top = pr2.p_lev[-1].copy()
-    pr2.p_lev = top + 2.0 * (pr2.p_lev - top)
-    _, ray2 = O.gas_optics(kd, pr2)
-    assert np.allclose(ray2, 2.0 * ray, rtol=1e-12)
+    vs = O.vmr_stack(pr)
+    for c in range(pr.ncol):
+        for l in range(pr.nlay):
+            dp = abs(pr.p_lev[l, c] - pr.p_lev[l + 1, c])
+            r = _interp(kd, pr.p_lay[l, c], pr.t_lay[l, c], vs[:, l, c], dp=dp)
+            ratio = ray[c, l] / (K * r["col"][0])
+            assert np.allclose(ratio, 1.0 + pr.vmr["h2o"][l, c], rtol=1e-13, atol=0)
 
 
 def test_cmp_tlev_matches_reference_fortran():

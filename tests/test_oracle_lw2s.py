@@ -116,3 +116,51 @@ def test_no_scattering_equals_noscat_solver_at_diffusivity(kd_lw, top_at_1):
     up2, dn2 = O.lw_solve_2str(kd_lw, nc, nl, tau, z, z, src, emis, top_at_1=top_at_1)
     assert np.allclose(up2, up1, rtol=1e-8, atol=1e-9)
     assert np.allclose(dn2, dn1, rtol=1e-8, atol=1e-9)
+
+
+@pytest.mark.parametrize("top_at_1", [True, False])
+def test_combined_level_sources_at_unequal_planck_fractions(kd_lw, top_at_1):
+    """lw_combine_sources where the Planck fraction changes from layer to
+    layer: the two-stream solver's level source between two layers is the
+    geometric mean of the two layers' sources at that level (each layer's
+    Planck fraction times the level's B), and the one layer's at the top and
+    bottom.  orc_lw_solve_2str (inc/dec per layer in, band and broadband
+    sums out) equals the two-stream equations integrated with those level
+    sources g-point by g-point; a swapped inc/dec or a level off by one
+    fails (the fractions differ by up to 5x between layers)."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(11)
+    nc, nl, ng, nb = 1, 10, kd_lw.ngpt, kd_lw.nband
+    tau = 10 ** rng.uniform(-3, 0.2, (nc, nl, ng))
+    ssa = rng.uniform(0.0, 0.95, (nc, nl, ng))
+    g = rng.uniform(0.0, 0.9, (nc, nl, ng))
+    pf = rng.uniform(0.02, 0.1, (nc, nl, ng))            # differs between layers
+    B_td = rng.uniform(1.0, 5.0, nl + 1)                  # level Planck function, top first
+    B = B_td if top_at_1 else B_td[::-1]                  # array order
+    inc = pf * B[None, 1:, None]                          # layer l's source at array level l+1
+    dec = pf * B[None, :-1, None]                         # ... at array level l
+    sfc = rng.uniform(0.1, 1.0, (nc, ng))
+    emis = np.full((nc, nb), 0.95)
+    src = dict(lev_inc=inc, lev_dec=dec, sfc_src=sfc)
+    up, dn = O.lw_solve_2str(kd_lw, nc, nl, tau, ssa, g, src, emis, top_at_1=top_at_1)
+    up_ref = np.zeros(nl + 1)
+    dn_ref = np.zeros(nl + 1)
+    for gp in range(ng):
+        p = pf[0, :, gp]
+        levs = np.empty(nl + 1)                            # array order
+        levs[0] = p[0] * B[0]
+        levs[nl] = p[nl - 1] * B[nl]
+        levs[1:nl] = np.sqrt(p[1:] * B[1:nl] * p[:-1] * B[1:nl])
+        t, s, gg = tau[0, :, gp], ssa[0, :, gp], g[0, :, gp]
+        if top_at_1:
+            u, d = _two_stream_bvp(t, s, gg, levs, 0.95, sfc[0, gp])
+        else:
+            u, d = _two_stream_bvp(t[::-1], s[::-1], gg[::-1], levs[::-1], 0.95, sfc[0, gp])
+            u, d = u[::-1], d[::-1]
+        up_ref += u
+        dn_ref += d
+    scale = np.abs(up_ref).max()
+    assert np.allclose(up[:, 0], up_ref, rtol=1e-9, atol=1e-10 * scale)
+    assert np.allclose(dn[:, 0], dn_ref, rtol=1e-9, atol=1e-10 * scale)
+    # the arithmetic mean would differ visibly: the geometric mean is what is pinned
+    assert np.abs(pf[0, 1:] - pf[0, :-1]).max() > 0.02

@@ -1,6 +1,7 @@
 """Pins of the oracle's SW solver (two-stream + adding) and cloud optics."""
 import numpy as np
 import pytest
+from scipy.linalg import expm
 
 
 def _solve1(kd, tau, ssa, g, mu0, adir, adif, nlay=1, toa=None):
@@ -107,3 +108,82 @@ def test_cloud_lut_linear_in_radius(lut_lw):
     assert np.allclose(g, ((tl * sl * gl + ti * si * gi) / (tl * sl + ti * si))[None], rtol=1e-12)
     with pytest.raises(ValueError):
         O.cloud_optics(lut, lwp + 1, iwp, r1 * 0 + lut.radliq_upr + 1, r2)
+
+
+def _sw_bvp(layers, mu0, adir, adif):
+    """Diffuse fluxes of the two-stream equations with a direct-beam source,
+    integrated independently of sw_two_stream / adding: with tau increasing
+    downward, I the beam flux normal to the beam (I(0) = 1 / mu0, so the
+    horizontal direct flux at the top is 1),
+      dF+/dtau =  g1 F+ - g2 F- - w g3 I
+      dF-/dtau =  g2 F+ - g1 F- + w g4 I
+      dI/dtau  = -I / mu0
+    with the PIFM coefficients g1 = (8 - w (5 + 3 g)) / 4, g2 = 3 w (1 - g) / 4,
+    g3 = (2 - 3 mu0 g) / 4, g4 = 1 - g3 (Zdunkowski et al. 1980).  Each
+    layer's map is the exponential of its constant matrix; F-(0) = 0 and
+    F+(bottom) = adif F-(bottom) + adir mu0 I(bottom) fix F+(0).
+    layers: (tau, w, g) top first.  Returns F+, F- (diffuse) at the levels,
+    top first."""
+    Es = []
+    for tau, w, g in layers:
+        g1 = (8.0 - w * (5.0 + 3.0 * g)) / 4.0
+        g2 = 3.0 * w * (1.0 - g) / 4.0
+        g3 = (2.0 - 3.0 * mu0 * g) / 4.0
+        M = np.array([[g1, -g2, -w * g3], [g2, -g1, w * (1.0 - g3)], [0.0, 0.0, -1.0 / mu0]])
+        Es.append(expm(M * tau))
+    P = np.eye(3)
+    for E in Es:
+        P = E @ P
+    # F+(bottom) - adif F-(bottom) - adir mu0 I(bottom) = 0, linear in u = F+(0)
+    c = P[0] - adif * P[1] - adir * mu0 * P[2]
+    u = -c[2] / (mu0 * c[0])
+    X = np.array([u, 0.0, 1.0 / mu0])
+    ups, dns = [X[0]], [X[1]]
+    for E in Es:
+        X = E @ X
+        ups.append(X[0])
+        dns.append(X[1])
+    return np.array(ups), np.array(dns)
+
+
+@pytest.mark.parametrize("tau,w,g,mu0", [(0.3, 0.5, 0.6, 0.7), (1.2, 0.9, 0.8, 0.35), (0.05, 0.99, 0.3, 0.9),
+                                         (2.5, 0.7, 0.85, 0.15), (0.8, 0.2, 0.0, 0.55)])
+def test_direct_beam_reflectance_and_transmittance(kd_sw, tau, w, g, mu0):
+    """One layer over a black surface: R_dir and T_dir of sw_two_stream
+    (the diffuse up flux at the top and down flux at the bottom per unit
+    direct flux at the top) against the two-stream equations integrated by
+    matrix exponentials, where the clamps to [0, 1 - T0] are inactive."""
+    ng = kd_sw.ngpt
+    up, dn, dr = _solve1(kd_sw, tau, w, g, mu0, 0.0, 0.0)
+    r_dir = up[-1] / (ng * mu0)
+    t_dir = (dn[0] - dr[0]) / (ng * mu0)
+    u_ref, d_ref = _sw_bvp([(tau, w, g)], mu0, 0.0, 0.0)
+    T0 = np.exp(-tau / mu0)
+    assert 0.0 < u_ref[0] < 1.0 - T0 and 0.0 < d_ref[-1] < 1.0 - T0 - u_ref[0]   # clamps inactive
+    assert r_dir == pytest.approx(u_ref[0], rel=1e-10)
+    assert t_dir == pytest.approx(d_ref[-1], rel=1e-10)
+
+
+@pytest.mark.parametrize("top_at_1", [False, True])
+def test_two_layer_composition_with_surface(kd_sw, top_at_1):
+    """Two different layers over a reflecting surface (direct and diffuse
+    albedos differ): the solver's diffuse fluxes at all three levels (adding
+    of the layers' R/T and direct-beam sources) equal the two-stream
+    equations integrated through both layers with the surface condition."""
+    from oracle import oracle as O
+    ng, nb = kd_sw.ngpt, kd_sw.nband
+    top, bot = (0.6, 0.85, 0.7), (1.4, 0.4, 0.5)
+    mu0, adir, adif = 0.45, 0.3, 0.6
+    lay_arr = [top, bot] if top_at_1 else [bot, top]
+    t = np.array([[[p[0]] * ng for p in lay_arr]])
+    s = np.array([[[p[1]] * ng for p in lay_arr]])
+    gg = np.array([[[p[2]] * ng for p in lay_arr]])
+    bu, bd, bdir = O.sw_solve(kd_sw, 1, 2, np.array([mu0]), t, s, gg, np.ones((1, ng)),
+                              np.full((1, nb), adir), np.full((1, nb), adif), top_at_1=top_at_1)
+    up, dn, dr = bu[0].sum(axis=1) / (ng * mu0), bd[0].sum(axis=1) / (ng * mu0), bdir[0].sum(axis=1) / (ng * mu0)
+    if not top_at_1:
+        up, dn, dr = up[::-1], dn[::-1], dr[::-1]        # top first
+    u_ref, d_ref = _sw_bvp([top, bot], mu0, adir, adif)
+    assert np.allclose(up, u_ref, rtol=1e-10, atol=1e-13)
+    assert np.allclose(dn - dr, d_ref, rtol=1e-10, atol=1e-13)
+    assert dr[-1] == pytest.approx(np.exp(-(top[0] + bot[0]) / mu0), rel=1e-12)
