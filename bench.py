@@ -39,9 +39,7 @@ METRIC = "LW+SW columns/s at 127 layers (table vs NN) @ HR RMSE<=0.1 K/day vs FP
 # --no-nn --copy-probe`), each scaled by the reading of a 1 GiB device copy
 # (FETCH x 2.0, WRITE x 1.0).  Counters cannot be read from inside this
 # process, so the last measurement of the current kernels is recorded here.
-TRAFFIC_GB = {"sw_sw_solve": 19.579, "lw_lw_solve": 22.667, "lw_gas_optics": 11.728, "sw_gas_optics": 10.670,
-              "lw_gas_interp": 3.026, "sw_gas_interp": 3.026, "lw_cloud_optics": 0.776, "sw_cloud_optics": 0.776,
-              "lw_mcica": 0.670, "sw_mcica": 0.670, "lw_solve_2str": 24.485}
+TRAFFIC_GB = {}
 # sha256 (first 16 hex digits) of the kernel sources the figures above were
 # measured with (kernel_source_sha()); on any other sources the bench line
 # reports traffic null rather than a stale figure.
@@ -260,7 +258,7 @@ def main():
         L.ccpprad_timing_enable(s.ctx, 1)
     step(serial=True)
     kt = {}
-    names = {6: "gas_interp", 0: "gas_optics", 1: "lw_solve", 2: "sw_solve", 3: "cloud_optics", 4: "mcica"}
+    names = {0: "gas_optics", 1: "lw_solve", 2: "sw_solve", 3: "cloud_optics", 4: "mcica"}
     for tag, s in (("lw", lw), ("sw", sw)):
         for kid, nm in names.items():
             tot, cnt = C.c_double(), C.c_int()
@@ -278,10 +276,8 @@ def main():
     per_col = {
         "lw_lw_solve": 269_308 + 2_048 + 3_072 + 1_016,
         "sw_sw_solve": 193_000 + 3_120 + 3_072 + 1_016,
-        "lw_gas_interp": 4_584,          # gas optics inputs (interpolation stage)
-        "lw_gas_optics": 261_120,        # tau + Planck fraction out (g-point gathers)
-        "sw_gas_interp": 4_576,
-        "sw_gas_optics": 171_136,        # tau_abs + tau_Rayleigh out
+        "lw_gas_optics": 4_584 + 261_120,   # gas-optics inputs in, tau + Planck fraction out
+        "sw_gas_optics": 4_576 + 171_136,   # inputs in, tau_abs + tau_Rayleigh out
         "lw_cloud_optics": 2_540 + 8_128,
         "sw_cloud_optics": 2_540 + 21_336,
     }
@@ -341,8 +337,8 @@ def main():
             L.ccpprad_timing_enable(lw_nn.ctx, 1)
             lw_nn.run_dev(lw_ints, nn_args, stream=sp, validate=False)
             nkt = {}
-            for kid, nm in {6: "gas_interp_fallback", 0: "gas_optics_fallback", 1: "lw_solve", 3: "cloud_optics",
-                            4: "mcica", 5: "mlp_infer"}.items():
+            for kid, nm in {6: "gas_optics_fallback", 1: "lw_solve", 3: "cloud_optics", 4: "mcica",
+                            5: "mlp_infer"}.items():
                 tot, cnt = C.c_double(), C.c_int()
                 L.ccpprad_timing_query(lw_nn.ctx, kid, C.byref(tot), C.byref(cnt))
                 if cnt.value:
@@ -397,7 +393,7 @@ def main():
         L.ccpprad_timing_enable(lw_s.ctx, 1)
         lw_s.run_dev(s_ints, s_args, stream=sp, validate=False)
         skt = {}
-        for kid, nm in {6: "gas_interp", 0: "gas_optics", 1: "lw_solve_2str", 3: "cloud_optics", 4: "mcica"}.items():
+        for kid, nm in {0: "gas_optics", 1: "lw_solve_2str", 3: "cloud_optics", 4: "mcica"}.items():
             tot, cnt = C.c_double(), C.c_int()
             L.ccpprad_timing_query(lw_s.ctx, kid, C.byref(tot), C.byref(cnt))
             if cnt.value:

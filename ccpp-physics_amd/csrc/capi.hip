@@ -71,7 +71,7 @@ struct Workspace {
     bool own_stream = false;       // created for a host thread's staging path
     std::thread::id owner;         // that thread
     DevBuf w_tau, w_aux, w_mask, w_ctau, w_sb, w_blk, w_colf, w_day, w_idx;
-    DevBuf w_oob, w_colm, w_rec;
+    DevBuf w_oob, w_colm;
     std::vector<DevBuf> stage;     // host-path staging
     int *err = nullptr;            // device error word, then the SW last-wipe block
     double *toa64 = nullptr;       // SW TOA source scaled to solcon
@@ -79,7 +79,7 @@ struct Workspace {
     void release()
     {
         DevBuf *bufs[] = {&w_tau, &w_aux, &w_mask, &w_ctau, &w_sb, &w_blk, &w_colf, &w_day, &w_idx,
-                          &w_oob, &w_colm, &w_rec};
+                          &w_oob, &w_colm};
         for (DevBuf *b : bufs) b->release();
         for (auto &b : stage) b.release();
         if (err) (void)hipFree(err);
@@ -134,7 +134,7 @@ static Workspace *ws_for_stream(ccpprad_ctx *c, hipStream_t st)
     return w;
 }
 
-enum { K_GAS = 0, K_LWSOLVE = 1, K_SWSOLVE = 2, K_CLOUD = 3, K_MCICA = 4, K_MLP = 5, K_INTERP = 6, K_N = 7 };
+enum { K_GAS = 0, K_LWSOLVE = 1, K_SWSOLVE = 2, K_CLOUD = 3, K_MCICA = 4, K_MLP = 5, K_GASFB = 6, K_N = 7 };
 
 static hipEvent_t ev_take(ccpprad_ctx *c)
 {
@@ -779,7 +779,6 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
     if (!w) { seterr(errmsg, errlen, "workspace allocation failed"); return 2; }
     HIPCHK(w->w_tau.ensure(sizeof(float) * (size_t)CH * nL * G));
     HIPCHK(w->w_aux.ensure(sizeof(float) * (size_t)CH * nL * G));
-    HIPCHK(w->w_rec.ensure(sizeof(unsigned) * (size_t)CH * nL * gas_record_words(T)));
     HIPCHK(w->w_mask.ensure(sizeof(unsigned) * (size_t)CH * nL * MASK_WORDS));
     HIPCHK(w->w_ctau.ensure((scat ? sizeof(float3) : sizeof(float)) * (size_t)CH * nL * B));
     HIPCHK(w->w_blk.ensure((size_t)nblk + 1));
@@ -830,11 +829,9 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
             launch_fill_u8(mi.colflag, (size_t)nc, st);
             { KScope k_(c, K_MLP, st); launch_mlp_infer(c->mlp, mi, st); }
             // layers outside the training hull: table gas optics
-            { KScope k_(c, K_INTERP, st); launch_gas_interp(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)w->w_rec.p, w->err, st, mi.colflag); }
-            { KScope k_(c, K_GAS, st); launch_gas_gather(T, nc, nL, (const unsigned *)w->w_rec.p, tau, pf, st, mi.oob, mi.colflag); }
+            { KScope k_(c, K_GASFB, st); launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, w->err, tau, pf, st, mi.oob, mi.colflag); }
         } else {
-            { KScope k_(c, K_INTERP, st); launch_gas_interp(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)w->w_rec.p, w->err, st); }
-            { KScope k_(c, K_GAS, st); launch_gas_gather(T, nc, nL, (const unsigned *)w->w_rec.p, tau, pf, st); }
+            { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, nullptr, c0, a->p_lay, a->p_lev, a->t_lay, vm, w->err, tau, pf, st); }
         }
         LwSolveArgs s{};
         s.nlay = nL; s.ncol_ld = nC; s.col0 = c0; s.top_at_1 = a->top_at_1;
@@ -928,7 +925,6 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
     const int nblk = nD / bs;
     HIPCHK(w->w_tau.ensure(sizeof(float) * (size_t)CH * nL * G));
     HIPCHK(w->w_aux.ensure(sizeof(float) * (size_t)CH * nL * G));
-    HIPCHK(w->w_rec.ensure(sizeof(unsigned) * (size_t)CH * nL * gas_record_words(T)));
     HIPCHK(w->w_mask.ensure(sizeof(unsigned) * (size_t)CH * nL * MASK_WORDS));
     HIPCHK(w->w_ctau.ensure(sizeof(float3) * (size_t)CH * nL * B));
     HIPCHK(w->w_sb.ensure(sizeof(double) * (size_t)nD * 3 * B));
@@ -973,8 +969,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
             const McicaArgs ma = sw_mcica_args(c, a, blk, c0, mask);
             { KScope k_(c, K_MCICA, st); launch_mcica(ma, nc, st); }
         }
-        { KScope k_(c, K_INTERP, st); launch_gas_interp(T, nc, nL, nC, a->idx, c0, a->p_lay, a->p_lev, a->t_lay, vm, (unsigned *)w->w_rec.p, w->err, st); }
-        { KScope k_(c, K_GAS, st); launch_gas_gather(T, nc, nL, (const unsigned *)w->w_rec.p, tabs, tray, st); }
+        { KScope k_(c, K_GAS, st); launch_gas_optics(T, nc, nL, nC, a->idx, c0, a->p_lay, a->p_lev, a->t_lay, vm, w->err, tabs, tray, st); }
         SwSolveArgs s{};
         s.nlay = nL; s.ncol_ld = nC; s.col0 = c0; s.top_at_1 = a->top_at_1; s.blksz = bs; s.isfc = a->iSFC - 1;
         s.write_clr_zero = !a->doSWclrsky; s.role = 0;
@@ -1162,7 +1157,6 @@ static int gas_optics_entry(ccpprad_ctx *c, int want_lw, int nCol, int nLay, con
     Workspace *w = ws_for_stream(c, st);
     if (!w) { seterr(errmsg, errlen, "workspace allocation failed"); return 2; }
     launch_fill_i32(w->err, 1, 0, st);
-    HIPCHK(w->w_rec.ensure(sizeof(unsigned) * (size_t)nCol * nLay * gas_record_words(c->T)));
     if (want_lw && c->mlp) {
         // the gas-optics stage of the LW run with the emulator attached:
         // MLP inference, then the table path on the layers outside its hull
@@ -1177,12 +1171,10 @@ static int gas_optics_entry(ccpprad_ctx *c, int want_lw, int nCol, int nLay, con
         mi.oob = (unsigned char *)w->w_oob.p; mi.colflag = (unsigned char *)w->w_colm.p;
         launch_fill_u8(mi.colflag, (size_t)nCol, st);
         launch_mlp_infer(c->mlp, mi, st);
-        launch_gas_interp(c->T, nCol, nLay, nCol, nullptr, 0, p_lay, p_lev, t_lay, vm, (unsigned *)w->w_rec.p, w->err, st,
+        launch_gas_optics(c->T, nCol, nLay, nCol, nullptr, 0, p_lay, p_lev, t_lay, vm, w->err, tau, aux, st, mi.oob,
                           mi.colflag);
-        launch_gas_gather(c->T, nCol, nLay, (const unsigned *)w->w_rec.p, tau, aux, st, mi.oob, mi.colflag);
     } else {
-        launch_gas_optics(c->T, nCol, nLay, nCol, nullptr, 0, p_lay, p_lev, t_lay, vm, (unsigned *)w->w_rec.p, tau, aux,
-                          w->err, st);
+        launch_gas_optics(c->T, nCol, nLay, nCol, nullptr, 0, p_lay, p_lev, t_lay, vm, w->err, tau, aux, st);
     }
     HIPCHK(hipGetLastError());
     return 0;

@@ -7,56 +7,55 @@
 // (interpolate2D), compute_Planck_source's Planck-fraction interpolation,
 // and for SW compute_tau_rayleigh.
 //
-// Two kernels:
-//   gas_interp_kernel (FP64), thread per (column, layer): col_dry and the gas
-//       columns, jtemp/ftemp, jpress/fpress, the tropopause flag, and per
-//       (flavor, T-corner) col_mix, jeta and the fminor weights, per minor
-//       interval the absorber scaling.  The discrete decisions (indices,
-//       tropopause) are taken in FP64 as in the FP64 reference.  The result
-//       is one compact record per (column, layer) (layout below), stored
-//       layer-major ([layer][column]) so a wavefront's 64 records are one
-//       contiguous block, assembled in LDS and written with 16-byte stores.
-//   gas_optics_kernel (FP32), block = (16-layer chunk, column), chunk slowest
-//       so co-resident blocks share k-table rows in L2, lane = g-point: the
-//       chunk's records are staged into LDS with the per-(layer, item) table
-//       offsets and weights built once per block, then 8 major-table corners
-//       (+8 Planck-fraction corners LW, +4 Rayleigh corners SW) and the minor
-//       intervals covering the g-point, two layers' gathers in flight.
+// One kernel, gas_optics_kernel: block = (16-layer chunk, column), lane =
+// g-point.
+//   1. Interpolation (FP64), one thread per layer of the chunk: col_dry and
+//      the gas columns, jtemp/ftemp, jpress/fpress, the tropopause flag, and
+//      per (flavor, T-corner) col_mix, jeta and the fminor weights, per
+//      minor interval the absorber scaling.  The discrete decisions
+//      (indices, tropopause) are taken in FP64 as in the FP64 reference.
+//      The result is one compact record per layer (layout below), built in
+//      LDS; it never goes to HBM.
+//   2. Per (layer, item) table offsets and weights, built once per block.
+//   3. Gathers (FP32): 8 major-table corners (+8 Planck-fraction corners LW,
+//      +4 Rayleigh corners SW) and the minor intervals covering the g-point,
+//      two layers' gathers in flight.
+// Blocks are numbered so that each of the 8 XCDs (which take workgroups in
+// turn) walks one contiguous range of (chunk, column): its L2 holds the
+// k-table rows of a few chunks only, and the CCPP input lines of a run of
+// neighbouring columns.
 // Output tau / pfrac / tau_ray : [col][lay][gpt] FP32.
 //
-// Record (32-bit words), NF2 = 2 nflav, NMN = the larger regime's minor count:
-//   [0]                 jtemp | jpress << 8 | itropo << 16
-//   [1] fpress  [2] cray (column of dry air + water vapour, Rayleigh)
-//   [3 + 3i .. 3i + 5]  fminor weights (eta lower, eta upper) and col_mix of
-//                       item i = 2 flavor + T-corner
-//   [JE + i/4]          jeta of item i, one byte each (JE = 3 + 3 NF2)
-//   [MS + m]            scaling of minor interval m of the layer's regime;
-//                       [MS + NMN] = 0, the scale of unused item slots
+// Per-layer record in LDS (32-bit words), NMN = the larger regime's minor
+// count:
+//   [0]       jtemp | jpress << 8 | itropo << 16
+//   [1]       fpress  [2] cray (column of dry air + water vapour, Rayleigh)
+//   [3 + m]   scaling of minor interval m of the layer's regime;
+//             [3 + NMN] = 0, the scale of unused item slots
+// and per (layer, item), item = 2 flavor + T-corner, two quads of gather
+// constants (interp_item).
 #include "kernels.h"
 
 namespace ccpprad {
 
 constexpr int LCHUNK = 16;
 
-// record geometry shared by both kernels
+// record geometry
 struct RecGeom {
-    int nf2, je, ms, nmn, words;
+    int nf2, ms, nmn, words;
 };
 __host__ __device__ inline RecGeom rec_geom(int nflav, int nminor0, int nminor1)
 {
     RecGeom r;
     r.nf2 = 2 * nflav;
-    r.je = 3 + 3 * r.nf2;
-    r.ms = r.je + (r.nf2 + 3) / 4;
+    r.ms = 3;
     r.nmn = nminor0 > nminor1 ? nminor0 : nminor1;
     r.words = 4 * ((r.ms + r.nmn + 1 + 3) / 4);
     return r;
 }
 
-int gas_record_words(const TablesView &T) { return rec_geom(T.nflav, T.nminor[0], T.nminor[1]).words; }
-
 // ---------------------------------------------------------------------------
-// interpolation: thread per (column, layer), FP64
+// interpolation, FP64
 // ---------------------------------------------------------------------------
 // v: the lane's gas columns in LDS (index 0 = dry air; a register array
 // indexed at run time would go to scratch); i outside 1..n-1 reads the
@@ -66,68 +65,22 @@ __device__ __forceinline__ double pick(const double *v, int n, int i)
     return v[(i >= 1 && i < n) ? i : 0];
 }
 
-// Reads of the read-only int tables at wavefront-uniform addresses, through
-// the scalar cache (constant address space): a vector load of one address
-// still returns 4 bytes for each of the 64 lanes through the texture path.
-typedef const __attribute__((address_space(4))) int cint;
-__device__ __forceinline__ int sld(const int *p, int i) { return ((cint *)p)[i]; }
 
-// One wavefront of 64 (column, layer) rows: each lane builds its record in
-// LDS, then the wavefront writes the 64 consecutive records as one
-// contiguous block with 16-byte stores.
-constexpr int INTERP_ROWS = 64;
+// Per-layer FP64 scratch of the interpolation in LDS: the gas columns
+// (index 0 = dry air), then ftemp, the density factor 0.01 p / T of the
+// minor scalings, 1 / col_dry and 1 / (1 + col_h2o / col_dry).
+constexpr int ROW_DBL = MAX_GAS + 1 + 4;
 
-__device__ __forceinline__ void interp_row(const TablesView &T, const RecGeom &G, unsigned i, int ncols, int nlay,
-                                           int ncol_ld, const int *col_idx, int col0, const double *p_lay,
-                                           const double *p_lev, const double *t_lay, const VmrPtrs &vmr,
-                                           unsigned *r, double *col, int *err);
-
-__global__ __launch_bounds__(INTERP_ROWS) void gas_interp_kernel(
-    TablesView T, int nlay, int ncol_ld, const int *__restrict__ col_idx, int col0, int ncols,
-    const double *__restrict__ p_lay, const double *__restrict__ p_lev, const double *__restrict__ t_lay,
-    VmrPtrs vmr, unsigned *__restrict__ rec, int *__restrict__ err, const unsigned char *__restrict__ col_mask)
+// Interpolation, part 1 (one thread per layer): get_col_dry, the
+// temperature and pressure indices and weights, the tropopause flag and the
+// Rayleigh column; record words 0-2 and the layer's scratch.
+__device__ __forceinline__ void interp_layer(const TablesView &T, int l, int d, int ncol_ld, const int *col_idx,
+                                             int col0, const double *p_lay, const double *p_lev, const double *t_lay,
+                                             const VmrPtrs &vmr, unsigned *r, double *rs, int *err)
 {
-    extern __shared__ unsigned srec[];   // [INTERP_ROWS][words]
-    __shared__ double scol[INTERP_ROWS][MAX_GAS + 1];
-    const RecGeom G = rec_geom(T.nflav, T.nminor[0], T.nminor[1]);
-    // a launch covers fewer than 2^32 rows: 32-bit row arithmetic
-    const unsigned nrow = (unsigned)ncols * (unsigned)nlay;
-    const unsigned i0 = blockIdx.x * INTERP_ROWS, i = i0 + threadIdx.x;
-    // fallback launch: blocks without a flagged column write nothing
-    if (col_mask && !__any(i < nrow && col_mask[i % (unsigned)ncols])) return;   // one wavefront: uniform
-    // every row of a flagged column gets a record: the gather kernel stages
-    // whole chunks, so no record it reads may be left unwritten (rows of
-    // unflagged columns are written with whatever the slot holds; nothing
-    // reads them)
-    if (i < nrow && (!col_mask || col_mask[i % (unsigned)ncols]))
-        interp_row(T, G, i, ncols, nlay, ncol_ld, col_idx, col0, p_lay, p_lev, t_lay, vmr,
-                   srec + threadIdx.x * G.words, scol[threadIdx.x], err);
-    __syncthreads();
-    const unsigned nr = nrow - i0 < (unsigned)INTERP_ROWS ? nrow - i0 : (unsigned)INTERP_ROWS;
-    const int n4 = (int)nr * G.words / 4;
-    uint4 *dst = (uint4 *)(rec + (size_t)i0 * G.words);
-    const uint4 *src = (const uint4 *)srec;
-    // 16-byte copies, 6 LDS reads in flight per lane
-    for (int k0 = threadIdx.x; k0 < n4; k0 += 6 * INTERP_ROWS) {
-        uint4 v[6];
-#pragma unroll
-        for (int u = 0; u < 6; ++u) v[u] = src[min(k0 + u * INTERP_ROWS, n4 - 1)];
-#pragma unroll
-        for (int u = 0; u < 6; ++u)
-            if (k0 + u * INTERP_ROWS < n4) dst[k0 + u * INTERP_ROWS] = v[u];
-    }
-}
-
-__device__ __forceinline__ void interp_row(const TablesView &T, const RecGeom &G, unsigned i, int ncols, int nlay,
-                                           int ncol_ld, const int *col_idx, int col0, const double *p_lay,
-                                           const double *p_lev, const double *t_lay, const VmrPtrs &vmr,
-                                           unsigned *r, double *col, int *err)
-{
-    const int l = (int)(i / (unsigned)ncols), d = (int)(i - (unsigned)l * (unsigned)ncols);   // column fastest: coalesced CCPP reads
     const int gc = col_idx ? col_idx[col0 + d] - 1 : col0 + d;
     const size_t q = (size_t)gc + (size_t)ncol_ld * l;
     const double tmin = T.temp_ref_min, dT = T.temp_ref_delta;
-
     const double pl = p_lay[q], tl = t_lay[q];
     if (!(pl >= T.press_min && pl <= T.press_max && tl >= T.temp_min && tl <= T.temp_max)) atomicOr(err, 1);
     const double dp = fabs(p_lev[q] - p_lev[q + ncol_ld]);
@@ -135,75 +88,78 @@ __device__ __forceinline__ void interp_row(const TablesView &T, const RecGeom &G
     const double fact = 1.0 / (1.0 + vh);
     const double m_air = (M_DRY + M_H2O * vh) * fact;
     const double col_dry = 10.0 * dp * AVOGAD / (1000.0 * m_air * 100.0 * GRAV) * fact;
-    col[0] = col_dry;
+    rs[0] = col_dry;
 #pragma unroll
-    for (int gs = 1; gs <= MAX_GAS; ++gs) col[gs] = gs <= T.ngas ? vmr.v[gs - 1][q] * col_dry : 0.0;
+    for (int gs = 1; gs <= MAX_GAS; ++gs) rs[gs] = gs <= T.ngas ? vmr.v[gs - 1][q] * col_dry : 0.0;
     int jt = (int)((tl - (tmin - dT)) / dT);
     jt = min(T.ntemp - 1, max(1, jt));
-    const double ftemp = (tl - T.temp_ref[jt - 1]) / dT;
     const double lpl = log(pl);
     const double locp = 1.0 + (T.press_ref_log1 - lpl) / T.press_ref_log_delta;
     int jp = (int)locp;
     jp = min(T.npres - 1, max(1, jp));
     const int itr = (lpl > T.press_ref_trop_log) ? 1 : 2;
-    const int ng1 = T.ngas + 1;
-
+    rs[MAX_GAS + 1] = (tl - T.temp_ref[jt - 1]) / dT;     // ftemp
+    rs[MAX_GAS + 2] = 0.01 * pl / tl;                      // density factor
+    rs[MAX_GAS + 3] = 1.0 / col_dry;
+    rs[MAX_GAS + 4] = 1.0 / (1.0 + rs[1] * rs[MAX_GAS + 3]);
     r[0] = (unsigned)jt | ((unsigned)jp << 8) | ((unsigned)itr << 16);
     r[1] = __float_as_uint((float)(locp - (double)jp));
-    r[2] = __float_as_uint((float)(col[1] + col_dry));
-    unsigned jeword = 0;
-    for (int f = 0; f < T.nflav; ++f) {
-        const int g1 = sld(T.flavor, 2 * f), g2 = sld(T.flavor, 2 * f + 1);
-        const double c1 = pick(col, ng1, g1), c2 = pick(col, ng1, g2);
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int it = 2 * f + t;
-            const double ratio = T.flav_ratio[((size_t)(jt - 1 + t) * T.nflav + f) * 2 + (itr - 1)];
-            const double cm = c1 + ratio * c2;
-            const double eta = (cm > 2.0 * 2.2250738585072014e-308) ? c1 / cm : 0.5;
-            const double loceta = eta * (double)(T.neta - 1);
-            const int jeta = min((int)loceta + 1, T.neta - 1);
-            const double feta = loceta - floor(loceta);   // fmod(loceta, 1) for loceta >= 0
-            const double ftt = t == 0 ? (1.0 - ftemp) : ftemp;
-            r[3 + 3 * it] = __float_as_uint((float)((1.0 - feta) * ftt));
-            r[3 + 3 * it + 1] = __float_as_uint((float)(feta * ftt));
-            r[3 + 3 * it + 2] = __float_as_uint((float)cm);
-            jeword |= (unsigned)jeta << (8 * (it & 3));
-            if ((it & 3) == 3) { r[G.je + it / 4] = jeword; jeword = 0; }
-        }
-    }
-    if (G.nf2 & 3) r[G.je + G.nf2 / 4] = jeword;
-    const int reg = itr - 1;
-    // per-layer factors of the minor scalings (the same FP64 expressions the
-    // interval loop would form each time)
-    const double dens = 0.01 * pl / tl;
-    const double vf = 1.0 / col_dry;
-    const double df = 1.0 / (1.0 + col[1] * vf);
-    for (int m = 0; m < G.nmn; ++m) {
-        // the interval's parameters in both regimes (uniform, scalar loads),
-        // then the lane's regime
-        int pr[2][4];
-#pragma unroll
-        for (int rg = 0; rg < 2; ++rg)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) pr[rg][k] = m < T.nminor[rg] ? sld(T.minor_par[rg], 5 * m + k) : 0;
-        const int p0 = reg ? pr[1][0] : pr[0][0], p1 = reg ? pr[1][1] : pr[0][1];
-        const int p2 = reg ? pr[1][2] : pr[0][2], p3 = reg ? pr[1][3] : pr[0][3];
-        double sc = 0.0;
-        if (m < T.nminor[reg]) {
-            sc = pick(col, ng1, p0);
-            if (p1) {
-                sc *= dens;
-                if (p2 > 0) {
-                    const double cs = pick(col, ng1, p2);
-                    if (p3) sc *= (1.0 - cs * vf * df);
-                    else sc *= cs * vf * df;
-                }
+    r[2] = __float_as_uint((float)(rs[1] + col_dry));
+}
+
+// Interpolation, part 2 (one thread per (layer, item), item = 2 flavor +
+// T-corner): col_mix, jeta and the eta / temperature weights, turned
+// straight into the gather constants of the item: quad 0 the major-table
+// element offset of the (jtemp + t, jpress, jeta) corner without the
+// g-point, fe0, fe1 (eta weights times the temperature weight), col_mix;
+// quad 1 (jtemp + t) * neta + jeta - 1 (row of the minor and Rayleigh
+// tables), fe0, fe1, 0.
+__device__ __forceinline__ void interp_item(const TablesView &T, const unsigned *r, const double *rs, int it,
+                                            uint4 *item)
+{
+    const int ng1 = T.ngas + 1, f = it >> 1, t = it & 1;
+    const unsigned w0 = r[0];
+    const int jt = (int)(w0 & 0xFFu), jp = (int)((w0 >> 8) & 0xFFu), itr = (int)(w0 >> 16);
+    const double ftemp = rs[MAX_GAS + 1];
+    const int g1 = T.flavor[2 * f], g2 = T.flavor[2 * f + 1];
+    const double c1 = pick(rs, ng1, g1), c2 = pick(rs, ng1, g2);
+    const double ratio = T.flav_ratio[((size_t)(jt - 1 + t) * T.nflav + f) * 2 + (itr - 1)];
+    const double cm = c1 + ratio * c2;
+    const double eta = (cm > 2.0 * 2.2250738585072014e-308) ? c1 / cm : 0.5;
+    const double loceta = eta * (double)(T.neta - 1);
+    const int jeta = min((int)loceta + 1, T.neta - 1);
+    const double feta = loceta - floor(loceta);   // fmod(loceta, 1) for loceta >= 0
+    const double ftt = t == 0 ? (1.0 - ftemp) : ftemp;
+    const unsigned fe0 = __float_as_uint((float)((1.0 - feta) * ftt));
+    const unsigned fe1 = __float_as_uint((float)(feta * ftt));
+    const unsigned jt0 = (unsigned)(jt - 1 + t), je0 = (unsigned)(jeta - 1);
+    const unsigned jp0 = (unsigned)(jp + itr - 2);   // 0-based lower pressure row
+    const unsigned neta = (unsigned)T.neta, np1 = (unsigned)(T.npres + 1), ngpt = (unsigned)T.ngpt;
+    const unsigned ko = __umul24(__umul24(__umul24(jt0, np1) + jp0, neta) + je0, ngpt);
+    item[0] = make_uint4(ko, fe0, fe1, __float_as_uint((float)cm));
+    item[1] = make_uint4(__umul24(jt0, neta) + je0, fe0, fe1, 0u);
+}
+
+// Interpolation, part 3 (one thread per (layer, minor interval)): the
+// absorber scaling of minor interval m of the layer's regime (0 for m past
+// the regime's count), record word MS + m.
+__device__ __forceinline__ void interp_minor(const TablesView &T, const RecGeom &G, unsigned *r, const double *rs,
+                                             int m)
+{
+    const int reg = (int)(r[0] >> 16) - 1, ng1 = T.ngas + 1;
+    double sc = 0.0;
+    if (m < T.nminor[reg]) {
+        const int *par = T.minor_par[reg] + 5 * m;
+        sc = pick(rs, ng1, par[0]);
+        if (par[1]) {
+            sc *= rs[MAX_GAS + 2];
+            if (par[2] > 0) {
+                const double x = pick(rs, ng1, par[2]) * rs[MAX_GAS + 3] * rs[MAX_GAS + 4];
+                sc *= par[3] ? (1.0 - x) : x;
             }
         }
-        r[G.ms + m] = __float_as_uint((float)sc);
     }
-    r[G.ms + G.nmn] = 0u;
+    r[G.ms + m] = __float_as_uint((float)sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -246,7 +202,7 @@ __device__ __forceinline__ float4 ld4(const float4 *base, unsigned idx)
 //           tables), fe0, fe1, 0
 __host__ __device__ inline size_t gather_smem(const RecGeom &G)
 {
-    return (size_t)LCHUNK * (sizeof(unsigned) * G.words + 2 * sizeof(uint4) * G.nf2);
+    return (size_t)LCHUNK * (sizeof(unsigned) * G.words + 2 * sizeof(uint4) * G.nf2 + sizeof(double) * ROW_DBL);
 }
 
 size_t gas_gather_smem(int is_lw, int nflav, int nminor0, int nminor1, int ngpt, int neta)
@@ -259,21 +215,43 @@ size_t gas_gather_smem(int is_lw, int nflav, int nminor0, int nminor1, int ngpt,
 // both LW and SW: fewer wavefronts fit)
 template <bool LW> constexpr int gather_layers() { return 2; }
 
-template <bool LW, int NM>
+// The block's (chunk, column): workgroup b runs on XCD b mod 8, which takes
+// the b / 8-th item of its own contiguous eighth of the nchunk * ncols
+// (chunk, column) pairs, column fastest.  Returns false for the few padding
+// blocks past the end.
+__device__ __forceinline__ bool block_item(int ncols, int nchunk, int &chunk, int &d)
+{
+    const unsigned nb = (unsigned)ncols * (unsigned)nchunk;
+    const unsigned per = (nb + 7u) >> 3;
+    const unsigned b = blockIdx.x;
+    const unsigned j = (b & 7u) * per + (b >> 3);
+    if (j >= nb) return false;
+    chunk = (int)(j / (unsigned)ncols);
+    d = (int)(j - (unsigned)chunk * (unsigned)ncols);
+    return true;
+}
+
+// FB: the emulator's fallback launch (only rows with row_mask set, of
+// columns with col_mask set), a separate instantiation so that profiles
+// show it apart from the table path.
+template <bool LW, int NM, bool FB>
 __global__ __launch_bounds__(128) void gas_optics_kernel(
-    TablesView T, int nlay, int ncols, const unsigned *__restrict__ rec, float *__restrict__ tau,
+    TablesView T, int nlay, int ncol_ld, const int *__restrict__ col_idx, int col0, int ncols,
+    const double *__restrict__ p_lay, const double *__restrict__ p_lev, const double *__restrict__ t_lay,
+    VmrPtrs vmr, int *__restrict__ err, float *__restrict__ tau,
     float *__restrict__ aux /* LW pfrac, SW tau_ray */, const unsigned char *__restrict__ row_mask,
     const unsigned char *__restrict__ col_mask)
 {
     extern __shared__ uint4 smem_q[];
-    const int chunk = blockIdx.x / ncols;
-    const int d = blockIdx.x - chunk * ncols;  // column within this launch
+    const int nchunk = (nlay + LCHUNK - 1) / LCHUNK;
+    int chunk, d;   // chunk of layers, column within this launch
+    if (!block_item(ncols, nchunk, chunk, d)) return;
     const int tid = threadIdx.x;
-    if (col_mask && !col_mask[d]) return;      // uniform per block
+    if (FB && !col_mask[d]) return;            // uniform per block
     const int l0 = chunk * LCHUNK;
     const int nl = min(LCHUNK, nlay - l0);
-    if (row_mask) {
-        // fallback launch: only chunks holding a flagged layer do any work
+    if (FB) {
+        // only chunks holding a flagged layer do any work
         bool any = false;
         for (int l = l0; l < l0 + nl; ++l) any |= row_mask[(size_t)d * nlay + l] != 0;
         if (!any) return;                      // uniform per block
@@ -282,17 +260,12 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
     const int ngpt = T.ngpt, neta = T.neta, np1 = T.npres + 1;
     uint4 *sitem = smem_q;                                                   // [LCHUNK][nf2][2]
     unsigned *srec = (unsigned *)(smem_q + (size_t)LCHUNK * G.nf2 * 2);    // [LCHUNK][words]
+    double *srow = (double *)(srec + (size_t)LCHUNK * G.words);            // [LCHUNK][ROW_DBL]
 
-    // stage the chunk's records (layer-major, one contiguous record per
-    // layer) into LDS, 16 B per thread
-    {
-        const int w4 = G.words / 4, n4 = nl * w4;
-        uint4 *dst = (uint4 *)srec;
-        for (int k = tid; k < n4; k += blockDim.x) {
-            const int j = k / w4, wq = k - j * w4;
-            dst[k] = ((const uint4 *)(rec + ((size_t)(l0 + j) * ncols + d) * G.words))[wq];
-        }
-    }
+    // interpolation of the chunk's layers, FP64: the per-layer part ...
+    if (tid < nl)
+        interp_layer(T, l0 + tid, d, ncol_ld, col_idx, col0, p_lay, p_lev, t_lay, vmr, srec + tid * G.words,
+                     srow + tid * ROW_DBL, err);
 
     // per-lane constants: flavors per regime and the minor-absorber items
     // covering this g-point; NM slots per regime (NM = the table's most items
@@ -319,20 +292,16 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
     const unsigned dp_ = (unsigned)(neta * ngpt), de = (unsigned)ngpt;   // jpress and eta strides of the major table
     __syncthreads();
 
-    // item constants of every (layer, item) of the chunk
+    // ... then the (layer, item) and (layer, minor interval) parts over the
+    // whole block
     for (int k = tid; k < nl * G.nf2; k += blockDim.x) {
-        const int j = k / G.nf2, it = k - j * G.nf2;
-        const unsigned *r = srec + j * G.words;
-        const unsigned w0 = r[0];
-        const unsigned itr = w0 >> 16;
-        const unsigned jt = (w0 & 0xFFu) - 1u + (unsigned)(it & 1);
-        const unsigned jp = ((w0 >> 8) & 0xFFu) + itr - 2u;   // 0-based lower pressure row
-        const unsigned je = ((r[G.je + (it >> 2)] >> (8 * (it & 3))) & 0xFFu) - 1u;
-        const unsigned row = __umul24(jt, (unsigned)neta) + je;
-        const unsigned ko = __umul24(__umul24(__umul24(jt, (unsigned)np1) + jp, (unsigned)neta) + je, (unsigned)ngpt);
-        const unsigned fe0 = r[3 + 3 * it], fe1 = r[4 + 3 * it], cm = r[5 + 3 * it];
-        sitem[2 * k] = make_uint4(ko, fe0, fe1, cm);
-        sitem[2 * k + 1] = make_uint4(row, fe0, fe1, 0u);
+        const int j = k / G.nf2;
+        interp_item(T, srec + j * G.words, srow + j * ROW_DBL, k - j * G.nf2, sitem + 2 * k);
+    }
+    for (int k = tid; k < nl * (G.nmn + 1); k += blockDim.x) {
+        const int j = k / (G.nmn + 1), m = k - j * (G.nmn + 1);
+        if (m < G.nmn) interp_minor(T, G, srec + j * G.words, srow + j * ROW_DBL, m);
+        else srec[j * G.words + G.ms + G.nmn] = 0u;   // the scale of unused item slots
     }
     __syncthreads();
     if (!act) return;
@@ -430,7 +399,7 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
                 tg += __uint_as_float(r[G.ms + mi_iv(v)]) * kvm;
             }
             if (j0 + u >= nl) continue;
-            if (row_mask && !row_mask[(size_t)d * nlay + l]) continue;
+            if (FB && !row_mask[(size_t)d * nlay + l]) continue;
             const size_t o = ((size_t)d * nlay + l) * ngpt + g;
             tau[o] = tg;
             aux[o] = LW ? pf : kv * __uint_as_float(r[2]);
@@ -438,14 +407,18 @@ __global__ __launch_bounds__(128) void gas_optics_kernel(
     }
 }
 
-template <bool LW>
-static void launch_nm(const TablesView &T, int nblocks, int nlay, int ncols, const unsigned *rec, float *tau,
-                      float *aux, hipStream_t st, const unsigned char *row_mask, const unsigned char *col_mask)
+template <bool LW, bool FB>
+static void launch_nm(const TablesView &T, int ncols, int nlay, int ncol_ld, const int *col_idx, int col0,
+                      const double *p_lay, const double *p_lev, const double *t_lay, const VmrPtrs &vmr, int *err,
+                      float *tau, float *aux, hipStream_t st, const unsigned char *row_mask,
+                      const unsigned char *col_mask)
 {
     const size_t smem = gather_smem(rec_geom(T.nflav, T.nminor[0], T.nminor[1]));
-#define GO_LAUNCH(NMV)                                                                                          \
-    hipLaunchKernelGGL((gas_optics_kernel<LW, NMV>), dim3(nblocks), dim3(128), smem, st, T, nlay, ncols, rec, \
-                       tau, aux, row_mask, col_mask)
+    const unsigned nb = (unsigned)ncols * (unsigned)((nlay + LCHUNK - 1) / LCHUNK);
+    const unsigned grid = 8u * ((nb + 7u) / 8u);
+#define GO_LAUNCH(NMV)                                                                                        \
+    hipLaunchKernelGGL((gas_optics_kernel<LW, NMV, FB>), dim3(grid), dim3(128), smem, st, T, nlay, ncol_ld,  \
+                       col_idx, col0, ncols, p_lay, p_lev, t_lay, vmr, err, tau, aux, row_mask, col_mask)
     const int nm = T.maxminor_g;
     if (nm <= 0) GO_LAUNCH(0);
     else if (nm == 1) GO_LAUNCH(1);
@@ -455,33 +428,22 @@ static void launch_nm(const TablesView &T, int nblocks, int nlay, int ncols, con
 #undef GO_LAUNCH
 }
 
-void launch_gas_interp(const TablesView &T, int ncols, int nlay, int ncol_ld, const int *col_idx, int col0,
-                       const double *p_lay, const double *p_lev, const double *t_lay, const VmrPtrs &vmr,
-                       unsigned *rec, int *err, hipStream_t st, const unsigned char *col_mask)
+void launch_gas_optics(const TablesView &T, int ncols, int nlay, int ncol_ld, const int *col_idx, int col0,
+                       const double *p_lay, const double *p_lev, const double *t_lay, const VmrPtrs &vmr, int *err,
+                       float *tau, float *aux, hipStream_t st, const unsigned char *row_mask,
+                       const unsigned char *col_mask)
 {
     if (ncols <= 0 || nlay <= 0) return;
-    const size_t nrow = (size_t)ncols * nlay;
-    const size_t smem = sizeof(unsigned) * INTERP_ROWS * gas_record_words(T);
-    hipLaunchKernelGGL(gas_interp_kernel, dim3((unsigned)((nrow + INTERP_ROWS - 1) / INTERP_ROWS)), dim3(INTERP_ROWS),
-                       smem, st, T, nlay, ncol_ld, col_idx, col0, ncols, p_lay, p_lev, t_lay, vmr, rec, err, col_mask);
-}
-
-void launch_gas_gather(const TablesView &T, int ncols, int nlay, const unsigned *rec, float *tau, float *aux,
-                       hipStream_t st, const unsigned char *row_mask, const unsigned char *col_mask)
-{
-    if (ncols <= 0 || nlay <= 0) return;
-    const int nblocks = ncols * ((nlay + LCHUNK - 1) / LCHUNK);
-    if (T.is_lw) launch_nm<true>(T, nblocks, nlay, ncols, rec, tau, aux, st, row_mask, col_mask);
-    else launch_nm<false>(T, nblocks, nlay, ncols, rec, tau, aux, st, row_mask, col_mask);
-}
-
-void launch_gas_optics(const TablesView &T, int ncols, int nlay, int ncol_ld, const int *col_idx,
-                       int col0, const double *p_lay, const double *p_lev, const double *t_lay,
-                       const VmrPtrs &vmr, unsigned *rec, float *tau, float *aux, int *err, hipStream_t st,
-                       const unsigned char *row_mask, const unsigned char *col_mask)
-{
-    launch_gas_interp(T, ncols, nlay, ncol_ld, col_idx, col0, p_lay, p_lev, t_lay, vmr, rec, err, st, col_mask);
-    launch_gas_gather(T, ncols, nlay, rec, tau, aux, st, row_mask, col_mask);
+    const bool fb = row_mask != nullptr;
+    if (T.is_lw) {
+        if (fb) launch_nm<true, true>(T, ncols, nlay, ncol_ld, col_idx, col0, p_lay, p_lev, t_lay, vmr, err, tau, aux, st,
+                                      row_mask, col_mask);
+        else launch_nm<true, false>(T, ncols, nlay, ncol_ld, col_idx, col0, p_lay, p_lev, t_lay, vmr, err, tau, aux, st,
+                                    nullptr, nullptr);
+    } else {
+        launch_nm<false, false>(T, ncols, nlay, ncol_ld, col_idx, col0, p_lay, p_lev, t_lay, vmr, err, tau, aux, st,
+                                nullptr, nullptr);
+    }
 }
 
 }  // namespace ccpprad

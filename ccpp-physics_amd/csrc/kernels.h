@@ -10,24 +10,18 @@ struct VmrPtrs {
     const double *v[MAX_GAS];  // gas g (1-based) at v[g-1]; each (ncol_ld, nlay)
 };
 
-// row_mask / col_mask (optional): only rows with row_mask[d*nlay+l] != 0 of
-// columns with col_mask[d] != 0 are written (the emulator's fallback rows).
-// rec: work buffer of ncols * nlay * gas_record_words(T) words (per-layer
-// interpolation records).
+// k-distribution gas optics of ncols columns (column col0 + d of the CCPP
+// arrays, or col_idx[col0 + d] - 1 with a gather map): tau and aux (LW
+// Planck fraction, SW Rayleigh depth), [d][l][g] FP32.  row_mask / col_mask
+// (both given or both null, LW only): the emulator's fallback, only rows
+// with row_mask[d*nlay+l] != 0 of columns with col_mask[d] != 0 are written.
+// err: bit 0 is set for a layer outside the table's p/T range.
 void launch_gas_optics(const TablesView &T, int ncols, int nlay, int ncol_ld, const int *col_idx, int col0,
-                       const double *p_lay, const double *p_lev, const double *t_lay, const VmrPtrs &vmr,
-                       unsigned *rec, float *tau, float *aux, int *err, hipStream_t st,
-                       const unsigned char *row_mask = nullptr, const unsigned char *col_mask = nullptr);
-int gas_record_words(const TablesView &T);
-// LDS per block of the g-point gather kernel
-size_t gas_gather_smem(int is_lw, int nflav, int nminor0, int nminor1, int ngpt, int neta);
-// the two stages of launch_gas_optics: interpolation records, then g-point gathers
-void launch_gas_interp(const TablesView &T, int ncols, int nlay, int ncol_ld, const int *col_idx, int col0,
-                       const double *p_lay, const double *p_lev, const double *t_lay, const VmrPtrs &vmr,
-                       unsigned *rec, int *err, hipStream_t st, const unsigned char *col_mask = nullptr);
-void launch_gas_gather(const TablesView &T, int ncols, int nlay, const unsigned *rec, float *tau, float *aux,
-                       hipStream_t st, const unsigned char *row_mask = nullptr,
+                       const double *p_lay, const double *p_lev, const double *t_lay, const VmrPtrs &vmr, int *err,
+                       float *tau, float *aux, hipStream_t st, const unsigned char *row_mask = nullptr,
                        const unsigned char *col_mask = nullptr);
+// LDS per block of the gas-optics kernel
+size_t gas_gather_smem(int is_lw, int nflav, int nminor0, int nminor1, int ngpt, int neta);
 
 struct LwSolveArgs {
     int nlay, ncol_ld, col0, top_at_1;

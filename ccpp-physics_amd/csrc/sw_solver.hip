@@ -35,17 +35,17 @@ struct TwoStream {
     float Rd, Td, Rdir, Tdir, T0;   // T0 = exp(-tau/mu0), the direct transmittance
 };
 
-// (e^{-a} - e^{-b}) / d for b - a = d * s, s > 0, without cancellation:
-// e^{-a} s phi(d s) with phi(x) = (1 - e^{-x}) / x from its series where
-// |d s| < 0.25 (truncation below 2e-9 relative), the direct difference
-// (whose terms then differ by a factor e^{0.25} or more) elsewhere.
-__device__ __forceinline__ float exp_diff_ratio(float a, float b, float d, float s)
+// (e^{-a} - e^{-b}) / d for b - a = d * s, s > 0, from ea = e^{-a} and
+// eb = e^{-b}, without cancellation: e^{-a} s phi(d s) with
+// phi(x) = (1 - e^{-x}) / x from its series where |d s| < 0.25 (truncation
+// below 2e-9 relative), the direct difference (whose terms then differ by a
+// factor e^{0.25} or more) elsewhere.
+__device__ __forceinline__ float diff_ratio(float ea, float eb, float d, float s)
 {
     const float x = d * s;
-    const float ea = exp_neg(a);
     const float phi = 1.0f - x * (0.5f - x * (1.0f / 6.0f - x * (1.0f / 24.0f - x * (1.0f / 120.0f -
                       x * (1.0f / 720.0f - x * (1.0f / 5040.0f))))));
-    return fabsf(x) < 0.25f ? ea * s * phi : (ea - exp_neg(b)) * frcp(d);
+    return fabsf(x) < 0.25f ? ea * s * phi : (ea - eb) * frcp(d);
 }
 
 // sw_two_stream for one layer in FP32 (Meador & Weaver 1980, Zdunkowski
@@ -62,6 +62,7 @@ __device__ __forceinline__ float exp_diff_ratio(float a, float b, float d, float
 //    both brackets and of 1 - (k mu0)^2, with
 //      T_R = (e^{-2k tau} - e^{-(k+1/mu0) tau}) / delta,
 //      T_T = (e^{-k tau} - e^{-tau/mu0}) / delta,
+//    both from the exponentials already formed (diff_ratio),
 //    Rdir = w RT / (1+k mu) [a2 (2G - E2 - 2 T_R) + k g3 (E2 + 2 T_R)]
 //    Tdir = -w RT / (1+k mu) [(a1 - k g4) E2 (1 - F) - 2 k (g4 + mu a1) T_T].
 // gamma1 - gamma2 = 2 (1 - w) comes from the co-albedo p.
@@ -94,8 +95,9 @@ __device__ __forceinline__ TwoStream two_stream32(float mu0, float mu0_inv, floa
         Tdir = -RT2 * (2.0f * k * (g4 + mu0 * a1) * (E1 - F) + (1.0f - kmu) * (a1 - k * g4) * E2 * (1.0f - F));
     } else {
         const float delta = 1.0f - kmu;
-        const float TR = exp_diff_ratio(2.0f * k * tau, k * tau + s, delta, s);
-        const float TT = exp_diff_ratio(k * tau, s, delta, s);
+        // e^{-2 k tau} = e^2, e^{-(k tau + s)} = e T0, e^{-k tau} = e, e^{-s} = T0
+        const float TR = diff_ratio(e * e, e * T0, delta, s);
+        const float TT = diff_ratio(e, T0, delta, s);
         const float pre = w * RT * frcp(1.0f + kmu);
         Rdir = pre * (a2 * (2.0f * G - E2 - 2.0f * TR) + k * g3 * (E2 + 2.0f * TR));
         Tdir = -pre * ((a1 - k * g4) * E2 * (1.0f - F) - 2.0f * k * (g4 + mu0 * a1) * TT);

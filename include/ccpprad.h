@@ -431,8 +431,8 @@ int ccpprad_lw_set_gas_optics_mlp(ccpprad_ctx *ctx, ccpprad_mlp *m);
 /* ---- observability (no counterpart in the reference, which has no timers) */
 
 /* Per-kernel device timing with HIP events recorded on the launch stream.
- * Kernel ids: 0 gas optics g-point gathers, 1 LW solver, 2 SW solver,
- * 3 cloud optics, 4 McICA, 5 MLP inference, 6 gas optics interpolation.
+ * Kernel ids: 0 gas optics, 1 LW solver, 2 SW solver, 3 cloud optics,
+ * 4 McICA, 5 MLP inference, 6 gas optics of the emulator's fallback rows.
  * enable != 0 starts recording (and clears). */
 int ccpprad_timing_enable(ccpprad_ctx *ctx, int enable);
 /* Synchronises the recorded events; total milliseconds and launch count. */
