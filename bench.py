@@ -36,14 +36,16 @@ UNIQUE = 32_768        # distinct synthetic columns, tiled to NCOL on the device
 METRIC = "LW+SW columns/s at 127 layers (table vs NN) @ HR RMSE<=0.1 K/day vs FP64"
 # HBM bytes per 65,536-column launch measured with rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE (separate runs of `bench.py --steps 1 --warmup 0 --no-cpu
-# --no-nn --copy-probe`), each scaled by the reading of a 1 GiB device copy
-# (FETCH x 2.0, WRITE x 1.0).  Counters cannot be read from inside this
+# --no-nn --no-e2e --no-callers --copy-probe`), each scaled by the reading of a
+# 1 GiB device copy (FETCH x 2.0, WRITE x 1.0).  Counters cannot be read from inside this
 # process, so the last measurement of the current kernels is recorded here.
-TRAFFIC_GB = {}
+TRAFFIC_GB = {"sw_sw_solve": 19.492, "lw_lw_solve": 22.665, "lw_gas_optics": 9.131, "sw_gas_optics": 8.072,
+              "lw_cloud_optics": 0.775, "sw_cloud_optics": 0.775, "lw_mcica": 0.670, "sw_mcica": 0.670,
+              "lw_solve_2str": 24.599}
 # sha256 (first 16 hex digits) of the kernel sources the figures above were
 # measured with (kernel_source_sha()); on any other sources the bench line
 # reports traffic null rather than a stale figure.
-TRAFFIC_SRC_SHA = "unmeasured"
+TRAFFIC_SRC_SHA = "c7e8c272ac7999a0"
 
 
 def kernel_source_sha():
