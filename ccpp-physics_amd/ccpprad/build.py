@@ -16,6 +16,7 @@ HIP_LIB = os.path.join(LIBDIR, "libccpprad_hip.so")
 F_LIB = os.path.join(LIBDIR, "libccpprad_f.so")
 F_DRIVER = os.path.join(LIBDIR, "ccpp_cap_driver")
 HR_DRIVER = os.path.join(LIBDIR, "hr_driver")
+CALLER_DRIVER = os.path.join(LIBDIR, "caller_driver")
 
 HIP_SOURCES = ["capi.hip", "host.hip", "gas_optics.hip", "lw_solver.hip", "lw_scat.hip", "sw_solver.hip",
                "clouds.hip", "util.hip", "mlp.hip", "prep.hip", "nc_tables.hip"]
@@ -52,7 +53,8 @@ def build_fortran(verbose=False):
     os.makedirs(LIBDIR, exist_ok=True)
     bdir = os.path.join(LIBDIR, "fmod")
     os.makedirs(bdir, exist_ok=True)
-    order = ["mpi_f08_stub.F90", "ccpprad_c.F90", "mo_heating_rates.F90", "rrtmgp_lw_main.F90", "rrtmgp_sw_main.F90"]
+    order = ["mpi_f08_stub.F90", "radiation_flux_types.F90", "ccpprad_c.F90", "mo_heating_rates.F90",
+             "rrtmgp_lw_main.F90", "rrtmgp_sw_main.F90", "GFS_rrtmgp_cloud_overlap.F90", "GFS_rrtmgp_post.F90"]
     objs = []
     for f in order:
         src = os.path.join(FSRC, f)
@@ -63,7 +65,8 @@ def build_fortran(verbose=False):
         objs.append(obj)
     _run([FLANG, "-shared", "-o", F_LIB] + objs + ["-L" + LIBDIR, "-lccpprad_hip",
                                                    "-Wl,-rpath,$ORIGIN"])
-    for name, exe in (("ccpp_cap_driver.F90", F_DRIVER), ("hr_driver.F90", HR_DRIVER)):
+    for name, exe in (("ccpp_cap_driver.F90", F_DRIVER), ("hr_driver.F90", HR_DRIVER),
+                      ("caller_driver.F90", CALLER_DRIVER)):
         drv = os.path.join(FSRC, name)
         if os.path.exists(drv):
             _run([FLANG, "-O2", "-I", bdir, "-module-dir", bdir, drv, "-o", exe, "-L" + LIBDIR,

@@ -394,6 +394,185 @@ extern "C" int ccpprad_sw_init_files_pade(const char *gas_file, const char *clou
 // the fluxes and pressures are copied to the device, the FP64 kernel runs
 // there and the heating rate is copied back.  Synchronous; the device
 // staging is allocated per call.
+namespace ccpprad {
+void launch_heating_rate(int ncol, int nlay, const double *up, const double *dn, const double *p_lev,
+                         double *htr, hipStream_t st);
+void launch_mark_days(int nday, const int *idx, unsigned char *day, hipStream_t st);           // lw_solver.hip
+void launch_heating_rate_day(int ncol, int nlay, const unsigned char *day, const double *up, const double *dn,
+                             const double *p_lev, double *htr, hipStream_t st);
+}
+
+namespace {
+
+// Staging of a context-free host entry: one device block and one private
+// stream per call.  Each Stage names a pointer field of the caller's
+// argument copy (a host pointer on entry, replaced by its device copy);
+// IN arrays are copied in, OUT arrays back, SCRATCH ones are device-only.
+enum { ST_IN = 1, ST_OUT = 2, ST_SCRATCH = 4 };
+struct Stage {
+    void **slot;
+    size_t bytes;
+    int mode;
+};
+
+template <typename Dev>
+int staged_call(const char *who, std::vector<Stage> &S, Dev &&dev, char *errmsg, int errlen)
+{
+    std::vector<void *> host(S.size());
+    std::vector<size_t> off(S.size());
+    size_t total = 0;
+    for (size_t i = 0; i < S.size(); ++i) {
+        host[i] = *S[i].slot;
+        off[i] = total;
+        if (host[i] || (S[i].mode & ST_SCRATCH)) total += (S[i].bytes + 255) & ~(size_t)255;
+    }
+    int rc = 0;
+    auto fail = [&](const char *what, hipError_t e) {
+        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "%s: %s: %s", who, what, hipGetErrorString(e));
+        rc = 2;
+    };
+    hipStream_t st = nullptr;
+    char *d = nullptr;
+    hipError_t e = hipStreamCreate(&st);
+    if (e != hipSuccess) { fail("stream creation", e); return rc; }
+    if (total && (e = hipMalloc(&d, total)) != hipSuccess) {
+        fail("device allocation", e);
+        (void)hipStreamDestroy(st);
+        return rc;
+    }
+    for (size_t i = 0; i < S.size() && !rc; ++i) {
+        if (!host[i] && !(S[i].mode & ST_SCRATCH)) continue;
+        *S[i].slot = d + off[i];
+        if ((S[i].mode & ST_IN) && S[i].bytes &&
+            (e = hipMemcpyAsync(d + off[i], host[i], S[i].bytes, hipMemcpyHostToDevice, st)) != hipSuccess)
+            fail("host to device copy", e);
+    }
+    if (!rc) rc = dev(st);
+    for (size_t i = 0; i < S.size() && !rc; ++i) {
+        if (!host[i] || !(S[i].mode & ST_OUT) || !S[i].bytes) continue;
+        if ((e = hipMemcpyAsync(host[i], d + off[i], S[i].bytes, hipMemcpyDeviceToHost, st)) != hipSuccess)
+            fail("device to host copy", e);
+    }
+    if (!rc && (e = hipStreamSynchronize(st)) != hipSuccess) fail("stream synchronisation", e);
+    (void)hipStreamSynchronize(st);
+    if (d) (void)hipFree(d);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+template <typename T>
+Stage stage(T *&p, size_t n, int mode) { return Stage{(void **)&p, n * sizeof(T), mode}; }
+
+}  // namespace
+
+extern "C" int ccpprad_cloud_overlap(const ccpprad_cloud_overlap_args *a, char *errmsg, int errlen)
+{
+    if (errmsg && errlen > 0) errmsg[0] = 0;
+    if (!a) {
+        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "cloud_overlap: null arguments");
+        return 1;
+    }
+    ccpprad_cloud_overlap_args d = *a;
+    if (d.nCol < 0 || d.nLev < 1 || d.nCol == 0 || !(d.doLWrad || d.doSWrad))   // checked or a no-op there
+        return ccpprad_cloud_overlap_dev(&d, nullptr, errmsg, errlen);
+    const size_t nc = (size_t)d.nCol, nl = (size_t)d.nLev;
+    std::vector<Stage> S = {
+        stage(d.lat, nc, ST_IN), stage(d.deltaZc, nc * nl, ST_IN), stage(d.cld_frac, nc * nl, ST_IN),
+        stage(d.cld_cnv_frac, nc * nl, ST_IN), stage(d.de_lgth, nc, ST_IN | ST_OUT),
+        stage(d.cloud_overlap_param, nc * nl, ST_IN | ST_OUT),
+        stage(d.cnv_cloud_overlap_param, nc * nl, ST_IN | ST_OUT),
+        stage(d.precip_overlap_param, nc * nl, ST_IN | ST_OUT)};
+    return staged_call("cloud_overlap", S,
+                       [&](hipStream_t st) { return ccpprad_cloud_overlap_dev(&d, (void *)st, errmsg, errlen); },
+                       errmsg, errlen);
+}
+
+extern "C" int ccpprad_post(const ccpprad_post_args *a, const ccpprad_post_hr_args *h, char *errmsg, int errlen)
+{
+    if (errmsg && errlen > 0) errmsg[0] = 0;
+    auto err = [&](const char *m) {
+        if (errmsg && errlen > 0) snprintf(errmsg, errlen, "post: %s", m);
+        return 1;
+    };
+    if (!a || !h) return err("null arguments");
+    if (a->nCol < 0 || a->nLev < 1) return err("bad dimensions");
+    if (a->nCol == 0 || !(a->doLWrad || a->doSWrad)) return 0;
+    ccpprad_post_args d = *a;
+    ccpprad_post_hr_args r = *h;
+    const int nc = a->nCol, nday = a->doSWrad ? a->nDay : 0;
+    if (!a->doLWrad) r.htrlw = r.htrlwc = nullptr;
+    if (!a->doSWrad) r.htrsw = r.htrswc = nullptr;
+    if (!r.do_lw_clrsky_hr) r.htrlwc = nullptr;
+    if (!r.do_sw_clrsky_hr) r.htrswc = nullptr;
+    if (a->doLWrad && !r.htrlw) return err("LW: htrlw is NULL");
+    if (a->doLWrad && r.do_lw_clrsky_hr && !r.htrlwc) return err("do_lw_clrsky_hr: htrlwc is NULL");
+    if (a->doSWrad && !r.htrsw) return err("SW: htrsw is NULL");
+    if (a->doSWrad && r.do_sw_clrsky_hr && !r.htrswc) return err("do_sw_clrsky_hr: htrswc is NULL");
+    if (a->doSWrad && (a->nDay < 0 || a->nDay > nc)) return err("nDay outside 0..nCol");
+    if (nday > 0) {
+        if (!r.idxday) return err("idxday is NULL");
+        std::vector<unsigned char> seen((size_t)nc, 0);
+        for (int i = 0; i < nday; ++i) {
+            const int c = r.idxday[i];
+            if (c < 1 || c > nc || seen[(size_t)c - 1]) return err("idxday(1:nDay) must be distinct columns in 1..nCol");
+            seen[(size_t)c - 1] = 1;
+        }
+    } else {
+        r.idxday = nullptr;
+    }
+    const size_t n = (size_t)nc, nl = (size_t)a->nLev, nv = n * (nl + 1);
+    unsigned char *day = nullptr;
+    std::vector<Stage> S = {
+        stage(d.fluxlwUP_allsky, nv, ST_IN), stage(d.fluxlwDOWN_allsky, nv, ST_IN),
+        stage(d.fluxlwUP_clrsky, nv, ST_IN), stage(d.fluxlwDOWN_clrsky, nv, ST_IN),
+        stage(d.fluxswUP_allsky, nv, ST_IN), stage(d.fluxswDOWN_allsky, nv, ST_IN),
+        stage(d.fluxswUP_clrsky, nv, ST_IN), stage(d.fluxswDOWN_clrsky, nv, ST_IN),
+        stage(d.scmpsw, 6 * n, ST_IN), stage(d.sfc_alb_nir_dir, n, ST_IN), stage(d.sfc_alb_nir_dif, n, ST_IN),
+        stage(d.sfc_alb_uvvis_dir, n, ST_IN), stage(d.sfc_alb_uvvis_dif, n, ST_IN), stage(d.tsfa, n, ST_IN),
+        stage(d.coszen, n, ST_IN), stage(d.coszdg, n, ST_IN), stage(d.p_lev, nv, ST_IN),
+        stage(d.cldsa, 5 * n, ST_IN), stage(d.cldtaulw, n * nl, ST_IN), stage(d.cldtausw, n * nl, ST_IN),
+        stage(d.aerodp, n * (size_t)(a->naerodp > 0 ? a->naerodp : 0), ST_IN), stage(d.mtopa, 3 * n, ST_IN),
+        stage(d.mbota, 3 * n, ST_IN), stage(d.topflw, 2 * n, ST_IN | ST_OUT), stage(d.sfcflw, 4 * n, ST_IN | ST_OUT),
+        stage(d.sfcdlw, n, ST_IN | ST_OUT), stage(d.sfculw, n, ST_IN | ST_OUT), stage(d.tsflw, n, ST_IN | ST_OUT),
+        stage(d.htrlwu, n * nl, ST_IN | ST_OUT), stage(d.topfsw, 3 * n, ST_IN | ST_OUT),
+        stage(d.sfcfsw, 4 * n, ST_IN | ST_OUT), stage(d.nirbmdi, n, ST_IN | ST_OUT),
+        stage(d.nirdfdi, n, ST_IN | ST_OUT), stage(d.visbmdi, n, ST_IN | ST_OUT), stage(d.visdfdi, n, ST_IN | ST_OUT),
+        stage(d.nirbmui, n, ST_IN | ST_OUT), stage(d.nirdfui, n, ST_IN | ST_OUT), stage(d.visbmui, n, ST_IN | ST_OUT),
+        stage(d.visdfui, n, ST_IN | ST_OUT), stage(d.sfcnsw, n, ST_IN | ST_OUT), stage(d.sfcdsw, n, ST_IN | ST_OUT),
+        stage(d.fluxr, n * (size_t)(a->nfxr > 0 ? a->nfxr : 0), ST_IN | ST_OUT),
+        stage(r.idxday, (size_t)(nday > 0 ? nday : 0), ST_IN), stage(r.htrlw, n * nl, ST_OUT),
+        stage(r.htrlwc, n * nl, ST_OUT), stage(r.htrsw, n * nl, ST_OUT), stage(r.htrswc, n * nl, ST_OUT),
+        stage(day, a->doSWrad ? n : 0, ST_SCRATCH)};
+    d.htrlw = nullptr;   // the LW heating rate this call computes (r.htrlw) feeds htrlwu
+    auto dev = [&](hipStream_t st) -> int {
+        const int nlay = a->nLev;
+        if (a->doLWrad) {
+            if (!d.fluxlwUP_allsky || !d.fluxlwDOWN_allsky || !d.p_lev ||
+                (r.htrlwc && (!d.fluxlwUP_clrsky || !d.fluxlwDOWN_clrsky)))
+                return err("LW heating rate: NULL flux or p_lev");
+            if (r.htrlwc) ccpprad::launch_heating_rate(nc, nlay, d.fluxlwUP_clrsky, d.fluxlwDOWN_clrsky, d.p_lev,
+                                                       r.htrlwc, st);
+            ccpprad::launch_heating_rate(nc, nlay, d.fluxlwUP_allsky, d.fluxlwDOWN_allsky, d.p_lev, r.htrlw, st);
+            d.htrlw = r.htrlw;
+        }
+        if (a->doSWrad) {
+            if (!d.fluxswUP_allsky || !d.fluxswDOWN_allsky || !d.p_lev ||
+                (r.htrswc && (!d.fluxswUP_clrsky || !d.fluxswDOWN_clrsky)))
+                return err("SW heating rate: NULL flux or p_lev");
+            hipError_t e = hipMemsetAsync(day, 0, n, st);
+            if (e != hipSuccess) return err("flag clear failed");
+            ccpprad::launch_mark_days(nday, r.idxday, day, st);
+            if (r.htrswc) ccpprad::launch_heating_rate_day(nc, nlay, day, d.fluxswUP_clrsky, d.fluxswDOWN_clrsky,
+                                                           d.p_lev, r.htrswc, st);
+            ccpprad::launch_heating_rate_day(nc, nlay, day, d.fluxswUP_allsky, d.fluxswDOWN_allsky, d.p_lev,
+                                             r.htrsw, st);
+        }
+        if (hipGetLastError() != hipSuccess) return err("heating-rate launch failed");
+        return ccpprad_post_dev(&d, (void *)st, errmsg, errlen);
+    };
+    return staged_call("post", S, dev, errmsg, errlen);
+}
+
 extern "C" int ccpprad_heating_rate(int nCol, int nLay, const double *flux_up, const double *flux_dn,
                                     const double *p_lev, double *htr, char *errmsg, int errlen)
 {

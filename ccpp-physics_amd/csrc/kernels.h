@@ -43,6 +43,10 @@ void launch_lw_solve_2str(const TablesView &T, const LwSolveArgs &A, int ncols, 
 size_t lw_solve_2str_smem(int nlay, int nband);
 void launch_heating_rate(int ncol, int nlay, const double *up, const double *dn, const double *p_lev,
                          double *htr, hipStream_t st);
+// SW heating rate over the daylit columns, 0 elsewhere (day: nCol flags)
+void launch_mark_days(int nday, const int *idx, unsigned char *day, hipStream_t st);
+void launch_heating_rate_day(int ncol, int nlay, const unsigned char *day, const double *up, const double *dn,
+                             const double *p_lev, double *htr, hipStream_t st);
 
 // GFS_rrtmgp_pre's preparation of the radiation inputs (prep.hip)
 struct PrepArgs {

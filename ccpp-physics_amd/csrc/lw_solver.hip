@@ -313,4 +313,39 @@ void launch_heating_rate(int ncol, int nlay, const double *up, const double *dn,
                        up, dn, p_lev, htr);
 }
 
+// The SW heating rate of GFS_rrtmgp_post_run (GFS_rrtmgp_post.F90:260-266):
+// zero, then the daylit rows.  Each row's value is the same expression as
+// heating_rate_kernel's, so it equals a call on the gathered daylit rows.
+__global__ void mark_days_kernel(int nday, const int *idx, unsigned char *day)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nday) day[idx[i] - 1] = 1;
+}
+
+__global__ void heating_rate_day_kernel(int ncol, int nlay, const unsigned char *day, const double *up,
+                                        const double *dn, const double *p_lev, double *htr)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)ncol * nlay) return;
+    const size_t c = i % ncol, l = i / ncol;
+    const size_t q0 = c + (size_t)ncol * l, q1 = q0 + ncol;
+    htr[q0] = day[c] ? (up[q1] - up[q0] - dn[q1] + dn[q0]) * GRAV / (CP_DRY * (p_lev[q1] - p_lev[q0])) : 0.0;
+}
+
+// day: nCol flags the caller has zeroed; idx: nday 1-based columns in 1..ncol.
+void launch_mark_days(int nday, const int *idx, unsigned char *day, hipStream_t st)
+{
+    if (nday <= 0) return;
+    hipLaunchKernelGGL(mark_days_kernel, dim3((unsigned)((nday + 255) / 256)), dim3(256), 0, st, nday, idx, day);
+}
+
+void launch_heating_rate_day(int ncol, int nlay, const unsigned char *day, const double *up, const double *dn,
+                             const double *p_lev, double *htr, hipStream_t st)
+{
+    const size_t n = (size_t)ncol * nlay;
+    if (!n) return;
+    hipLaunchKernelGGL(heating_rate_day_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ncol, nlay,
+                       day, up, dn, p_lev, htr);
+}
+
 }  // namespace ccpprad

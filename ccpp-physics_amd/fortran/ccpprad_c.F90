@@ -41,7 +41,53 @@ module ccpprad_c
      type(c_ptr) :: cldtausw, htrsw, htrswc
   end type ccpprad_sw_args_t
 
+  type, bind(C) :: ccpprad_post_args_t
+     integer(c_int) :: nCol, nLev, nDay, iSFC, iTOA
+     integer(c_int) :: doLWrad, doSWrad, save_diag, naerodp, nfxr
+     real(c_double) :: fhlwr, fhswr, raddt
+     type(c_ptr) :: fluxlwUP_allsky, fluxlwDOWN_allsky, fluxlwUP_clrsky, fluxlwDOWN_clrsky
+     type(c_ptr) :: fluxswUP_allsky, fluxswDOWN_allsky, fluxswUP_clrsky, fluxswDOWN_clrsky
+     type(c_ptr) :: scmpsw
+     type(c_ptr) :: sfc_alb_nir_dir, sfc_alb_nir_dif, sfc_alb_uvvis_dir, sfc_alb_uvvis_dif
+     type(c_ptr) :: tsfa, coszen, coszdg, htrlw, p_lev
+     type(c_ptr) :: cldsa, cldtaulw, cldtausw, aerodp
+     type(c_ptr) :: mtopa, mbota
+     type(c_ptr) :: topflw, sfcflw, sfcdlw, sfculw, tsflw, htrlwu
+     type(c_ptr) :: topfsw, sfcfsw
+     type(c_ptr) :: nirbmdi, nirdfdi, visbmdi, visdfdi, nirbmui, nirdfui, visbmui, visdfui
+     type(c_ptr) :: sfcnsw, sfcdsw, fluxr
+  end type ccpprad_post_args_t
+
+  type, bind(C) :: ccpprad_post_hr_args_t
+     integer(c_int) :: do_lw_clrsky_hr, do_sw_clrsky_hr
+     type(c_ptr) :: idxday
+     type(c_ptr) :: htrlw, htrlwc, htrsw, htrswc
+  end type ccpprad_post_hr_args_t
+
+  type, bind(C) :: ccpprad_cloud_overlap_args_t
+     integer(c_int) :: nCol, nLev, yearlen, doSWrad, doLWrad, literal_r8
+     integer(c_int) :: idcor, idcor_con, idcor_hogan, idcor_oreopoulos
+     integer(c_int) :: iovr, iovr_convcld, iovr_dcorr, iovr_exp, iovr_exprand
+     integer(c_int) :: imfdeepcnv, imfdeepcnv_gf, imfdeepcnv_samf
+     real(c_double) :: julian, con_pi, dcorr_con
+     type(c_ptr) :: lat, deltaZc, cld_frac, cld_cnv_frac
+     type(c_ptr) :: de_lgth, cloud_overlap_param, cnv_cloud_overlap_param, precip_overlap_param
+  end type ccpprad_cloud_overlap_args_t
+
   interface
+     integer(c_int) function ccpprad_post(args, hr, errmsg, errlen) bind(C, name="ccpprad_post")
+       import :: c_int, c_char, ccpprad_post_args_t, ccpprad_post_hr_args_t
+       type(ccpprad_post_args_t), intent(in) :: args
+       type(ccpprad_post_hr_args_t), intent(in) :: hr
+       character(kind=c_char), dimension(*) :: errmsg
+       integer(c_int), value :: errlen
+     end function ccpprad_post
+     integer(c_int) function ccpprad_cloud_overlap(args, errmsg, errlen) bind(C, name="ccpprad_cloud_overlap")
+       import :: c_int, c_char, ccpprad_cloud_overlap_args_t
+       type(ccpprad_cloud_overlap_args_t), intent(in) :: args
+       character(kind=c_char), dimension(*) :: errmsg
+       integer(c_int), value :: errlen
+     end function ccpprad_cloud_overlap
      integer(c_int) function ccpprad_lw_init_files(gas_file, cloud_file, nrghice, gas_names, ngases, nLay, &
           blksz, ctx, errmsg, errlen) bind(C, name="ccpprad_lw_init_files")
        import :: c_int, c_ptr, c_char

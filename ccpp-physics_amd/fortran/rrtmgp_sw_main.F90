@@ -1,15 +1,9 @@
 ! CCPP scheme rrtmgp_sw_main on the MI355X: same module name and
 ! rrtmgp_sw_main_init / rrtmgp_sw_main_run argument lists as
 ! physics/Radiation/RRTMGP/rrtmgp_sw_main.F90:34-65 and :87-189.  scmpsw is
-! cmpfsw_type(nCol) from Radiation/RRTMG/radsw_param.f:111-118, declared
-! here with the same six components.  Errors set errflg = 1.
-module module_radsw_parameters
-  use iso_c_binding, only: c_double
-  implicit none
-  type, bind(C) :: cmpfsw_type
-     real(c_double) :: uvbfc, uvbf0, nirbm, nirdf, visbm, visdf
-  end type cmpfsw_type
-end module module_radsw_parameters
+! cmpfsw_type(nCol) of module_radsw_parameters (Radiation/RRTMG/
+! radsw_param.f:111-118; radiation_flux_types.F90 for builds without the
+! host's).  Errors set errflg = 1.
 
 module rrtmgp_sw_main
   use mpi_f08,   only: MPI_Comm

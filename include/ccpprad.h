@@ -8,13 +8,14 @@
  *        (physics/Radiation/RRTMGP/rrtmgp_sw_main.F90:34-79, :87-683)
  *   ccpprad_heating_rate               <- compute_heating_rate as called by
  *        GFS_rrtmgp_post_run (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_post.F90:159-171,251-266)
- *   ccpprad_post_dev                   <- GFS_rrtmgp_post_run's flux bookkeeping
- *        (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_post.F90:173-389)
+ *   ccpprad_post / ccpprad_post_dev    <- GFS_rrtmgp_post_run
+ *        (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_post.F90:35-389): heating
+ *        rates and flux bookkeeping (host pointers) / the bookkeeping (device)
  *   ccpprad_prepare_inputs_dev         <- the radiation-input preparation of
  *        GFS_rrtmgp_pre_run (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_pre.F90:217-274,349-371)
  *   ccpprad_layer_geometry_dev         <- the virtual temperature and layer thicknesses of
  *        GFS_rrtmgp_pre_run (GFS_rrtmgp_pre.F90:289,294-341)
- *   ccpprad_cloud_overlap_dev          <- GFS_rrtmgp_cloud_overlap_run
+ *   ccpprad_cloud_overlap(_dev)        <- GFS_rrtmgp_cloud_overlap_run
  *        (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_cloud_overlap.F90:31-138)
  *   ccpprad_*_gas_optics_dev           <- ty_gas_optics_rrtmgp%gas_optics
  *        (call sites rrtmgp_lw_main.F90:337-345, rrtmgp_sw_main.F90:346-353)
@@ -300,6 +301,25 @@ typedef struct ccpprad_post_args {
 /* GFS_rrtmgp_post_run's flux bookkeeping on the device (see the struct). */
 int ccpprad_post_dev(const ccpprad_post_args *args, void *stream, char *errmsg, int errlen);
 
+/* The heating-rate half of GFS_rrtmgp_post_run (GFS_rrtmgp_post.F90:152-171,
+ * 242-266, 298-316) for ccpprad_post: LW all-sky always and clear-sky when
+ * do_lw_clrsky_hr; SW zeroed, then the daylit rows idxday(1:nDay) (1-based,
+ * distinct, in 1..nCol); htrswc likewise when do_sw_clrsky_hr.  (nCol, nLev)
+ * FP64 in K/s, written only where the reference writes them. */
+typedef struct ccpprad_post_hr_args {
+    int do_lw_clrsky_hr, do_sw_clrsky_hr;
+    const int *idxday;
+    double *htrlw, *htrlwc, *htrsw, *htrswc;
+} ccpprad_post_hr_args;
+
+/* GFS_rrtmgp_post_run on HOST pointers: the heating rates (FP64 on the GPU,
+ * from the level fluxes of args) and then the bookkeeping of
+ * ccpprad_post_dev, whose htrlw input is the freshly computed LW heating
+ * rate (args->htrlw is ignored).  Every output of args is intent(inout):
+ * it is copied in and back, so what the reference leaves alone keeps its
+ * value.  A private stream; returns after the copies back. */
+int ccpprad_post(const ccpprad_post_args *args, const ccpprad_post_hr_args *hr, char *errmsg, int errlen);
+
 /* The radiation-input preparation of GFS_rrtmgp_pre_run
  * (physics/Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_pre.F90:217-274,349-371),
  * FP64 on the device: q_lay = max(qgrs_h2o, 1e-6); p_lev = prsi; p_lay, t_lay
@@ -367,6 +387,11 @@ typedef struct ccpprad_cloud_overlap_args {
 
 int ccpprad_cloud_overlap_dev(const ccpprad_cloud_overlap_args *args, void *stream, char *errmsg,
                               int errlen);
+/* The same on HOST pointers (GFS_rrtmgp_cloud_overlap_run's own argument
+ * kinds): the outputs are copied in and back, so the ones the reference
+ * leaves unwritten keep their values.  A private stream; returns after the
+ * copies back. */
+int ccpprad_cloud_overlap(const ccpprad_cloud_overlap_args *args, char *errmsg, int errlen);
 
 /* ty_gas_optics_rrtmgp%gas_optics, LW (rrtmgp_lw_main.F90:337-345).  Device
  * pointers.  vmr is (nCol, nLay, ngas) in the k-distribution's gas order.

@@ -37,7 +37,7 @@ def test_fortran_modules_built_with_reference_names():
         pytest.skip("flang not available")
     out = subprocess.run(["nm", "-D", "--defined-only", build.F_LIB], capture_output=True, text=True).stdout
     for sym in ("rrtmgp_lw_main_init", "rrtmgp_lw_main_run", "rrtmgp_sw_main_init", "rrtmgp_sw_main_run",
-                "compute_heating_rate"):
+                "compute_heating_rate", "gfs_rrtmgp_post_run", "gfs_rrtmgp_cloud_overlap_run"):
         assert sym in out, sym
     # the device release is no CCPP phase name (the unchanged .meta has no table for it)
     assert "rrtmgp_lw_main_finalize" not in out and "rrtmgp_lw_main_release" in out
