@@ -19,7 +19,11 @@
 // A-fragments (transposed: W^T is the A operand, activations the B operand),
 // so the D registers of one layer are, after bias + softplus + BF16 rounding,
 // the B registers of the next: the next layer's K order is permuted to
-// match, by storing its weights accordingly.  One wavefront = 32 rows.
+// match, by storing its weights accordingly.  One wavefront carries one
+// 32-row tile at a time; the hidden layers work in base 2 (softplus / ln2,
+// one exp2 and one log2 per activation), the output layer's products for
+// one pair of feature tiles overlap the previous pair's transform and
+// stores, and the next tile's inputs are loaded during the current one.
 #include "kernels.h"
 
 #include <type_traits>
@@ -30,7 +34,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int MLP_IN = MLP_NIN, MLP_H = 128, MLP_OUT = 256, MLP_G = 128;
-constexpr int MLP_K1 = 14;   // layer-1 K: hi and lo BF16 parts of the 7 inputs
+constexpr int MLP_K1 = 14;
+constexpr double LOG2E_D = 1.4426950408889634, LN2_D = 0.6931471805599453;   // layer-1 K: hi and lo BF16 parts of the 7 inputs
 constexpr int MLP_NPARAM = 7 * 128 + 128 + 2 * (128 * 128 + 128) + 128 * 256 + 256;  // 67072
 
 __device__ __forceinline__ float softplus_f(float z)
@@ -39,6 +44,14 @@ __device__ __forceinline__ float softplus_f(float z)
     const float a = fabsf(z);
     const float e = __builtin_amdgcn_exp2f(-a * 1.4426950408889634f);
     return fmaxf(z, 0.0f) + __builtin_amdgcn_logf(1.0f + e) * 0.6931471805599453f;
+}
+
+// log2(1 + 2^z) = max(z, 0) + log2(1 + 2^-|z|), the hardware exp2/log2: the
+// softplus of z ln2, divided by ln2
+__device__ __forceinline__ float softplus2_f(float z)
+{
+    const float e = __builtin_amdgcn_exp2f(-fabsf(z));
+    return fmaxf(z, 0.0f) + __builtin_amdgcn_logf(1.0f + e);
 }
 
 __device__ __forceinline__ float sigmoid_f(float z) { return 1.0f / (1.0f + __expf(-z)); }
@@ -232,6 +245,42 @@ constexpr int NFRAG = 4 * 1 + 4 * 8 + 4 * 8 + 8 * 8;              // 132
 constexpr int MLP_LDS_BYTES = NFRAG * FRAG_BYTES + (3 * 128 + 256 + 2 * 256) * 4;
 
 
+// The raw inputs of one tile row (the 9 FP64 loads a lane makes for its row),
+// fetched one tile ahead so their latency hides behind the current tile.
+struct MlpRow {
+    double t, p, v[5], pl0, pl1;
+    long orow;      // output row, [col][lay] order
+    int dcol;
+    bool valid;
+};
+
+__device__ __forceinline__ void mlp_load_row(const MlpInferArgs &A, long nrows, long tile, int rl, MlpRow &R)
+{
+    const long row = tile * 32 + rl;
+    R.valid = row < nrows;
+    const long rr = R.valid ? row : nrows - 1;
+    // rows are ordered column-fastest within a layer, so the 32 rows of a tile
+    // read consecutive columns of the (nCol, nLay) inputs
+    const int d = (int)(rr % A.ncols), lay = (int)(rr / A.ncols);
+    const long q = (long)(A.col0 + d) + (long)A.ncol_ld * lay;
+    R.orow = (long)d * A.nlay + lay;
+    R.dcol = d;
+    R.t = A.t_lay[q];
+    R.p = A.p_lay[q];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) R.v[i] = A.vmr[i][q];
+    R.pl0 = A.p_lev[q];
+    R.pl1 = A.p_lev[q + A.ncol_ld];
+}
+
+// The hidden layers run in the base-2 domain: with z' = z log2(e) the
+// softplus is ln2 * softplus2(z'), so the standardised features and the
+// biases of layers 1-3 carry a factor log2(e), each hidden activation is
+// softplus2(z') (the next layer's weights then see activations 1/ln2 times
+// the natural ones, which the factor log2(e) of its bias and pre-activation
+// absorbs exactly), and the output scale ystd carries the factor ln2.
+// One wavefront carries one 32-row tile at a time; 8 wavefronts per block
+// share the LDS weight image.
 __global__ __launch_bounds__(512) void mlp_infer_kernel(MlpInferArgs A)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
@@ -248,49 +297,39 @@ __global__ __launch_bounds__(512) void mlp_infer_kernel(MlpInferArgs A)
     const bf16x8 *F3 = F2 + 32 * 64;
     const bf16x8 *F4 = F3 + 32 * 64;
     const float *bias = (const float *)(lds + NFRAG * FRAG_BYTES);
+    const float *ysd = bias + 640, *yc = bias + 896;
     const int half = l >> 5, rl = l & 31;
     const long nrows = (long)A.ncols * A.nlay;
     const long ntiles = (nrows + 31) / 32;
     const long stride = (long)gridDim.x * 8;
-    // Each wavefront carries two 32-row tiles through the network together:
-    // the two tiles' MFMA chains and softplus work are independent, so one
-    // tile's arithmetic runs while the other's matrix products are in flight.
-    for (long tile0 = (long)blockIdx.x * 8 + w; tile0 < ntiles; tile0 += 2 * stride) {
-        long orow[2];
-        int dcol[2];
-        bool valid[2], out[2];
-        float col_dry[2];
-        bf16x8 h[2][8];   // B operands of the next layer (8 K-chunks of 16), per tile
+    long tile = (long)blockIdx.x * 8 + w;
+    MlpRow cur;
+    if (tile < ntiles) mlp_load_row(A, nrows, tile, rl, cur);
+    for (; tile < ntiles; tile += stride) {
+        // features (standardised, split into BF16 hi + lo) and dry-air column
+        float f[MLP_NIN];
+        f[0] = (float)cur.t;
+        f[1] = __logf((float)cur.p);
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const long row = (tile0 + s * stride) * 32 + rl;
-            valid[s] = row < nrows;
-            const long rr = valid[s] ? row : nrows - 1;
-            // rows are ordered column-fastest within a layer, so the 32 rows of
-            // a tile read consecutive columns of the (nCol, nLay) inputs
-            const int d = (int)(rr % A.ncols), lay = (int)(rr / A.ncols);
-            const long q = (long)(A.col0 + d) + (long)A.ncol_ld * lay;
-            orow[s] = (long)d * A.nlay + lay;   // output row, [col][lay] order
-            dcol[s] = d;
-            // features (standardised, split into BF16 hi + lo) and dry-air column
-            const double vh = A.vmr[0][q];
-            float f[MLP_NIN];
-            f[0] = (float)A.t_lay[q];
-            f[1] = __logf((float)A.p_lay[q]);
-            f[2] = __logf((float)vh);
+        for (int i = 0; i < 5; ++i) f[2 + i] = __logf((float)cur.v[i]);
+        bool oob = false;
 #pragma unroll
-            for (int i = 1; i < 5; ++i) f[2 + i] = __logf((float)A.vmr[i][q]);
-            bool o = false;
-#pragma unroll
-            for (int i = 0; i < MLP_NIN; ++i) {
-                o |= !(f[i] >= A.lo[i] && f[i] <= A.hi[i]);
-                f[i] = (f[i] - A.xmean[i]) * A.xistd[i];
-            }
-            out[s] = o;
-            const double dp = fabs(A.p_lev[q] - A.p_lev[q + A.ncol_ld]);
-            const double fact = 1.0 / (1.0 + vh);
-            col_dry[s] = (float)(10.0 * dp * AVOGAD / (1000.0 * ((M_DRY + M_H2O * vh) * fact) * 100.0 * GRAV) * fact);
-            // layer 1 B operand: slot k = hi_k (k < 7), lo_{k-7} (7 <= k < 14), 0
+        for (int i = 0; i < MLP_NIN; ++i) {
+            oob |= !(f[i] >= A.lo[i] && f[i] <= A.hi[i]);
+            f[i] = (f[i] - A.xmean[i]) * A.xistd[i];
+        }
+        const double vh = cur.v[0], fact = 1.0 / (1.0 + vh);
+        const float col_dry = (float)(10.0 * fabs(cur.pl0 - cur.pl1) * AVOGAD /
+                                      (1000.0 * ((M_DRY + M_H2O * vh) * fact) * 100.0 * GRAV) * fact);
+        const long orow = cur.orow;
+        const int dcol = cur.dcol;
+        const bool valid = cur.valid;
+        // the next tile's inputs, in flight during this tile's products
+        if (tile + stride < ntiles) mlp_load_row(A, nrows, tile + stride, rl, cur);
+
+        // layer 1: B operand slot k = hi_k (k < 7), lo_{k-7} (7 <= k < 14), 0
+        bf16x8 h[8];   // B operands of the next layer (8 K-chunks of 16)
+        {
             bf16x8 b1;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
@@ -312,100 +351,114 @@ __global__ __launch_bounds__(512) void mlp_infer_kernel(MlpInferArgs A)
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     const int n0 = (2 * np) * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
-                    h[s][4 * np + (r >> 3)][r & 7] = f2bf(softplus_f(a0[r] + bias[n0]));
-                    h[s][4 * np + 2 + (r >> 3)][r & 7] = f2bf(softplus_f(a1[r] + bias[n0 + 32]));
+                    h[4 * np + (r >> 3)][r & 7] = f2bf(softplus2_f(a0[r] + bias[n0]));
+                    h[4 * np + 2 + (r >> 3)][r & 7] = f2bf(softplus2_f(a1[r] + bias[n0 + 32]));
                 }
             }
         }
-        // layers 2, 3
+        // layers 2, 3: both halves of the output features' MFMA chains, then
+        // their softplus, scheduled so that each of the second half's 16
+        // products issues beside 12 VALU operations of the first half's
+        // softplus
 #pragma unroll 1
         for (int L = 0; L < 2; ++L) {
             const bf16x8 *Fw = L == 0 ? F2 : F3;
             const float *bl = bias + 128 * (L + 1);
-            bf16x8 hn[2][8];
+            f32x16 a[2][2];
 #pragma unroll
             for (int np = 0; np < 2; ++np) {
-                f32x16 a[2][2] = {{{}, {}}, {{}, {}}};
+                a[np][0] = f32x16{};
+                a[np][1] = f32x16{};
 #pragma unroll
                 for (int kc = 0; kc < 8; ++kc) {
                     const bf16x8 w0 = Fw[((2 * np) * 8 + kc) * 64 + l], w1 = Fw[((2 * np + 1) * 8 + kc) * 64 + l];
-#pragma unroll
-                    for (int s = 0; s < 2; ++s) {
-                        a[s][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, h[s][kc], a[s][0], 0, 0, 0);
-                        a[s][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, h[s][kc], a[s][1], 0, 0, 0);
-                    }
+                    a[np][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, h[kc], a[np][0], 0, 0, 0);
+                    a[np][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, h[kc], a[np][1], 0, 0, 0);
                 }
-#pragma unroll
-                for (int s = 0; s < 2; ++s)
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const int n0 = (2 * np) * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
-                        hn[s][4 * np + (r >> 3)][r & 7] = f2bf(softplus_f(a[s][0][r] + bl[n0]));
-                        hn[s][4 * np + 2 + (r >> 3)][r & 7] = f2bf(softplus_f(a[s][1][r] + bl[n0 + 32]));
-                    }
             }
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
+            for (int np = 0; np < 2; ++np)
 #pragma unroll
-                for (int kc = 0; kc < 8; ++kc) h[s][kc] = hn[s][kc];
+                for (int r = 0; r < 16; ++r) {
+                    const int n0 = (2 * np) * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+                    h[4 * np + (r >> 3)][r & 7] = f2bf(softplus2_f(a[np][0][r] + bl[n0]));
+                    h[4 * np + 2 + (r >> 3)][r & 7] = f2bf(softplus2_f(a[np][1][r] + bl[n0 + 32]));
+                }
+            __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);     // MFMA
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x002, 12, 0); // VALU
+            }
         }
-        // layer 4 (linear), two output tiles at a time, fused output transform:
+        // layer 4 (linear), fused output transform:
         // y = acc ystd + (b4 ystd + ymean); tau = e^y col_dry, pfrac = max(y, 0).
         // The operands are swapped (rows x K times K x features): the layer-3
         // output already sits in the A-operand layout, and the weight fragments
         // in the B-operand layout, so the result has one feature per lane and
         // 16 rows per lane, and each store writes one 128-byte line per half
         // wavefront (32 consecutive features of one row) instead of 32 partial
-        // lines.  A lane's rows come from the lanes that prepared them.
-        const float *ysd = bias + 640, *yc = bias + 896;
-        // one tile at a time (unrolled: h[s] must stay in registers): its 16
-        // rows per lane (output row, dry-air column) from the lanes that
-        // prepared them, then its 8 output tiles
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            int orr[16];
-            float cdr[16];
-            const int mine = valid[s] ? (int)orow[s] : -1;
+        // lines.  A lane's 16 rows (output row, dry-air column) come from the
+        // lanes that prepared them.
+        int orr[16];
+        float cdr[16];
+        {
+            const int mine = valid ? (int)orow : -1;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int row = 8 * (r >> 2) + 4 * half + (r & 3);
                 orr[r] = __shfl(mine, row);
-                cdr[r] = __shfl(col_dry[s], row);
+                cdr[r] = __shfl(col_dry, row);
             }
-            // output tiles 2 np, 2 np + 1: tau for np < 2 (TAU), Planck fraction after
-            auto layer4 = [&](auto TAU, int np) {
-                f32x16 acc[2] = {{}, {}};
-#pragma unroll
-                for (int kc = 0; kc < 8; ++kc) {
-                    const bf16x8 w0 = F4[((2 * np) * 8 + kc) * 64 + l], w1 = F4[((2 * np + 1) * 8 + kc) * 64 + l];
-                    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(h[s][kc], w0, acc[0], 0, 0, 0);
-                    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(h[s][kc], w1, acc[1], 0, 0, 0);
-                }
-                float *out_p = TAU.value ? A.tau : A.pfrac;
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    const int feat = (2 * np + u) * 32 + rl;
-                    const float sc = ysd[feat], cc = yc[feat];
-                    const int col = feat - (TAU.value ? 0 : MLP_G);
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const float y = acc[u][r] * sc + cc;
-                        const float v = TAU.value ? __expf(y) * cdr[r] : fmaxf(y, 0.0f);
-                        if (orr[r] >= 0) out_p[(long)orr[r] * MLP_G + col] = v;
-                    }
-                }
-            };
-#pragma unroll 1
-            for (int np = 0; np < 2; ++np) layer4(std::true_type{}, np);
-#pragma unroll 1
-            for (int np = 2; np < 4; ++np) layer4(std::false_type{}, np);
         }
+        // output tiles 2 np, 2 np + 1 (tau for np < 2, Planck fraction after):
+        // the products of one pair run while the previous pair's outputs are
+        // transformed and stored
+        auto chain = [&](int np, f32x16 (&acc)[2]) {
+            acc[0] = f32x16{};
+            acc[1] = f32x16{};
 #pragma unroll
-        for (int s = 0; s < 2; ++s)
-            if (valid[s] && half == 0) {
-                A.oob[orow[s]] = out[s] ? 1 : 0;
-                if (out[s]) A.colflag[dcol[s]] = 1;
+            for (int kc = 0; kc < 8; ++kc) {
+                const bf16x8 w0 = F4[((2 * np) * 8 + kc) * 64 + l], w1 = F4[((2 * np + 1) * 8 + kc) * 64 + l];
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(h[kc], w0, acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(h[kc], w1, acc[1], 0, 0, 0);
             }
+        };
+        auto store = [&](int np, const f32x16 (&acc)[2]) {
+            const bool tau = np < 2;
+            float *out_p = tau ? A.tau : A.pfrac;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int feat = (2 * np + u) * 32 + rl;
+                const float sc = ysd[feat], cc = yc[feat];
+                const int col = feat - (tau ? 0 : MLP_G);
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const float y = acc[u][r] * sc + cc;
+                    const float v = tau ? __expf(y) * cdr[r] : fmaxf(y, 0.0f);
+                    if (orr[r] >= 0) out_p[(long)orr[r] * MLP_G + col] = v;
+                }
+            }
+        };
+        f32x16 acc[2][2];
+        chain(0, acc[0]);
+#pragma unroll
+        for (int np = 1; np < 4; ++np) {
+            chain(np, acc[np & 1]);
+            store(np - 1, acc[(np - 1) & 1]);
+            __builtin_amdgcn_sched_group_barrier(0x008, 2, 1);      // MFMA
+#pragma unroll
+            for (int i = 0; i < 14; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 1);  // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x002, 6, 1);  // VALU
+                __builtin_amdgcn_sched_group_barrier(0x040, 2, 1);  // VMEM write
+            }
+        }
+        store(3, acc[1]);
+        if (valid && half == 0) {
+            A.oob[orow] = oob ? 1 : 0;
+            if (oob) A.colflag[dcol] = 1;
+        }
     }
 }
 
@@ -519,13 +572,15 @@ static void build_image(const std::vector<float> &P, const float *ystd, const fl
                     }
     }
     float *b = (float *)(img.data() + NFRAG * FRAG_BYTES);
+    // hidden-layer biases in the base-2 domain of the inference kernel
     for (int i = 0; i < 3; ++i)
-        for (int n = 0; n < 128; ++n) b[128 * i + n] = P[boff(i) + n];
+        for (int n = 0; n < 128; ++n) b[128 * i + n] = (float)((double)P[boff(i) + n] * LOG2E_D);
     for (int n = 0; n < 256; ++n) b[384 + n] = P[boff(3) + n];
     // output de-standardisation folded with the layer-4 bias:
-    // y = (acc + b4) ystd + ymean = acc ystd + (b4 ystd + ymean)
+    // y = (acc + b4) ystd + ymean = acc ystd + (b4 ystd + ymean), where the
+    // kernel's acc is the natural one over ln2 (base-2 hidden activations)
     for (int n = 0; n < 256; ++n) {
-        b[640 + n] = ystd[n];
+        b[640 + n] = (float)((double)ystd[n] * LN2_D);
         b[896 + n] = (float)((double)P[boff(3) + n] * ystd[n] + ymean[n]);
     }
 }
@@ -708,7 +763,7 @@ void launch_mlp_infer(const ::ccpprad_mlp *m, MlpInferArgs &a, hipStream_t st)
     a.image = m->image;
     a.ymean = m->ymean;
     a.ystd = m->ystd;
-    for (int i = 0; i < MLP_NIN; ++i) { a.xmean[i] = m->xmean[i]; a.xistd[i] = m->xistd[i]; a.lo[i] = m->lo[i]; a.hi[i] = m->hi[i]; }
+    for (int i = 0; i < MLP_NIN; ++i) { a.xmean[i] = m->xmean[i]; a.xistd[i] = (float)((double)m->xistd[i] * LOG2E_D); a.lo[i] = m->lo[i]; a.hi[i] = m->hi[i]; }
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute((const void *)mlp_infer_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, MLP_LDS_BYTES);

@@ -80,7 +80,16 @@ def _profiles_dev(kd, ncol, nlay, seed):
 
 
 def test_fused_inference_matches_training_forward(kd_lw):
+    """The fused inference kernel and the training forward against the FP64
+    forward of the same network (BF16 weights, unrounded activations).  Both
+    GPU paths round every hidden activation to BF16 (8-bit significand), the
+    fused kernel in its base-2 domain (softplus / ln2), so their roundings
+    differ; each carries an error of a few BF16 ulp through the untrained
+    network.  The fused path must be as accurate as the training path: its
+    error on the standardised outputs within twice the training path's (plus
+    1e-3 of the output scale), and below the absolute bounds."""
     from ccpprad import mlp
+    from oracle import mlp_oracle as M
     e, p = _emu(5)
     ncol, nlay = 40, 127
     pr, arr, d = _profiles_dev(kd_lw, ncol, nlay, 9)
@@ -92,7 +101,8 @@ def test_fused_inference_matches_training_forward(kd_lw):
     ys[128:] = 0.05
     e.set_norm(xm, xs, F.min(0) - 1, F.max(0) + 1, ym, ys)
     e.finalize()
-    X = torch.from_numpy(((F - xm) / xs).astype(np.float32)).cuda()
+    Xs = ((F - xm) / xs).astype(np.float32)
+    X = torch.from_numpy(Xs).cuda()
     Y = torch.zeros((X.shape[0], 256), dtype=torch.float32, device="cuda")
     e.forward(X, Y)
     tau = torch.zeros((ncol, nlay, 128), dtype=torch.float32, device="cuda")
@@ -101,22 +111,23 @@ def test_fused_inference_matches_training_forward(kd_lw):
     cf = torch.zeros((ncol,), dtype=torch.uint8, device="cuda")
     e.infer(ncol, nlay, d, tau, pf, oob, cf)
     torch.cuda.synchronize()
-    y = Y.cpu().numpy() * ys + ym
+    params = [(_bf16(W), b) for W, b in _unflatten(p)]
+    ref, _, _ = M.forward(params, Xs.astype(np.float64))         # standardised outputs, FP64
     cd = mlp.col_dry(pr.p_lev, pr.vmr["h2o"]).T.reshape(-1)
-    tau_ref = np.exp(y[:, :128]) * cd[:, None]
-    pf_ref = np.maximum(y[:, 128:], 0)
-    # Both paths round every hidden activation to BF16 (8-bit significand);
-    # an activation whose rounding differs between them moves by one BF16 ulp
-    # and an untrained network carries that to the outputs, so the bound is on
-    # the standardised outputs: a few ulp at the worst row, far less typical.
-    ys_inf = (np.log(tau.cpu().numpy().reshape(-1, 128) / cd[:, None]) - ym[:128]) / ys[:128]
-    Ystd = Y.cpu().numpy()
-    scale = np.sqrt((Ystd[:, :128] ** 2).mean())
-    dtau = np.abs(ys_inf - Ystd[:, :128]) / scale
-    dpf = np.abs(pf.cpu().numpy().reshape(-1, 128) - pf_ref)
-    print("ln tau std diff max", dtau.max(), "mean", dtau.mean(), "pfrac abs max", dpf.max())
-    assert dtau.max() < 5e-2 and dtau.mean() < 5e-3
-    assert dpf.max() < 5e-3
+    y_inf = (np.log(tau.cpu().numpy().reshape(-1, 128) / cd[:, None]) - ym[:128]) / ys[:128]
+    y_fwd = Y.cpu().numpy()[:, :128]
+    scale = np.sqrt((ref[:, :128] ** 2).mean())
+    e_inf = np.abs(y_inf - ref[:, :128]) / scale
+    e_fwd = np.abs(y_fwd - ref[:, :128]) / scale
+    pf_ref = np.maximum(ref[:, 128:] * ys[128:] + ym[128:], 0)
+    d_inf = np.abs(pf.cpu().numpy().reshape(-1, 128) - pf_ref)
+    d_fwd = np.abs(np.maximum(Y.cpu().numpy()[:, 128:] * ys[128:] + ym[128:], 0) - pf_ref)
+    print("ln tau error / scale: fused max %.3g mean %.3g, training max %.3g mean %.3g" %
+          (e_inf.max(), e_inf.mean(), e_fwd.max(), e_fwd.mean()))
+    print("pfrac abs error: fused max %.3g, training max %.3g" % (d_inf.max(), d_fwd.max()))
+    assert e_inf.mean() < 2 * e_fwd.mean() + 1e-3 and e_inf.max() < 2 * e_fwd.max() + 1e-3
+    assert e_inf.max() < 5e-2 and e_inf.mean() < 5e-3
+    assert d_inf.max() < 2 * d_fwd.max() + 1e-4 and d_inf.max() < 5e-3
     assert oob.sum().item() == 0
 
 
