@@ -124,3 +124,36 @@ def test_pade_bounds_that_need_the_regime_clamp_are_rejected():
     buf, n = capi.errbuf()
     rc = L.ccpprad_lw_init(C.addressof(hk.struct), C.addressof(hl.struct), None, 0, 127, 1, C.byref(ctx), buf, n)
     assert rc != 0 and b"b2 - b3 <= b1" in buf.value
+
+
+REF_PHYSICS = "/root/reference/physics"
+DROP_INS = [
+    ("rrtmgp_lw_main.F90", "Radiation/RRTMGP/rrtmgp_lw_main.F90", "rrtmgp_lw_main_init"),
+    ("rrtmgp_lw_main.F90", "Radiation/RRTMGP/rrtmgp_lw_main.F90", "rrtmgp_lw_main_run"),
+    ("rrtmgp_sw_main.F90", "Radiation/RRTMGP/rrtmgp_sw_main.F90", "rrtmgp_sw_main_init"),
+    ("rrtmgp_sw_main.F90", "Radiation/RRTMGP/rrtmgp_sw_main.F90", "rrtmgp_sw_main_run"),
+    ("GFS_rrtmgp_post.F90", "Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_post.F90", "GFS_rrtmgp_post_run"),
+    ("GFS_rrtmgp_cloud_overlap.F90", "Interstitials/UFS_SCM_NEPTUNE/GFS_rrtmgp_cloud_overlap.F90",
+     "GFS_rrtmgp_cloud_overlap_run"),
+]
+
+
+@pytest.mark.parametrize("ours,theirs,name", DROP_INS)
+def test_fortran_drop_ins_keep_the_reference_argument_lists(ours, theirs, name):
+    """The scheme subroutines' dummy-argument lists, name for name and in
+    order, against the reference's sources (read as text where they are
+    mounted), so the reference's .meta tables bind them unchanged."""
+    import re
+    from ccpprad import build
+    ref = os.path.join(REF_PHYSICS, theirs)
+    if not os.path.exists(ref):
+        pytest.skip("reference sources not mounted")
+
+    def args(path):
+        txt = open(path).read()
+        m = re.search(r"subroutine\s+" + name + r"\s*\((.*?)\)", txt, re.S | re.I)
+        assert m, (path, name)
+        body = re.sub(r"!.*", "", m.group(1))
+        return [a.lower() for a in re.findall(r"\w+", body)]
+
+    assert args(os.path.join(build.FSRC, ours)) == args(ref)
