@@ -411,7 +411,8 @@ def main():
     # ---- end to end through the host-pointer entries (the ones the Fortran
     # scheme modules call): host arrays in pageable memory, as a Fortran
     # host holds them; every input copied in and every output copied out
-    # each call (intent(inout) outputs are copied both ways), synchronous.
+    # each call (intent(inout) outputs are copied both ways, except those the
+    # scheme writes in every column), synchronous.
     # LW then SW, one after the other.
     e2e = None
     if not a.no_e2e:
@@ -424,11 +425,21 @@ def main():
         hl.update({k: np.zeros(tuple(v.shape), dtype=np.float64) for k, v in lw_out.items()})
         hs = dict(host)
         hs.update({k: np.zeros(tuple(v.shape), dtype=np.float64) for k, v in sw_out.items()})
+        # bytes the host entries copy: each input of the scheme's argument
+        # list once, each output back, and in as well unless the scheme writes
+        # it in every column (the LW fluxes and heating rate)
+        lw_whole = {"fluxlwUP_allsky", "fluxlwDOWN_allsky", "fluxlwUP_clrsky", "fluxlwDOWN_clrsky",
+                    "fluxlwUP_radtime", "fluxlwDOWN_radtime", "htrlw"}
         moved = 0
-        for d_, outs in ((hl, set(lw_out)), (hs, set(sw_out))):
+        for d_, ins, outs, whole in ((hl, set(capi.LW_INPUT_FIELDS) | {"icseed_lw"}, set(lw_out), lw_whole),
+                                     (hs, set(capi.SW_INPUT_FIELDS) | {"idx", "icseed_sw"}, set(sw_out), set())):
             for k, v in d_.items():
-                if isinstance(v, np.ndarray):
-                    moved += v.nbytes * (2 if k in outs else 1)
+                if not isinstance(v, np.ndarray):
+                    continue
+                if k in outs:
+                    moved += v.nbytes * (1 if k in whole else 2)
+                elif k in ins:
+                    moved += v.nbytes
         lw.run(lw_ints, hl)
         sw.run(sw_ints, hs)
         t0 = time.perf_counter()

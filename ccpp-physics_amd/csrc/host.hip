@@ -5,7 +5,10 @@
 // list to device staging buffers (outputs too: the CCPP outputs are
 // intent(inout), rrtmgp_lw_main.F90:183-190 / rrtmgp_sw_main.F90:175-182, and
 // columns the scheme does not touch keep their values), run the device entry
-// on a private stream, copy the outputs back and synchronise.
+// on a private stream, copy the outputs back and synchronise.  Outputs the
+// scheme overwrites in every column are not copied in: the LW level fluxes
+// (the reference zeroes or writes them for every block,
+// rrtmgp_lw_main.F90:281-292,529-533,604-609) and the LW heating rate.
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdio.h>
@@ -35,9 +38,11 @@ struct Field {
     Shape shape;
     bool is_int;
     bool out;
+    bool whole = false;   // an output written in every column: not copied in
 };
 
 #define LWF(name, sh, isint, out) {offsetof(ccpprad_lw_args, name), sh, isint, out}
+#define LWW(name) {offsetof(ccpprad_lw_args, name), S_LEV, false, true, true}
 #define SWF(name, sh, isint, out) {offsetof(ccpprad_sw_args, name), sh, isint, out}
 
 const Field LW_FIELDS[] = {
@@ -55,10 +60,9 @@ const Field LW_FIELDS[] = {
     LWF(cld_pbl_iwp, S_LAY, false, false), LWF(cld_pbl_reice, S_LAY, false, false),
     LWF(cloud_overlap_param, S_LAY, false, false), LWF(aerlw_tau, S_BAND, false, false),
     LWF(aerlw_ssa, S_BAND, false, false), LWF(aerlw_g, S_BAND, false, false),
-    LWF(fluxlwUP_allsky, S_LEV, false, true), LWF(fluxlwDOWN_allsky, S_LEV, false, true),
-    LWF(fluxlwUP_clrsky, S_LEV, false, true), LWF(fluxlwDOWN_clrsky, S_LEV, false, true),
-    LWF(fluxlwUP_jac, S_LEV, false, true), LWF(fluxlwUP_radtime, S_LEV, false, true),
-    LWF(fluxlwDOWN_radtime, S_LEV, false, true), LWF(htrlw, S_LAY, false, true), LWF(htrlwc, S_LAY, false, true),
+    LWW(fluxlwUP_allsky), LWW(fluxlwDOWN_allsky), LWW(fluxlwUP_clrsky), LWW(fluxlwDOWN_clrsky),
+    LWF(fluxlwUP_jac, S_LEV, false, true), LWW(fluxlwUP_radtime), LWW(fluxlwDOWN_radtime),
+    {offsetof(ccpprad_lw_args, htrlw), S_LAY, false, true, true}, LWF(htrlwc, S_LAY, false, true),
 };
 
 const Field SW_FIELDS[] = {
@@ -122,7 +126,7 @@ int run_host(ccpprad_ctx *c, const Args *a, const Field (&F)[N], int ncol, int n
             if (errmsg && errlen > 0) snprintf(errmsg, errlen, "device staging allocation failed");
             return 2;
         }
-        if (bytes && hipMemcpyAsync(dp, *hp, bytes, hipMemcpyHostToDevice, st) != hipSuccess) {
+        if (bytes && !F[i].whole && hipMemcpyAsync(dp, *hp, bytes, hipMemcpyHostToDevice, st) != hipSuccess) {
             if (errmsg && errlen > 0) snprintf(errmsg, errlen, "host to device copy failed");
             return 2;
         }

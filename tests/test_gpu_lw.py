@@ -119,23 +119,38 @@ def test_lw_overlap_variants(kd_lw, lut_lw, iovr):
 
 def test_lw_host_entry_matches_device_entry(kd_lw, lut_lw):
     """ccpprad_lw_run (host arrays, the Fortran drop-in path) equals the
-    device entry bit for bit."""
+    device entry bit for bit.  The outputs written in every column (fluxes,
+    radtime copies, heating rate) are not copied in: they start as NaN on the
+    host and must come back complete; the Jacobian, not requested, keeps its
+    host values.  Two calls with different profiles on one context, so a
+    staging buffer's contents from the first call cannot pass for the
+    second's."""
     from ccpprad import lib, device, capi
     from oracle import oracle as O
     ncol, nlay = 64, 127
-    pr, arr = _setup(kd_lw, lut_lw, ncol, nlay, True)
     ints = capi.lw_default_ints(ncol, nlay, nangle=3)
     s = lib.LWScheme(kd_lw, lut_lw, nlay, blksz=ncol)
-    host = dict(arr)
-    host.update(O.lw_outputs(ncol, nlay))
-    s.run(ints, host)
-    dev = device.to_device(arr)
-    dev.update(device.lw_outputs(ncol, nlay))
-    s.run_dev(ints, dev)
-    torch.cuda.synchronize()
-    out = device.to_host(dev)
-    for n in ("fluxlwUP_allsky", "fluxlwDOWN_allsky", "htrlw"):
-        assert np.array_equal(out[n], host[n]), n
+    for seed in (3, 4):
+        pr, arr = _setup(kd_lw, lut_lw, ncol, nlay, True, seed=seed)
+        host = dict(arr)
+        outs = O.lw_outputs(ncol, nlay)
+        sentinel = {}
+        for k in outs:
+            outs[k][:] = np.nan
+        outs["fluxlwUP_jac"][:] = 7.0
+        sentinel["fluxlwUP_jac"] = outs["fluxlwUP_jac"].copy()
+        host.update(outs)
+        s.run(ints, host)
+        dev = device.to_device(arr)
+        dev.update(device.lw_outputs(ncol, nlay))
+        s.run_dev(ints, dev)
+        torch.cuda.synchronize()
+        out = device.to_host(dev)
+        for n in ("fluxlwUP_allsky", "fluxlwDOWN_allsky", "fluxlwUP_clrsky", "fluxlwDOWN_clrsky",
+                  "fluxlwUP_radtime", "fluxlwDOWN_radtime", "htrlw"):
+            assert not np.isnan(host[n]).any(), n
+            assert np.array_equal(out[n], host[n]), n
+        assert np.array_equal(host["fluxlwUP_jac"], sentinel["fluxlwUP_jac"])
 
 
 def test_lw_aerosols(kd_lw, lut_lw):
