@@ -234,8 +234,12 @@ __device__ __forceinline__ bool block_item(int ncols, int nchunk, int &chunk, in
 // FB: the emulator's fallback launch (only rows with row_mask set, of
 // columns with col_mask set), a separate instantiation so that profiles
 // show it apart from the table path.
+// At most 72 VGPRs, 7 wavefronts per SIMD, as many as the block's LDS allows
+// with 9 flavors: without the bound the LW kernel used 73 (allocated in
+// steps of 8: 80), which left 6 per SIMD (LW gas optics 27.3 -> 26.2 ms per
+// 262,144-column step).
 template <bool LW, int NM, bool FB>
-__global__ __launch_bounds__(128) void gas_optics_kernel(
+__global__ __launch_bounds__(128, 7) void gas_optics_kernel(
     TablesView T, int nlay, int ncol_ld, const int *__restrict__ col_idx, int col0, int ncols,
     const double *__restrict__ p_lay, const double *__restrict__ p_lev, const double *__restrict__ t_lay,
     VmrPtrs vmr, int *__restrict__ err, float *__restrict__ tau,
