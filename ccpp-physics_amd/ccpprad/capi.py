@@ -103,6 +103,18 @@ class PostArgs(C.Structure):
     _fields_ = _POST_FIELDS
 
 
+class PostHrArgs(C.Structure):
+    """ccpprad_post_hr_args (GFS_rrtmgp_post_run's heating rates)."""
+    _fields_ = [("do_lw_clrsky_hr", C.c_int), ("do_sw_clrsky_hr", C.c_int), ("idxday", P_I),
+                ("htrlw", P_D), ("htrlwc", P_D), ("htrsw", P_D), ("htrswc", P_D)]
+
+
+def post_hr_struct(ints: dict, arrays: dict) -> Holder:
+    """Fill a PostHrArgs from the two flags and arrays (idxday int32, the
+    heating rates FP64; missing names become NULL)."""
+    return _fill(PostHrArgs, ints, {}, arrays)
+
+
 _OVL_FIELDS = (
     [(n, C.c_int) for n in ("nCol", "nLev", "yearlen", "doSWrad", "doLWrad", "literal_r8",
                             "idcor", "idcor_con", "idcor_hogan", "idcor_oreopoulos",

@@ -27,7 +27,8 @@ EXPORTS = ["ccpprad_lw_init", "ccpprad_sw_init", "ccpprad_lw_init_files", "ccppr
            "ccpprad_prepare_inputs_dev", "ccpprad_post_dev", "ccpprad_layer_geometry_dev",
            "ccpprad_cloud_overlap_dev", "ccpprad_lw_init_files_pade", "ccpprad_sw_init_files_pade",
            "ccpprad_heating_rate", "ccpprad_set_launch_columns", "ccpprad_launch_columns",
-           "ccpprad_lw_mcica_mask_dev", "ccpprad_sw_mcica_mask_dev", "ccpprad_lw_mcica_mask_noalloc_dev"]
+           "ccpprad_lw_mcica_mask_dev", "ccpprad_sw_mcica_mask_dev", "ccpprad_lw_mcica_mask_noalloc_dev",
+           "ccpprad_post", "ccpprad_cloud_overlap"]
 
 
 def load():
@@ -47,6 +48,8 @@ def load():
     L.ccpprad_sw_init_files.argtypes = L.ccpprad_lw_init_files.argtypes
     L.ccpprad_lw_init_files_pade.argtypes = L.ccpprad_lw_init_files.argtypes
     L.ccpprad_sw_init_files_pade.argtypes = L.ccpprad_lw_init_files.argtypes
+    L.ccpprad_post.argtypes = [vp, vp, C.c_char_p, C.c_int]
+    L.ccpprad_cloud_overlap.argtypes = [vp, C.c_char_p, C.c_int]
     L.ccpprad_lw_run.argtypes = [vp, vp, C.c_char_p, C.c_int]
     L.ccpprad_sw_run.argtypes = [vp, vp, C.c_char_p, C.c_int]
     L.ccpprad_lw_run_dev.argtypes = [vp, vp, vp, C.c_int, C.c_char_p, C.c_int]

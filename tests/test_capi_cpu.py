@@ -1,5 +1,6 @@
 """CPU checks of the C-ABI library: it builds, loads without a GPU and
 exports every entry point include/ccpprad.h declares."""
+import ctypes as C
 import os
 import re
 import subprocess
@@ -157,3 +158,51 @@ def test_fortran_drop_ins_keep_the_reference_argument_lists(ours, theirs, name):
         return [a.lower() for a in re.findall(r"\w+", body)]
 
     assert args(os.path.join(build.FSRC, ours)) == args(ref)
+
+
+def _post_call(ints, hr_ints, arrays):
+    from ccpprad import lib, capi
+    L = lib.load()
+    a = capi.post_struct(ints, dict(fhlwr=3600.0, fhswr=3600.0, raddt=1800.0), arrays)
+    h = capi.post_hr_struct(hr_ints, arrays)
+    buf, n = capi.errbuf()
+    rc = L.ccpprad_post(C.addressof(a.struct), C.addressof(h.struct), buf, n)
+    return rc, buf.value
+
+
+def test_post_host_entry_argument_errors():
+    """ccpprad_post (GFS_rrtmgp_post_run on host arrays) checks its heating-rate
+    arguments on the host before any device work: a requested clear-sky
+    heating rate without its array, nDay outside 0..nCol, daylit indices
+    outside 1..nCol or repeated; a call with neither LW nor SW does nothing."""
+    import numpy as np
+    nc, nl = 8, 4
+    ints = dict(nCol=nc, nLev=nl, nDay=3, iSFC=1, iTOA=nl + 1, doLWrad=1, doSWrad=1, save_diag=0, naerodp=6,
+                nfxr=45)
+    z = lambda *s: np.zeros(s)
+    arrays = dict(htrlw=z(nl, nc), htrsw=z(nl, nc), idxday=np.array([1, 4, 6], dtype=np.int32))
+    rc, msg = _post_call(ints, dict(do_lw_clrsky_hr=1), arrays)
+    assert rc == 1 and b"htrlwc is NULL" in msg
+    rc, msg = _post_call(ints, dict(do_sw_clrsky_hr=1), arrays)
+    assert rc == 1 and b"htrswc is NULL" in msg
+    rc, msg = _post_call(dict(ints, nDay=9), {}, arrays)
+    assert rc == 1 and b"nDay" in msg
+    for bad in ([1, 4, 9], [0, 4, 6], [4, 4, 6]):
+        rc, msg = _post_call(ints, {}, dict(arrays, idxday=np.array(bad, dtype=np.int32)))
+        assert rc == 1 and b"idxday" in msg, bad
+    rc, msg = _post_call(dict(ints, doLWrad=0), {}, dict(arrays, htrsw=None))
+    assert rc == 1 and b"htrsw is NULL" in msg
+    rc, _ = _post_call(dict(ints, doLWrad=0, doSWrad=0), {}, {})
+    assert rc == 0
+
+
+def test_cloud_overlap_host_entry_argument_errors():
+    """ccpprad_cloud_overlap rejects bad dimensions and does nothing without
+    radiation, on the host."""
+    from ccpprad import lib, capi
+    L = lib.load()
+    for ints, want in ((dict(nCol=-1, nLev=4, doLWrad=1), 1), (dict(nCol=4, nLev=0, doLWrad=1), 1),
+                       (dict(nCol=4, nLev=4, doLWrad=0, doSWrad=0), 0), (dict(nCol=0, nLev=4, doLWrad=1), 0)):
+        a = capi.overlap_struct(ints, {}, {})
+        buf, n = capi.errbuf()
+        assert L.ccpprad_cloud_overlap(C.addressof(a.struct), buf, n) == want, ints
