@@ -271,10 +271,13 @@ def lw_default_ints(ncol, nlay, blksz=None, nangle=3, clrsky=False, jacobian=Fal
     return d
 
 
-def sw_default_ints(ncol, nday, nlay, blksz=None, clrsky=False, iovr="iovr_maxrand", isubc=1):
-    d = dict(doSWrad=1, doSWclrsky=int(clrsky), top_at_1=0, doGP_sgs_cnv=0, doGP_sgs_pbl=0,
+def sw_default_ints(ncol, nday, nlay, blksz=None, clrsky=False, iovr="iovr_maxrand", isubc=1, top_at_1=False):
+    # iSFC: the surface level, nLay + 1 when the vertical index runs from the
+    # top (as GFS_rrtmgp_pre sets it)
+    d = dict(doSWrad=1, doSWclrsky=int(clrsky), top_at_1=int(top_at_1), doGP_sgs_cnv=0, doGP_sgs_pbl=0,
              nCol=ncol, nDay=nday, nLay=nlay, nGases=6, rrtmgp_phys_blksz=blksz or nday,
-             iovr=IOVR[iovr], iovr_convcld=IOVR["iovr_maxrand"], isubc_sw=isubc, iSFC=1)
+             iovr=IOVR[iovr], iovr_convcld=IOVR["iovr_maxrand"], isubc_sw=isubc,
+             iSFC=nlay + 1 if top_at_1 else 1)
     d.update(IOVR)
     return d
 

@@ -12,7 +12,7 @@ HR_TOL = 0.01
 
 
 def _run_both(kd, lut, ncol, nlay, clouds, clrsky, blksz=None, nday=None, iovr="iovr_maxrand", seed=20261203,
-              aer=False):
+              aer=False, top_at_1=False):
     from ccpprad import synthetic as syn, capi, lib, device
     from oracle import oracle as O
     pr = syn.make_profiles(ncol, nlay, seed=seed, clouds=clouds, kd=kd)
@@ -24,11 +24,14 @@ def _run_both(kd, lut, ncol, nlay, clouds, clrsky, blksz=None, nday=None, iovr="
         arr["aersw_tau"] = 10 ** rng.uniform(-5, -1, shp)
         arr["aersw_ssa"] = rng.uniform(0.8, 1.0, shp)
         arr["aersw_g"] = rng.uniform(0.5, 0.8, shp)
+    if top_at_1:
+        from test_gpu_lw import _flip
+        arr = _flip(arr, nlay)
     nday = ncol if nday is None else nday
     rng = np.random.default_rng(seed + 5)
     idx = np.sort(rng.choice(ncol, nday, replace=False)) + 1 if nday < ncol else np.arange(1, ncol + 1)
     arr["idx"] = idx.astype(np.int32)
-    ints = capi.sw_default_ints(ncol, nday, nlay, blksz=blksz, clrsky=clrsky, iovr=iovr)
+    ints = capi.sw_default_ints(ncol, nday, nlay, blksz=blksz, clrsky=clrsky, iovr=iovr, top_at_1=top_at_1)
     ref = dict(arr)
     ref.update(O.sw_outputs(ncol, nlay))
     O.sw_main_run(kd, lut, ints, ref)
@@ -95,3 +98,18 @@ def test_sw_aerosols(kd_sw, lut_sw):
     (increment_2stream_by_2stream)."""
     ref, out, cols = _run_both(kd_sw, lut_sw, 128, 127, clouds=True, clrsky=True, blksz=64, aer=True)
     _check(ref, out, cols)
+
+
+def test_sw_top_at_1(kd_sw, lut_sw):
+    """The same kind of atmosphere stored top first (top_at_1, iSFC = nLay + 1),
+    all and clear sky with a partial day, against the oracle; and the clear-sky
+    fluxes (no McICA draws) equal the surface-first run's with the levels
+    reversed, to the flux bar."""
+    ref, out, cols = _run_both(kd_sw, lut_sw, 128, 127, clouds=True, clrsky=True, blksz=32, nday=96,
+                               top_at_1=True)
+    _check(ref, out, cols)
+    ref0, out0, cols0 = _run_both(kd_sw, lut_sw, 128, 127, clouds=True, clrsky=True, blksz=32, nday=96)
+    assert np.array_equal(cols, cols0)
+    for n in ("fluxswUP_clrsky", "fluxswDOWN_clrsky"):
+        mx, _ = report(n + " flipped", out[n][::-1][:, cols], out0[n][:, cols], "W/m2")
+        assert mx < FLUX_TOL, n
