@@ -113,3 +113,35 @@ def test_sw_top_at_1(kd_sw, lut_sw):
     for n in ("fluxswUP_clrsky", "fluxswDOWN_clrsky"):
         mx, _ = report(n + " flipped", out[n][::-1][:, cols], out0[n][:, cols], "W/m2")
         assert mx < FLUX_TOL, n
+
+
+def test_sw_no_daylit_columns(kd_sw, lut_sw):
+    """nDay = 0 (rrtmgp_sw_main.F90:676-682): the level fluxes and scmpsw are
+    zeroed, cldtausw keeps its values; the same through the host entry (the
+    Fortran path) and as the oracle."""
+    from ccpprad import synthetic as syn, capi, lib, device
+    from oracle import oracle as O
+    ncol, nlay = 64, 127
+    pr = syn.make_profiles(ncol, nlay, clouds=True, kd=kd_sw)
+    arr = capi.profile_fields(pr)
+    arr["idx"] = np.zeros(1, dtype=np.int32)
+    ints = capi.sw_default_ints(ncol, 0, nlay, blksz=16)
+    s = lib.SWScheme(kd_sw, lut_sw, nlay, blksz=16)
+    dev = device.to_device(arr)
+    outs = device.sw_outputs(ncol, nlay)
+    for v in outs.values():
+        v.fill_(5.0)
+    dev.update(outs)
+    s.run_dev(ints, dev)
+    torch.cuda.synchronize()
+    got = device.to_host(dev)
+    host = dict(arr)
+    host.update({k: np.full(v.shape, 5.0) for k, v in O.sw_outputs(ncol, nlay).items()})
+    s.run(ints, host)
+    ref = dict(arr)
+    ref.update({k: np.full(v.shape, 5.0) for k, v in O.sw_outputs(ncol, nlay).items()})
+    O.sw_main_run(kd_sw, lut_sw, ints, ref)
+    for n in ("fluxswUP_allsky", "fluxswDOWN_allsky", "fluxswUP_clrsky", "fluxswDOWN_clrsky", "scmpsw"):
+        assert not got[n].any() and not host[n].any() and not ref[n].any(), n
+    for d in (got, host, ref):
+        assert (d["cldtausw"] == 5.0).all()
