@@ -13,7 +13,9 @@ The JSON line also carries:
                  time (HIP events on the launch stream) against 8 TB/s HBM
   cpu_baseline : the FP64 oracle timed on the host cores over a bounded
                  sample of the same workload, and the flux / heating-rate
-                 error of the GPU result on that sample against it
+                 error of the GPU result on that sample against it; also
+                 on one core, and its FP32 build on all cores (with that
+                 build's error against the FP64 run)
 """
 from __future__ import annotations
 
@@ -493,6 +495,30 @@ def main():
         O.set_threads(nth)
         cpu["one_core"] = {"value": n1 / t1c, "unit": "columns/s", "cores": 1,
                            "sample": f"first {n1} distinct columns, {t1c:.2f} s"}
+        # the same source built with FP32 per-g-point optics and transport,
+        # all cores, the same sample; its error against the FP64 run
+        O.lib32()
+        O.set_threads(nth)
+        l32 = dict(sarr)
+        l32.update(O.lw_outputs(ns, NLAY))
+        w32 = dict(sarr)
+        w32.update(O.sw_outputs(ns, NLAY))
+        t0 = time.perf_counter()
+        O.lw_main_run(kd_lw, lut_lw, capi.lw_default_ints(ns, NLAY, blksz=blksz, nangle=3), l32, fp32=True)
+        O.sw_main_run(kd_sw, lut_sw, capi.sw_default_ints(ns, ns, NLAY, blksz=blksz), w32, fp32=True)
+        t32 = time.perf_counter() - t0
+
+        def mx(x, y, scale=1.0):
+            return float(np.abs(x - y).max() * scale)
+
+        cpu["fp32"] = {
+            "value": ns / t32, "unit": "columns/s", "cores": O.lib32().orc_nthreads(),
+            "sample": f"FP32 build of the oracle, same {ns} columns, {t32:.2f} s",
+            "lw_flux_maxabs_vs_fp64_Wm2": max(mx(l32[k], lw_ref[k]) for k in ("fluxlwUP_allsky", "fluxlwDOWN_allsky")),
+            "sw_flux_maxabs_vs_fp64_Wm2": max(mx(w32[k], sw_ref[k]) for k in ("fluxswUP_allsky", "fluxswDOWN_allsky")),
+            "lw_hr_maxabs_vs_fp64_Kday": mx(l32["htrlw"], lw_ref["htrlw"], 86400.0),
+            "sw_hr_maxabs_vs_fp64_Kday": mx(w32["htrsw"], sw_ref["htrsw"], 86400.0)}
+        del l32, w32
         # GPU results on the same columns (the first ns columns of the device run)
         g = {k: v[..., :ns].cpu().numpy() if v.dim() == 2 else v for k, v in
              {"lwu": lw_out["fluxlwUP_allsky"], "lwd": lw_out["fluxlwDOWN_allsky"], "lwh": lw_out["htrlw"],

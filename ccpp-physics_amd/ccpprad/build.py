@@ -78,7 +78,7 @@ def build_fortran(verbose=False):
 
 def build_oracle(verbose=False):
     odir = os.path.join(REPO, "oracle")
-    _run(["make", "-s", "-C", odir, "liboracle.so"])
+    _run(["make", "-s", "-C", odir, "liboracle.so", "liboracle32.so"])
     if os.path.isdir("/root/reference/physics"):
         try:
             _run(["make", "-s", "-C", odir, "ref"])

@@ -67,6 +67,27 @@ def lib():
     return _lib
 
 
+_lib32 = None
+
+
+def lib32():
+    """The FP32 build of the same source (liboracle32.so: per-g-point optics
+    and transport in FP32, reductions in FP64), for the FP32 CPU baseline.
+    Only its scheme entries are bound; it is timed, never used as a checker."""
+    global _lib32
+    if _lib32 is None:
+        path = os.path.join(_HERE, "liboracle32.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle32.so is not built (run make -C oracle)")
+        L = C.CDLL(path)
+        L.orc_lw_main_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
+        L.orc_sw_main_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
+        L.orc_nthreads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
+        _lib32 = L
+    return _lib32
+
+
 def ref_lib():
     """The reference's own Fortran (mersenne_twister.f, radiation_tools.F90)
     built by `make -C oracle ref`; None when not built."""
@@ -112,6 +133,8 @@ def _p(a, t=P_D):
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
+    if _lib32 is not None:
+        _lib32.orc_set_threads(int(n))
 
 
 def nthreads():
@@ -316,14 +339,17 @@ def _run(fn, kd, lut, cls, ints, arrays, out_names, solcon=None):
     return {k: arrays[k] for k in out_names if k in arrays}
 
 
-def lw_main_run(kd, lut, ints, arrays):
+def lw_main_run(kd, lut, ints, arrays, fp32=False):
     """rrtmgp_lw_main_run on host numpy arrays; outputs are written in place
-    into the arrays dict (which must hold the output buffers)."""
-    return _run(lib().orc_lw_main_run, kd, lut, capi.LWArgs, ints, arrays, capi.LW_OUTPUT_FIELDS)
+    into the arrays dict (which must hold the output buffers).  fp32: the
+    FP32 build (baseline timing only)."""
+    L = lib32() if fp32 else lib()
+    return _run(L.orc_lw_main_run, kd, lut, capi.LWArgs, ints, arrays, capi.LW_OUTPUT_FIELDS)
 
 
-def sw_main_run(kd, lut, ints, arrays, solcon=1361.0):
-    return _run(lib().orc_sw_main_run, kd, lut, capi.SWArgs, ints, arrays, capi.SW_OUTPUT_FIELDS,
+def sw_main_run(kd, lut, ints, arrays, solcon=1361.0, fp32=False):
+    L = lib32() if fp32 else lib()
+    return _run(L.orc_sw_main_run, kd, lut, capi.SWArgs, ints, arrays, capi.SW_OUTPUT_FIELDS,
                 solcon=solcon)
 
 

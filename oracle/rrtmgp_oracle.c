@@ -43,7 +43,24 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+#include <tgmath.h>
 #include "../include/ccpprad.h"
+
+/* Working precision of the per-g-point optics and transport (the arrays
+ * shaped (ngpt, nlay, ncol) and the solvers' recurrences): double, or float
+ * for the FP32 CPU baseline build (-DORC_REAL=float, liboracle32.so).  The
+ * interpolation indices and weights, the cloud optics by band, McICA and
+ * the band / broadband reductions stay FP64 in both builds.  <tgmath.h>
+ * picks exp / sqrt / fmax ... by argument type; R() makes every literal of
+ * the working precision, so the double build is the same program as before
+ * and the float build computes the per-g-point work in FP32. */
+#ifndef ORC_REAL
+#define ORC_REAL double
+#endif
+typedef ORC_REAL real;
+#define R(x) ((real)(x))
+#define R_EPS ((real)(sizeof(real) == sizeof(float) ? (double)FLT_EPSILON : DBL_EPSILON))
+#define R_MIN ((real)(sizeof(real) == sizeof(float) ? (double)FLT_MIN : DBL_MIN))
 
 /* rte-rrtmgp mo_gas_optics_constants (recalled; parity unpinned) */
 #define AVOGAD 6.02214076e23
@@ -126,40 +143,40 @@ static void interp_layer(const ccpprad_kdist *k, double play, double tlay, doubl
 }
 
 /* interpolate3D on a (ngpt, neta, npres+1, ntemp) table; jp is jpress+itropo. */
-static double interp3d(const ccpprad_kdist *k, const double *tab, const layer_interp *L,
-                       int f, int g, const double scale[2])
+static real interp3d(const ccpprad_kdist *k, const double *tab, const layer_interp *L,
+                     int f, int g, const double scale[2])
 {
     const int ng = k->ngpt, ne = k->neta, np1 = k->npres + 1;
     int jp = L->jpress + L->itropo; /* 1-based upper row index */
-    double fp0 = 1.0 - L->fpress, fp1 = L->fpress;
-    double res = 0.0;
+    real fp0 = R(1.0 - L->fpress), fp1 = R(L->fpress);
+    real res = R(0.0);
     for (int t = 0; t < 2; ++t) {
         int je = L->jeta[f][t], jt = L->jtemp + t;
-        double fe0 = L->fminor[f][t][0], fe1 = L->fminor[f][t][1];
-        double s = fp0 * fe0 * F4(tab, g, je - 1, jp - 2, jt - 1, ng, ne, np1) +
-                   fp1 * fe0 * F4(tab, g, je - 1, jp - 1, jt - 1, ng, ne, np1) +
-                   fp0 * fe1 * F4(tab, g, je, jp - 2, jt - 1, ng, ne, np1) +
-                   fp1 * fe1 * F4(tab, g, je, jp - 1, jt - 1, ng, ne, np1);
-        res += scale[t] * s;
+        real fe0 = R(L->fminor[f][t][0]), fe1 = R(L->fminor[f][t][1]);
+        real s = fp0 * fe0 * R(F4(tab, g, je - 1, jp - 2, jt - 1, ng, ne, np1)) +
+                 fp1 * fe0 * R(F4(tab, g, je - 1, jp - 1, jt - 1, ng, ne, np1)) +
+                 fp0 * fe1 * R(F4(tab, g, je, jp - 2, jt - 1, ng, ne, np1)) +
+                 fp1 * fe1 * R(F4(tab, g, je, jp - 1, jt - 1, ng, ne, np1));
+        res += R(scale[t]) * s;
     }
     return res;
 }
 
 /* interpolate2D on (ncontrib, neta, ntemp) or (ngpt, neta, ntemp). */
-static double interp2d(int n1, int neta, const double *tab, const layer_interp *L, int f, int idx)
+static real interp2d(int n1, int neta, const double *tab, const layer_interp *L, int f, int idx)
 {
-    double res = 0.0;
+    real res = R(0.0);
     for (int t = 0; t < 2; ++t) {
         int je = L->jeta[f][t], jt = L->jtemp + t;
-        res += L->fminor[f][t][0] * F3(tab, idx, je - 1, jt - 1, n1, neta) +
-               L->fminor[f][t][1] * F3(tab, idx, je, jt - 1, n1, neta);
+        res += R(L->fminor[f][t][0]) * R(F3(tab, idx, je - 1, jt - 1, n1, neta)) +
+               R(L->fminor[f][t][1]) * R(F3(tab, idx, je, jt - 1, n1, neta));
     }
     return res;
 }
 
 /* Major + minor absorption (and Planck fraction / Rayleigh) for one layer. */
 static void gas_optics_layer(const ccpprad_kdist *k, const layer_interp *L, double play,
-                             double tlay, double *tau, double *pfrac, double *tau_ray)
+                             double tlay, real *tau, real *pfrac, real *tau_ray)
 {
     const int ng = k->ngpt;
     const int it = L->itropo;
@@ -188,15 +205,15 @@ static void gas_optics_layer(const ccpprad_kdist *k, const layer_interp *L, doub
         int gs = F2(m->limits_gpt, 0, i, 2) - 1, ge = F2(m->limits_gpt, 1, i, 2) - 1;
         int f = F2(k->gpoint_flavor, it - 1, gs, 2) - 1;
         for (int g = gs; g <= ge; ++g)
-            tau[g] += scaling * interp2d(m->ncontrib, k->neta, m->kminor, L, f,
-                                         m->kminor_start[i] - 1 + (g - gs));
+            tau[g] += R(scaling) * interp2d(m->ncontrib, k->neta, m->kminor, L, f,
+                                            m->kminor_start[i] - 1 + (g - gs));
     }
     if (tau_ray) {
         const double *kr = (it == 1) ? k->krayl_lower : k->krayl_upper;
         double cr = L->col[IH2O] + L->col[0];
         for (int g = 0; g < ng; ++g) {
             int f = F2(k->gpoint_flavor, it - 1, g, 2) - 1;
-            tau_ray[g] = interp2d(ng, k->neta, kr, L, f, g) * cr;
+            tau_ray[g] = interp2d(ng, k->neta, kr, L, f, g) * R(cr);
         }
     }
 }
@@ -204,8 +221,8 @@ static void gas_optics_layer(const ccpprad_kdist *k, const layer_interp *L, doub
 /* vmr: (ncol, nlay, ngas).  Outputs (ngpt, nlay, ncol).  pfrac LW only,
  * tau_ray SW only (NULL otherwise). */
 void orc_gas_optics(const ccpprad_kdist *k, int ncol, int nlay, const double *p_lay,
-                    const double *p_lev, const double *t_lay, const double *vmr, double *tau,
-                    double *pfrac, double *tau_ray)
+                    const double *p_lev, const double *t_lay, const double *vmr, real *tau,
+                    real *pfrac, real *tau_ray)
 {
     const int ng = k->ngpt;
 #pragma omp parallel for schedule(static)
@@ -264,8 +281,8 @@ double orc_planck_band(const ccpprad_kdist *k, int b, double T) { return planck1
  * band-sum invariant sum_{g in b} lay_src = B_b(T) and the Jacobian slope. */
 void orc_lw_sources(const ccpprad_kdist *k, int ncol, int nlay, int top_at_1,
                     const double *t_lay, const double *t_lev, const double *tsfc,
-                    const double *pfrac, double *lay_src, double *lev_inc, double *lev_dec,
-                    double *sfc_src, double *sfc_jac)
+                    const real *pfrac, real *lay_src, real *lev_inc, real *lev_dec,
+                    real *sfc_src, real *sfc_jac)
 {
     const int ng = k->ngpt, nb = k->nband;
     const int sfc_lay = top_at_1 ? nlay - 1 : 0;
@@ -278,9 +295,9 @@ void orc_lw_sources(const ccpprad_kdist *k, int ncol, int nlay, int top_at_1,
         }
         for (int b = 0; b < nb; ++b)
             for (int g = F2(k->band_lims_gpt, 0, b, 2) - 1; g < F2(k->band_lims_gpt, 1, b, 2); ++g) {
-                double p = pfrac[g + (size_t)ng * (sfc_lay + (size_t)nlay * c)];
-                sfc_src[g + (size_t)ng * c] = p * pf0[b];
-                sfc_jac[g + (size_t)ng * c] = p * (pf1[b] - pf0[b]);
+                real p = pfrac[g + (size_t)ng * (sfc_lay + (size_t)nlay * c)];
+                sfc_src[g + (size_t)ng * c] = p * R(pf0[b]);
+                sfc_jac[g + (size_t)ng * c] = p * R(pf1[b] - pf0[b]);
             }
         double plo[MAXB], phi[MAXB], play_[MAXB];
         for (int b = 0; b < nb; ++b) plo[b] = planck1d(k, b, F2(t_lev, c, 0, ncol));
@@ -292,9 +309,9 @@ void orc_lw_sources(const ccpprad_kdist *k, int ncol, int nlay, int top_at_1,
             for (int b = 0; b < nb; ++b)
                 for (int g = F2(k->band_lims_gpt, 0, b, 2) - 1; g < F2(k->band_lims_gpt, 1, b, 2); ++g) {
                     size_t o = g + (size_t)ng * (l + (size_t)nlay * c);
-                    lay_src[o] = pfrac[o] * play_[b];
-                    lev_inc[o] = pfrac[o] * phi[b];
-                    lev_dec[o] = pfrac[o] * plo[b];
+                    lay_src[o] = pfrac[o] * R(play_[b]);
+                    lev_inc[o] = pfrac[o] * R(phi[b]);
+                    lev_dec[o] = pfrac[o] * R(plo[b]);
                 }
             for (int b = 0; b < nb; ++b) plo[b] = phi[b];
         }
@@ -366,39 +383,39 @@ void orc_gauss(int n, double *D, double *w)
 /* lw_source_noscat + lw_transport_noscat for one (column, g-point, angle).
  * Adds 2*pi*w*I into up/dn (nlev) and the Jacobian into jac.
  * tau, lay, inc, dec are strided by ngpt over layers. */
-static void lw_one(int nlay, int ngpt, int top_at_1, double D, double w, const double *tau,
-                   const double *lay, const double *inc, const double *dec, double emis,
-                   double sfc, double sfcj, double *up, double *dn, double *jac, double *tr,
-                   double *su, double *sd, double *Iu, double *Id)
+static void lw_one(int nlay, int ngpt, int top_at_1, real D, real w, const real *tau,
+                   const real *lay, const real *inc, const real *dec, real emis,
+                   real sfc, real sfcj, real *up, real *dn, real *jac, real *tr,
+                   real *su, real *sd, real *Iu, real *Id)
 {
-    const double tau_thresh = sqrt(DBL_EPSILON);
+    const real tau_thresh = sqrt(R_EPS);
     for (int l = 0; l < nlay; ++l) {
-        double t = tau[(size_t)l * ngpt] * D;
-        double trans = exp(-t);
-        double fact = (t > tau_thresh) ? (1.0 - trans) / t - trans : t * (0.5 - t / 3.0);
-        double ls = lay[(size_t)l * ngpt];
-        double lup = top_at_1 ? dec[(size_t)l * ngpt] : inc[(size_t)l * ngpt];
-        double ldn = top_at_1 ? inc[(size_t)l * ngpt] : dec[(size_t)l * ngpt];
+        real t = tau[(size_t)l * ngpt] * D;
+        real trans = exp(-t);
+        real fact = (t > tau_thresh) ? (R(1.0) - trans) / t - trans : t * (R(0.5) - t / R(3.0));
+        real ls = lay[(size_t)l * ngpt];
+        real lup = top_at_1 ? dec[(size_t)l * ngpt] : inc[(size_t)l * ngpt];
+        real ldn = top_at_1 ? inc[(size_t)l * ngpt] : dec[(size_t)l * ngpt];
         tr[l] = trans;
-        sd[l] = (1.0 - trans) * ldn + 2.0 * fact * (ls - ldn);
-        su[l] = (1.0 - trans) * lup + 2.0 * fact * (ls - lup);
+        sd[l] = (R(1.0) - trans) * ldn + R(2.0) * fact * (ls - ldn);
+        su[l] = (R(1.0) - trans) * lup + R(2.0) * fact * (ls - lup);
     }
-    double alb = 1.0 - emis, ssrc = emis * sfc;
+    real alb = R(1.0) - emis, ssrc = emis * sfc;
     if (top_at_1) {
-        Id[0] = 0.0;
+        Id[0] = R(0.0);
         for (int v = 1; v <= nlay; ++v) Id[v] = tr[v - 1] * Id[v - 1] + sd[v - 1];
         Iu[nlay] = Id[nlay] * alb + ssrc;
         for (int v = nlay - 1; v >= 0; --v) Iu[v] = tr[v] * Iu[v + 1] + su[v];
     } else {
-        Id[nlay] = 0.0;
+        Id[nlay] = R(0.0);
         for (int v = nlay - 1; v >= 0; --v) Id[v] = tr[v] * Id[v + 1] + sd[v];
         Iu[0] = Id[0] * alb + ssrc;
         for (int v = 1; v <= nlay; ++v) Iu[v] = tr[v - 1] * Iu[v - 1] + su[v - 1];
     }
-    double f = 2.0 * M_PI * w;
+    real f = R(2.0 * M_PI) * w;
     for (int v = 0; v <= nlay; ++v) { up[v] += f * Iu[v]; dn[v] += f * Id[v]; }
     if (jac) {
-        double J = emis * sfcj;
+        real J = emis * sfcj;
         if (top_at_1) {
             Iu[nlay] = J;
             for (int v = nlay - 1; v >= 0; --v) Iu[v] = tr[v] * Iu[v + 1];
@@ -415,9 +432,9 @@ static void lw_one(int nlay, int ngpt, int top_at_1, double D, double w, const d
  * reduction (sum_byband then the CCPP sum over bands) into flux_up/flux_dn
  * (ncol, nlev) and optional flux_jac.  emis_b (nband, ncol). */
 void orc_lw_solve(const ccpprad_kdist *k, int ncol, int nlay, int top_at_1, int nang,
-                  const double *Ds, const double *tau, const double *lay_src,
-                  const double *lev_inc, const double *lev_dec, const double *emis_b,
-                  const double *sfc_src, const double *sfc_jac, double *flux_up,
+                  const real *Ds, const real *tau, const real *lay_src,
+                  const real *lev_inc, const real *lev_dec, const double *emis_b,
+                  const real *sfc_src, const real *sfc_jac, double *flux_up,
                   double *flux_dn, double *flux_jac)
 {
     const int ng = k->ngpt, nb = k->nband, nlev = nlay + 1;
@@ -425,28 +442,28 @@ void orc_lw_solve(const ccpprad_kdist *k, int ncol, int nlay, int top_at_1, int 
     orc_gauss(nang, Dq, wq);
 #pragma omp parallel
     {
-        double *gup = malloc(sizeof(double) * nlev * ng), *gdn = malloc(sizeof(double) * nlev * ng);
-        double *gj = malloc(sizeof(double) * nlev * ng);
-        double *tr = malloc(sizeof(double) * nlay), *su = malloc(sizeof(double) * nlay);
-        double *sd = malloc(sizeof(double) * nlay);
-        double *Iu = malloc(sizeof(double) * nlev), *Id = malloc(sizeof(double) * nlev);
+        real *gup = malloc(sizeof(real) * nlev * ng), *gdn = malloc(sizeof(real) * nlev * ng);
+        real *gj = malloc(sizeof(real) * nlev * ng);
+        real *tr = malloc(sizeof(real) * nlay), *su = malloc(sizeof(real) * nlay);
+        real *sd = malloc(sizeof(real) * nlay);
+        real *Iu = malloc(sizeof(real) * nlev), *Id = malloc(sizeof(real) * nlev);
 #pragma omp for schedule(static)
         for (int c = 0; c < ncol; ++c) {
-            memset(gup, 0, sizeof(double) * nlev * ng);
-            memset(gdn, 0, sizeof(double) * nlev * ng);
-            memset(gj, 0, sizeof(double) * nlev * ng);
+            memset(gup, 0, sizeof(real) * nlev * ng);
+            memset(gdn, 0, sizeof(real) * nlev * ng);
+            memset(gj, 0, sizeof(real) * nlev * ng);
             for (int b = 0; b < nb; ++b) {
-                double emis = F2(emis_b, b, c, nb);
+                real emis = R(F2(emis_b, b, c, nb));
                 for (int g = F2(k->band_lims_gpt, 0, b, 2) - 1; g < F2(k->band_lims_gpt, 1, b, 2); ++g) {
                     size_t o = g + (size_t)ng * (size_t)nlay * c;
-                    double s = sfc_src[g + (size_t)ng * c], sj = sfc_jac[g + (size_t)ng * c];
-                    double *u = gup + (size_t)g * nlev, *d = gdn + (size_t)g * nlev, *j = gj + (size_t)g * nlev;
+                    real s = sfc_src[g + (size_t)ng * c], sj = sfc_jac[g + (size_t)ng * c];
+                    real *u = gup + (size_t)g * nlev, *d = gdn + (size_t)g * nlev, *j = gj + (size_t)g * nlev;
                     if (Ds) {
-                        lw_one(nlay, ng, top_at_1, Ds[g + (size_t)ng * c], 0.5, tau + o, lay_src + o,
+                        lw_one(nlay, ng, top_at_1, Ds[g + (size_t)ng * c], R(0.5), tau + o, lay_src + o,
                                lev_inc + o, lev_dec + o, emis, s, sj, u, d, flux_jac ? j : NULL, tr, su, sd, Iu, Id);
                     } else {
                         for (int a = 0; a < nang; ++a)
-                            lw_one(nlay, ng, top_at_1, Dq[a], wq[a], tau + o, lay_src + o, lev_inc + o,
+                            lw_one(nlay, ng, top_at_1, R(Dq[a]), R(wq[a]), tau + o, lay_src + o, lev_inc + o,
                                    lev_dec + o, emis, s, sj, u, d, flux_jac ? j : NULL, tr, su, sd, Iu, Id);
                     }
                 }
@@ -455,12 +472,12 @@ void orc_lw_solve(const ccpprad_kdist *k, int ncol, int nlay, int top_at_1, int 
                 double su_ = 0.0, sd_ = 0.0, sj_ = 0.0;
                 for (int b = 0; b < nb; ++b) {
                     int g0 = F2(k->band_lims_gpt, 0, b, 2) - 1, g1 = F2(k->band_lims_gpt, 1, b, 2);
-                    double bu = gup[(size_t)g0 * nlev + v], bd = gdn[(size_t)g0 * nlev + v];
-                    double bj = gj[(size_t)g0 * nlev + v];
+                    double bu = (double)gup[(size_t)g0 * nlev + v], bd = (double)gdn[(size_t)g0 * nlev + v];
+                    double bj = (double)gj[(size_t)g0 * nlev + v];
                     for (int g = g0 + 1; g < g1; ++g) {
-                        bu += gup[(size_t)g * nlev + v];
-                        bd += gdn[(size_t)g * nlev + v];
-                        bj += gj[(size_t)g * nlev + v];
+                        bu += (double)gup[(size_t)g * nlev + v];
+                        bd += (double)gdn[(size_t)g * nlev + v];
+                        bj += (double)gj[(size_t)g * nlev + v];
                     }
                     su_ += bu; sd_ += bd; sj_ += bj;
                 }
@@ -770,16 +787,16 @@ void orc_post(const ccpprad_post_args *a)
 }
 
 /* compute_optimal_angles: Ds(g,c) = fit1(b) * exp(-sum_l tau) + fit2(b). */
-void orc_optimal_angles(const ccpprad_kdist *k, int ncol, int nlay, const double *tau, double *Ds)
+void orc_optimal_angles(const ccpprad_kdist *k, int ncol, int nlay, const real *tau, real *Ds)
 {
     const int ng = k->ngpt;
     for (int c = 0; c < ncol; ++c)
         for (int b = 0; b < k->nband; ++b)
             for (int g = F2(k->band_lims_gpt, 0, b, 2) - 1; g < F2(k->band_lims_gpt, 1, b, 2); ++g) {
-                double s = 0.0;
+                real s = R(0.0);
                 for (int l = 0; l < nlay; ++l) s += tau[g + (size_t)ng * (l + (size_t)nlay * c)];
-                Ds[g + (size_t)ng * c] = F2(k->optimal_angle_fit, 0, b, 2) * exp(-s) +
-                                         F2(k->optimal_angle_fit, 1, b, 2);
+                Ds[g + (size_t)ng * c] = R(F2(k->optimal_angle_fit, 0, b, 2)) * exp(-s) +
+                                         R(F2(k->optimal_angle_fit, 1, b, 2));
             }
 }
 
@@ -1056,6 +1073,17 @@ static inline void inc2s(double t1, double s1, double g1, double *t2, double *s2
     *t2 = t12;
 }
 
+/* inc2s on the per-g-point optics, in the working precision. */
+static inline void inc2s_r(real t1, real s1, real g1, real *t2, real *s2, real *g2)
+{
+    const real eps = R(3.0) * R_MIN;
+    real t12 = t1 + *t2;
+    real ts12 = t1 * s1 + *t2 * *s2;
+    *g2 = (t1 * s1 * g1 + *t2 * *s2 * *g2) / fmax(eps, ts12);
+    *s2 = ts12 / fmax(eps, t12);
+    *t2 = t12;
+}
+
 /* ===================================================================== */
 /* adding (Shonk & Hogan 2008, rte's adding()), shared by the SW and the  */
 /* LW two-stream solvers.  Per array layer l: diffuse reflectance Rd,     */
@@ -1064,20 +1092,20 @@ static inline void inc2s(double t1, double s1, double g1, double *t2, double *s2
 /* source src_sfc.  No diffuse flux enters at the top.  up/dn at the      */
 /* nlay+1 levels; alb/src (nlev) and den (nlay) are scratch.              */
 /* ===================================================================== */
-static void adding(int nlay, int top_at_1, double alb_sfc, double src_sfc, const double *Rd,
-                   const double *Td, const double *sup, const double *sdn, double *up, double *dn,
-                   double *alb, double *src, double *den)
+static void adding(int nlay, int top_at_1, real alb_sfc, real src_sfc, const real *Rd,
+                   const real *Td, const real *sup, const real *sdn, real *up, real *dn,
+                   real *alb, real *src, real *den)
 {
     const int nlev = nlay + 1, vs = top_at_1 ? nlay : 0;
     alb[vs] = alb_sfc;
     src[vs] = src_sfc;
     if (top_at_1) {
         for (int l = nlay - 1; l >= 0; --l) {
-            den[l] = 1.0 / (1.0 - Rd[l] * alb[l + 1]);
+            den[l] = R(1.0) / (R(1.0) - Rd[l] * alb[l + 1]);
             alb[l] = Rd[l] + Td[l] * Td[l] * alb[l + 1] * den[l];
             src[l] = sup[l] + Td[l] * den[l] * (src[l + 1] + alb[l + 1] * sdn[l]);
         }
-        dn[0] = 0.0;
+        dn[0] = R(0.0);
         up[0] = dn[0] * alb[0] + src[0];
         for (int v = 1; v < nlev; ++v) {
             dn[v] = (Td[v - 1] * dn[v - 1] + Rd[v - 1] * src[v] + sdn[v - 1]) * den[v - 1];
@@ -1085,11 +1113,11 @@ static void adding(int nlay, int top_at_1, double alb_sfc, double src_sfc, const
         }
     } else {
         for (int l = 0; l < nlay; ++l) {
-            den[l] = 1.0 / (1.0 - Rd[l] * alb[l]);
+            den[l] = R(1.0) / (R(1.0) - Rd[l] * alb[l]);
             alb[l + 1] = Rd[l] + Td[l] * Td[l] * alb[l] * den[l];
             src[l + 1] = sup[l] + Td[l] * den[l] * (src[l] + alb[l] * sdn[l]);
         }
-        dn[nlay] = 0.0;
+        dn[nlay] = R(0.0);
         up[nlay] = dn[nlay] * alb[nlay] + src[nlay];
         for (int v = nlay - 1; v >= 0; --v) {
             dn[v] = (Td[v] * dn[v + 1] + Rd[v] * src[v] + sdn[v]) * den[v];
@@ -1112,43 +1140,43 @@ static void adding(int nlay, int top_at_1, double alb_sfc, double src_sfc, const
  * matrix exponentials, layer by layer, with the boundary conditions solved
  * as one linear system (no adding), and against lw_solver_noscat at one
  * angle (1.66, 0.5) when ssa = 0. */
-static void lw2s_core(int nlay, int top_at_1, const double *tau, const double *ssa, const double *gg,
-                      const double *levs, double emis, double sfc, double *up, double *dn, double *ws)
+static void lw2s_core(int nlay, int top_at_1, const real *tau, const real *ssa, const real *gg,
+                      const real *levs, real emis, real sfc, real *up, real *dn, real *ws)
 {
-    const double D = 1.66;
+    const real D = R(1.66);
     const int nlev = nlay + 1;
-    double *Rd = ws, *Td = ws + nlev, *sup = ws + 2 * nlev, *sdn = ws + 3 * nlev;
-    double *alb = ws + 4 * nlev, *src = ws + 5 * nlev, *den = ws + 6 * nlev;
+    real *Rd = ws, *Td = ws + nlev, *sup = ws + 2 * nlev, *sdn = ws + 3 * nlev;
+    real *alb = ws + 4 * nlev, *src = ws + 5 * nlev, *den = ws + 6 * nlev;
     for (int l = 0; l < nlay; ++l) {
-        const double t = tau[l], w = ssa[l], g = gg[l];
+        const real t = tau[l], w = ssa[l], g = gg[l];
         /* lw_two_stream */
-        const double g1 = D * (1.0 - 0.5 * w * (1.0 + g));
-        const double g2 = D * 0.5 * w * (1.0 - g);
-        const double kk = sqrt(fmax((g1 - g2) * (g1 + g2), 1.0e-12));
-        const double e = exp(-t * kk), e2 = e * e;
-        const double RT = 1.0 / (kk * (1.0 + e2) + g1 * (1.0 - e2));
-        Rd[l] = RT * g2 * (1.0 - e2);
-        Td[l] = RT * 2.0 * kk * e;
+        const real g1 = D * (R(1.0) - R(0.5) * w * (R(1.0) + g));
+        const real g2 = D * R(0.5) * w * (R(1.0) - g);
+        const real kk = sqrt(fmax((g1 - g2) * (g1 + g2), R(1.0e-12)));
+        const real e = exp(-t * kk), e2 = e * e;
+        const real RT = R(1.0) / (kk * (R(1.0) + e2) + g1 * (R(1.0) - e2));
+        Rd[l] = RT * g2 * (R(1.0) - e2);
+        Td[l] = RT * R(2.0) * kk * e;
         /* lw_source_2str */
-        if (t > 1.0e-8) {
-            const double st = top_at_1 ? levs[l] : levs[l + 1];
-            const double sb = top_at_1 ? levs[l + 1] : levs[l];
-            const double Z = (sb - st) / (t * (g1 + g2));
-            const double zut = Z + st, zub = Z + sb, zdt = -Z + st, zdb = -Z + sb;
-            sup[l] = M_PI * (zut - Rd[l] * zdt - Td[l] * zub);
-            sdn[l] = M_PI * (zdb - Rd[l] * zub - Td[l] * zdt);
+        if (t > R(1.0e-8)) {
+            const real st = top_at_1 ? levs[l] : levs[l + 1];
+            const real sb = top_at_1 ? levs[l + 1] : levs[l];
+            const real Z = (sb - st) / (t * (g1 + g2));
+            const real zut = Z + st, zub = Z + sb, zdt = -Z + st, zdb = -Z + sb;
+            sup[l] = R(M_PI) * (zut - Rd[l] * zdt - Td[l] * zub);
+            sdn[l] = R(M_PI) * (zdb - Rd[l] * zub - Td[l] * zdt);
         } else {
-            sup[l] = 0.0;
-            sdn[l] = 0.0;
+            sup[l] = R(0.0);
+            sdn[l] = R(0.0);
         }
     }
-    adding(nlay, top_at_1, 1.0 - emis, M_PI * emis * sfc, Rd, Td, sup, sdn, up, dn, alb, src, den);
+    adding(nlay, top_at_1, R(1.0) - emis, R(M_PI) * emis * sfc, Rd, Td, sup, sdn, up, dn, alb, src, den);
 }
 
-void orc_lw2s_gpt(int nlay, int top_at_1, const double *tau, const double *ssa, const double *g,
-                  const double *levs, double emis, double sfc, double *up, double *dn)
+void orc_lw2s_gpt(int nlay, int top_at_1, const real *tau, const real *ssa, const real *g,
+                  const real *levs, real emis, real sfc, real *up, real *dn)
 {
-    double *ws = malloc(sizeof(double) * 7 * (size_t)(nlay + 1));
+    real *ws = malloc(sizeof(real) * 7 * (size_t)(nlay + 1));
     lw2s_core(nlay, top_at_1, tau, ssa, g, levs, emis, sfc, up, dn, ws);
     free(ws);
 }
@@ -1156,7 +1184,7 @@ void orc_lw2s_gpt(int nlay, int top_at_1, const double *tau, const double *ssa, 
 /* lw_combine_sources: the level source at array level v is
  * sqrt(lev_dec(v) * lev_inc(v-1)) between two layers, lev_dec(0) at the
  * first level and lev_inc(nlay-1) at the last (inc/dec strided by ngpt). */
-static void lw_combine_sources(int nlay, int ngpt, const double *inc, const double *dec, double *levs)
+static void lw_combine_sources(int nlay, int ngpt, const real *inc, const real *dec, real *levs)
 {
     levs[0] = dec[0];
     for (int v = 1; v < nlay; ++v) levs[v] = sqrt(dec[(size_t)v * ngpt] * inc[(size_t)(v - 1) * ngpt]);
@@ -1166,21 +1194,21 @@ static void lw_combine_sources(int nlay, int ngpt, const double *inc, const doub
 /* lw_solver_2stream over ncol columns with the band / broadband reduction of
  * orc_lw_solve.  tau/ssa/g/lev_inc/lev_dec (ngpt, nlay, ncol), emis_b
  * (nband, ncol), sfc_src (ngpt, ncol); flux_up/flux_dn (ncol, nlev). */
-void orc_lw_solve_2str(const ccpprad_kdist *k, int ncol, int nlay, int top_at_1, const double *tau,
-                       const double *ssa, const double *g, const double *lev_inc, const double *lev_dec,
-                       const double *emis_b, const double *sfc_src, double *flux_up, double *flux_dn)
+void orc_lw_solve_2str(const ccpprad_kdist *k, int ncol, int nlay, int top_at_1, const real *tau,
+                       const real *ssa, const real *g, const real *lev_inc, const real *lev_dec,
+                       const double *emis_b, const real *sfc_src, double *flux_up, double *flux_dn)
 {
     const int ng = k->ngpt, nb = k->nband, nlev = nlay + 1;
 #pragma omp parallel
     {
-        double *gup = malloc(sizeof(double) * nlev * ng), *gdn = malloc(sizeof(double) * nlev * ng);
-        double *ws = malloc(sizeof(double) * 7 * nlev), *levs = malloc(sizeof(double) * nlev);
-        double *t1 = malloc(sizeof(double) * nlay), *s1 = malloc(sizeof(double) * nlay);
-        double *g1 = malloc(sizeof(double) * nlay);
+        real *gup = malloc(sizeof(real) * nlev * ng), *gdn = malloc(sizeof(real) * nlev * ng);
+        real *ws = malloc(sizeof(real) * 7 * nlev), *levs = malloc(sizeof(real) * nlev);
+        real *t1 = malloc(sizeof(real) * nlay), *s1 = malloc(sizeof(real) * nlay);
+        real *g1 = malloc(sizeof(real) * nlay);
 #pragma omp for schedule(static)
         for (int c = 0; c < ncol; ++c) {
             for (int b = 0; b < nb; ++b) {
-                double emis = F2(emis_b, b, c, nb);
+                real emis = R(F2(emis_b, b, c, nb));
                 for (int gp = F2(k->band_lims_gpt, 0, b, 2) - 1; gp < F2(k->band_lims_gpt, 1, b, 2); ++gp) {
                     size_t o = gp + (size_t)ng * (size_t)nlay * c;
                     for (int l = 0; l < nlay; ++l) {
@@ -1197,10 +1225,10 @@ void orc_lw_solve_2str(const ccpprad_kdist *k, int ncol, int nlay, int top_at_1,
                 double su_ = 0.0, sd_ = 0.0;
                 for (int b = 0; b < nb; ++b) {
                     int g0 = F2(k->band_lims_gpt, 0, b, 2) - 1, gl = F2(k->band_lims_gpt, 1, b, 2);
-                    double bu = gup[(size_t)g0 * nlev + v], bd = gdn[(size_t)g0 * nlev + v];
+                    double bu = (double)gup[(size_t)g0 * nlev + v], bd = (double)gdn[(size_t)g0 * nlev + v];
                     for (int gp = g0 + 1; gp < gl; ++gp) {
-                        bu += gup[(size_t)gp * nlev + v];
-                        bd += gdn[(size_t)gp * nlev + v];
+                        bu += (double)gup[(size_t)gp * nlev + v];
+                        bd += (double)gdn[(size_t)gp * nlev + v];
                     }
                     su_ += bu; sd_ += bd;
                 }
@@ -1215,44 +1243,44 @@ void orc_lw_solve_2str(const ccpprad_kdist *k, int ncol, int nlay, int top_at_1,
 /* ===================================================================== */
 /* SW solver (sw_two_stream + sw_source_2str + adding), one column, one g */
 /* ===================================================================== */
-static void sw_one(int nlay, int ngpt, int top_at_1, double mu0, double toa, double adir,
-                   double adif, const double *tau, const double *ssa, const double *gg,
-                   double *up, double *dn, double *dir, double *Rd, double *Td, double *sup,
-                   double *sdn, double *alb, double *src, double *den)
+static void sw_one(int nlay, int ngpt, int top_at_1, real mu0, real toa, real adir,
+                   real adif, const real *tau, const real *ssa, const real *gg,
+                   real *up, real *dn, real *dir, real *Rd, real *Td, real *sup,
+                   real *sdn, real *alb, real *src, real *den)
 {
     /* two-stream coefficients (Meador & Weaver 1980, Zdunkowski PIFM) */
-    double Rdir_l, Tdir_l, T0;
-    double fdir_top = toa * mu0;
+    real Rdir_l, Tdir_l, T0;
+    real fdir_top = toa * mu0;
     /* surface-first (top_at_1 false): layer l spans levels l (bottom) .. l+1 (top) */
     int nlev = nlay + 1;
-    double *fd = dir; /* direct flux at levels */
+    real *fd = dir; /* direct flux at levels */
     if (top_at_1) fd[0] = fdir_top; else fd[nlay] = fdir_top;
     for (int ii = 0; ii < nlay; ++ii) {
         int l = top_at_1 ? ii : nlay - 1 - ii;
-        double t = tau[(size_t)l * ngpt], w = ssa[(size_t)l * ngpt], g = gg[(size_t)l * ngpt];
-        double mu0_inv = 1.0 / mu0;
-        double g1 = (8.0 - w * (5.0 + 3.0 * g)) * 0.25;
-        double g2 = 3.0 * (w * (1.0 - g)) * 0.25;
-        double g3 = (2.0 - 3.0 * mu0 * g) * 0.25;
-        double g4 = 1.0 - g3;
-        double a1 = g1 * g4 + g2 * g3;
-        double a2 = g1 * g3 + g2 * g4;
-        double kk = sqrt(fmax((g1 - g2) * (g1 + g2), 1.0e-12));
-        double e = exp(-t * kk);
-        double e2 = e * e;
-        double RT = 1.0 / (kk * (1.0 + e2) + g1 * (1.0 - e2));
-        Rd[l] = RT * g2 * (1.0 - e2);
-        Td[l] = RT * 2.0 * kk * e;
+        real t = tau[(size_t)l * ngpt], w = ssa[(size_t)l * ngpt], g = gg[(size_t)l * ngpt];
+        real mu0_inv = R(1.0) / mu0;
+        real g1 = (R(8.0) - w * (R(5.0) + R(3.0) * g)) * R(0.25);
+        real g2 = R(3.0) * (w * (R(1.0) - g)) * R(0.25);
+        real g3 = (R(2.0) - R(3.0) * mu0 * g) * R(0.25);
+        real g4 = R(1.0) - g3;
+        real a1 = g1 * g4 + g2 * g3;
+        real a2 = g1 * g3 + g2 * g4;
+        real kk = sqrt(fmax((g1 - g2) * (g1 + g2), R(1.0e-12)));
+        real e = exp(-t * kk);
+        real e2 = e * e;
+        real RT = R(1.0) / (kk * (R(1.0) + e2) + g1 * (R(1.0) - e2));
+        Rd[l] = RT * g2 * (R(1.0) - e2);
+        Td[l] = RT * R(2.0) * kk * e;
         T0 = exp(-t * mu0_inv);
-        double kmu = kk * mu0, kg3 = kk * g3, kg4 = kk * g4;
-        double d = 1.0 - kmu * kmu;
-        double RT2 = w * RT / (fabs(d) >= DBL_EPSILON ? d : DBL_EPSILON);
-        Rdir_l = RT2 * ((1.0 - kmu) * (a2 + kg3) - (1.0 + kmu) * (a2 - kg3) * e2 -
-                        2.0 * (kg3 - a2 * kmu) * e * T0);
-        Tdir_l = -RT2 * ((1.0 + kmu) * (a1 + kg4) * T0 - (1.0 - kmu) * (a1 - kg4) * e2 * T0 -
-                         2.0 * (kg4 + a1 * kmu) * e);
-        Rdir_l = fmax(0.0, fmin(Rdir_l, 1.0 - T0));
-        Tdir_l = fmax(0.0, fmin(Tdir_l, 1.0 - T0 - Rdir_l));
+        real kmu = kk * mu0, kg3 = kk * g3, kg4 = kk * g4;
+        real d = R(1.0) - kmu * kmu;
+        real RT2 = w * RT / (fabs(d) >= R_EPS ? d : R_EPS);
+        Rdir_l = RT2 * ((R(1.0) - kmu) * (a2 + kg3) - (R(1.0) + kmu) * (a2 - kg3) * e2 -
+                        R(2.0) * (kg3 - a2 * kmu) * e * T0);
+        Tdir_l = -RT2 * ((R(1.0) + kmu) * (a1 + kg4) * T0 - (R(1.0) - kmu) * (a1 - kg4) * e2 * T0 -
+                         R(2.0) * (kg4 + a1 * kmu) * e);
+        Rdir_l = fmax(R(0.0), fmin(Rdir_l, R(1.0) - T0));
+        Tdir_l = fmax(R(0.0), fmin(Tdir_l, R(1.0) - T0 - Rdir_l));
         /* sw_source_2str */
         int vin = top_at_1 ? l : l + 1, vout = top_at_1 ? l + 1 : l;
         sup[l] = Rdir_l * fd[vin];
@@ -1268,17 +1296,17 @@ static void sw_one(int nlay, int ngpt, int top_at_1, double mu0, double toa, dou
  * already normalised; alb_dir/dif (nband, ncol).  Band fluxes out
  * (nband, nlev, ncol) for up, dn, dn_dir. */
 void orc_sw_solve(const ccpprad_kdist *k, int ncol, int nlay, int top_at_1, const double *mu0,
-                  const double *tau, const double *ssa, const double *g, const double *toa,
+                  const real *tau, const real *ssa, const real *g, const real *toa,
                   const double *alb_dir, const double *alb_dif, double *bup, double *bdn,
                   double *bdir)
 {
     const int ng = k->ngpt, nb = k->nband, nlev = nlay + 1;
 #pragma omp parallel
     {
-        double *w = malloc(sizeof(double) * 11 * (nlev + 1));
-        double *up = w, *dn = w + nlev + 1, *dir = w + 2 * (nlev + 1), *Rd = w + 3 * (nlev + 1);
-        double *Td = w + 4 * (nlev + 1), *sup = w + 5 * (nlev + 1), *sdn = w + 6 * (nlev + 1);
-        double *alb = w + 7 * (nlev + 1), *src = w + 8 * (nlev + 1), *den = w + 9 * (nlev + 1);
+        real *w = malloc(sizeof(real) * 11 * (nlev + 1));
+        real *up = w, *dn = w + nlev + 1, *dir = w + 2 * (nlev + 1), *Rd = w + 3 * (nlev + 1);
+        real *Td = w + 4 * (nlev + 1), *sup = w + 5 * (nlev + 1), *sdn = w + 6 * (nlev + 1);
+        real *alb = w + 7 * (nlev + 1), *src = w + 8 * (nlev + 1), *den = w + 9 * (nlev + 1);
 #pragma omp for schedule(static)
         for (int c = 0; c < ncol; ++c) {
             for (size_t i = 0; i < (size_t)nb * nlev; ++i) {
@@ -1290,13 +1318,13 @@ void orc_sw_solve(const ccpprad_kdist *k, int ncol, int nlay, int top_at_1, cons
                 int g0 = F2(k->band_lims_gpt, 0, b, 2) - 1, g1 = F2(k->band_lims_gpt, 1, b, 2);
                 for (int gp = g0; gp < g1; ++gp) {
                     size_t o = gp + (size_t)ng * (size_t)nlay * c;
-                    sw_one(nlay, ng, top_at_1, mu0[c], toa[gp + (size_t)ng * c],
-                           F2(alb_dir, b, c, nb), F2(alb_dif, b, c, nb), tau + o, ssa + o, g + o,
+                    sw_one(nlay, ng, top_at_1, R(mu0[c]), toa[gp + (size_t)ng * c],
+                           R(F2(alb_dir, b, c, nb)), R(F2(alb_dif, b, c, nb)), tau + o, ssa + o, g + o,
                            up, dn, dir, Rd, Td, sup, sdn, alb, src, den);
                     for (int v = 0; v < nlev; ++v) {
                         size_t q = b + (size_t)nb * (v + (size_t)nlev * c);
-                        if (gp == g0) { bup[q] = up[v]; bdn[q] = dn[v]; bdir[q] = dir[v]; }
-                        else { bup[q] += up[v]; bdn[q] += dn[v]; bdir[q] += dir[v]; }
+                        if (gp == g0) { bup[q] = (double)up[v]; bdn[q] = (double)dn[v]; bdir[q] = (double)dir[v]; }
+                        else { bup[q] += (double)up[v]; bdn[q] += (double)dn[v]; bdir[q] += (double)dir[v]; }
                     }
                 }
             }
@@ -1343,13 +1371,14 @@ int orc_lw_main_run(const ccpprad_kdist *k, const ccpprad_cloud_lut *lut, const 
     double *vmr = malloc(sizeof(double) * cl * 6), *pl = malloc(sizeof(double) * cl);
     double *tl = malloc(sizeof(double) * cl), *pv = malloc(sizeof(double) * bs * nV);
     double *tv = malloc(sizeof(double) * bs * nV);
-    double *tau = malloc(sizeof(double) * cg), *pf = malloc(sizeof(double) * cg);
-    double *lay = malloc(sizeof(double) * cg), *inc = malloc(sizeof(double) * cg);
-    double *dec = malloc(sizeof(double) * cg), *ss = malloc(sizeof(double) * bs * ng);
-    double *sj = malloc(sizeof(double) * bs * ng), *emis = malloc(sizeof(double) * bs * nb);
+    real *tau = malloc(sizeof(real) * cg), *pf = malloc(sizeof(real) * cg);
+    real *lay = malloc(sizeof(real) * cg), *inc = malloc(sizeof(real) * cg);
+    real *dec = malloc(sizeof(real) * cg), *ss = malloc(sizeof(real) * bs * ng);
+    real *sj = malloc(sizeof(real) * bs * ng);
+    double *emis = malloc(sizeof(double) * bs * nb);
     double *ctau = malloc(sizeof(double) * cl * nb), *cssa = malloc(sizeof(double) * cl * nb);
     double *cg_ = malloc(sizeof(double) * cl * nb), *tmp = malloc(sizeof(double) * cl * nb * 3);
-    double *Ds = malloc(sizeof(double) * bs * ng), *tau_all = malloc(sizeof(double) * cg);
+    real *Ds = malloc(sizeof(real) * bs * ng), *tau_all = malloc(sizeof(real) * cg);
     double *fu = malloc(sizeof(double) * bs * nV), *fd = malloc(sizeof(double) * bs * nV);
     double *fj = malloc(sizeof(double) * bs * nV), *cf = malloc(sizeof(double) * cl);
     double *ov = malloc(sizeof(double) * cl);
@@ -1458,7 +1487,7 @@ int orc_lw_main_run(const ccpprad_kdist *k, const ccpprad_cloud_lut *lut, const 
                     for (int b = 0; b < nb; ++b) {
                         double at = F3(a->aerlw_tau, c0 + i, l, b, nC, nL);
                         for (int gp = F2(k->band_lims_gpt, 0, b, 2) - 1; gp < F2(k->band_lims_gpt, 1, b, 2); ++gp)
-                            tau[gp + (size_t)ng * (l + (size_t)nL * i)] += at;
+                            tau[gp + (size_t)ng * (l + (size_t)nL * i)] += R(at);
                     }
         }
         /* clear sky */
@@ -1486,8 +1515,8 @@ int orc_lw_main_run(const ccpprad_kdist *k, const ccpprad_cloud_lut *lut, const 
              * the McICA-sampled clouds (2str), increment_2stream_by_1scalar:
              * tau = tau_c + tau, ssa = tau_c ssa_c / tau (0 unless tau > eps),
              * g = g_c; then lw_solver_2stream */
-            const double eps = 3.0 * DBL_MIN;
-            double *ssa_all = lay, *g_all = pf;   /* pfrac and the layer source are no longer needed */
+            const real eps = R(3.0) * R_MIN;
+            real *ssa_all = lay, *g_all = pf;   /* pfrac and the layer source are no longer needed */
             for (int i = 0; i < bs; ++i)
                 for (int l = 0; l < nL; ++l)
                     for (int b = 0; b < nb; ++b) {
@@ -1495,9 +1524,9 @@ int orc_lw_main_run(const ccpprad_kdist *k, const ccpprad_cloud_lut *lut, const 
                         for (int gp = F2(k->band_lims_gpt, 0, b, 2) - 1; gp < F2(k->band_lims_gpt, 1, b, 2); ++gp) {
                             size_t o = gp + (size_t)ng * (l + (size_t)nL * i);
                             int m = mask[l + (size_t)nL * gp + (size_t)nL * ng * i];
-                            double t1 = m ? ctau[q] : 0.0, s1 = m ? cssa[q] : 0.0, g1 = m ? cg_[q] : 0.0;
-                            double t12 = t1 + tau[o];
-                            ssa_all[o] = t12 > eps ? t1 * s1 / t12 : 0.0;
+                            real t1 = m ? R(ctau[q]) : R(0.0), s1 = m ? R(cssa[q]) : R(0.0), g1 = m ? R(cg_[q]) : R(0.0);
+                            real t12 = t1 + tau[o];
+                            ssa_all[o] = t12 > eps ? t1 * s1 / t12 : R(0.0);
                             g_all[o] = g1;
                             tau_all[o] = t12;
                         }
@@ -1522,8 +1551,8 @@ int orc_lw_main_run(const ccpprad_kdist *k, const ccpprad_cloud_lut *lut, const 
                     for (int gp = F2(k->band_lims_gpt, 0, b, 2) - 1; gp < F2(k->band_lims_gpt, 1, b, 2); ++gp) {
                         size_t o = gp + (size_t)ng * (l + (size_t)nL * i);
                         int m = mask[l + (size_t)nL * gp + (size_t)nL * ng * i];
-                        double t1 = m ? ctau[q] : 0.0, s1 = m ? cssa[q] : 0.0;
-                        tau_all[o] = tau[o] + t1 * (1.0 - s1);
+                        real t1 = m ? R(ctau[q]) : R(0.0), s1 = m ? R(cssa[q]) : R(0.0);
+                        tau_all[o] = tau[o] + t1 * (R(1.0) - s1);
                     }
                 }
         orc_lw_solve(k, bs, nL, a->top_at_1, a->nGauss_angles, NULL, tau_all, lay, inc, dec, emis, ss, sj, fu, fd,
@@ -1572,9 +1601,10 @@ int orc_sw_main_run(const ccpprad_kdist *k, const ccpprad_cloud_lut *lut, const 
     double *vmr = malloc(sizeof(double) * cl * 6), *pl = malloc(sizeof(double) * cl);
     double *tl = malloc(sizeof(double) * cl), *pv = malloc(sizeof(double) * bs * nV);
     double *cf = malloc(sizeof(double) * cl), *ov = malloc(sizeof(double) * cl);
-    double *tau = malloc(sizeof(double) * cg), *ray = malloc(sizeof(double) * cg);
-    double *ssa = malloc(sizeof(double) * cg), *gg = malloc(sizeof(double) * cg);
-    double *toa = malloc(sizeof(double) * bs * ng), *mu0 = malloc(sizeof(double) * bs);
+    real *tau = malloc(sizeof(real) * cg), *ray = malloc(sizeof(real) * cg);
+    real *ssa = malloc(sizeof(real) * cg), *gg = malloc(sizeof(real) * cg);
+    real *toa = malloc(sizeof(real) * bs * ng);
+    double *mu0 = malloc(sizeof(double) * bs);
     double *adir = malloc(sizeof(double) * bs * nb), *adif = malloc(sizeof(double) * bs * nb);
     double *ctau = malloc(sizeof(double) * cl * nb), *cssa = malloc(sizeof(double) * cl * nb);
     double *cgg = malloc(sizeof(double) * cl * nb), *tmp = malloc(sizeof(double) * cl * nb * 3);
@@ -1631,15 +1661,15 @@ int orc_sw_main_run(const ccpprad_kdist *k, const ccpprad_cloud_lut *lut, const 
         orc_gas_optics(k, bs, nL, pl, pv, tl, vmr, tau, NULL, ray);
         /* combine_and_reorder_2str */
         for (size_t i = 0; i < cg; ++i) {
-            double t = tau[i] + ray[i];
-            if (t > 2.0 * DBL_MIN) { tau[i] = t; ssa[i] = ray[i] / t; }
-            else { tau[i] = t; ssa[i] = 0.0; }
-            gg[i] = 0.0;
+            real t = tau[i] + ray[i];
+            if (t > R(2.0) * R_MIN) { tau[i] = t; ssa[i] = ray[i] / t; }
+            else { tau[i] = t; ssa[i] = R(0.0); }
+            gg[i] = R(0.0);
         }
         for (int i = 0; i < bs; ++i) {
             double s = 0.0;
             for (int gp = 0; gp < ng; ++gp) s += k->solar_src[gp];
-            for (int gp = 0; gp < ng; ++gp) toa[gp + (size_t)ng * i] = k->solar_src[gp] * a->solcon / s;
+            for (int gp = 0; gp < ng; ++gp) toa[gp + (size_t)ng * i] = R(k->solar_src[gp] * a->solcon / s);
             mu0[i] = a->coszen[cols[i]];
             for (int b = 0; b < nb; ++b) {
                 double lo = F2(k->band_lims_wvn, 0, b, 2);
@@ -1736,7 +1766,7 @@ int orc_sw_main_run(const ccpprad_kdist *k, const ccpprad_cloud_lut *lut, const 
                         double ag = F3(a->aersw_g, c, l, b, nC, nL);
                         for (int gp = F2(k->band_lims_gpt, 0, b, 2) - 1; gp < F2(k->band_lims_gpt, 1, b, 2); ++gp) {
                             size_t o = gp + (size_t)ng * (l + (size_t)nL * i);
-                            inc2s(at, as, ag, &tau[o], &ssa[o], &gg[o]);
+                            inc2s_r(R(at), R(as), R(ag), &tau[o], &ssa[o], &gg[o]);
                         }
                     }
         }
@@ -1780,7 +1810,8 @@ int orc_sw_main_run(const ccpprad_kdist *k, const ccpprad_cloud_lut *lut, const 
                         for (int gp = F2(k->band_lims_gpt, 0, b, 2) - 1; gp < F2(k->band_lims_gpt, 1, b, 2); ++gp) {
                             size_t o = gp + (size_t)ng * (l + (size_t)nL * i);
                             int m = mask[l + (size_t)nL * gp + (size_t)nL * ng * i];
-                            inc2s(m ? ctau[q] : 0.0, m ? cssa[q] : 0.0, m ? cgg[q] : 0.0, &tau[o], &ssa[o], &gg[o]);
+                            inc2s_r(m ? R(ctau[q]) : R(0.0), m ? R(cssa[q]) : R(0.0), m ? R(cgg[q]) : R(0.0),
+                                    &tau[o], &ssa[o], &gg[o]);
                         }
                     }
             orc_sw_solve(k, bs, nL, a->top_at_1, mu0, tau, ssa, gg, toa, adir, adif, aup, adn, adr);
