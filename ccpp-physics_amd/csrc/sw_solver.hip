@@ -65,51 +65,68 @@ __device__ __forceinline__ float diff_ratio(float ea, float eb, float d, float s
 //    both from the exponentials already formed (diff_ratio),
 //    Rdir = w RT / (1+k mu) [a2 (2G - E2 - 2 T_R) + k g3 (E2 + 2 T_R)]
 //    Tdir = -w RT / (1+k mu) [(a1 - k g4) E2 (1 - F) - 2 k (g4 + mu a1) T_T].
-// gamma1 - gamma2 = 2 (1 - w) comes from the co-albedo p.
+// gamma1 - gamma2 = 2 (1 - w) comes from the co-albedo p.  Away from
+// k mu0 = 1 one reciprocal serves RT = 1 / X and RT / (1 - (k mu0)^2).
+//
+// G0: the layer's asymmetry is 0 (gas and Rayleigh only).  Then
+// g3 = g4 = 1/2 and a1 = a2 = (g1 + g2) / 2, and the terms that vanish are
+// left out; every remaining operation is the one the general form performs
+// with g = 0 (the dropped ones are exact: x * 0 + y = y, and
+// g1 / 2 + g2 / 2 rounds as (g1 + g2) / 2), so both forms give the same
+// bits.  Contraction is off so the two forms round alike.
+template <bool G0>
 __device__ __forceinline__ TwoStream two_stream32(float mu0, float mu0_inv, float c3, const Opt2s &o)
 {
+#pragma clang fp contract(off)
     const float tau = o.t, w = o.w, p = o.p, g = o.g;
-    // (8 - w (5 + 3 g)) / 4, 3 w (1 - g) / 4, (2 - 3 mu0 g) / 4 and 1 - g3,
-    // each in one or two FMAs (c3 = 3 mu0 / 4 is the column's)
-    const float g1 = fmaf(-w, fmaf(0.75f, g, 1.25f), 2.0f);
+    // (8 - w (5 + 3 g)) / 4, 3 w (1 - g) / 4, (2 - 3 mu0 g) / 4 and 1 - g3
+    // (c3 = 3 mu0 / 4 is the column's)
+    const float g1 = fmaf(-w, G0 ? 1.25f : fmaf(0.75f, g, 1.25f), 2.0f);
     const float w34 = 0.75f * w;
-    const float g2 = fmaf(-w34, g, w34);
-    const float g3 = fmaf(-c3, g, 0.5f);
-    const float g4 = fmaf(c3, g, 0.5f);
-    const float a1 = g1 * g4 + g2 * g3;
-    const float a2 = g1 * g3 + g2 * g4;
-    const float k = __builtin_amdgcn_sqrtf(fmaxf(2.0f * p * (g1 + g2), 1.0e-12f));
+    const float g2 = G0 ? w34 : fmaf(-w34, g, w34);
+    const float g3 = G0 ? 0.5f : fmaf(-c3, g, 0.5f);
+    const float g4 = G0 ? 0.5f : fmaf(c3, g, 0.5f);
+    const float g12 = g1 + g2;
+    const float a1 = G0 ? 0.5f * g12 : fmaf(g1, g4, g2 * g3);
+    const float a2 = G0 ? a1 : fmaf(g1, g3, g2 * g4);
+    const float k = __builtin_amdgcn_sqrtf(fmaxf((2.0f * p) * g12, 1.0e-12f));
     const float kmu = k * mu0;
-    const float dd = 1.0f - kmu * kmu;
+    const float dd = fmaf(-kmu, kmu, 1.0f);
     const float s = tau * mu0_inv;
     float e, E1, T0, F;
     exp_pair(tau * k, e, E1);
     exp_pair(s, T0, F);
     const float E2 = E1 * (2.0f - E1);
-    const float G = E1 + F - E1 * F;
-    const float RT = frcp(k * (1.0f + e * e) + g1 * E2);
+    const float G = fmaf(-E1, F, E1 + F);
+    const float X = fmaf(k, fmaf(e, e, 1.0f), g1 * E2);
+    const bool reg = fabsf(dd) >= 1.0e-2f;
+    const float r = frcp(reg ? X * dd : X);
+    const float RT = reg ? r * dd : r;
     float Rdir, Tdir;
-    if (fabsf(dd) >= 1.0e-2f) {
-        const float RT2 = w * RT * frcp(dd);
-        Rdir = RT2 * ((1.0f + kmu) * (a2 - k * g3) * E2 + 2.0f * k * (g3 - a2 * mu0) * G);
-        Tdir = -RT2 * (2.0f * k * (g4 + mu0 * a1) * (E1 - F) + (1.0f - kmu) * (a1 - k * g4) * E2 * (1.0f - F));
+    if (reg) {
+        const float RT2 = w * r;
+        const float br = fmaf((1.0f + kmu) * fmaf(-k, g3, a2), E2, ((2.0f * k) * fmaf(-a2, mu0, g3)) * G);
+        const float bt = fmaf((2.0f * k) * fmaf(mu0, a1, g4), E1 - F,
+                              (((1.0f - kmu) * fmaf(-k, g4, a1)) * E2) * (1.0f - F));
+        Rdir = RT2 * br;
+        Tdir = -RT2 * bt;
     } else {
         const float delta = 1.0f - kmu;
         // e^{-2 k tau} = e^2, e^{-(k tau + s)} = e T0, e^{-k tau} = e, e^{-s} = T0
         const float TR = diff_ratio(e * e, e * T0, delta, s);
         const float TT = diff_ratio(e, T0, delta, s);
-        const float pre = w * RT * frcp(1.0f + kmu);
-        Rdir = pre * (a2 * (2.0f * G - E2 - 2.0f * TR) + k * g3 * (E2 + 2.0f * TR));
-        Tdir = -pre * ((a1 - k * g4) * E2 * (1.0f - F) - 2.0f * k * (g4 + mu0 * a1) * TT);
+        const float pre = (w * RT) * frcp(1.0f + kmu);
+        Rdir = pre * fmaf(a2, fmaf(2.0f, G - TR, -E2), (k * g3) * fmaf(2.0f, TR, E2));
+        Tdir = -pre * fmaf((fmaf(-k, g4, a1) * E2), 1.0f - F, -(((2.0f * k) * fmaf(mu0, a1, g4)) * TT));
     }
-    TwoStream r;
+    TwoStream rs;
     // max(0, min(x, hi)) for hi >= 0, one v_med3 each
-    r.Rdir = __builtin_amdgcn_fmed3f(Rdir, 0.0f, F);
-    r.Tdir = __builtin_amdgcn_fmed3f(Tdir, 0.0f, F - r.Rdir);
-    r.Rd = RT * g2 * E2;
-    r.Td = RT * 2.0f * k * e;
-    r.T0 = T0;
-    return r;
+    rs.Rdir = __builtin_amdgcn_fmed3f(Rdir, 0.0f, F);
+    rs.Tdir = __builtin_amdgcn_fmed3f(Tdir, 0.0f, F - rs.Rdir);
+    rs.Rd = (RT * g2) * E2;
+    rs.Td = (RT * (2.0f * k)) * e;
+    rs.T0 = T0;
+    return rs;
 }
 
 // Layers per segment.  The column is walked in segments of SW_SEG layers;
@@ -255,10 +272,6 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
             o[j].g = tsg * frcp(fmaxf(EPS3, ts));
         }
     };
-    auto optics_seg = [&](int sg, Opt2s *o) {
-        if (seg_cloudy(sg)) load_seg(sg, o, std::true_type{});
-        else load_seg(sg, o, std::false_type{});
-    };
     const float toa = T.toa_norm[gg];
 
     // ---- pass 1, bottom-up: adding albedo A and the direct-beam source
@@ -268,19 +281,23 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     // bounded by 1 (the upwelling a direct beam can feed), and this pass
     // needs no direct beam.  A and q are kept at segment boundaries.
     float Ab = adif, qb = adir;
-    for (int s = nseg - 1; s >= 0; --s) {
+    auto pass1_seg = [&](int s, auto cl) {
         Opt2s o[SW_SEG];
-        optics_seg(s, o);
+        load_seg(s, o, cl);
 #pragma unroll
         for (int j = SW_SEG - 1; j >= 0; --j) {
             const int k = s * SW_SEG + j;
             if (k >= nlay) continue;
-            const TwoStream ts = two_stream32(mu0, mu0_inv, c3, o[j]);
+            const TwoStream ts = two_stream32<!decltype(cl)::value && !AER>(mu0, mu0_inv, c3, o[j]);
             const float den = frcp(1.0f - ts.Rd * Ab);
             const float Aa = ts.Rd + ts.Td * ts.Td * Ab * den;
             const float qa = ts.Rdir + ts.Td * den * (qb * ts.T0 + Ab * ts.Tdir);
             Ab = Aa; qb = qa;
         }
+    };
+    for (int s = nseg - 1; s >= 0; --s) {
+        if (seg_cloudy(s)) pass1_seg(s, std::true_type{});
+        else pass1_seg(s, std::false_type{});
         if (s > 0) { cX[(size_t)(s - 1) * 128 + g] = Ab; cQ[(size_t)(s - 1) * 128 + g] = qb; }
     }
 
@@ -299,27 +316,31 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     // surface level is the TOA)
     float sfc_dn = fdir, sfc_dir = fdir;
     for (int s = 0; s < nseg; ++s) {
-        Opt2s o[SW_SEG];
         // per layer: diffuse down below = al * above + be * F_dir above,
         // direct transmittance T0, and A, q below the layer
         float al[SW_SEG], be[SW_SEG], T0[SW_SEG], Abl[SW_SEG], qbl[SW_SEG];
-        optics_seg(s, o);
-        float a = adif, q = adir;   // the surface below the last segment
-        if (s + 1 < nseg) { a = cX[(size_t)s * 128 + g]; q = cQ[(size_t)s * 128 + g]; }
+        auto pass2_terms = [&](auto cl) {
+            Opt2s o[SW_SEG];
+            load_seg(s, o, cl);
+            float a = adif, q = adir;   // the surface below the last segment
+            if (s + 1 < nseg) { a = cX[(size_t)s * 128 + g]; q = cQ[(size_t)s * 128 + g]; }
 #pragma unroll
-        for (int j = SW_SEG - 1; j >= 0; --j) {
-            const int k = s * SW_SEG + j;
-            if (k >= nlay) continue;
-            const TwoStream ts = two_stream32(mu0, mu0_inv, c3, o[j]);
-            const float den = frcp(1.0f - ts.Rd * a);
-            // dn_dif below = (Td dn_dif + Rd Q + Tdir F_dir) den with Q = q T0 F_dir
-            al[j] = ts.Td * den;
-            be[j] = (ts.Rd * q * ts.T0 + ts.Tdir) * den;
-            T0[j] = ts.T0; Abl[j] = a; qbl[j] = q;
-            const float qa = ts.Rdir + ts.Td * den * (q * ts.T0 + a * ts.Tdir);
-            a = ts.Rd + ts.Td * ts.Td * a * den;
-            q = qa;
-        }
+            for (int j = SW_SEG - 1; j >= 0; --j) {
+                const int k = s * SW_SEG + j;
+                if (k >= nlay) continue;
+                const TwoStream ts = two_stream32<!decltype(cl)::value && !AER>(mu0, mu0_inv, c3, o[j]);
+                const float den = frcp(1.0f - ts.Rd * a);
+                // dn_dif below = (Td dn_dif + Rd Q + Tdir F_dir) den with Q = q T0 F_dir
+                al[j] = ts.Td * den;
+                be[j] = (ts.Rd * q * ts.T0 + ts.Tdir) * den;
+                T0[j] = ts.T0; Abl[j] = a; qbl[j] = q;
+                const float qa = ts.Rdir + ts.Td * den * (q * ts.T0 + a * ts.Tdir);
+                a = ts.Rd + ts.Td * ts.Td * a * den;
+                q = qa;
+            }
+        };
+        if (seg_cloudy(s)) pass2_terms(std::true_type{});
+        else pass2_terms(std::false_type{});
         // the segment's fluxes at the levels below its layers: up in
         // fl[0..7], down in fl[8..15], summed over the g-points together
         float fl[2 * SW_SEG];
