@@ -16,7 +16,13 @@
 //   2 top-down : direct beam F_dir by products of T_noscat (as the
 //                reference); per segment the layer terms and A, q below each
 //                layer rebuilt in registers from the checkpoint at the segment
-//                bottom; diffuse down/up fluxes; F_dn += F_dir
+//                bottom; downward fluxes and each layer's absorbed flux
+// The upward level fluxes are the downward ones minus the net flux, which
+// falls by each layer's absorbed flux from the TOA down (FP64); the heating
+// rate is the absorbed flux over the layer's pressure thickness.  Forming the
+// absorbed flux from the layer's absorptances, rather than as the difference
+// of two FP32 level fluxes that agree to 1e-4 in a thin layer, keeps the
+// heating rate's error relative to itself.
 // Optical properties carry the co-albedo (1 - ssa) next to ssa, so the
 // absorption of nearly conservative layers is not lost to FP32 rounding.
 #include "kernels.h"
@@ -33,6 +39,9 @@ struct Opt2s {
 
 struct TwoStream {
     float Rd, Td, Rdir, Tdir, T0;   // T0 = exp(-tau/mu0), the direct transmittance
+    // absorptances, formed without cancellation: of the direct beam
+    // Adir = 1 - T0 - Rdir - Tdir, of diffuse light Adif = 1 - Rd - Td
+    float Adir, Adif;
 };
 
 // (e^{-a} - e^{-b}) / d for b - a = d * s, s > 0, from ea = e^{-a} and
@@ -67,6 +76,10 @@ __device__ __forceinline__ float diff_ratio(float ea, float eb, float d, float s
 //    Tdir = -w RT / (1+k mu) [(a1 - k g4) E2 (1 - F) - 2 k (g4 + mu a1) T_T].
 // gamma1 - gamma2 = 2 (1 - w) comes from the co-albedo p.  Away from
 // k mu0 = 1 one reciprocal serves RT = 1 / X and RT / (1 - (k mu0)^2).
+// The layer's absorptances: of diffuse light
+//   1 - Rd - Td = RT (k (1 + e^2) - 2 k e + (g1 - g2) E2) = RT (k E1^2 + 2 p E2),
+// a sum of positive terms; of the direct beam F - Rdir - Tdir, which the
+// clamps keep >= 0 and whose rounding is relative to F.
 //
 // G0: the layer's asymmetry is 0 (gas and Rayleigh only).  Then
 // g3 = g4 = 1/2 and a1 = a2 = (g1 + g2) / 2, and the terms that vanish are
@@ -126,6 +139,8 @@ __device__ __forceinline__ TwoStream two_stream32(float mu0, float mu0_inv, floa
     rs.Rd = (RT * g2) * E2;
     rs.Td = (RT * (2.0f * k)) * e;
     rs.T0 = T0;
+    rs.Adif = RT * fmaf(k * E1, E1, (2.0f * p) * E2);
+    rs.Adir = (F - rs.Rdir) - rs.Tdir;
     return rs;
 }
 
@@ -142,16 +157,17 @@ static __host__ __device__ inline int sw_nckpt(int nseg) { return nseg > 1 ? nse
 size_t sw_solve_smem(int nlay, int nband)
 {
     const int nseg = (nlay + SW_SEG - 1) / SW_SEG;
-    return sizeof(double) * (2 * (size_t)(nlay + 1) + 6 * (size_t)nband) +
+    return sizeof(double) * (2 * (size_t)(nlay + 1) + (size_t)nlay + 6 * (size_t)nband) +
            sizeof(float) * 2 * (size_t)sw_nckpt(nseg) * 128 + 2 * sizeof(unsigned long long);
 }
 
 template <bool CLOUDS, bool AER>
 __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveArgs A)
 {
-    // LDS: [2 waves][2][nlev] level sums, [2 waves][3][nband] band partials,
-    // then per lane [2][nseg + 1] checkpoints: the adding albedo A (slot X)
-    // and the normalised source q (slot Q) at every segment boundary
+    // LDS: [2][nlev] level sums and [nlay] layer absorption sums of both
+    // wavefronts, [2 waves][3][nband] band partials, then per lane
+    // [2][nseg - 1] checkpoints: the adding albedo A (slot X) and the
+    // normalised source q (slot Q) at segment boundaries
     extern __shared__ double red[];
     const int d = blockIdx.x;
     const int od = A.col0 + d;
@@ -165,14 +181,14 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     const bool cloudy = CLOUDS && A.blkflag[od / A.blksz];
     const int top = A.top_at_1;
     const int lev_toa = top ? 0 : nlay;
-    // level sums of both wavefronts: each slot gets exactly two FP64 adds
-    // onto 0, so the total is the same in either order
-    double *rup = red, *rdn = red + nlev;
-    double *rb = red + 2 * (size_t)nlev;  // [wave][3][nband]
-    float *cX = (float *)(red + 2 * (size_t)nlev + 6 * (size_t)nb);   // checkpoint of boundary k at k - 1
+    // level and layer sums of both wavefronts: each slot gets exactly two
+    // FP64 adds onto 0, so the total is the same in either order
+    double *rup = red, *rdn = red + nlev, *rab = red + 2 * (size_t)nlev;
+    double *rb = rab + nlay;  // [wave][3][nband]
+    float *cX = (float *)(rb + 6 * (size_t)nb);   // checkpoint of boundary k at k - 1
     float *cQ = cX + (size_t)sw_nckpt(nseg) * 128;
     unsigned long long *sflag = (unsigned long long *)(cQ + (size_t)sw_nckpt(nseg) * 128);  // [2 waves]
-    for (int v = g; v < 2 * nlev; v += blockDim.x) red[v] = 0.0;
+    for (int v = g; v < 2 * nlev + nlay; v += blockDim.x) red[v] = 0.0;
     __syncthreads();
 
     const double mu0d = A.coszen[c];
@@ -316,9 +332,10 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     // surface level is the TOA)
     float sfc_dn = fdir, sfc_dir = fdir;
     for (int s = 0; s < nseg; ++s) {
-        // per layer: diffuse down below = al * above + be * F_dir above,
-        // direct transmittance T0, and A, q below the layer
-        float al[SW_SEG], be[SW_SEG], T0[SW_SEG], Abl[SW_SEG], qbl[SW_SEG];
+        // per layer: diffuse down below = al * above + be * F_dir above, the
+        // direct transmittance T0, and the absorbed flux
+        // ab = c1 * (diffuse down above) + c2 * F_dir above
+        float al[SW_SEG], be[SW_SEG], T0[SW_SEG], c1[SW_SEG], c2[SW_SEG];
         auto pass2_terms = [&](auto cl) {
             Opt2s o[SW_SEG];
             load_seg(s, o, cl);
@@ -333,7 +350,12 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
                 // dn_dif below = (Td dn_dif + Rd Q + Tdir F_dir) den with Q = q T0 F_dir
                 al[j] = ts.Td * den;
                 be[j] = (ts.Rd * q * ts.T0 + ts.Tdir) * den;
-                T0[j] = ts.T0; Abl[j] = a; qbl[j] = q;
+                T0[j] = ts.T0;
+                // absorbed = Adir F_dir + Adif (dn_dif above + up below), with
+                // up below = a dn_dif below + q T0 F_dir: sums of non-negative
+                // terms, so the absorbed flux keeps its relative accuracy
+                c1[j] = ts.Adif * fmaf(a, al[j], 1.0f);
+                c2[j] = fmaf(ts.Adif, fmaf(a, be[j], q * ts.T0), ts.Adir);
                 const float qa = ts.Rdir + ts.Td * den * (q * ts.T0 + a * ts.Tdir);
                 a = ts.Rd + ts.Td * ts.Td * a * den;
                 q = qa;
@@ -341,8 +363,13 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
         };
         if (seg_cloudy(s)) pass2_terms(std::true_type{});
         else pass2_terms(std::false_type{});
-        // the segment's fluxes at the levels below its layers: up in
-        // fl[0..7], down in fl[8..15], summed over the g-points together
+        // the segment's downward fluxes at the levels below its layers
+        // (fl[0..7]) and each layer's absorbed flux, the net flux at its top
+        // minus that at its bottom (fl[8..15]), summed over the g-points
+        // together.  The absorbed flux comes from the layer's absorptances,
+        // so the heating rate of a thin layer does not rest on the difference
+        // of two nearly equal FP32 fluxes; the upward level fluxes follow
+        // from it in the epilogue.
         float fl[2 * SW_SEG];
 #pragma unroll
         for (int j = 0; j < SW_SEG; ++j) {
@@ -350,16 +377,17 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
             const int k = s * SW_SEG + j;
             if (k >= nlay) continue;
             const float fd = T0[j] * fdir;            // direct beam at the level below
+            const float ab = fmaf(dnd, c1[j], fdir * c2[j]);
             dnd = al[j] * dnd + be[j] * fdir;
-            const float up = dnd * Abl[j] + qbl[j] * fd;
             const float dn = dnd + fd;
-            fdir = fd;
-            if (act) { fl[j] = up; fl[SW_SEG + j] = dn; }
             if (lev(k + 1) == A.isfc) { sfc_dn = dn; sfc_dir = fd; }
+            fdir = fd;
+            if (act) { fl[j] = dn; fl[SW_SEG + j] = ab; }
         }
         const double sum = wave_sum_n<2 * SW_SEG>(fl);
         const int idx = wave_sum_idx<2 * SW_SEG>(lane), k = s * SW_SEG + (idx & (SW_SEG - 1));
-        if (wave_sum_writer<2 * SW_SEG>(lane) && k < nlay) atomicAdd(&(idx < SW_SEG ? rup : rdn)[lev(k + 1)], sum);
+        if (wave_sum_writer<2 * SW_SEG>(lane) && k < nlay)
+            atomicAdd(idx < SW_SEG ? &rdn[lev(k + 1)] : &rab[lay(k)], sum);
     }
     {
         // per-band surface sums for scmpsw
@@ -371,7 +399,16 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
         }
     }
     __syncthreads();
-    // ---- epilogue
+    // ---- epilogue: the upward fluxes below the TOA from the net flux,
+    // net(below layer) = net(above) - absorbed(layer), in FP64
+    if (g == 0) {
+        double net = rdn[lev_toa] - rup[lev_toa];
+        for (int k = 0; k < nlay; ++k) {
+            net -= rab[lay(k)];
+            rup[lev(k + 1)] = rdn[lev(k + 1)] - net;
+        }
+    }
+    __syncthreads();
     const bool clr_too = (A.role == 1) || !cloudy;  // fluxes equal the clear-sky ones
     for (int v = g; v < nlev; v += blockDim.x) {
         const double up = rup[v], dn = rdn[v];
@@ -388,11 +425,11 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     }
     double *h_all = A.role == 0 ? A.htr : nullptr, *h_clr = (clr_too || A.role == 1) ? A.htr_clr : nullptr;
     if (h_all || h_clr) {
+        // compute_heating_rate: (net flux at the layer top - at its bottom)
+        // g / (cp dp), the flux difference taken as the layer's absorbed flux
         for (int l = g; l < nlay; l += blockDim.x) {
-            const double u0 = rup[l], u1 = rup[l + 1];
-            const double d0 = rdn[l], d1 = rdn[l + 1];
             const double p0 = A.p_lev[(size_t)c + (size_t)ld * l], p1 = A.p_lev[(size_t)c + (size_t)ld * (l + 1)];
-            const double hr = (u1 - u0 - d1 + d0) * GRAV / (CP_DRY * (p1 - p0));
+            const double hr = rab[l] * GRAV / (CP_DRY * fabs(p1 - p0));
             if (h_all) h_all[(size_t)c + (size_t)ld * l] = hr;
             if (h_clr) h_clr[(size_t)c + (size_t)ld * l] = hr;
         }
