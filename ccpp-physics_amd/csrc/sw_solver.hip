@@ -311,11 +311,43 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
             Ab = Aa; qb = qa;
         }
     };
-    for (int s = nseg - 1; s >= 0; --s) {
+    // Pass 2's terms of one segment, from A, q below it (a, q, advanced to
+    // the segment top): per layer, diffuse down below = al * above + be *
+    // F_dir above, the direct transmittance T0, and the absorbed flux
+    // ab = c1 * (diffuse down above) + c2 * F_dir above.
+    float al[SW_SEG], be[SW_SEG], T0[SW_SEG], c1[SW_SEG], c2[SW_SEG];
+    auto pass2_terms = [&](int s, auto cl, float &a, float &q) {
+        Opt2s o[SW_SEG];
+        load_seg(s, o, cl);
+#pragma unroll
+        for (int j = SW_SEG - 1; j >= 0; --j) {
+            const int k = s * SW_SEG + j;
+            if (k >= nlay) continue;
+            const TwoStream ts = two_stream32<!decltype(cl)::value && !AER>(mu0, mu0_inv, c3, o[j]);
+            const float den = frcp(1.0f - ts.Rd * a);
+            // dn_dif below = (Td dn_dif + Rd Q + Tdir F_dir) den with Q = q T0 F_dir
+            al[j] = ts.Td * den;
+            be[j] = (ts.Rd * q * ts.T0 + ts.Tdir) * den;
+            T0[j] = ts.T0;
+            // absorbed = Adir F_dir + Adif (dn_dif above + up below), with
+            // up below = a dn_dif below + q T0 F_dir: sums of non-negative
+            // terms, so the absorbed flux keeps its relative accuracy
+            c1[j] = ts.Adif * fmaf(a, al[j], 1.0f);
+            c2[j] = fmaf(ts.Adif, fmaf(a, be[j], q * ts.T0), ts.Adir);
+            const float qa = ts.Rdir + ts.Td * den * (q * ts.T0 + a * ts.Tdir);
+            a = ts.Rd + ts.Td * ts.Td * a * den;
+            q = qa;
+        }
+    };
+    for (int s = nseg - 1; s >= 1; --s) {
         if (seg_cloudy(s)) pass1_seg(s, std::true_type{});
         else pass1_seg(s, std::false_type{});
-        if (s > 0) { cX[(size_t)(s - 1) * 128 + g] = Ab; cQ[(size_t)(s - 1) * 128 + g] = qb; }
+        cX[(size_t)(s - 1) * 128 + g] = Ab; cQ[(size_t)(s - 1) * 128 + g] = qb;
     }
+    // the top segment, last of pass 1 and first of pass 2: its pass-2 terms
+    // are formed here once, and A, q reach the TOA
+    if (seg_cloudy(0)) pass2_terms(0, std::true_type{}, Ab, qb);
+    else pass2_terms(0, std::false_type{}, Ab, qb);
 
     // ---- pass 2, top-down: direct beam by products of T_noscat (as the
     // reference) and fluxes.  Per segment the layer terms and A, q below each
@@ -332,37 +364,14 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     // surface level is the TOA)
     float sfc_dn = fdir, sfc_dir = fdir;
     for (int s = 0; s < nseg; ++s) {
-        // per layer: diffuse down below = al * above + be * F_dir above, the
-        // direct transmittance T0, and the absorbed flux
-        // ab = c1 * (diffuse down above) + c2 * F_dir above
-        float al[SW_SEG], be[SW_SEG], T0[SW_SEG], c1[SW_SEG], c2[SW_SEG];
-        auto pass2_terms = [&](auto cl) {
-            Opt2s o[SW_SEG];
-            load_seg(s, o, cl);
-            float a = adif, q = adir;   // the surface below the last segment
+        if (s > 0) {
+            // the segment's terms, from the checkpoint at its bottom (the
+            // surface below the last segment)
+            float a = adif, q = adir;
             if (s + 1 < nseg) { a = cX[(size_t)s * 128 + g]; q = cQ[(size_t)s * 128 + g]; }
-#pragma unroll
-            for (int j = SW_SEG - 1; j >= 0; --j) {
-                const int k = s * SW_SEG + j;
-                if (k >= nlay) continue;
-                const TwoStream ts = two_stream32<!decltype(cl)::value && !AER>(mu0, mu0_inv, c3, o[j]);
-                const float den = frcp(1.0f - ts.Rd * a);
-                // dn_dif below = (Td dn_dif + Rd Q + Tdir F_dir) den with Q = q T0 F_dir
-                al[j] = ts.Td * den;
-                be[j] = (ts.Rd * q * ts.T0 + ts.Tdir) * den;
-                T0[j] = ts.T0;
-                // absorbed = Adir F_dir + Adif (dn_dif above + up below), with
-                // up below = a dn_dif below + q T0 F_dir: sums of non-negative
-                // terms, so the absorbed flux keeps its relative accuracy
-                c1[j] = ts.Adif * fmaf(a, al[j], 1.0f);
-                c2[j] = fmaf(ts.Adif, fmaf(a, be[j], q * ts.T0), ts.Adir);
-                const float qa = ts.Rdir + ts.Td * den * (q * ts.T0 + a * ts.Tdir);
-                a = ts.Rd + ts.Td * ts.Td * a * den;
-                q = qa;
-            }
-        };
-        if (seg_cloudy(s)) pass2_terms(std::true_type{});
-        else pass2_terms(std::false_type{});
+            if (seg_cloudy(s)) pass2_terms(s, std::true_type{}, a, q);
+            else pass2_terms(s, std::false_type{}, a, q);
+        }
         // the segment's downward fluxes at the levels below its layers
         // (fl[0..7]) and each layer's absorbed flux, the net flux at its top
         // minus that at its bottom (fl[8..15]), summed over the g-points
