@@ -194,36 +194,13 @@ __global__ __launch_bounds__(128) void lw_solve_2str_kernel(TablesView T, LwSolv
     const float asfc = 1.0f - emis;
     const float ssfc = 3.14159265358979f * emis * (pf_c[(unsigned)lay(nlay - 1) * ngpt + gg] * Bsfc[b]);
 
-    // ---- pass 1, bottom-up: adding albedo A and upward source S below each level
-    float Ab = asfc, Sb = ssfc;
-    for (int s = nseg - 1; s >= 0; --s) {
+    // Pass 2's terms of one segment from A, S below it (a, q, advanced to the
+    // segment top): per layer, diffuse down below = al * above + be, and A,
+    // S below it.
+    float al[L2_SEG], be[L2_SEG], Abl[L2_SEG], Sbl[L2_SEG];
+    auto pass2_terms = [&](int s, float &a, float &q) {
         Lw2Layer o[L2_SEG];
         seg_terms(s, o);
-#pragma unroll
-        for (int j = L2_SEG - 1; j >= 0; --j) {
-            const int k = s * L2_SEG + j;
-            if (k >= nlay) continue;
-            const float den = frcp(1.0f - o[j].R * Ab);
-            const float An = o[j].R + o[j].T * o[j].T * Ab * den;
-            const float Sn = o[j].su + o[j].T * den * (Sb + Ab * o[j].sd);
-            Ab = An; Sb = Sn;
-        }
-        if (s > 0) { cA[(size_t)(s - 1) * 128 + g] = Ab; cS[(size_t)(s - 1) * 128 + g] = Sb; }
-    }
-
-    // ---- pass 2, top-down: no diffuse flux enters at the TOA
-    {
-        const double su = wave_sum(act ? (double)Sb : 0.0);   // up at the TOA: the end of pass 1
-        if (lane == 0) atomicAdd(&rup[lev_toa], su);
-    }
-    float dn = 0.0f;
-    for (int s = 0; s < nseg; ++s) {
-        Lw2Layer o[L2_SEG];
-        seg_terms(s, o);
-        // per layer: diffuse down below = al * above + be, and A, S below it
-        float al[L2_SEG], be[L2_SEG], Abl[L2_SEG], Sbl[L2_SEG];
-        float a = asfc, q = ssfc;   // the surface below the last segment
-        if (s + 1 < nseg) { a = cA[(size_t)s * 128 + g]; q = cS[(size_t)s * 128 + g]; }
 #pragma unroll
         for (int j = L2_SEG - 1; j >= 0; --j) {
             const int k = s * L2_SEG + j;
@@ -235,6 +212,40 @@ __global__ __launch_bounds__(128) void lw_solve_2str_kernel(TablesView T, LwSolv
             const float an = o[j].R + o[j].T * o[j].T * a * den;
             q = o[j].su + o[j].T * den * (q + a * o[j].sd);
             a = an;
+        }
+    };
+
+    // ---- pass 1, bottom-up: adding albedo A and upward source S below each level
+    float Ab = asfc, Sb = ssfc;
+    for (int s = nseg - 1; s >= 1; --s) {
+        Lw2Layer o[L2_SEG];
+        seg_terms(s, o);
+#pragma unroll
+        for (int j = L2_SEG - 1; j >= 0; --j) {
+            const int k = s * L2_SEG + j;
+            if (k >= nlay) continue;
+            const float den = frcp(1.0f - o[j].R * Ab);
+            const float An = o[j].R + o[j].T * o[j].T * Ab * den;
+            const float Sn = o[j].su + o[j].T * den * (Sb + Ab * o[j].sd);
+            Ab = An; Sb = Sn;
+        }
+        cA[(size_t)(s - 1) * 128 + g] = Ab; cS[(size_t)(s - 1) * 128 + g] = Sb;
+    }
+    // the top segment, last of pass 1 and first of pass 2: its pass-2 terms
+    // are formed here once, and A, S reach the TOA
+    pass2_terms(0, Ab, Sb);
+
+    // ---- pass 2, top-down: no diffuse flux enters at the TOA
+    {
+        const double su = wave_sum(act ? (double)Sb : 0.0);   // up at the TOA: the end of pass 1
+        if (lane == 0) atomicAdd(&rup[lev_toa], su);
+    }
+    float dn = 0.0f;
+    for (int s = 0; s < nseg; ++s) {
+        if (s > 0) {
+            float a = asfc, q = ssfc;   // the surface below the last segment
+            if (s + 1 < nseg) { a = cA[(size_t)s * 128 + g]; q = cS[(size_t)s * 128 + g]; }
+            pass2_terms(s, a, q);
         }
         // fluxes at the levels below the segment's layers: up in fl[0..7],
         // down in fl[8..15], summed over the g-points together
