@@ -41,13 +41,28 @@ METRIC = "LW+SW columns/s at 127 layers (table vs NN) @ HR RMSE<=0.1 K/day vs FP
 # --no-nn --no-e2e --no-callers --copy-probe`), each scaled by the reading of a
 # 1 GiB device copy (FETCH x 2.0, WRITE x 1.0).  Counters cannot be read from inside this
 # process, so the last measurement of the current kernels is recorded here.
-TRAFFIC_GB = {"sw_sw_solve": 19.484, "lw_lw_solve": 22.677, "lw_gas_optics": 9.131, "sw_gas_optics": 8.071,
+TRAFFIC_GB = {"sw_sw_solve": 19.393, "lw_lw_solve": 22.659, "lw_gas_optics": 9.131, "sw_gas_optics": 8.070,
               "lw_cloud_optics": 0.775, "sw_cloud_optics": 0.775, "lw_mcica": 0.670, "sw_mcica": 0.670,
-              "lw_solve_2str": 24.556}
+              "lw_solve_2str": 24.217}
 # sha256 (first 16 hex digits) of the kernel sources the figures above were
 # measured with (kernel_source_sha()); on any other sources the bench line
 # reports traffic null rather than a stale figure.
-TRAFFIC_SRC_SHA = "e6c686ed5fc47b33"
+TRAFFIC_SRC_SHA = "de4137567f5ab191"
+
+
+# Algorithmic bytes per column of each stage: the compulsory I/O of the
+# reference's own data flow (every array it materialises, counted once; FP32
+# optics, FP64 fluxes / heating rate), at 127 layers, LW 128 g / 16 bands, SW
+# 112 g / 14 bands.  The solvers include the fused heating-rate stage
+# (3,072 B in, 1,016 B out).
+ALG_BYTES_PER_COLUMN = {
+    "lw_lw_solve": 269_308 + 2_048 + 3_072 + 1_016,
+    "sw_sw_solve": 193_000 + 3_120 + 3_072 + 1_016,
+    "lw_gas_optics": 4_584 + 261_120,   # gas-optics inputs in, tau + Planck sources out
+    "sw_gas_optics": 4_576 + 171_136,   # inputs in, tau_abs + tau_Rayleigh out
+    "lw_cloud_optics": 2_540 + 8_128,
+    "sw_cloud_optics": 2_540 + 21_336,
+}
 
 
 def kernel_source_sha():
@@ -270,21 +285,8 @@ def main():
             if cnt.value:
                 kt[f"{tag}_{nm}"] = (tot.value, cnt.value)
         L.ccpprad_timing_enable(s.ctx, 0)
-    # Algorithmic bytes per column of each stage: the compulsory I/O of the
-    # reference's own data flow (every array it materialises, counted once;
-    # FP32 optics, FP64 fluxes / heating rate), at 127 layers, LW 128 g /
-    # 16 bands, SW 112 g / 14 bands.  The solvers include the fused
-    # heating-rate stage (3,072 B in, 1,016 B out).  One launch processes a
-    # chunk of 65,536 columns at most.
-    chunk = min(ncol, 65536)
-    per_col = {
-        "lw_lw_solve": 22.677_308 + 2_048 + 3_072 + 1_016,
-        "sw_sw_solve": 19.484_000 + 3_120 + 3_072 + 1_016,
-        "lw_gas_optics": 4_584 + 261_120,   # gas-optics inputs in, tau + Planck fraction out
-        "sw_gas_optics": 8.071_576 + 171_136,   # inputs in, tau_abs + tau_Rayleigh out
-        "lw_cloud_optics": 2_540 + 8_128,
-        "sw_cloud_optics": 2_540 + 21_336,
-    }
+    chunk = min(ncol, 65536)   # one launch processes a chunk of 65,536 columns at most
+    per_col = ALG_BYTES_PER_COLUMN
     src_sha = kernel_source_sha()
     dom = max(kt.items(), key=lambda kv: kv[1][0])
     dname, (dtot, dcnt) = dom
