@@ -71,7 +71,7 @@ struct Workspace {
     bool own_stream = false;       // created for a host thread's staging path
     std::thread::id owner;         // that thread
     DevBuf w_tau, w_aux, w_mask, w_ctau, w_sb, w_blk, w_colf, w_day, w_idx;
-    DevBuf w_oob, w_colm;
+    DevBuf w_oob, w_colm, w_aer;
     std::vector<DevBuf> stage;     // host-path staging
     int *err = nullptr;            // device error word, then the SW last-wipe block
     double *toa64 = nullptr;       // SW TOA source scaled to solcon
@@ -79,7 +79,7 @@ struct Workspace {
     void release()
     {
         DevBuf *bufs[] = {&w_tau, &w_aux, &w_mask, &w_ctau, &w_sb, &w_blk, &w_colf, &w_day, &w_idx,
-                          &w_oob, &w_colm};
+                          &w_oob, &w_colm, &w_aer};
         for (DevBuf *b : bufs) b->release();
         for (auto &b : stage) b.release();
         if (err) (void)hipFree(err);
@@ -787,6 +787,7 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
         HIPCHK(w->w_oob.ensure((size_t)CH * nL));
         HIPCHK(w->w_colm.ensure((size_t)CH + 1));
     }
+    if (a->aerlw_tau) HIPCHK(w->w_aer.ensure(sizeof(float) * (size_t)CH * nL * B));
     unsigned char *blk = (unsigned char *)w->w_blk.p;
     launch_fill_i32(w->err, 1, 0, st);
     if (have_clouds) launch_block_flags(nC, nL, nC, nullptr, bs, a->cld_frac, blk, (unsigned char *)w->w_colf.p, st);
@@ -836,12 +837,14 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
         LwSolveArgs s{};
         s.nlay = nL; s.ncol_ld = nC; s.col0 = c0; s.top_at_1 = a->top_at_1;
         s.tau = tau; s.pfrac = pf; s.mask = mask; s.ctau = ctau; s.copt = copt; s.aer_tau = a->aerlw_tau;
+        s.aer1 = a->aerlw_tau ? (const float *)w->w_aer.p : nullptr;
         s.t_lay = a->t_lay; s.t_lev = a->t_lev; s.tsfc = a->tsfg; s.semis = a->semis; s.p_lev = a->p_lev;
         for (int i = 0; i < MAX_ANG; ++i) { s.D[i] = D[i]; s.W[i] = W[i]; }
         s.flux_up = a->fluxlwUP_allsky; s.flux_dn = a->fluxlwDOWN_allsky;
         s.flux_up2 = a->fluxlwUP_radtime; s.flux_dn2 = a->fluxlwDOWN_radtime;
         s.flux_jac = a->use_LW_jacobian ? a->fluxlwUP_jac : nullptr;
         s.htr = a->htrlw;
+        if (a->aerlw_tau) launch_lw_aer_pack(s, nc, B, (float *)w->w_aer.p, st);
         if (scat) {
             KScope k_(c, K_LWSOLVE, st);
             launch_lw_solve_2str(T, s, nc, have_clouds, st);
@@ -910,6 +913,9 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
     for (const void *p : req2) if (!need(p)) { seterr(errmsg, errlen, "required array argument is NULL"); return 1; }
     if (a->iSFC < 1 || a->iSFC > nV) { seterr(errmsg, errlen, "iSFC out of range"); return 1; }
     if (a->isubc_sw == 2 && !a->icseed_sw) { seterr(errmsg, errlen, "isubc_sw=2 needs icseed_sw"); return 1; }
+    if ((a->aersw_tau != nullptr) != (a->aersw_ssa != nullptr) || (a->aersw_tau != nullptr) != (a->aersw_g != nullptr)) {
+        seterr(errmsg, errlen, "aersw_tau, aersw_ssa and aersw_g: give all three or none"); return 1;
+    }
     if ((a->iovr == a->iovr_exp || a->iovr == a->iovr_exprand || a->iovr == a->iovr_dcorr) && !a->cloud_overlap_param) {
         seterr(errmsg, errlen, "exponential/decorrelation overlap needs cloud_overlap_param"); return 1;
     }
@@ -931,6 +937,8 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
     HIPCHK(w->w_blk.ensure((size_t)nblk + 1));
     HIPCHK(w->w_colf.ensure((size_t)nD + 1));
     HIPCHK(w->w_day.ensure((size_t)nC + 1));
+    const bool have_aer = a->aersw_tau != nullptr;
+    if (have_aer) HIPCHK(w->w_aer.ensure(sizeof(float4) * (size_t)CH * nL * B));
     unsigned char *blk = (unsigned char *)w->w_blk.p, *colf = (unsigned char *)w->w_colf.p;
     launch_fill_i32(w->err, 1, 0, st);
     launch_fill_i32((w->err + 1), 1, -1, st);
@@ -975,12 +983,14 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
         s.write_clr_zero = !a->doSWclrsky; s.role = 0;
         s.col_idx = a->idx; s.tau_abs = tabs; s.tau_ray = tray; s.mask = mask; s.copt = copt;
         s.blkflag = blk; s.aer_tau = a->aersw_tau; s.aer_ssa = a->aersw_ssa; s.aer_g = a->aersw_g;
+        s.aer4 = have_aer ? (const float4 *)w->w_aer.p : nullptr;
         s.coszen = a->coszen; s.alb_nir_dir = a->sfc_alb_nir_dir; s.alb_nir_dif = a->sfc_alb_nir_dif;
         s.alb_uv_dir = a->sfc_alb_uvvis_dir; s.alb_uv_dif = a->sfc_alb_uvvis_dif; s.p_lev = a->p_lev;
         s.alb_cat = c->alb_cat;
         s.up_all = a->fluxswUP_allsky; s.dn_all = a->fluxswDOWN_allsky; s.up_clr = a->fluxswUP_clrsky;
         s.dn_clr = a->fluxswDOWN_clrsky; s.sb = (double *)w->w_sb.p + (size_t)c0 * 3 * B; s.htr = a->htrsw;
         s.htr_clr = a->doSWclrsky ? a->htrswc : nullptr;
+        if (have_aer) launch_sw_aer_pack(s, nc, B, (float4 *)w->w_aer.p, st);
         { KScope k_(c, K_SWSOLVE, st); launch_sw_solve(T, s, nc, have_clouds, st); }
         if (a->doSWclrsky && have_clouds) {
             SwSolveArgs s2 = s;

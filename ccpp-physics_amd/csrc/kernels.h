@@ -31,6 +31,7 @@ struct LwSolveArgs {
     const float *ctau;     // [l][d][nband] cloud absorption depth (d = column of the launch) or null
     const float3 *copt;    // [l][d][nband] cloud (tau, co-albedo, g) for the scattering solver, or null
     const double *aer_tau; // Fortran (ncol_ld, nlay, nband) or null
+    const float *aer1;     // the chunk's aerosol depths packed by launch_lw_aer_pack, [d][l][nband] FP32, or null
     const double *t_lay, *t_lev, *tsfc, *semis, *p_lev;
     float D[MAX_ANG], W[MAX_ANG];
     double *flux_up, *flux_dn, *flux_up2, *flux_dn2, *flux_jac, *htr;  // Fortran (ncol_ld, nlev)
@@ -38,6 +39,9 @@ struct LwSolveArgs {
 void launch_lw_solve(const TablesView &T, const LwSolveArgs &A, int ncols, int nang, bool clouds, bool jac,
                      bool optang, hipStream_t st);
 size_t lw_solve_smem(int nlay, int nband);
+// Packs the aerosol depths of the chunk's columns (A.col0 ..) into
+// out[d][l][nband] in FP32, the form the LW solvers read.
+void launch_lw_aer_pack(const LwSolveArgs &A, int ncols, int nband, float *out, hipStream_t st);
 // the two-stream solver with scattering (doGP_lwscat): all-sky fluxes, no Jacobian
 void launch_lw_solve_2str(const TablesView &T, const LwSolveArgs &A, int ncols, bool clouds, hipStream_t st);
 size_t lw_solve_2str_smem(int nlay, int nband);
@@ -105,6 +109,9 @@ struct SwSolveArgs {
     const float3 *copt;                    // [l][d][nband] (tau, co-albedo 1 - ssa, g) or null
     const unsigned char *blkflag;
     const double *aer_tau, *aer_ssa, *aer_g;  // (ncol_ld, nlay, nband) or null
+    // the chunk's aerosol depths packed by launch_sw_aer_pack, [d][l][nband]:
+    // (tau, tau ssa, tau (1 - ssa), tau ssa g) in FP32; null without aerosols
+    const float4 *aer4;
     const double *coszen, *alb_nir_dir, *alb_nir_dif, *alb_uv_dir, *alb_uv_dif, *p_lev;
     const int *alb_cat;                    // [nband] 0 NIR, 1 average, 2 UV-VIS, 3 unset
     double *up_all, *dn_all, *up_clr, *dn_clr;  // (ncol_ld, nlev)
@@ -112,6 +119,9 @@ struct SwSolveArgs {
     double *htr, *htr_clr;                 // (ncol_ld, nlay) or null
 };
 void launch_sw_solve(const TablesView &T, const SwSolveArgs &A, int ncols, bool clouds, hipStream_t st);
+// Packs the aerosol optics of the chunk's columns (A.col0 .., A.col_idx) into
+// out[d][l][nband], the form the solver reads.
+void launch_sw_aer_pack(const SwSolveArgs &A, int ncols, int nband, float4 *out, hipStream_t st);
 size_t sw_solve_smem(int nlay, int nband);
 
 struct ScmpArgs {

@@ -129,7 +129,8 @@ __global__ __launch_bounds__(128) void lw_solve_2str_kernel(TablesView T, LwSolv
     const float3 *co_c = CLOUDS ? A.copt + (size_t)d * nb : nullptr;
     const unsigned co_ls = gridDim.x * (unsigned)nb;
     const unsigned mbit = 1u << (gg & 31), mword = (unsigned)gg >> 5;
-    const bool aer = A.aer_tau != nullptr;
+    const bool aer = A.aer1 != nullptr;
+    const float *ae_c = aer ? A.aer1 + (size_t)d * nlay * nb : nullptr;   // [l][band]
     const float EPS3 = 3.0f * 1.17549435e-38f;
 
     // One segment's layer terms.  Every load is issued before the first use,
@@ -140,12 +141,12 @@ __global__ __launch_bounds__(128) void lw_solve_2str_kernel(TablesView T, LwSolv
     auto seg_terms = [&](int s, Lw2Layer *o) {
         float tau[L2_SEG], pf[L2_SEG + 2], cx[L2_SEG], cy[L2_SEG], cz[L2_SEG];
         unsigned mw[L2_SEG];
-        double at[L2_SEG];
+        float at[L2_SEG];
         if (aer) {
 #pragma unroll
             for (int j = 0; j < L2_SEG; ++j) {
                 const int l = lay(min(s * L2_SEG + j, nlay - 1));
-                at[j] = A.aer_tau[(size_t)c + (size_t)ld * (l + (size_t)nlay * b)];
+                at[j] = ldu(ae_c, (unsigned)l * nb + b);
             }
         }
 #pragma unroll
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(128) void lw_solve_2str_kernel(TablesView T, LwSolv
             // increment_2stream_by_1scalar: the clear sky (gas + aerosol,
             // absorbing) added to the sampled cloud
             float tcl = tau[j];
-            if (aer) tcl += (float)at[j];
+            if (aer) tcl += at[j];
             float tc = 0.0f, pc = 1.0f, gc = 0.0f;
             if (CLOUDS) {
                 const bool m = mw[j] & mbit;

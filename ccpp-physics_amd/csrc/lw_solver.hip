@@ -52,7 +52,7 @@ size_t lw_solve_smem(int nlay, int nband)
     return sizeof(double) * 3 * (size_t)(nlay + 1) + sizeof(float) * (size_t)nband * (2 * (size_t)nlay + 3);
 }
 
-template <int NANG, bool CLOUDS, bool JAC, bool OPTANG>
+template <int NANG, bool CLOUDS, bool JAC, bool OPTANG, bool AER>
 __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs A)
 {
     extern __shared__ double red[];
@@ -103,20 +103,21 @@ __global__ __launch_bounds__(128) void lw_solve_kernel(TablesView T, LwSolveArgs
     const float *ct_c = CLOUDS ? A.ctau + (size_t)d * nb : nullptr;
     const unsigned ct_ls = gridDim.x * (unsigned)nb;
     const unsigned mbit = 1u << (gg & 31), mword = (unsigned)gg >> 5;
+    const float *ae_c = AER ? A.aer1 + (size_t)d * nlay * nb : nullptr;   // [l][band]
 
     // Per-layer inputs, loaded for LW_BATCH layers at a time with no load
     // depending on another (the cloud depth is fetched unconditionally and
     // selected by the mask bit), so a batch costs one memory latency.
-    struct LayerIn { float tau, pf, ct; unsigned mw; double aer; };
+    struct LayerIn { float tau, pf, ct, aer; unsigned mw; };
     auto load_layer = [&](int l, LayerIn &x) {
         x.tau = ldu(tau_c, (unsigned)l * ngpt + gg);
         x.pf = ldu(pf_c, (unsigned)l * ngpt + gg);
-        x.aer = A.aer_tau ? A.aer_tau[(size_t)c + (size_t)ld * (l + (size_t)nlay * b)] : 0.0;
+        x.aer = AER ? ldu(ae_c, (unsigned)l * nb + b) : 0.0f;
         if (CLOUDS) { x.mw = ldu(mk_c, (unsigned)l * MASK_WORDS + mword); x.ct = ldu(ct_c, (unsigned)l * ct_ls + b); }
     };
     auto layer_tau = [&](const LayerIn &x) -> float {
         float t = x.tau;
-        if (A.aer_tau) t += (float)x.aer;
+        if (AER) t += x.aer;
         if (CLOUDS) { if (x.mw & mbit) t += x.ct; }
         return t;
     };
@@ -261,7 +262,10 @@ template <int NANG, bool CLOUDS, bool JAC, bool OPTANG>
 static void launch_one(const TablesView &T, const LwSolveArgs &A, int ncols, hipStream_t st)
 {
     const size_t smem = lw_solve_smem(A.nlay, T.nband);
-    hipLaunchKernelGGL((lw_solve_kernel<NANG, CLOUDS, JAC, OPTANG>), dim3(ncols), dim3(128), smem, st, T, A);
+    if (A.aer1)
+        hipLaunchKernelGGL((lw_solve_kernel<NANG, CLOUDS, JAC, OPTANG, true>), dim3(ncols), dim3(128), smem, st, T, A);
+    else
+        hipLaunchKernelGGL((lw_solve_kernel<NANG, CLOUDS, JAC, OPTANG, false>), dim3(ncols), dim3(128), smem, st, T, A);
 }
 
 template <int NANG>
@@ -346,6 +350,33 @@ void launch_heating_rate_day(int ncol, int nlay, const unsigned char *day, const
     if (!n) return;
     hipLaunchKernelGGL(heating_rate_day_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ncol, nlay,
                        day, up, dn, p_lev, htr);
+}
+
+// Aerosol depths of a chunk in the solvers' order.  The CCPP array is
+// (ncol, nlay, nband) with the column fastest; the solvers read, per layer,
+// the column's nband values together.  Block = (64 columns, one layer): the
+// FP64 reads run along the columns, the FP32 values go out through LDS as
+// each column's contiguous run of nband values.
+__global__ __launch_bounds__(256) void lw_aer_pack_kernel(LwSolveArgs A, int ncols, int nb, float *out)
+{
+    extern __shared__ float tile1[];   // [64 columns][nb]
+    const int d0 = blockIdx.x * 64, l = blockIdx.y, nlay = A.nlay, ld = A.ncol_ld;
+    for (int i = threadIdx.x; i < 64 * nb; i += blockDim.x) {
+        const int b = i >> 6, dd = i & 63, d = d0 + dd;
+        if (d < ncols) tile1[dd * nb + b] = (float)A.aer_tau[(size_t)(A.col0 + d) + (size_t)ld * (l + (size_t)nlay * b)];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 64 * nb; i += blockDim.x) {
+        const int dd = i / nb, b = i - dd * nb, d = d0 + dd;
+        if (d < ncols) out[((size_t)d * nlay + l) * nb + b] = tile1[i];
+    }
+}
+
+void launch_lw_aer_pack(const LwSolveArgs &A, int ncols, int nband, float *out, hipStream_t st)
+{
+    if (ncols <= 0 || A.nlay <= 0) return;
+    hipLaunchKernelGGL(lw_aer_pack_kernel, dim3((ncols + 63) / 64, A.nlay), dim3(256),
+                       sizeof(float) * 64 * (size_t)nband, st, A, ncols, nband, out);
 }
 
 }  // namespace ccpprad

@@ -211,6 +211,7 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     const unsigned *mk_c = CLOUDS ? A.mask + (size_t)d * nlay * MASK_WORDS : nullptr;
     // cloud records [layer][column][band]; the grid has one block per column
     const float3 *co_c = CLOUDS ? A.copt + (size_t)d * nb : nullptr;
+    const float4 *ae_c = AER ? A.aer4 + (size_t)d * nlay * nb : nullptr;   // [l][band]
     const unsigned co_ls = gridDim.x * (unsigned)nb;
     const unsigned mbit = 1u << (gg & 31), mword = (unsigned)gg >> 5;
 
@@ -250,7 +251,8 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     // read the last layer again (their results are not used), and cloud
     // values are selected by the mask bit rather than loaded under it.
     auto load_seg = [&](int sg, Opt2s *o, auto cl) {
-        float ta[SW_SEG], tr[SW_SEG], at[SW_SEG], aw[SW_SEG], ap[SW_SEG], ag[SW_SEG];
+        float ta[SW_SEG], tr[SW_SEG];
+        float4 ae[SW_SEG];
         unsigned mw[SW_SEG];
         float3 cv[SW_SEG];
 #pragma unroll
@@ -258,11 +260,7 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
             const int l = lay(min(sg * SW_SEG + j, nlay - 1));
             const unsigned og = (unsigned)l * ngpt + gg;
             ta[j] = ldu(ta_c, og); tr[j] = ldu(tr_c, og);
-            if (AER) {
-                const size_t q = (size_t)c + (size_t)ld * (l + (size_t)nlay * b);
-                const double sd = A.aer_ssa[q];
-                at[j] = (float)A.aer_tau[q]; aw[j] = (float)sd; ap[j] = (float)(1.0 - sd); ag[j] = (float)A.aer_g[q];
-            }
+            if (AER) ae[j] = ldu(ae_c, (unsigned)l * nb + b);
             if (cl.value) {
                 mw[j] = ldu(mk_c, (unsigned)l * MASK_WORDS + mword);
                 cv[j] = ldu(co_c, (unsigned)l * co_ls + b);
@@ -271,10 +269,7 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
 #pragma unroll
         for (int j = 0; j < SW_SEG; ++j) {
             float te = ta[j] + tr[j], ts = tr[j], tab = ta[j], tsg = 0.0f;
-            if (AER) {
-                const float s1 = at[j] * aw[j];
-                te = at[j] + te; ts += s1; tab += at[j] * ap[j]; tsg += s1 * ag[j];
-            }
+            if (AER) { te = ae[j].x + te; ts += ae[j].y; tab += ae[j].z; tsg += ae[j].w; }
             if (cl.value) {
                 const bool m = mw[j] & mbit;
                 const float tc = m ? cv[j].x : 0.0f, pc = m ? cv[j].y : 0.0f, gc = m ? cv[j].z : 0.0f;
@@ -449,7 +444,7 @@ void launch_sw_solve(const TablesView &T, const SwSolveArgs &A, int ncols, bool 
 {
     if (ncols <= 0) return;
     const size_t smem = sw_solve_smem(A.nlay, T.nband);
-    const bool aer = A.aer_tau != nullptr;
+    const bool aer = A.aer4 != nullptr;
     if (clouds && aer)
         hipLaunchKernelGGL((sw_solve_kernel<true, true>), dim3(ncols), dim3(128), smem, st, T, A);
     else if (clouds)
@@ -532,6 +527,42 @@ void launch_scmpsw(const ScmpArgs &A, int ncol, unsigned char *daymap, int *last
     hipLaunchKernelGGL(scmpsw_kernel, dim3((A.ndays + 255) / 256), dim3(256), 0, st, A);
     hipLaunchKernelGGL(daymap_kernel, dim3((A.ndays + 255) / 256), dim3(256), 0, st, A.ndays, A.col_idx, daymap);
     hipLaunchKernelGGL(scmpsw_night_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, ncol, daymap, last_wipe, A.scmpsw);
+}
+
+// Aerosol optics of a chunk in the solver's order.  The CCPP arrays are
+// (ncol, nlay, nband) with the column fastest; the solver reads, per layer,
+// the column's nband values together.  Block = (64 columns, one layer): the
+// FP64 reads run along the columns, the FP32 records go out through LDS as
+// each column's contiguous run of nband records.  A record holds the
+// aerosol's increments to the extinction, scattering, absorption and
+// scattering x asymmetry depths (increment_2stream_by_2stream).
+__global__ __launch_bounds__(256) void sw_aer_pack_kernel(SwSolveArgs A, int ncols, int nb, float4 *out)
+{
+    extern __shared__ float4 tile[];   // [64 columns][nb]
+    const int d0 = blockIdx.x * 64, l = blockIdx.y, nlay = A.nlay, ld = A.ncol_ld;
+    for (int i = threadIdx.x; i < 64 * nb; i += blockDim.x) {
+        const int b = i >> 6, dd = i & 63, d = d0 + dd;
+        if (d >= ncols) continue;
+        const int od = A.col0 + d;
+        const int c = A.col_idx ? A.col_idx[od] - 1 : od;
+        const size_t q = (size_t)c + (size_t)ld * (l + (size_t)nlay * b);
+        const double sd = A.aer_ssa[q];
+        const float at = (float)A.aer_tau[q], aw = (float)sd, ap = (float)(1.0 - sd), ag = (float)A.aer_g[q];
+        const float s1 = at * aw;
+        tile[dd * nb + b] = make_float4(at, s1, at * ap, s1 * ag);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 64 * nb; i += blockDim.x) {
+        const int dd = i / nb, b = i - dd * nb, d = d0 + dd;
+        if (d < ncols) out[((size_t)d * nlay + l) * nb + b] = tile[i];
+    }
+}
+
+void launch_sw_aer_pack(const SwSolveArgs &A, int ncols, int nband, float4 *out, hipStream_t st)
+{
+    if (ncols <= 0 || A.nlay <= 0) return;
+    hipLaunchKernelGGL(sw_aer_pack_kernel, dim3((ncols + 63) / 64, A.nlay), dim3(256),
+                       sizeof(float4) * 64 * (size_t)nband, st, A, ncols, nband, out);
 }
 
 }  // namespace ccpprad

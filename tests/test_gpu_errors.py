@@ -99,6 +99,19 @@ def test_null_required_array(kd_lw, lut_lw, missing):
         _run(s, arr, ints, True)
 
 
+@pytest.mark.parametrize("missing", ["aersw_ssa", "aersw_g"])
+def test_sw_aerosols_all_or_none(kd_sw, lut_sw, missing):
+    """SW aerosol optics come as the three fields together: one left out is
+    an argument error, not a read through a NULL pointer."""
+    from ccpprad import capi
+    s, arr, ints = _sw_setup(kd_sw, lut_sw)
+    shp = (kd_sw.nband, ints["nLay"], ints["nCol"])
+    arr.update(aersw_tau=np.full(shp, 0.01), aersw_ssa=np.full(shp, 0.9), aersw_g=np.full(shp, 0.6))
+    del arr[missing]
+    with pytest.raises(capi.CcppradError, match="aersw_tau, aersw_ssa and aersw_g"):
+        _run(s, arr, ints, False)
+
+
 def test_sw_null_output(kd_sw, lut_sw):
     from ccpprad import capi, device
     s, arr, ints = _sw_setup(kd_sw, lut_sw)
