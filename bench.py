@@ -41,13 +41,13 @@ METRIC = "LW+SW columns/s at 127 layers (table vs NN) @ HR RMSE<=0.1 K/day vs FP
 # --no-nn --no-e2e --no-callers --copy-probe`), each scaled by the reading of a
 # 1 GiB device copy (FETCH x 2.0, WRITE x 1.0).  Counters cannot be read from inside this
 # process, so the last measurement of the current kernels is recorded here.
-TRAFFIC_GB = {"sw_sw_solve": 19.393, "lw_lw_solve": 22.659, "lw_gas_optics": 9.131, "sw_gas_optics": 8.070,
-              "lw_cloud_optics": 0.775, "sw_cloud_optics": 0.775, "lw_mcica": 0.670, "sw_mcica": 0.670,
-              "lw_solve_2str": 24.217}
+TRAFFIC_GB = {"sw_sw_solve": 19.365, "lw_lw_solve": 22.656, "lw_gas_optics": 9.131, "sw_gas_optics": 8.070,
+              "lw_cloud_optics": 0.756, "sw_cloud_optics": 0.756, "lw_mcica": 0.670, "sw_mcica": 0.670,
+              "lw_solve_2str": 24.259}
 # sha256 (first 16 hex digits) of the kernel sources the figures above were
 # measured with (kernel_source_sha()); on any other sources the bench line
 # reports traffic null rather than a stale figure.
-TRAFFIC_SRC_SHA = "de4137567f5ab191"
+TRAFFIC_SRC_SHA = "7e81b9ed81469774"
 
 
 # Algorithmic bytes per column of each stage: the compulsory I/O of the
@@ -110,6 +110,7 @@ def parse():
                    help="trained emulator weights (.npz); default: the packaged ones")
     p.add_argument("--no-nn", action="store_true", help="skip the learned LW gas-optics leg")
     p.add_argument("--no-lwscat", action="store_true", help="skip the LW cloud-scattering (doGP_lwscat) leg")
+    p.add_argument("--no-aer", action="store_true", help="skip the leg with aerosol optics")
     p.add_argument("--no-callers", action="store_true",
                    help="skip the leg timing the device callers (pre, geometry, overlap, post)")
     p.add_argument("--no-e2e", action="store_true",
@@ -410,6 +411,58 @@ def main():
                "kernel_ms_per_step": skt,
                "solver": "two-stream adding with cloud scattering (all sky), no-scattering clear sky off"}
 
+    # ---- the same step with aerosol optics, which GFS hands both schemes on
+    # every call (per band tau, ssa, g of every column, FP64, generated on
+    # the device); the library packs them per launch before the solvers
+    aer = None
+    if not a.no_aer:
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(20261204)
+
+        def afields(pre, nb):
+            shp = (nb, NLAY, ncol)
+
+            def u(lo, hi):
+                return torch.rand(shp, generator=gen, dtype=torch.float64, device="cuda") * (hi - lo) + lo
+            return {pre + "tau": torch.pow(10.0, u(-5.0, -1.0)), pre + "ssa": u(0.8, 1.0), pre + "g": u(0.5, 0.8)}
+        # own outputs: the table-path results are checked against the oracle below
+        lwa = dict(dev)
+        lwa.update(device.lw_outputs(ncol, NLAY))
+        lwa.update(afields("aerlw_", kd_lw.nband))
+        swa = dict(dev)
+        swa.update(device.sw_outputs(ncol, NLAY))
+        swa.update(afields("aersw_", kd_sw.nband))
+
+        def step_aer(validate=False, serial=False):
+            lw.run_dev(lw_ints, lwa, stream=sp, validate=validate)
+            sw.run_dev(sw_ints, swa, stream=sp if serial else sp_sw, validate=validate)
+
+        step_aer(validate=True)
+        for _ in range(a.warmup):
+            step_aer()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            step_aer()
+        torch.cuda.synchronize()
+        ms_aer = (time.perf_counter() - t0) * 1e3 / a.steps
+        for s in (lw, sw):
+            L.ccpprad_timing_enable(s.ctx, 1)
+        step_aer(serial=True)
+        akt = {}
+        for tag, s in (("lw", lw), ("sw", sw)):
+            for kid, nm in {**names, 7: "aer_pack"}.items():
+                tot, cnt = C.c_double(), C.c_int()
+                L.ccpprad_timing_query(s.ctx, kid, C.byref(tot), C.byref(cnt))
+                if cnt.value:
+                    akt[f"{tag}_{nm}"] = tot.value
+            L.ccpprad_timing_enable(s.ctx, 0)
+        aer = {"value": ncol / (ms_aer / 1e3), "unit": "columns/s", "ms_per_step": ms_aer,
+               "kernel_ms_per_step": akt,
+               "aerosols": "LW and SW, every column and band: tau 1e-5..1e-1 log-uniform, ssa 0.8..1, g 0.5..0.8"}
+        print(f"# with aerosols: {ms_aer:.3f} ms/step", file=sys.stderr, flush=True)
+        del lwa, swa
+
     callers = None if a.no_callers else time_callers(a, torch, lw, dev, ncol)
 
     # ---- end to end through the host-pointer entries (the ones the Fortran
@@ -570,7 +623,7 @@ def main():
                                f"FP32 optics/transport, FP64 reductions and heating rate)",
                    "ncol": ncol, "nlay": NLAY, "path": "table", "blksz": blksz},
         "roofline": roof, "cpu_baseline": cpu, "accuracy_vs_fp64_oracle": acc, "nn": nn,
-        "lw_scattering": lws, "callers": callers, "end_to_end": e2e,
+        "lw_scattering": lws, "with_aerosols": aer, "callers": callers, "end_to_end": e2e,
         "library": {"path": os.path.relpath(lib_path, REPO) if lib_path.startswith(REPO) else lib_path,
                     "sha256_16": file_sha(lib_path)},
     }

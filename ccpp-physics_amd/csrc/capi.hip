@@ -134,7 +134,7 @@ static Workspace *ws_for_stream(ccpprad_ctx *c, hipStream_t st)
     return w;
 }
 
-enum { K_GAS = 0, K_LWSOLVE = 1, K_SWSOLVE = 2, K_CLOUD = 3, K_MCICA = 4, K_MLP = 5, K_GASFB = 6, K_N = 7 };
+enum { K_GAS = 0, K_LWSOLVE = 1, K_SWSOLVE = 2, K_CLOUD = 3, K_MCICA = 4, K_MLP = 5, K_GASFB = 6, K_AER = 7, K_N = 8 };
 
 static hipEvent_t ev_take(ccpprad_ctx *c)
 {
@@ -844,7 +844,7 @@ extern "C" int ccpprad_lw_run_dev(ccpprad_ctx *c, const ccpprad_lw_args *a, void
         s.flux_up2 = a->fluxlwUP_radtime; s.flux_dn2 = a->fluxlwDOWN_radtime;
         s.flux_jac = a->use_LW_jacobian ? a->fluxlwUP_jac : nullptr;
         s.htr = a->htrlw;
-        if (a->aerlw_tau) launch_lw_aer_pack(s, nc, B, (float *)w->w_aer.p, st);
+        if (a->aerlw_tau) { KScope k_(c, K_AER, st); launch_lw_aer_pack(s, nc, B, (float *)w->w_aer.p, st); }
         if (scat) {
             KScope k_(c, K_LWSOLVE, st);
             launch_lw_solve_2str(T, s, nc, have_clouds, st);
@@ -990,7 +990,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
         s.up_all = a->fluxswUP_allsky; s.dn_all = a->fluxswDOWN_allsky; s.up_clr = a->fluxswUP_clrsky;
         s.dn_clr = a->fluxswDOWN_clrsky; s.sb = (double *)w->w_sb.p + (size_t)c0 * 3 * B; s.htr = a->htrsw;
         s.htr_clr = a->doSWclrsky ? a->htrswc : nullptr;
-        if (have_aer) launch_sw_aer_pack(s, nc, B, (float4 *)w->w_aer.p, st);
+        if (have_aer) { KScope k_(c, K_AER, st); launch_sw_aer_pack(s, nc, B, (float4 *)w->w_aer.p, st); }
         { KScope k_(c, K_SWSOLVE, st); launch_sw_solve(T, s, nc, have_clouds, st); }
         if (a->doSWclrsky && have_clouds) {
             SwSolveArgs s2 = s;

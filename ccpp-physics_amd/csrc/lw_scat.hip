@@ -78,8 +78,12 @@ size_t lw_solve_2str_smem(int nlay, int nband)
            sizeof(float) * 2 * (size_t)l2_nckpt(nseg) * 128;
 }
 
+// At 127 layers a block takes 25.6 KB of LDS: 6 blocks per CU, 3 wavefronts
+// per SIMD, the residency the waves-per-EU attribute gives the compiler
+// (the LDS size is known only at launch).
 template <bool CLOUDS>
-__global__ __launch_bounds__(128) void lw_solve_2str_kernel(TablesView T, LwSolveArgs A)
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3, 3))) void lw_solve_2str_kernel(TablesView T,
+                                                                                                        LwSolveArgs A)
 {
     extern __shared__ double red[];
     const int d = blockIdx.x;

@@ -457,7 +457,8 @@ int ccpprad_lw_set_gas_optics_mlp(ccpprad_ctx *ctx, ccpprad_mlp *m);
 
 /* Per-kernel device timing with HIP events recorded on the launch stream.
  * Kernel ids: 0 gas optics, 1 LW solver, 2 SW solver, 3 cloud optics,
- * 4 McICA, 5 MLP inference, 6 gas optics of the emulator's fallback rows.
+ * 4 McICA, 5 MLP inference, 6 gas optics of the emulator's fallback rows,
+ * 7 packing of the aerosol optics.
  * enable != 0 starts recording (and clears). */
 int ccpprad_timing_enable(ccpprad_ctx *ctx, int enable);
 /* Synchronises the recorded events; total milliseconds and launch count. */
