@@ -153,12 +153,16 @@ def test_lw_host_entry_matches_device_entry(kd_lw, lut_lw):
         assert np.array_equal(host["fluxlwUP_jac"], sentinel["fluxlwUP_jac"])
 
 
-def test_lw_aerosols(kd_lw, lut_lw):
-    """Aerosol depths by band added to the clear and all sky (1scl += 1scl)."""
-    ref, out = _run_both(kd_lw, lut_lw, 128, 127, clouds=True, nangle=3, clrsky=True, jac=False, blksz=64,
+@pytest.mark.parametrize("nangle,jac", [(1, False), (2, True), (3, False), (4, True)])
+def test_lw_aerosols(kd_lw, lut_lw, nangle, jac):
+    """Aerosol depths by band added to the clear and all sky (1scl += 1scl),
+    for each angle count, with and without the surface Jacobian (the
+    solver's aerosol instantiations; one angle also runs the clear sky at
+    the optimal angle)."""
+    ref, out = _run_both(kd_lw, lut_lw, 128, 127, clouds=True, nangle=nangle, clrsky=True, jac=jac, blksz=64,
                          aer=True)
-    _check(ref, out, ["fluxlwUP_allsky", "fluxlwDOWN_allsky", "fluxlwUP_clrsky", "fluxlwDOWN_clrsky"],
-           ("htrlw", "htrlwc"))
+    _check(ref, out, ["fluxlwUP_allsky", "fluxlwDOWN_allsky", "fluxlwUP_clrsky", "fluxlwDOWN_clrsky"]
+           + (["fluxlwUP_jac"] if jac else []), ("htrlw", "htrlwc"))
 
 
 def test_lw_top_at_1(kd_lw, lut_lw):
