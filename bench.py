@@ -111,6 +111,8 @@ def parse():
     p.add_argument("--no-nn", action="store_true", help="skip the learned LW gas-optics leg")
     p.add_argument("--no-lwscat", action="store_true", help="skip the LW cloud-scattering (doGP_lwscat) leg")
     p.add_argument("--no-aer", action="store_true", help="skip the leg with aerosol optics")
+    p.add_argument("--launch-columns", type=int, default=65536,
+                   help="columns per kernel launch (the library's default and its maximum is 65,536)")
     p.add_argument("--no-callers", action="store_true",
                    help="skip the leg timing the device callers (pre, geometry, overlap, post)")
     p.add_argument("--no-e2e", action="store_true",
@@ -240,6 +242,7 @@ def main():
     pr, arr, dev = build_inputs(kd_lw, kd_sw, ncol, torch)
     lw = lib.LWScheme(kd_lw, lut_lw, NLAY, blksz=blksz)
     sw = lib.SWScheme(kd_sw, lut_sw, NLAY, blksz=blksz)
+    launch = min(lw.set_launch_columns(a.launch_columns), sw.set_launch_columns(a.launch_columns))
     lw_out = device.lw_outputs(ncol, NLAY)
     sw_out = device.sw_outputs(ncol, NLAY)
     lw_args = dict(dev)
@@ -286,7 +289,7 @@ def main():
             if cnt.value:
                 kt[f"{tag}_{nm}"] = (tot.value, cnt.value)
         L.ccpprad_timing_enable(s.ctx, 0)
-    chunk = min(ncol, 65536)   # one launch processes a chunk of 65,536 columns at most
+    chunk = min(ncol, launch)   # columns per launch
     per_col = ALG_BYTES_PER_COLUMN
     src_sha = kernel_source_sha()
     dom = max(kt.items(), key=lambda kv: kv[1][0])
@@ -314,6 +317,7 @@ def main():
         if os.path.exists(wpath):
             emu = mlp.LWEmulator.load(wpath)
             lw_nn = lib.LWScheme(kd_lw, lut_lw, NLAY, blksz=blksz)
+            lw_nn.set_launch_columns(launch)
             emu.attach(lw_nn)
             nn_out = device.lw_outputs(ncol, NLAY)
             nn_args = dict(dev)
@@ -380,6 +384,7 @@ def main():
     lws_out = None
     if not a.no_lwscat:
         lw_s = lib.LWScheme(kd_lw, lut_lw, NLAY, blksz=blksz)
+        lw_s.set_launch_columns(launch)
         lws_out = device.lw_outputs(ncol, NLAY)
         s_args = dict(dev)
         s_args.update(lws_out)
