@@ -938,7 +938,7 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
     HIPCHK(w->w_colf.ensure((size_t)nD + 1));
     HIPCHK(w->w_day.ensure((size_t)nC + 1));
     const bool have_aer = a->aersw_tau != nullptr;
-    if (have_aer) HIPCHK(w->w_aer.ensure(sizeof(float4) * (size_t)CH * nL * B));
+    if (have_aer) HIPCHK(w->w_aer.ensure(sizeof(float3) * (size_t)CH * nL * B));
     unsigned char *blk = (unsigned char *)w->w_blk.p, *colf = (unsigned char *)w->w_colf.p;
     launch_fill_i32(w->err, 1, 0, st);
     launch_fill_i32((w->err + 1), 1, -1, st);
@@ -983,14 +983,14 @@ extern "C" int ccpprad_sw_run_dev(ccpprad_ctx *c, const ccpprad_sw_args *a, void
         s.write_clr_zero = !a->doSWclrsky; s.role = 0;
         s.col_idx = a->idx; s.tau_abs = tabs; s.tau_ray = tray; s.mask = mask; s.copt = copt;
         s.blkflag = blk; s.aer_tau = a->aersw_tau; s.aer_ssa = a->aersw_ssa; s.aer_g = a->aersw_g;
-        s.aer4 = have_aer ? (const float4 *)w->w_aer.p : nullptr;
+        s.aer4 = have_aer ? (const float3 *)w->w_aer.p : nullptr;
         s.coszen = a->coszen; s.alb_nir_dir = a->sfc_alb_nir_dir; s.alb_nir_dif = a->sfc_alb_nir_dif;
         s.alb_uv_dir = a->sfc_alb_uvvis_dir; s.alb_uv_dif = a->sfc_alb_uvvis_dif; s.p_lev = a->p_lev;
         s.alb_cat = c->alb_cat;
         s.up_all = a->fluxswUP_allsky; s.dn_all = a->fluxswDOWN_allsky; s.up_clr = a->fluxswUP_clrsky;
         s.dn_clr = a->fluxswDOWN_clrsky; s.sb = (double *)w->w_sb.p + (size_t)c0 * 3 * B; s.htr = a->htrsw;
         s.htr_clr = a->doSWclrsky ? a->htrswc : nullptr;
-        if (have_aer) { KScope k_(c, K_AER, st); launch_sw_aer_pack(s, nc, B, (float4 *)w->w_aer.p, st); }
+        if (have_aer) { KScope k_(c, K_AER, st); launch_sw_aer_pack(s, nc, B, (float3 *)w->w_aer.p, st); }
         { KScope k_(c, K_SWSOLVE, st); launch_sw_solve(T, s, nc, have_clouds, st); }
         if (a->doSWclrsky && have_clouds) {
             SwSolveArgs s2 = s;

@@ -110,8 +110,9 @@ struct SwSolveArgs {
     const unsigned char *blkflag;
     const double *aer_tau, *aer_ssa, *aer_g;  // (ncol_ld, nlay, nband) or null
     // the chunk's aerosol depths packed by launch_sw_aer_pack, [d][l][nband]:
-    // (tau, tau ssa, tau (1 - ssa), tau ssa g) in FP32; null without aerosols
-    const float4 *aer4;
+    // (tau ssa, tau (1 - ssa), tau ssa g) in FP32, the extinction being the
+    // sum of the first two; null without aerosols
+    const float3 *aer4;
     const double *coszen, *alb_nir_dir, *alb_nir_dif, *alb_uv_dir, *alb_uv_dif, *p_lev;
     const int *alb_cat;                    // [nband] 0 NIR, 1 average, 2 UV-VIS, 3 unset
     double *up_all, *dn_all, *up_clr, *dn_clr;  // (ncol_ld, nlev)
@@ -121,7 +122,7 @@ struct SwSolveArgs {
 void launch_sw_solve(const TablesView &T, const SwSolveArgs &A, int ncols, bool clouds, hipStream_t st);
 // Packs the aerosol optics of the chunk's columns (A.col0 .., A.col_idx) into
 // out[d][l][nband], the form the solver reads.
-void launch_sw_aer_pack(const SwSolveArgs &A, int ncols, int nband, float4 *out, hipStream_t st);
+void launch_sw_aer_pack(const SwSolveArgs &A, int ncols, int nband, float3 *out, hipStream_t st);
 size_t sw_solve_smem(int nlay, int nband);
 
 struct ScmpArgs {

@@ -211,7 +211,7 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     const unsigned *mk_c = CLOUDS ? A.mask + (size_t)d * nlay * MASK_WORDS : nullptr;
     // cloud records [layer][column][band]; the grid has one block per column
     const float3 *co_c = CLOUDS ? A.copt + (size_t)d * nb : nullptr;
-    const float4 *ae_c = AER ? A.aer4 + (size_t)d * nlay * nb : nullptr;   // [l][band]
+    const float3 *ae_c = AER ? A.aer4 + (size_t)d * nlay * nb : nullptr;   // [l][band]
     const unsigned co_ls = gridDim.x * (unsigned)nb;
     const unsigned mbit = 1u << (gg & 31), mword = (unsigned)gg >> 5;
 
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     // values are selected by the mask bit rather than loaded under it.
     auto load_seg = [&](int sg, Opt2s *o, auto cl) {
         float ta[SW_SEG], tr[SW_SEG];
-        float4 ae[SW_SEG];
+        float3 ae[SW_SEG];
         unsigned mw[SW_SEG];
         float3 cv[SW_SEG];
 #pragma unroll
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
 #pragma unroll
         for (int j = 0; j < SW_SEG; ++j) {
             float te = ta[j] + tr[j], ts = tr[j], tab = ta[j], tsg = 0.0f;
-            if (AER) { te = ae[j].x + te; ts += ae[j].y; tab += ae[j].z; tsg += ae[j].w; }
+            if (AER) { te = (ae[j].x + ae[j].y) + te; ts += ae[j].x; tab += ae[j].y; tsg += ae[j].z; }
             if (cl.value) {
                 const bool m = mw[j] & mbit;
                 const float tc = m ? cv[j].x : 0.0f, pc = m ? cv[j].y : 0.0f, gc = m ? cv[j].z : 0.0f;
@@ -534,11 +534,12 @@ void launch_scmpsw(const ScmpArgs &A, int ncol, unsigned char *daymap, int *last
 // the column's nband values together.  Block = (64 columns, one layer): the
 // FP64 reads run along the columns, the FP32 records go out through LDS as
 // each column's contiguous run of nband records.  A record holds the
-// aerosol's increments to the extinction, scattering, absorption and
-// scattering x asymmetry depths (increment_2stream_by_2stream).
-__global__ __launch_bounds__(256) void sw_aer_pack_kernel(SwSolveArgs A, int ncols, int nb, float4 *out)
+// aerosol's increments to the scattering, absorption and scattering x
+// asymmetry depths (increment_2stream_by_2stream); their first two add up
+// to its extinction depth.
+__global__ __launch_bounds__(256) void sw_aer_pack_kernel(SwSolveArgs A, int ncols, int nb, float3 *out)
 {
-    extern __shared__ float4 tile[];   // [64 columns][nb]
+    extern __shared__ float3 tile[];   // [64 columns][nb]
     const int d0 = blockIdx.x * 64, l = blockIdx.y, nlay = A.nlay, ld = A.ncol_ld;
     for (int i = threadIdx.x; i < 64 * nb; i += blockDim.x) {
         const int b = i >> 6, dd = i & 63, d = d0 + dd;
@@ -549,7 +550,7 @@ __global__ __launch_bounds__(256) void sw_aer_pack_kernel(SwSolveArgs A, int nco
         const double sd = A.aer_ssa[q];
         const float at = (float)A.aer_tau[q], aw = (float)sd, ap = (float)(1.0 - sd), ag = (float)A.aer_g[q];
         const float s1 = at * aw;
-        tile[dd * nb + b] = make_float4(at, s1, at * ap, s1 * ag);
+        tile[dd * nb + b] = make_float3(s1, at * ap, s1 * ag);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < 64 * nb; i += blockDim.x) {
@@ -558,11 +559,11 @@ __global__ __launch_bounds__(256) void sw_aer_pack_kernel(SwSolveArgs A, int nco
     }
 }
 
-void launch_sw_aer_pack(const SwSolveArgs &A, int ncols, int nband, float4 *out, hipStream_t st)
+void launch_sw_aer_pack(const SwSolveArgs &A, int ncols, int nband, float3 *out, hipStream_t st)
 {
     if (ncols <= 0 || A.nlay <= 0) return;
     hipLaunchKernelGGL(sw_aer_pack_kernel, dim3((ncols + 63) / 64, A.nlay), dim3(256),
-                       sizeof(float4) * 64 * (size_t)nband, st, A, ncols, nband, out);
+                       sizeof(float3) * 64 * (size_t)nband, st, A, ncols, nband, out);
 }
 
 }  // namespace ccpprad
