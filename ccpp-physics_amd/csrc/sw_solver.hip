@@ -179,6 +179,10 @@ __global__ __launch_bounds__(128, 4) void sw_solve_kernel(TablesView T, SwSolveA
     const int gg = act ? g : 0;
     const int b = T.g2b[gg];
     const bool cloudy = CLOUDS && A.blkflag[od / A.blksz];
+    // the clear-sky pass after an all-sky one: a block without clouds already
+    // has its clear-sky fluxes, band sums and heating rate from the all-sky
+    // pass (the same clear-segment arithmetic, so the same bits)
+    if (A.role == 1 && A.blkflag && !A.blkflag[od / A.blksz]) return;
     const int top = A.top_at_1;
     const int lev_toa = top ? 0 : nlay;
     // level and layer sums of both wavefronts: each slot gets exactly two

@@ -145,3 +145,41 @@ def test_sw_no_daylit_columns(kd_sw, lut_sw):
         assert not got[n].any() and not host[n].any() and not ref[n].any(), n
     for d in (got, host, ref):
         assert (d["cldtausw"] == 5.0).all()
+
+
+@pytest.mark.parametrize("aer", [False, True])
+def test_sw_clear_blocks_with_clrsky(kd_sw, lut_sw, aer):
+    """Blocks without clouds next to cloudy ones, with clear-sky fluxes on: the
+    clear-sky pass skips the cloud-free blocks, whose clear-sky values the
+    all-sky pass wrote; all against the oracle, and all sky equal to clear
+    sky bit for bit in those blocks."""
+    from ccpprad import synthetic as syn, capi, lib, device
+    from oracle import oracle as O
+    ncol, nlay, blk = 256, 127, 64
+    arr = capi.profile_fields(syn.make_profiles(ncol, nlay, seed=41, clouds=True, kd=kd_sw))
+    clear = np.r_[blk:2 * blk, 3 * blk:4 * blk]
+    arr["cld_frac"][:, clear] = 0.0
+    if aer:
+        rng = np.random.default_rng(42)
+        shp = (kd_sw.nband, nlay, ncol)
+        arr.update(aersw_tau=10 ** rng.uniform(-5, -1, shp), aersw_ssa=rng.uniform(0.8, 1.0, shp),
+                   aersw_g=rng.uniform(0.5, 0.8, shp))
+    arr["idx"] = np.arange(1, ncol + 1, dtype=np.int32)
+    ints = capi.sw_default_ints(ncol, ncol, nlay, blksz=blk, clrsky=True)
+    ref = dict(arr)
+    ref.update(O.sw_outputs(ncol, nlay))
+    O.sw_main_run(kd_sw, lut_sw, ints, ref)
+    s = lib.SWScheme(kd_sw, lut_sw, nlay, blksz=blk)
+    dev = device.to_device(arr)
+    dev.update(device.sw_outputs(ncol, nlay))
+    s.run_dev(ints, dev)
+    torch.cuda.synchronize()
+    out = device.to_host(dev)
+    _check(ref, out, np.arange(ncol))
+    mx, _ = report("htrswc", out["htrswc"] * 86400, ref["htrswc"] * 86400, "K/day")
+    assert mx < HR_TOL
+    for a, c in (("fluxswUP_allsky", "fluxswUP_clrsky"), ("fluxswDOWN_allsky", "fluxswDOWN_clrsky"),
+                 ("htrsw", "htrswc")):
+        assert np.array_equal(out[a][:, clear], out[c][:, clear]), a
+    cloudy = np.setdiff1d(np.arange(ncol), clear)
+    assert not np.array_equal(out["fluxswUP_allsky"][:, cloudy], out["fluxswUP_clrsky"][:, cloudy])
